@@ -1,0 +1,245 @@
+/*
+ * mdn_mech.h -- C ABI of the MI355X (gfx950) mechanical-equilibration library.
+ *
+ * The library replaces the mechanical-equilibration path of MEDYAN 5.4.0:
+ *   - ForceFieldManager energy / force evaluation over six force fields
+ *     (FilamentStretching<Harmonic>, FilamentBending<Cosine>, LinkerStretching<Harmonic>,
+ *      MotorGhostStretching<Harmonic>, CylinderExclVolume<CylinderExclVolRepulsion>,
+ *      BoundaryCylinderRepulsion<BoundaryCylinderRepulsionExp>),
+ *   - the cylinder-cylinder hybrid neighbor list and the boundary-cylinder neighbor list,
+ *   - CGMethod::minimize (steepest descent, Fletcher-Reeves, Polak-Ribiere) with the
+ *     backtracking line search, safe mode and recovery move.
+ *
+ * Conventions (all entry points):
+ *   - every function returns an int status, MDN_OK (0) on success; mdn_last_error() gives
+ *     a message for the last failing call on a context;
+ *   - host buffers are owned by the caller; device buffers are owned by the context;
+ *   - coordinates and forces are the reference's flat `coord` vector: 3*n_beads doubles,
+ *     bead-major (x,y,z per bead), in bead index order (src/Structure/DofSerializer.hpp:64-187);
+ *   - one context serves one caller thread (the reference path is single-threaded,
+ *     src/Controller/MController.h:108-211); contexts are not thread-safe.
+ *
+ * Citations are relative to medyan-5.4.0/ in the reference tree.
+ */
+#ifndef MDN_MECH_H
+#define MDN_MECH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDN_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------------------ */
+enum {
+    MDN_OK = 0,
+    MDN_ERR_ARG = 1,          /* invalid argument / inconsistent system description          */
+    MDN_ERR_HIP = 2,          /* HIP runtime error (no device, launch failure, ...)          */
+    MDN_ERR_STATE = 3,        /* call order violated (e.g. energy before upload)             */
+    MDN_ERR_NONFINITE = 4,    /* maxF non-finite: CGMethod.hpp:202-212 throws "maxF error"   */
+    MDN_ERR_LINESEARCH = 5,   /* line search failed with recovery disabled                   */
+    MDN_ERR_NL_BOUNDS = 6,    /* cylinder midpoint outside the neighbor-list grid            */
+    MDN_ERR_RECOVERY = 7,     /* recovery step infinite: MController.h:312-316               */
+    MDN_ERR_NOMEM = 8,
+    MDN_ERR_UNSUPPORTED = 9,
+    MDN_ERR_CAPACITY = 10     /* caller buffer too small                                     */
+};
+
+/* ---- force-field slots, in ForceFieldManager order (src/Controller/MController.cpp:70-178) */
+enum {
+    MDN_FF_STRETCH = 0,   /* FilamentStretching<FilamentStretchingHarmonic>              */
+    MDN_FF_BEND = 1,      /* FilamentBending<FilamentBendingCosine>                      */
+    MDN_FF_LINKER = 2,    /* LinkerStretching<LinkerStretchingHarmonic>                  */
+    MDN_FF_MOTOR = 3,     /* MotorGhostStretching<MotorGhostStretchingHarmonic>          */
+    MDN_FF_EXVOL = 4,     /* CylinderExclVolume<CylinderExclVolRepulsion>                */
+    MDN_FF_BOUNDARY = 5,  /* BoundaryCylinderRepulsion<BoundaryCylinderRepulsionExp>     */
+    MDN_NUM_FF = 6
+};
+
+/* ---- system description (the data serializeDof + vectorize read) ----------------------- */
+
+/* A linker or motor ghost: a spring between the point at relative position pos1 on
+ * cylinder cyl1 and pos2 on cyl2 (Linker.cpp:60-71, LinkerStretching.cpp:30-57). */
+typedef struct mdn_crosslink {
+    int32_t cyl1, cyl2;   /* global cylinder indices */
+    double pos1, pos2;    /* relative positions in [0,1] (adjustedrelativeposition = identity) */
+    double kstr, eql;     /* stretching constant (pN/nm), rest length (nm) */
+} mdn_crosslink;
+
+enum { MDN_BND_PLANE = 0, MDN_BND_SPHERE = 1 };
+
+/* A boundary element (src/Structure/BoundaryElementImpl.h:30-197).
+ * plane : p = {a, b, c, d}, distance = (a x + b y + c z + d) / |(a,b,c)|, normal = (a,b,c)
+ * sphere: p = {cx, cy, cz, radius}, distance = radius - |x - c|, normal = (c - x)/|c - x| */
+typedef struct mdn_boundary_element {
+    int32_t kind;
+    int32_t reserved;
+    double p[4];
+    double krep;          /* BoundaryK (pN)            */
+    double slen;          /* BScreenLength (nm)        */
+} mdn_boundary_element;
+
+/* The mechanical system. Cylinders are numbered filament-major: cylinder k (0-based) of
+ * filament f joins beads fil_beads[off_f + k] and fil_beads[off_f + k + 1] with
+ * off_f = fil_bead_offset[f]; its global index is off_f - f + k. The global index is also the
+ * cylinder id used by the neighbor-list half-list rule (HybridNeighborListImpl.cpp:413), and k
+ * its position on the filament (:415-418). Every filament has at least two beads. */
+typedef struct mdn_system_desc {
+    int32_t n_beads;
+    int32_t n_filaments;
+    const int32_t* fil_bead_offset;   /* [n_filaments + 1]                                   */
+    const int32_t* fil_beads;         /* bead indices, minus end first                       */
+    const int32_t* fil_type;          /* [n_filaments] filament type, or NULL (all type 0)   */
+    const double* cyl_kstr;           /* [n_cyl] FStretchingK * cylinderSize / eqLength      */
+    const double* cyl_eql;            /* [n_cyl] cylinder equilibrium length                 */
+    const double* cyl_kbend;          /* [n_cyl] bending constant; the bending term whose
+                                         second cylinder is c uses cyl_kbend[c]
+                                         (FilamentBending.cpp:29-60)                         */
+    const double* cyl_eqtheta;        /* [n_cyl] equilibrium bending angle                   */
+    const double* cyl_kexvol;         /* [n_cyl] excluded-volume constant                    */
+    int32_t n_linkers;
+    const mdn_crosslink* linkers;
+    int32_t n_motors;
+    const mdn_crosslink* motors;
+    int32_t n_boundary;
+    const mdn_boundary_element* boundary;   /* registry order = interaction order           */
+    double boundary_cutoff;           /* BoundaryCutoff (strict <, NeighborListImpl.cpp:748)  */
+    double exvol_rmax;                /* VolumeCutoff, passed through float like the
+                                         reference (HybridNeighborListImpl.h:74-80)          */
+    double exvol_rmin;
+    double grid_size[3];              /* neighbor-list grid covers [0, grid_size) per axis   */
+} mdn_system_desc;
+
+typedef struct mdn_energy_report {
+    double total;                     /* sum of ff[] in order; +inf on a failure sentinel    */
+    double ff[MDN_NUM_FF];
+} mdn_energy_report;
+
+enum { MDN_CG_STEEPEST = 0, MDN_CG_FLETCHER_REEVES = 1, MDN_CG_POLAK_RIBIERE = 2 };
+enum { MDN_PREC_FP64 = 0, MDN_PREC_MIXED = 1 };
+
+/* ConjugateGradientParams (CGMethod.hpp:44-63) plus the precision selector. */
+typedef struct mdn_cg_params {
+    int32_t method;               /* MDN_CG_*                                               */
+    int32_t try_recover;          /* tryToRecoverInLineSearchError                          */
+    double max_dist;              /* maxDist                                                */
+    double lambda_max;            /* lambdaMax                                              */
+    double grad_tol;              /* gradTol (max component |F|)                            */
+    double energy_rel_tol;        /* energyRelTol                                           */
+    int32_t force_call_limit;     /* raised to max(limit, 3 * n_beads) (CGMethod.hpp:855-858) */
+    int32_t energy_call_limit;    /* 0 = no limit                                           */
+    int32_t report_on_call_limit; /* restore minimum-energy coordinates on call limit       */
+    int32_t precision;            /* MDN_PREC_*                                             */
+} mdn_cg_params;
+
+enum { MDN_MIN_ERR_LINESEARCH = 1 };   /* MinimizationResult::errorLineSearch */
+
+/* MinimizationResult (MinimizationTypes.hpp:10-64) plus path statistics. */
+typedef struct mdn_min_result {
+    mdn_energy_report energies_before;
+    mdn_energy_report energies_after;
+    uint32_t error_flags;
+    int32_t num_energy_call;
+    int32_t num_force_call;
+    int32_t force_call_limit;
+    int32_t energy_call_limit;
+    int32_t num_iter;
+    int32_t num_safe_mode;        /* line-search failures that entered safe mode            */
+    int32_t num_recovery;         /* recovery moves                                         */
+    double grad_inf_norm;
+    double grad_tol;
+    double energy_rel_change;
+    double energy_rel_tol;
+    int32_t converged_grad;       /* gradInfNorm < gradTol fired                            */
+    int32_t converged_energy;     /* energyRelChange < energyRelTol fired                   */
+    int32_t call_limit_exceeded;
+    int32_t restored_min_energy;  /* copybackupcoordinates() restored coordMinE             */
+} mdn_min_result;
+
+/* ---- library ------------------------------------------------------------------------ */
+typedef struct mdn_ctx mdn_ctx;
+
+/* ABI version (MDN_ABI_VERSION). */
+int mdn_abi_version(void);
+
+/* Create a context bound to HIP device `device` with its own stream. */
+int mdn_ctx_create(int device, mdn_ctx** out);
+int mdn_ctx_destroy(mdn_ctx* ctx);
+const char* mdn_last_error(const mdn_ctx* ctx);
+
+/* Upload the system and its coordinates, and build the bonded and crosslink tables
+ * (ForceFieldManager::vectorizeAllForceFields, ForceFieldManager.cpp:35-50, for the
+ * stretching, bending, linker and motor force fields). */
+int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* sys, const double* coord);
+
+/* Counts: n_beads, n_cylinders, and interaction counts per force field (after mdn_nl_build
+ * for MDN_FF_EXVOL and MDN_FF_BOUNDARY). out must hold 2 + MDN_NUM_FF int64 values. */
+int mdn_system_counts(mdn_ctx* ctx, int64_t* out);
+
+/* Rebuild the excluded-volume neighbor list (HybridCylinderCylinderNL::reset,
+ * HybridNeighborListImpl.cpp:527-556) and the boundary neighbor list
+ * (BoundaryCylinderNL::reset, NeighborListImpl.cpp:801-809) from `coord` (host, or NULL for
+ * the context's current coordinates) and vectorize the two force fields
+ * (CylinderExclVolume.cpp:33-138, BoundaryCylinderRepulsion.cpp:32-87).
+ * n_pairs receives the number of excluded-volume pairs. */
+int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs);
+
+/* Copy the excluded-volume pair list (cylinder index pairs (c, c'), c > c', ordered by c then
+ * c') into out[2*cap]. */
+int mdn_nl_pairs(mdn_ctx* ctx, int32_t* out, int64_t cap, int64_t* n_pairs);
+
+/* Copy the boundary interaction list: bead index per interaction, element-major
+ * (BoundaryCylinderRepulsion.cpp:56-84), and per-element counts (nneighbors). */
+int mdn_boundary_list(mdn_ctx* ctx, int32_t* out_beads, int64_t cap, int32_t* out_nneighbors,
+                      int64_t* n_interactions);
+
+/* ForceFieldManager::computeIndividualEnergies (ForceFieldManager.cpp:209-227): per-FF
+ * energies at `coord` (host; NULL = current coordinates). A force field that reports the
+ * failure sentinel (energy <= -1, ForceFieldManager.cpp:150-161) makes total = +inf. */
+int mdn_energy(mdn_ctx* ctx, const double* coord, mdn_energy_report* out);
+
+/* ForceFieldManager::computeForces (ForceFieldManager.cpp:242-290): force[0..n_var) is
+ * zero-filled, then every force field accumulates in order. n_var = 3 * n_beads. */
+int mdn_forces(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
+
+/* ForceField::computeForces of one force field (ForceField.h:34-99): ACCUMULATES (+=) the
+ * forces of force field `ff` (MDN_FF_*) into force[0..n_var). */
+int mdn_ff_forces(mdn_ctx* ctx, int32_t ff, const double* coord, double* force, int32_t n_var);
+
+/* Linker / motor stretchForce after the last force evaluation
+ * (LinkerStretchingHarmonic.cpp:291, ForceField::assignforcemags). */
+int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor_out);
+
+/* CGMethod::minimize (CGMethod.hpp:826-1759) driven by MController::run
+ * (MController.h:108-211): minimizes from `coord` (host, in/out), writes the final forces to
+ * force_out (3*n_beads, may be NULL) and the statistics to res. The neighbor list must have
+ * been built (mdn_nl_build) and stays frozen during the minimization, as in the reference. */
+int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* params, double* coord, double* force_out,
+                 mdn_min_result* res);
+
+/* ---- device-resident variants (benchmarking, zero host traffic) ---------------------- */
+
+/* Time `iters` force evaluations (mdn_forces semantics) on the context's current device
+ * coordinates with HIP events on the context's stream; *ms_per_eval receives the mean
+ * wall time per evaluation and *kernel_ms the mean duration of the dominant kernel. */
+int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval,
+                     double* kernel_ms);
+
+/* Time `iters` total-energy evaluations the same way. */
+int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval);
+
+/* Set the current device coordinates from host memory (no NL rebuild). */
+int mdn_set_coords(mdn_ctx* ctx, const double* coord);
+int mdn_get_coords(mdn_ctx* ctx, double* coord);
+
+/* Mixed-precision force evaluation (FP32 force arithmetic on FP64-differenced bond vectors,
+ * FP64 accumulation); same semantics as mdn_forces. */
+int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDN_MECH_H */

@@ -1,0 +1,261 @@
+/*
+ * cg.c -- oracle restatement of CGMethod::minimize with the backtracking line search, and
+ * of the MController wiring with the recovery move. TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+/* CGMethod.hpp:98-107 (double build: LAMBDATOL fixed at 1e-8) */
+#define LAMBDAREDUCE 0.5
+#define LAMBDATOL 1e-8
+#define BACKTRACKSLOPE 0.4
+
+typedef struct {
+    int ndof, nvar;
+    double *coordLS, *coordMinE, *force, *forcePrev, *searchDir;
+    int have_min_e;
+    double minimumE, totalEnergy, maxForceBackup;
+} cg_state;
+
+static double dot_n(const double* a, const double* b, int n) {
+    double r = 0.0;
+    for (int i = 0; i < n; i++) r += a[i] * b[i];
+    return r;
+}
+
+/* maxF (CGMethod.hpp:202-212): component-wise max |F| over numDof; non-finite is an error */
+static int max_f(const cg_state* st, double* out) {
+    double m = 0;
+    for (int i = 0; i < st->ndof; i++) m = fmax(m, fabs(st->force[i]));
+    if (!isfinite(m)) return MDN_ERR_NONFINITE;
+    *out = m;
+    return MDN_OK;
+}
+
+static int below_relaxed_tol(const cg_state* st, double factor, double tol) {
+    for (int i = 0; i < st->ndof; i++)
+        if (fabs(st->force[i]) > factor * tol) return 0;
+    return 1;
+}
+
+static void shift_search_dir(cg_state* st, double d) {
+    /* CGMethod.cpp:356-360 */
+    for (int i = 0; i < st->ndof; i++) st->searchDir[i] = st->force[i] + d * st->searchDir[i];
+}
+
+typedef struct { double lambda; int numEnergyCall; double newEnergy; } ls_result;
+
+/* backtrackingLineSearch (CGMethod.hpp:238-373); the running-average manager is a fresh object
+ * per call with probability 0, so it never changes lambda (LineSearch.hpp:21-48). */
+static ls_result backtracking(cg_state* st, const double* coord, const or_cg_callbacks* cb,
+                              double MAXDIST, double maxForce, double LAMBDAMAX, double armijo) {
+    ls_result res = {0.0, 0, 0.0};
+    if (maxForce == 0.0) return res;
+    double lambda = fmin(LAMBDAMAX, MAXDIST / maxForce);
+    const double currentEnergy = cb->energy(cb->user, coord);
+    ++res.numEnergyCall;
+    const double expectedSlope = dot_n(st->searchDir, st->force, st->ndof);
+    const double reducedSlope = armijo * expectedSlope;
+    int conv = 0;
+    double energyChange = 0.0, energyLambda = 0.0;
+    while (!conv) {
+        for (int i = 0; i < st->ndof; i++) st->coordLS[i] = coord[i] + lambda * st->searchDir[i];
+        energyLambda = cb->energy(cb->user, st->coordLS);
+        ++res.numEnergyCall;
+        const double idealEnergyChange = -reducedSlope * lambda;
+        if (lambda < 0 || idealEnergyChange > 0) {
+            conv = 1;
+            lambda = 0.0;
+        }
+        energyChange = energyLambda - currentEnergy;
+        if (energyChange <= idealEnergyChange) conv = 1;
+        else lambda *= LAMBDAREDUCE;
+        if (lambda <= 0.0 || lambda <= LAMBDATOL) {
+            conv = 1;
+            lambda = 0.0;
+            energyLambda = currentEnergy;
+            energyChange = 0;
+        }
+    }
+    (void)energyChange;
+    st->totalEnergy = energyLambda;
+    res.lambda = lambda;
+    res.newEnergy = energyLambda;
+    return res;
+}
+
+static int call_limit_exceeded(const mdn_min_result* r) {
+    return (r->energy_call_limit != 0 && r->num_energy_call > r->energy_call_limit) ||
+           (r->force_call_limit != 0 && r->num_force_call > r->force_call_limit);
+}
+static int converged(const mdn_min_result* r) {
+    return r->grad_inf_norm < r->grad_tol || r->energy_rel_change < r->energy_rel_tol;
+}
+
+int or_cg_minimize(const mdn_cg_params* p, double* coord, int ndof, int nvar,
+                   const or_cg_callbacks* cb, double* force_out, mdn_min_result* res) {
+    memset(res, 0, sizeof(*res));
+    res->energy_call_limit = p->energy_call_limit;
+    res->grad_tol = p->grad_tol;
+    res->energy_rel_tol = p->energy_rel_tol;
+    {
+        int beadMaxStep = 3 * cb->n_beads;
+        res->force_call_limit = p->force_call_limit > beadMaxStep ? p->force_call_limit : beadMaxStep;
+    }
+    res->grad_inf_norm = INFINITY;
+    res->energy_rel_change = INFINITY;
+    const int method = p->method;
+
+    cg_state st;
+    memset(&st, 0, sizeof(st));
+    st.ndof = ndof;
+    st.nvar = nvar;
+    st.forcePrev = calloc(ndof > 0 ? ndof : 1, sizeof(double));
+    st.searchDir = calloc(ndof > 0 ? ndof : 1, sizeof(double));
+    st.coordLS = calloc(nvar > 0 ? nvar : 1, sizeof(double));
+    st.force = calloc(nvar > 0 ? nvar : 1, sizeof(double));
+    st.coordMinE = calloc(nvar > 0 ? nvar : 1, sizeof(double));
+    st.minimumE = INFINITY;  /* startMinimization (CGMethod.cpp:362-366) */
+    int status = MDN_OK;
+
+    cb->energy_individual(cb->user, coord, &res->energies_before);
+    ++res->num_energy_call;
+    cb->force(cb->user, coord, st.force, nvar);
+    ++res->num_force_call;
+    if ((status = max_f(&st, &res->grad_inf_norm)) != MDN_OK) goto done;
+    double prevEnergy = res->energies_before.total;
+
+    int safeMode = 0, numIter = 0;
+    double lastFDotF = 0;
+#define RESET_CG()                                                                 \
+    do {                                                                           \
+        shift_search_dir(&st, 0);                                                  \
+        if (method == MDN_CG_FLETCHER_REEVES || method == MDN_CG_POLAK_RIBIERE)    \
+            lastFDotF = dot_n(st.force, st.force, st.nvar);                        \
+        if (method == MDN_CG_POLAK_RIBIERE)                                        \
+            memcpy(st.forcePrev, st.force, sizeof(double) * st.ndof);              \
+    } while (0)
+    RESET_CG();
+
+    while (!call_limit_exceeded(res) && !converged(res)) {
+        double beta = 0, newGrad = 0, prevGrad = 0;
+        numIter++;
+        ls_result lsr = backtracking(&st, coord, cb, p->max_dist, res->grad_inf_norm,
+                                     p->lambda_max, safeMode ? 0 : BACKTRACKSLOPE);
+        const double lambda = lsr.lambda;
+        res->num_energy_call += lsr.numEnergyCall;
+        if (lambda == 0) {
+            if (p->try_recover) {
+                if (safeMode) {
+                    if (cb->recovery) {
+                        status = cb->recovery(cb->user, coord, st.searchDir, ndof);
+                        if (status != MDN_OK) goto done;
+                    }
+                    res->num_recovery++;
+                    cb->force(cb->user, coord, st.force, nvar);
+                    ++res->num_force_call;
+                    if ((status = max_f(&st, &res->grad_inf_norm)) != MDN_OK) goto done;
+                    RESET_CG();
+                    safeMode = 0;
+                } else {
+                    RESET_CG();
+                    safeMode = 1;
+                    res->num_safe_mode++;
+                }
+            } else {
+                res->error_flags |= MDN_MIN_ERR_LINESEARCH;
+                res->num_iter = numIter;
+                goto done_early;
+            }
+        } else {
+            for (int i = 0; i < ndof; i++) coord[i] += lambda * st.searchDir[i];
+            cb->force(cb->user, coord, st.force, nvar);
+            ++res->num_force_call;
+            if ((status = max_f(&st, &res->grad_inf_norm)) != MDN_OK) goto done;
+            res->energy_rel_change = fabs(lsr.newEnergy - prevEnergy) / fabs(prevEnergy);
+            prevEnergy = lsr.newEnergy;
+            if (method == MDN_CG_FLETCHER_REEVES) {
+                newGrad = dot_n(st.force, st.force, nvar);
+                beta = newGrad / lastFDotF;
+            } else if (method == MDN_CG_POLAK_RIBIERE) {
+                newGrad = dot_n(st.force, st.force, nvar);
+                prevGrad = dot_n(st.force, st.forcePrev, ndof);
+                beta = fmax(0.0, (newGrad - prevGrad) / lastFDotF);
+            }
+            shift_search_dir(&st, beta);
+            if (dot_n(st.searchDir, st.force, ndof) <= 0) RESET_CG();
+            if (method == MDN_CG_POLAK_RIBIERE) memcpy(st.forcePrev, st.force, sizeof(double) * ndof);
+            if (below_relaxed_tol(&st, 10, p->grad_tol) && numIter > res->force_call_limit / 2) {
+                /* copycoordsifminimumE (CGMethod.hpp:794-803) */
+                if (st.totalEnergy <= st.minimumE) {
+                    st.minimumE = st.totalEnergy;
+                    st.maxForceBackup = res->grad_inf_norm;
+                    memcpy(st.coordMinE, coord, sizeof(double) * nvar);
+                    st.have_min_e = 1;
+                }
+            }
+            if (method == MDN_CG_FLETCHER_REEVES || method == MDN_CG_POLAK_RIBIERE) lastFDotF = newGrad;
+        }
+    }
+#undef RESET_CG
+    res->num_iter = numIter;
+    res->call_limit_exceeded = call_limit_exceeded(res);
+    if (p->report_on_call_limit && res->call_limit_exceeded) {
+        if (st.have_min_e) {  /* copybackupcoordinates (CGMethod.hpp:805-814) */
+            memcpy(coord, st.coordMinE, sizeof(double) * nvar);
+            res->restored_min_energy = 1;
+        }
+    }
+    cb->energy_individual(cb->user, coord, &res->energies_after);
+    ++res->num_energy_call;
+done_early:
+    res->converged_grad = res->grad_inf_norm < res->grad_tol;
+    res->converged_energy = res->energy_rel_change < res->energy_rel_tol;
+    res->call_limit_exceeded = call_limit_exceeded(res);
+    if (force_out) memcpy(force_out, st.force, sizeof(double) * nvar);
+done:
+    free(st.forcePrev); free(st.searchDir); free(st.coordLS); free(st.force); free(st.coordMinE);
+    return status;
+}
+
+/* ---- MController wiring ----------------------------------------------------------- */
+
+typedef struct { or_system* s; double max_dist; } sys_user;
+
+static double sys_energy(void* u, const double* coord) {
+    return or_ffm_energy(((sys_user*)u)->s, coord, NULL);
+}
+static void sys_force(void* u, const double* coord, double* f, int nvar) {
+    (void)nvar;
+    or_ffm_forces(((sys_user*)u)->s, coord, f);
+}
+static void sys_energy_ind(void* u, const double* coord, mdn_energy_report* rep) {
+    or_ffm_energy(((sys_user*)u)->s, coord, rep);
+}
+/* moveAlongDirectionLimitingBeadMove_ (MController.h:273-339) */
+static int sys_recovery(void* u, double* coord, const double* dir, int ndof) {
+    double step = INFINITY;
+    double maxd = ((sys_user*)u)->max_dist;
+    for (int i = 0; i < ndof; i++) {
+        double v = maxd / fabs(dir[i]);
+        if (v < step) step = v;
+    }
+    if (step == INFINITY) return MDN_ERR_RECOVERY;
+    for (int i = 0; i < ndof; i++) coord[i] += step * dir[i];
+    return MDN_OK;
+}
+
+int or_system_minimize(or_system* s, const mdn_cg_params* p, double* coord, double* force_out,
+                       mdn_min_result* res) {
+    sys_user u = {s, p->max_dist};
+    or_cg_callbacks cb = {sys_energy, sys_force, sys_energy_ind, sys_recovery, &u, s->n_beads};
+    int ndof = 3 * s->n_beads;
+    int st = or_cg_minimize(p, coord, ndof, ndof, &cb, force_out, res);
+    if (st != MDN_OK) return st;
+    /* MController.h:142-155: a line-search error runs a diagnosis and throws */
+    if (res->error_flags & MDN_MIN_ERR_LINESEARCH) return MDN_ERR_LINESEARCH;
+    return MDN_OK;
+}
