@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark of the mechanical-equilibration hot path on one MI355X.
+
+Workload: the single large network (2,000 filaments x 20 cylinders = 40,000 cylinders /
+42,000 beads in a 5 um cube, 8,000 linkers, 4,000 motors, seeded synthetic coordinates),
+neighbor list built once from the start coordinates, as the reference freezes it per
+minimization.
+
+A "step" is one force evaluation (ForceFieldManager::computeForces: six force fields, the
+gather into the 3 x 42,000 force vector) with the coordinates resident in HBM. `value` is
+force-evaluations per second over the K timed steps. The same JSON line also carries the CG
+minimizations/s of one full minimization from the seeded start to the reference criterion,
+the roofline of the dominant kernel (HIP events on the library's stream) and the
+single-core CPU baseline (the oracle built with the reference's release flags, timed on a
+bounded sample of the same workload).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-min]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from picgo_amd import abi, mech, synth  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+WORKLOAD = "config1: 2000 filaments x 20 cylinders (42000 beads), 8000 linkers, 4000 motors, 5 um cube"
+
+
+def algorithmic_bytes(counts, n_beads, sx=8, sf=8, sp=8, idx=4):
+    """Bytes a force evaluation must move at minimum: coordinates in, forces out, each
+    interaction table read once: (bead indices, parameters) = stretch (2,2), bend (3,2),
+    linker / motor (4,4), excluded volume (4,3), boundary (1,2)."""
+    per = {"FilamentStretching": (2, 2), "FilamentBending": (3, 2), "LinkerStretching": (4, 4),
+           "MotorGhostStretching": (4, 4), "CylinderExclVolume": (4, 3), "BoundaryCylinderRepulsion": (1, 2)}
+    b = 3 * n_beads * (sx + sf)
+    for name, (ni, npar) in per.items():
+        b += counts[name] * (idx * ni + sp * npar)
+    return b
+
+
+def cpu_baseline(net, target_s=15.0):
+    """Oracle (liboracle_fast.so: the reference's release flags) on one core: force
+    evaluations of the same config-1 network for about target_s seconds."""
+    from oracle import oracle_ffi as of
+    o = of.OracleSystem(net.desc, fast=True)
+    o.build_nl(net.coord)
+    o.forces(net.coord)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.forces(net.coord)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= target_s:
+            break
+    return {"value": n / el, "unit": "force-evals/s", "cores": 1, "kind": "port",
+            "sample": f"{n} force evaluations of the config-1 network ({el:.1f} s, oracle built "
+                      f"-O2 -funroll-loops -flto -ffast-math, single thread)"}
+
+
+def load_traffic(kernel):
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_config1.json")
+    if os.path.exists(p):
+        with open(p) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-min", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    net = synth.Network.config(1)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    n_pairs = ffm.reset_neighbor_lists(net.coord)
+    counts = ffm.counts()
+    print(f"# {WORKLOAD}; {n_pairs} excluded-volume pairs; counts {counts}", file=sys.stderr, flush=True)
+
+    # warm-up, then exactly K timed force evaluations (library syncs its stream at the end)
+    ffm.bench_forces(max(1, args.warmup))
+    t0 = time.perf_counter()
+    ms_eval, ms_kernel = ffm.bench_forces(args.steps)
+    wall = time.perf_counter() - t0
+    evals_per_s = 1000.0 / ms_eval
+    bytes_alg = algorithmic_bytes(counts, net.n_beads)
+    achieved = bytes_alg / (ms_kernel * 1e-3) / 1e9
+    print(f"# force eval {ms_eval * 1e3:.1f} us (terms kernel {ms_kernel * 1e3:.1f} us), "
+          f"{args.steps} steps in {wall:.3f} s wall", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_min:
+        p = abi.default_cg_params()
+        ffm.minimize(abi.default_cg_params(energy_call_limit=50), net.coord)   # warm
+        t1 = time.perf_counter()
+        x, f, res = ffm.minimize(p, net.coord)
+        tmin = time.perf_counter() - t1
+        extra = {"minimizations_per_s": 1.0 / tmin, "minimization_s": tmin,
+                 "cg_iterations": res.num_iter, "energy_calls": res.num_energy_call,
+                 "force_calls": res.num_force_call,
+                 "criterion": "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit"),
+                 "final_max_abs_force_pN": res.grad_inf_norm}
+        print(f"# minimization {tmin:.3f} s, {res.num_iter} iterations, criterion {extra['criterion']}",
+              file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
+        "value": evals_per_s,
+        "unit": "force-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_eval,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded generator, config 1)",
+        "config": {"workload": WORKLOAD, "n_beads": net.n_beads, "n_cylinders": net.n_cylinders,
+                   "exvol_pairs": n_pairs, "precision_path": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("k_forces_terms"),
+                     "kernel": "k_forces_terms", "algorithmic_bytes": bytes_alg},
+    }
+    line.update(extra)
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(net, args.cpu_seconds)
+        line["cpu_baseline"]["host_nproc"] = os.cpu_count()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

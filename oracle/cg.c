@@ -13,6 +13,10 @@
 #define LAMBDATOL 1e-8
 #define BACKTRACKSLOPE 0.4
 
+/* std::max / std::min semantics */
+static inline double rmax(double a, double b) { return (a < b) ? b : a; }
+static inline double rmin(double a, double b) { return (b < a) ? b : a; }
+
 typedef struct {
     int ndof, nvar;
     double *coordLS, *coordMinE, *force, *forcePrev, *searchDir;
@@ -29,7 +33,7 @@ static double dot_n(const double* a, const double* b, int n) {
 /* maxF (CGMethod.hpp:202-212): component-wise max |F| over numDof; non-finite is an error */
 static int max_f(const cg_state* st, double* out) {
     double m = 0;
-    for (int i = 0; i < st->ndof; i++) m = fmax(m, fabs(st->force[i]));
+    for (int i = 0; i < st->ndof; i++) m = rmax(m, fabs(st->force[i]));
     if (!isfinite(m)) return MDN_ERR_NONFINITE;
     *out = m;
     return MDN_OK;
@@ -54,7 +58,7 @@ static ls_result backtracking(cg_state* st, const double* coord, const or_cg_cal
                               double MAXDIST, double maxForce, double LAMBDAMAX, double armijo) {
     ls_result res = {0.0, 0, 0.0};
     if (maxForce == 0.0) return res;
-    double lambda = fmin(LAMBDAMAX, MAXDIST / maxForce);
+    double lambda = rmin(LAMBDAMAX, MAXDIST / maxForce);
     const double currentEnergy = cb->energy(cb->user, coord);
     ++res.numEnergyCall;
     const double expectedSlope = dot_n(st->searchDir, st->force, st->ndof);
@@ -183,7 +187,7 @@ int or_cg_minimize(const mdn_cg_params* p, double* coord, int ndof, int nvar,
             } else if (method == MDN_CG_POLAK_RIBIERE) {
                 newGrad = dot_n(st.force, st.force, nvar);
                 prevGrad = dot_n(st.force, st.forcePrev, ndof);
-                beta = fmax(0.0, (newGrad - prevGrad) / lastFDotF);
+                beta = rmax(0.0, (newGrad - prevGrad) / lastFDotF);
             }
             shift_search_dir(&st, beta);
             if (dot_n(st.searchDir, st.force, ndof) <= 0) RESET_CG();

@@ -31,6 +31,9 @@ static inline void mid_point(double* v, const double* v1, const double* v2, doub
     v[2] = (v1[2] * beta + alpha * v2[2]);
 }
 
+/* std::max / std::min semantics: max(a, b) = (a < b) ? b : a, min(a, b) = (b < a) ? b : a */
+static inline double rmax(double a, double b) { return (a < b) ? b : a; }
+
 /* areEqual with ZERO_PREC = 1e-6 for double (src/utility.h:58-77) */
 static inline int are_equal(double a, double b) { return fabs(a - b) < 1e-6; }
 
@@ -99,7 +102,7 @@ double or_bend_energy(const double* coord, int nint, const int* beadSet, const d
             U_i = kbend[i] * (1 - x);
         } else {
             double cosA = x;
-            double sinA = fmax(sqrt(1 - cosA * cosA), 0.0);
+            double sinA = rmax(sqrt(1 - cosA * cosA), 0.0);
             double cosAminusB = cosA * cos(eqt[i]) + sinA * sin(eqt[i]);
             U_i = kbend[i] * (1 - cosAminusB);
         }
@@ -133,7 +136,7 @@ void or_bend_forces(const double* coord, double* f, int nint, const int* beadSet
             if (x < -1.0) x = -1.0;
             else if (x > 1.0) x = 1.0;
             double cosp = x;
-            double sinp = sqrt(fmax((1 - cosp * cosp), 0.0));
+            double sinp = sqrt(rmax((1 - cosp * cosp), 0.0));
             double sinpminusq = sinp * cos(eqt[i]) - cosp * sin(eqt[i]);
             k = kbend[i] * sinpminusq / sinp;
         }
@@ -280,12 +283,12 @@ static void exvol_terms(const double* c1, const double* c2, const double* c3, co
     double bc = sqrt(b * c);
     double aD = sqrt(a * sqmag_D);
     double bE = sqrt(b * sqmag_E);
-    t->AA = sqrt(fmax(0.0, (ac + e) * (ac - e)));
-    t->BB = sqrt(fmax(0.0, (bc + F) * (bc - F)));
+    t->AA = sqrt(rmax(0.0, (ac + e) * (ac - e)));
+    t->BB = sqrt(rmax(0.0, (bc + F) * (bc - F)));
     t->CC = d * e - a * F;
     t->DD = b * e - d * F;
-    t->EE = sqrt(fmax(0.0, (aD + t->g) * (aD - t->g)));
-    t->FF = sqrt(fmax(0.0, (bE + t->h) * (bE - t->h)));
+    t->EE = sqrt(rmax(0.0, (aD + t->g) * (aD - t->g)));
+    t->FF = sqrt(rmax(0.0, (bE + t->h) * (bE - t->h)));
     t->GG = d * t->g - a * t->I;
     t->HH = t->CC + t->GG - t->DD;
     cross3(cp, vA, vB);

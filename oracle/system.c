@@ -154,7 +154,7 @@ int or_system_nl(or_system* s, const double* coord) {
         s->ev_eqlen[2 * p] = s->cyl_eql[ci];
         s->ev_eqlen[2 * p + 1] = s->cyl_eql[cn];
         if (s->cyl_type[ci] != s->cyl_type[cn])
-            s->ev_krep[p] = fmax(s->cyl_kexvol[ci], s->cyl_kexvol[cn]);
+            s->ev_krep[p] = (s->cyl_kexvol[ci] < s->cyl_kexvol[cn]) ? s->cyl_kexvol[cn] : s->cyl_kexvol[ci];
         else
             s->ev_krep[p] = s->cyl_kexvol[ci];
     }

@@ -1,0 +1,951 @@
+// mdn_mech.hip -- host side of the gfx950 mechanical-equilibration library: context, system
+// upload (vectorize), neighbor-list rebuild, energy / force entry points and the
+// device-driven conjugate-gradient minimizer, behind the C ABI of include/mdn_mech.h.
+//
+// Citations are relative to medyan-5.4.0/ in the reference tree.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mdn_mech.h"
+#include "mdn_kernels.h"
+
+using namespace mdn;
+
+namespace {
+
+constexpr int SLOTS_PER_TERM[NFF] = {2, 3, 4, 4, 4, 1};
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        size_t m = std::max<size_t>(want, 1);
+        hipError_t e = hipMalloc(&p, m * sizeof(T));
+        if (e == hipSuccess) n = m;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct HipErr {
+    hipError_t e;
+    const char* what;
+};
+
+}  // namespace
+
+struct mdn_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool uploaded = false, nl_built = false;
+
+    // system (host copies used by vectorize)
+    int n_beads = 0, n_cyl = 0, n_fil = 0;
+    std::vector<int> cyl_b1, cyl_b2, cyl_fil, cyl_pos, cyl_type;
+    std::vector<char> cyl_minus;
+    std::vector<double> cyl_kexvol, cyl_eql;
+    std::vector<mdn_boundary_element> elems;
+    double bnd_cutoff = 0, rmax = 0, rmin = 0, grid[3] = {0, 0, 0};
+
+    // device tables
+    DBuf<int> str_b, bend_b, lnk_b, mot_b, ev_b, bnd_b, bnd_e;
+    DBuf<double> str_k, str_eql, bend_k, bend_eqt, lnk_k, lnk_eql, lnk_p1, lnk_p2, lnk_sf;
+    DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
+    DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor;
+    // cylinder tables for the NL
+    DBuf<int> d_cyl_b1, d_cyl_b2, d_cyl_fil, d_cyl_pos, d_cyl_type;
+    DBuf<char> d_cyl_minus;
+    DBuf<double> d_cyl_kexvol, d_cyl_eql, d_krep_e, d_slen_e;
+    // NL workspace
+    DBuf<double> mid;
+    DBuf<int> bin, bin_count, bin_start, bin_cursor, members, nl_count, nl_off, pairs, bnd_cnt, bnd_off;
+    DBuf<int> nl_err;
+    DBuf<unsigned char> cub_tmp;
+    long n_pairs = 0;
+    int n_bint = 0;
+    std::vector<int> bnd_nneighbors;
+
+    // state vectors
+    DBuf<double> x[2], f, fp, d, xmin, scratch_f, ext;
+    DBuf<D3> slots;
+    DBuf<double> partial;
+    DBuf<int> partial_i, bad_count, bad_list;
+    DBuf<unsigned> ticket;
+    DBuf<double> eout;
+    DBuf<CgState> cg;
+    DBuf<double> fallback_partial;
+    DBuf<int> fallback_ok;
+    CgState* h_cg = nullptr;   // pinned
+    double* h_eout = nullptr;  // pinned
+    int* h_int = nullptr;      // pinned
+
+    DevSys sys{};
+    int cur = 0;   // which x buffer holds the current coordinates
+
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_slots = 0;
+};
+
+namespace {
+
+int fail(mdn_ctx* c, int code, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(ctx, MDN_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                                    \
+    } while (0)
+
+template <typename T>
+hipError_t upload(mdn_ctx* ctx, DBuf<T>& b, const std::vector<T>& v) {
+    hipError_t e = b.ensure(v.size());
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+    return e;
+}
+
+int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Block and slot layout of the term / bead grids for the current interaction counts.
+void layout(mdn_ctx* ctx) {
+    DevSys& s = ctx->sys;
+    s.n_beads = ctx->n_beads;
+    s.blk[0] = 0;
+    s.slot0[0] = 0;
+    for (int ff = 0; ff < NFF; ff++) {
+        s.blk[ff + 1] = s.blk[ff] + ceil_div(s.n[ff], BLOCK);
+        s.slot0[ff + 1] = s.slot0[ff] + (long)SLOTS_PER_TERM[ff] * s.n[ff];
+    }
+    s.blk[NFF + 1] = s.blk[NFF] + ceil_div(ctx->n_beads, BLOCK);
+}
+
+int bead_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, BLOCK)); }
+
+void refresh_ptrs(mdn_ctx* ctx) {
+    DevSys& s = ctx->sys;
+    s.str_b = ctx->str_b.p; s.str_k = ctx->str_k.p; s.str_eql = ctx->str_eql.p;
+    s.bend_b = ctx->bend_b.p; s.bend_k = ctx->bend_k.p; s.bend_eqt = ctx->bend_eqt.p;
+    s.lnk_b = ctx->lnk_b.p; s.lnk_k = ctx->lnk_k.p; s.lnk_eql = ctx->lnk_eql.p; s.lnk_p1 = ctx->lnk_p1.p;
+    s.lnk_p2 = ctx->lnk_p2.p; s.lnk_sf = ctx->lnk_sf.p;
+    s.mot_b = ctx->mot_b.p; s.mot_k = ctx->mot_k.p; s.mot_eql = ctx->mot_eql.p; s.mot_p1 = ctx->mot_p1.p;
+    s.mot_p2 = ctx->mot_p2.p; s.mot_sf = ctx->mot_sf.p;
+    s.ev_b = ctx->ev_b.p; s.ev_ks = ctx->ev_ks.p;
+    s.bnd_b = ctx->bnd_b.p; s.bnd_e = ctx->bnd_e.p; s.bnd_krep = ctx->bnd_krep.p; s.bnd_slen = ctx->bnd_slen.p;
+    s.inc_s_start = ctx->inc_s_start.p; s.inc_s = ctx->inc_s.p;
+    s.inc_d_start = ctx->inc_d_start.p; s.inc_d = ctx->inc_d.p;
+    s.n_elem = (int)ctx->elems.size();
+    for (int e = 0; e < s.n_elem; e++) {
+        s.elem[e].kind = ctx->elems[e].kind;
+        for (int q = 0; q < 4; q++) s.elem[e].p[q] = ctx->elems[e].p[q];
+    }
+}
+
+// Size the per-evaluation buffers (slots, partials) for the current layout.
+int ensure_eval_buffers(mdn_ctx* ctx) {
+    const DevSys& s = ctx->sys;
+    HIPCHK(ctx->slots.ensure((size_t)s.slot0[NFF]));
+    HIPCHK(ctx->partial.ensure((size_t)(s.blk[NFF + 1] + 1) * 16));
+    HIPCHK(ctx->partial_i.ensure((size_t)(s.blk[NFF + 1] + 1) * 16));
+    HIPCHK(ctx->bad_list.ensure((size_t)std::max(1, s.n[FF_EXVOL])));
+    return MDN_OK;
+}
+
+NLDev nl_view(mdn_ctx* ctx) {
+    NLDev n{};
+    n.n_cyl = ctx->n_cyl;
+    n.cyl_b1 = ctx->d_cyl_b1.p; n.cyl_b2 = ctx->d_cyl_b2.p; n.cyl_fil = ctx->d_cyl_fil.p;
+    n.cyl_pos = ctx->d_cyl_pos.p; n.cyl_type = ctx->d_cyl_type.p;
+    n.cyl_kexvol = ctx->d_cyl_kexvol.p; n.cyl_eql = ctx->d_cyl_eql.p; n.cyl_minus = ctx->d_cyl_minus.p;
+    // setneighborsearchparameters(..., float rMax, float rMin): squares formed in float
+    // (HybridNeighborListImpl.h:74-80)
+    float rmaxf = (float)ctx->rmax, rminf = (float)ctx->rmin;
+    float rmaxsq = rmaxf * rmaxf, rminsq = rminf * rminf;
+    n.rmaxsq = rmaxsq;
+    n.rminsq = rminsq;
+    n.side = 1.125 * std::sqrt((double)rmaxsq);   // HybridNeighborListImpl.cpp:180
+    for (int q = 0; q < 3; q++) {
+        n.grid[q] = ctx->grid[q];
+        n.nb[q] = std::max(1, (int)std::ceil(ctx->grid[q] / n.side));
+    }
+    return n;
+}
+
+int exclusive_scan(mdn_ctx* ctx, const int* in, int* out, long n) {
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream));
+    HIPCHK(ctx->cub_tmp.ensure(bytes));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(ctx->cub_tmp.p, bytes, in, out, (int)n, ctx->stream));
+    return MDN_OK;
+}
+
+int need_system(mdn_ctx* ctx, bool need_nl) {
+    if (!ctx) return MDN_ERR_ARG;
+    if (!ctx->uploaded) return fail(ctx, MDN_ERR_STATE, "no system uploaded (mdn_system_upload)");
+    if (need_nl && !ctx->nl_built) return fail(ctx, MDN_ERR_STATE, "neighbor lists not built (mdn_nl_build)");
+    return MDN_OK;
+}
+
+int set_coords_dev(mdn_ctx* ctx, const double* coord) {
+    if (coord)
+        HIPCHK(hipMemcpyAsync(ctx->x[ctx->cur].p, coord, sizeof(double) * 3 * ctx->n_beads,
+                              hipMemcpyHostToDevice, ctx->stream));
+    return MDN_OK;
+}
+
+// Term forces + numerical fallback for non-finite excluded-volume pairs, into ctx->slots, at
+// device coordinates x.
+int eval_slots(mdn_ctx* ctx, const double* x) {
+    const DevSys& s = ctx->sys;
+    HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+    if (s.blk[NFF] > 0)
+        hipLaunchKernelGGL(k_forces_terms, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x, ctx->slots.p,
+                           ctx->bad_count.p, ctx->bad_list.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int nbad = ctx->h_int[0];
+    if (nbad == 0) return MDN_OK;
+    std::vector<int> bad(nbad);
+    HIPCHK(hipMemcpy(bad.data(), ctx->bad_list.p, sizeof(int) * nbad, hipMemcpyDeviceToHost));
+    std::sort(bad.begin(), bad.end());
+    const int G = 240;
+    HIPCHK(ctx->fallback_partial.ensure(6 * G));
+    HIPCHK(ctx->fallback_ok.ensure(1));
+    for (int t : bad) {
+        // forceN, retried once with beads 1 and 2 moved out of plane (:1198-1204)
+        int ok = 0;
+        for (int mv = 0; mv < 2 && !ok; mv++) {
+            hipLaunchKernelGGL(k_exvol_forceN_rows, dim3(G), dim3(BLOCK), 0, ctx->stream, s, x, t, mv,
+                               ctx->fallback_partial.p);
+            hipLaunchKernelGGL(k_exvol_forceN_finish, dim3(1), dim3(64), 0, ctx->stream, s, x, t, mv, G,
+                               ctx->fallback_partial.p, ctx->slots.p, ctx->fallback_ok.p);
+            HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->fallback_ok.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            ok = ctx->h_int[0];
+        }
+        if (!ok)
+            return fail(ctx, MDN_ERR_NONFINITE,
+                        "excluded-volume pair %d: force non-finite after the numerical fallback", t);
+    }
+    return MDN_OK;
+}
+
+int gather(mdn_ctx* ctx, long lo, long hi, const double* base, double* out) {
+    hipLaunchKernelGGL(k_gather_plain, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                       (const D3*)ctx->slots.p, lo, hi, base, out);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+}
+
+// Per-FF energies at device coordinates x into ctx->h_eout[0..7] (synchronous).
+int energy_report(mdn_ctx* ctx, const double* x) {
+    const DevSys& s = ctx->sys;
+    int grid = std::max(1, s.blk[NFF]);
+    hipLaunchKernelGGL(k_energy_report, dim3(grid), dim3(BLOCK), 0, ctx->stream, s, x, ctx->partial.p,
+                       ctx->partial_i.p, ctx->ticket.p + 7, ctx->eout.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->h_eout, ctx->eout.p, sizeof(double) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+void fill_report(const double* e, mdn_energy_report* out) {
+    for (int ff = 0; ff < NFF; ff++) out->ff[ff] = e[ff];
+    out->total = e[6];
+}
+
+}  // namespace
+
+// =========================================================================================
+// C ABI
+// =========================================================================================
+extern "C" {
+
+int mdn_abi_version(void) { return MDN_ABI_VERSION; }
+
+const char* mdn_last_error(const mdn_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mdn_ctx_create(int device, mdn_ctx** out) {
+    if (!out) return MDN_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MDN_ERR_HIP;
+    if (device < 0 || device >= n) return MDN_ERR_ARG;
+    mdn_ctx* ctx = new mdn_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MDN_ERR_HIP;
+    }
+    if (hipHostMalloc(&ctx->h_cg, sizeof(CgState)) != hipSuccess ||
+        hipHostMalloc(&ctx->h_eout, sizeof(double) * 8) != hipSuccess ||
+        hipHostMalloc(&ctx->h_int, sizeof(int) * 16) != hipSuccess) {
+        delete ctx;
+        return MDN_ERR_HIP;
+    }
+    if (ctx->ticket.ensure(8) != hipSuccess || ctx->eout.ensure(8) != hipSuccess ||
+        ctx->bad_count.ensure(1) != hipSuccess || ctx->cg.ensure(1) != hipSuccess ||
+        ctx->nl_err.ensure(1) != hipSuccess) {
+        delete ctx;
+        return MDN_ERR_HIP;
+    }
+    (void)hipMemset(ctx->ticket.p, 0, sizeof(unsigned) * 8);
+    *out = ctx;
+    return MDN_OK;
+}
+
+int mdn_ctx_destroy(mdn_ctx* ctx) {
+    if (!ctx) return MDN_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    DBuf<int>* ib[] = {&ctx->str_b, &ctx->bend_b, &ctx->lnk_b, &ctx->mot_b, &ctx->ev_b, &ctx->bnd_b, &ctx->bnd_e,
+                       &ctx->inc_s_start, &ctx->inc_s, &ctx->inc_d_start, &ctx->inc_d, &ctx->inc_cnt,
+                       &ctx->inc_cursor, &ctx->d_cyl_b1, &ctx->d_cyl_b2, &ctx->d_cyl_fil, &ctx->d_cyl_pos,
+                       &ctx->d_cyl_type, &ctx->bin, &ctx->bin_count, &ctx->bin_start, &ctx->bin_cursor,
+                       &ctx->members, &ctx->nl_count, &ctx->nl_off, &ctx->pairs, &ctx->bnd_cnt, &ctx->bnd_off,
+                       &ctx->nl_err, &ctx->partial_i, &ctx->bad_count, &ctx->bad_list, &ctx->fallback_ok};
+    for (auto* b : ib) b->release();
+    DBuf<double>* db[] = {&ctx->str_k, &ctx->str_eql, &ctx->bend_k, &ctx->bend_eqt, &ctx->lnk_k, &ctx->lnk_eql,
+                          &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
+                          &ctx->mot_p2, &ctx->mot_sf, &ctx->ev_ks, &ctx->bnd_krep, &ctx->bnd_slen,
+                          &ctx->d_cyl_kexvol, &ctx->d_cyl_eql, &ctx->d_krep_e, &ctx->d_slen_e, &ctx->mid,
+                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d, &ctx->xmin, &ctx->scratch_f,
+                          &ctx->ext, &ctx->partial, &ctx->eout, &ctx->fallback_partial};
+    for (auto* b : db) b->release();
+    ctx->d_cyl_minus.release();
+    ctx->cub_tmp.release();
+    ctx->slots.release();
+    ctx->ticket.release();
+    ctx->cg.release();
+    if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
+    if (ctx->h_eout) (void)hipHostFree(ctx->h_eout);
+    if (ctx->h_int) (void)hipHostFree(ctx->h_int);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MDN_OK;
+}
+
+int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coord) {
+    if (!ctx || !d || !coord) return fail(ctx, MDN_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (d->n_beads <= 0 || d->n_filaments < 0) return fail(ctx, MDN_ERR_ARG, "bad bead / filament counts");
+    if (d->n_boundary > MAXELEM) return fail(ctx, MDN_ERR_UNSUPPORTED, "at most %d boundary elements", MAXELEM);
+    ctx->uploaded = ctx->nl_built = false;
+    ctx->n_beads = d->n_beads;
+    ctx->n_fil = d->n_filaments;
+    int n_cyl = 0;
+    for (int f = 0; f < d->n_filaments; f++) {
+        int nb = d->fil_bead_offset[f + 1] - d->fil_bead_offset[f];
+        if (nb < 2) return fail(ctx, MDN_ERR_ARG, "filament %d has %d beads (needs >= 2)", f, nb);
+        n_cyl += nb - 1;
+    }
+    ctx->n_cyl = n_cyl;
+    ctx->cyl_b1.assign(n_cyl, 0); ctx->cyl_b2.assign(n_cyl, 0); ctx->cyl_fil.assign(n_cyl, 0);
+    ctx->cyl_pos.assign(n_cyl, 0); ctx->cyl_type.assign(n_cyl, 0); ctx->cyl_minus.assign(n_cyl, 0);
+    ctx->cyl_kexvol.assign(n_cyl, 0); ctx->cyl_eql.assign(n_cyl, 0);
+    // FilamentStretching::vectorize (FilamentStretching.cpp:26-45)
+    std::vector<int> str_b(2 * (size_t)n_cyl);
+    std::vector<double> str_k(n_cyl), str_eql(n_cyl);
+    int c = 0;
+    for (int f = 0; f < d->n_filaments; f++) {
+        int off = d->fil_bead_offset[f], nb = d->fil_bead_offset[f + 1] - off;
+        for (int k = 0; k < nb - 1; k++, c++) {
+            int b1 = d->fil_beads[off + k], b2 = d->fil_beads[off + k + 1];
+            if (b1 < 0 || b1 >= d->n_beads || b2 < 0 || b2 >= d->n_beads)
+                return fail(ctx, MDN_ERR_ARG, "filament %d bead index out of range", f);
+            ctx->cyl_b1[c] = b1; ctx->cyl_b2[c] = b2; ctx->cyl_fil[c] = f; ctx->cyl_pos[c] = k;
+            ctx->cyl_type[c] = d->fil_type ? d->fil_type[f] : 0;
+            ctx->cyl_minus[c] = (k == 0);
+            ctx->cyl_kexvol[c] = d->cyl_kexvol[c];
+            ctx->cyl_eql[c] = d->cyl_eql[c];
+            str_b[2 * c] = b1; str_b[2 * c + 1] = b2;
+            str_k[c] = d->cyl_kstr[c]; str_eql[c] = d->cyl_eql[c];
+        }
+    }
+    // FilamentBending::vectorize (FilamentBending.cpp:29-60)
+    int n_bend = n_cyl - d->n_filaments;
+    std::vector<int> bend_b(3 * (size_t)std::max(0, n_bend));
+    std::vector<double> bend_k(std::max(0, n_bend)), bend_eqt(std::max(0, n_bend));
+    {
+        int i = 0;
+        c = 0;
+        for (int f = 0; f < d->n_filaments; f++) {
+            int nb = d->fil_bead_offset[f + 1] - d->fil_bead_offset[f];
+            for (int k = 1; k < nb - 1; k++, i++) {
+                int cp = c + k - 1, cc = c + k;
+                bend_b[3 * i] = ctx->cyl_b1[cp];
+                bend_b[3 * i + 1] = ctx->cyl_b1[cc];
+                bend_b[3 * i + 2] = ctx->cyl_b2[cc];
+                bend_k[i] = d->cyl_kbend[cc];
+                bend_eqt[i] = d->cyl_eqtheta[cc];
+            }
+            c += nb - 1;
+        }
+    }
+    // LinkerStretching / MotorGhostStretching::vectorize (LinkerStretching.cpp:30-57)
+    std::vector<int> xb[2];
+    std::vector<double> xk[2], xe[2], xp1[2], xp2[2];
+    for (int w = 0; w < 2; w++) {
+        int n = w ? d->n_motors : d->n_linkers;
+        const mdn_crosslink* xl = w ? d->motors : d->linkers;
+        xb[w].resize(4 * (size_t)n);
+        xk[w].resize(n); xe[w].resize(n); xp1[w].resize(n); xp2[w].resize(n);
+        for (int j = 0; j < n; j++) {
+            int c1 = xl[j].cyl1, c2 = xl[j].cyl2;
+            if (c1 < 0 || c1 >= n_cyl || c2 < 0 || c2 >= n_cyl)
+                return fail(ctx, MDN_ERR_ARG, "%s %d: cylinder index out of range", w ? "motor" : "linker", j);
+            xb[w][4 * j] = ctx->cyl_b1[c1]; xb[w][4 * j + 1] = ctx->cyl_b2[c1];
+            xb[w][4 * j + 2] = ctx->cyl_b1[c2]; xb[w][4 * j + 3] = ctx->cyl_b2[c2];
+            xk[w][j] = xl[j].kstr; xe[w][j] = xl[j].eql; xp1[w][j] = xl[j].pos1; xp2[w][j] = xl[j].pos2;
+        }
+    }
+    ctx->elems.assign(d->boundary, d->boundary + d->n_boundary);
+    for (auto& e : ctx->elems)
+        if (e.kind != MDN_BND_PLANE && e.kind != MDN_BND_SPHERE)
+            return fail(ctx, MDN_ERR_ARG, "unknown boundary element kind %d", e.kind);
+    ctx->bnd_cutoff = d->boundary_cutoff;
+    ctx->rmax = d->exvol_rmax;
+    ctx->rmin = d->exvol_rmin;
+    for (int q = 0; q < 3; q++) ctx->grid[q] = d->grid_size[q];
+    if (!(ctx->rmax > 0)) return fail(ctx, MDN_ERR_ARG, "exvol_rmax must be > 0");
+
+    DevSys& s = ctx->sys;
+    s = DevSys{};
+    s.n[FF_STRETCH] = n_cyl;
+    s.n[FF_BEND] = n_bend;
+    s.n[FF_LINKER] = d->n_linkers;
+    s.n[FF_MOTOR] = d->n_motors;
+    s.n[FF_EXVOL] = 0;
+    s.n[FF_BOUNDARY] = 0;
+    layout(ctx);
+
+    // static incidence CSR: per bead, slots of stretching / bending / linker / motor ascending
+    std::vector<int> cnt(ctx->n_beads + 1, 0);
+    auto each_static = [&](auto&& fn) {
+        for (int t = 0; t < n_cyl; t++) { fn(str_b[2 * t], s.slot0[0] + 2L * t); fn(str_b[2 * t + 1], s.slot0[0] + 2L * t + 1); }
+        for (int t = 0; t < n_bend; t++)
+            for (int q = 0; q < 3; q++) fn(bend_b[3 * t + q], s.slot0[1] + 3L * t + q);
+        for (int w = 0; w < 2; w++)
+            for (size_t t = 0; t < xk[w].size(); t++)
+                for (int q = 0; q < 4; q++) fn(xb[w][4 * t + q], s.slot0[2 + w] + 4L * t + q);
+    };
+    each_static([&](int b, long) { cnt[b + 1]++; });
+    for (int b = 0; b < ctx->n_beads; b++) cnt[b + 1] += cnt[b];
+    std::vector<int> inc(cnt[ctx->n_beads]);
+    {
+        std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+        each_static([&](int b, long sl) { inc[cur[b]++] = (int)sl; });   // visited in slot order
+    }
+    if (s.slot0[NFF] > 0x7fffffffL) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many interactions");
+
+    HIPCHK(upload(ctx, ctx->str_b, str_b)); HIPCHK(upload(ctx, ctx->str_k, str_k)); HIPCHK(upload(ctx, ctx->str_eql, str_eql));
+    HIPCHK(upload(ctx, ctx->bend_b, bend_b)); HIPCHK(upload(ctx, ctx->bend_k, bend_k)); HIPCHK(upload(ctx, ctx->bend_eqt, bend_eqt));
+    HIPCHK(upload(ctx, ctx->lnk_b, xb[0])); HIPCHK(upload(ctx, ctx->lnk_k, xk[0])); HIPCHK(upload(ctx, ctx->lnk_eql, xe[0]));
+    HIPCHK(upload(ctx, ctx->lnk_p1, xp1[0])); HIPCHK(upload(ctx, ctx->lnk_p2, xp2[0]));
+    HIPCHK(ctx->lnk_sf.ensure(std::max<size_t>(1, xk[0].size())));
+    HIPCHK(upload(ctx, ctx->mot_b, xb[1])); HIPCHK(upload(ctx, ctx->mot_k, xk[1])); HIPCHK(upload(ctx, ctx->mot_eql, xe[1]));
+    HIPCHK(upload(ctx, ctx->mot_p1, xp1[1])); HIPCHK(upload(ctx, ctx->mot_p2, xp2[1]));
+    HIPCHK(ctx->mot_sf.ensure(std::max<size_t>(1, xk[1].size())));
+    HIPCHK(upload(ctx, ctx->inc_s_start, cnt)); HIPCHK(upload(ctx, ctx->inc_s, inc));
+    // cylinder tables for the neighbor lists
+    HIPCHK(upload(ctx, ctx->d_cyl_b1, ctx->cyl_b1)); HIPCHK(upload(ctx, ctx->d_cyl_b2, ctx->cyl_b2));
+    HIPCHK(upload(ctx, ctx->d_cyl_fil, ctx->cyl_fil)); HIPCHK(upload(ctx, ctx->d_cyl_pos, ctx->cyl_pos));
+    HIPCHK(upload(ctx, ctx->d_cyl_type, ctx->cyl_type)); HIPCHK(upload(ctx, ctx->d_cyl_minus, ctx->cyl_minus));
+    HIPCHK(upload(ctx, ctx->d_cyl_kexvol, ctx->cyl_kexvol)); HIPCHK(upload(ctx, ctx->d_cyl_eql, ctx->cyl_eql));
+    {
+        std::vector<double> ke(ctx->elems.size()), se(ctx->elems.size());
+        for (size_t e = 0; e < ctx->elems.size(); e++) { ke[e] = ctx->elems[e].krep; se[e] = ctx->elems[e].slen; }
+        HIPCHK(upload(ctx, ctx->d_krep_e, ke)); HIPCHK(upload(ctx, ctx->d_slen_e, se));
+    }
+    // empty dynamic CSR until the neighbor lists are built
+    {
+        std::vector<int> z(ctx->n_beads + 1, 0);
+        HIPCHK(upload(ctx, ctx->inc_d_start, z));
+        HIPCHK(ctx->inc_d.ensure(1));
+    }
+    // state vectors
+    size_t nv = 3 * (size_t)ctx->n_beads;
+    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d, &ctx->xmin, &ctx->scratch_f, &ctx->ext})
+        HIPCHK(b->ensure(nv));
+    ctx->cur = 0;
+    HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
+    refresh_ptrs(ctx);
+    int st = ensure_eval_buffers(ctx);
+    if (st) return st;
+    refresh_ptrs(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->uploaded = true;
+    return MDN_OK;
+}
+
+int mdn_system_counts(mdn_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return MDN_ERR_ARG;
+    out[0] = ctx->n_beads;
+    out[1] = ctx->n_cyl;
+    for (int ff = 0; ff < NFF; ff++) out[2 + ff] = ctx->sys.n[ff];
+    return MDN_OK;
+}
+
+int mdn_set_coords(mdn_ctx* ctx, const double* coord) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    if (!coord) return fail(ctx, MDN_ERR_ARG, "null coord");
+    st = set_coords_dev(ctx, coord);
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+int mdn_get_coords(mdn_ctx* ctx, double* coord) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(coord, ctx->x[ctx->cur].p, sizeof(double) * 3 * ctx->n_beads, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    HIPCHK(hipSetDevice(ctx->device));
+    if ((st = set_coords_dev(ctx, coord))) return st;
+    const double* x = ctx->x[ctx->cur].p;
+    NLDev n = nl_view(ctx);
+    const long nbins = (long)n.nb[0] * n.nb[1] * n.nb[2];
+    if (nbins > (1L << 30)) return fail(ctx, MDN_ERR_UNSUPPORTED, "neighbor-list grid too large (%ld bins)", nbins);
+    const int nc = ctx->n_cyl;
+    HIPCHK(ctx->mid.ensure(3 * (size_t)std::max(1, nc)));
+    HIPCHK(ctx->bin.ensure(std::max(1, nc)));
+    HIPCHK(ctx->bin_count.ensure(nbins + 1));
+    HIPCHK(ctx->bin_start.ensure(nbins + 1));
+    HIPCHK(ctx->bin_cursor.ensure(nbins + 1));
+    HIPCHK(ctx->members.ensure(std::max(1, nc)));
+    HIPCHK(ctx->nl_count.ensure(nc + 1));
+    HIPCHK(ctx->nl_off.ensure(nc + 1));
+    HIPCHK(hipMemsetAsync(ctx->bin_count.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->bin_cursor.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->nl_err.p, 0, sizeof(int), ctx->stream));
+    const int G = std::max(1, ceil_div(nc, BLOCK));
+    hipLaunchKernelGGL(k_nl_bin, dim3(G), dim3(BLOCK), 0, ctx->stream, n, x, ctx->mid.p, ctx->bin.p,
+                       ctx->bin_count.p, ctx->nl_err.p);
+    HIPCHK(hipGetLastError());
+    if ((st = exclusive_scan(ctx, ctx->bin_count.p, ctx->bin_start.p, nbins + 1))) return st;
+    hipLaunchKernelGGL(k_nl_fill, dim3(G), dim3(BLOCK), 0, ctx->stream, nc, ctx->bin.p, ctx->bin_start.p,
+                       ctx->bin_cursor.p, ctx->members.p);
+    hipLaunchKernelGGL(k_nl_pairs<0>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, ctx->mid.p, ctx->bin.p,
+                       ctx->bin_start.p, ctx->members.p, ctx->nl_count.p, (const int*)nullptr, (int*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(ctx->nl_count.p + nc, 0, sizeof(int), ctx->stream));
+    if ((st = exclusive_scan(ctx, ctx->nl_count.p, ctx->nl_off.p, nc + 1))) return st;
+    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->nl_off.p + nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_int + 1, ctx->nl_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_int[1] != 0) {
+        ctx->nl_built = false;
+        return fail(ctx, MDN_ERR_NL_BOUNDS, "%d cylinder midpoints outside the neighbor-list grid [0, %g)x[0, %g)x[0, %g)",
+                    ctx->h_int[1], ctx->grid[0], ctx->grid[1], ctx->grid[2]);
+    }
+    const long np = ctx->h_int[0];
+    HIPCHK(ctx->pairs.ensure(2 * (size_t)std::max(1L, np)));
+    hipLaunchKernelGGL(k_nl_pairs<1>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, ctx->mid.p, ctx->bin.p,
+                       ctx->bin_start.p, ctx->members.p, ctx->nl_count.p, ctx->nl_off.p, ctx->pairs.p);
+    HIPCHK(hipGetLastError());
+    // CylinderExclVolume::vectorize
+    HIPCHK(ctx->ev_b.ensure(4 * (size_t)std::max(1L, np)));
+    HIPCHK(ctx->ev_ks.ensure((size_t)std::max(1L, np)));
+    if (np > 0)
+        hipLaunchKernelGGL(k_ev_vectorize, dim3(ceil_div(np, BLOCK)), dim3(BLOCK), 0, ctx->stream, n, np,
+                           ctx->pairs.p, ctx->ev_b.p, ctx->ev_ks.p);
+    ctx->n_pairs = np;
+    // boundary NL + BoundaryCylinderRepulsion::vectorize
+    const int ne = (int)ctx->elems.size();
+    const long nbc = (long)ne * nc;
+    ctx->sys.n_elem = ne;
+    HIPCHK(ctx->bnd_cnt.ensure(nbc + 1));
+    HIPCHK(ctx->bnd_off.ensure(nbc + 1));
+    if (nbc > 0)
+        hipLaunchKernelGGL(k_bnd_count, dim3(ceil_div(nbc, BLOCK)), dim3(BLOCK), 0, ctx->stream, n, ctx->sys,
+                           ctx->bnd_cutoff, ctx->mid.p, ctx->bnd_cnt.p);
+    HIPCHK(hipMemsetAsync(ctx->bnd_cnt.p + nbc, 0, sizeof(int), ctx->stream));
+    if ((st = exclusive_scan(ctx, ctx->bnd_cnt.p, ctx->bnd_off.p, nbc + 1))) return st;
+    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bnd_off.p + nbc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    // per-element counts (nneighbors) come from the offsets at element boundaries
+    std::vector<int> eoff(ne + 1, 0);
+    for (int e = 0; e <= ne; e++)
+        HIPCHK(hipMemcpyAsync(&eoff[e], ctx->bnd_off.p + (long)e * nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int nbi = ctx->h_int[0];
+    ctx->bnd_nneighbors.assign(ne, 0);
+    for (int e = 0; e < ne; e++) ctx->bnd_nneighbors[e] = eoff[e + 1] - eoff[e];
+    HIPCHK(ctx->bnd_b.ensure(std::max(1, nbi)));
+    HIPCHK(ctx->bnd_e.ensure(std::max(1, nbi)));
+    HIPCHK(ctx->bnd_krep.ensure(std::max(1, nbi)));
+    HIPCHK(ctx->bnd_slen.ensure(std::max(1, nbi)));
+    if (nbc > 0)
+        hipLaunchKernelGGL(k_bnd_emit, dim3(ceil_div(nbc, BLOCK)), dim3(BLOCK), 0, ctx->stream, n, ctx->sys,
+                           ctx->bnd_cnt.p, ctx->bnd_off.p, ctx->d_krep_e.p, ctx->d_slen_e.p, ctx->bnd_b.p,
+                           ctx->bnd_e.p, ctx->bnd_krep.p, ctx->bnd_slen.p);
+    ctx->n_bint = nbi;
+    // new layout
+    DevSys& s = ctx->sys;
+    if (np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many excluded-volume pairs");
+    s.n[FF_EXVOL] = (int)np;
+    s.n[FF_BOUNDARY] = nbi;
+    layout(ctx);
+    if ((st = ensure_eval_buffers(ctx))) return st;
+    refresh_ptrs(ctx);
+    // dynamic incidence CSR
+    HIPCHK(ctx->inc_cnt.ensure(ctx->n_beads + 1));
+    HIPCHK(ctx->inc_cursor.ensure(ctx->n_beads + 1));
+    HIPCHK(ctx->inc_d_start.ensure(ctx->n_beads + 1));
+    const long ndyn = 4L * np + nbi;
+    HIPCHK(ctx->inc_d.ensure(std::max(1L, ndyn)));
+    HIPCHK(hipMemsetAsync(ctx->inc_cnt.p, 0, sizeof(int) * (ctx->n_beads + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->inc_cursor.p, 0, sizeof(int) * (ctx->n_beads + 1), ctx->stream));
+    refresh_ptrs(ctx);
+    if (ndyn > 0)
+        hipLaunchKernelGGL(k_inc_count, dim3(ceil_div(ndyn, BLOCK)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                           ctx->inc_cnt.p);
+    if ((st = exclusive_scan(ctx, ctx->inc_cnt.p, ctx->inc_d_start.p, ctx->n_beads + 1))) return st;
+    if (ndyn > 0) {
+        hipLaunchKernelGGL(k_inc_fill, dim3(ceil_div(ndyn, BLOCK)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                           ctx->inc_d_start.p, ctx->inc_cursor.p, ctx->inc_d.p);
+        hipLaunchKernelGGL(k_inc_sort, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
+                           ctx->inc_d_start.p, ctx->inc_d.p);
+    }
+    HIPCHK(hipGetLastError());
+    refresh_ptrs(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->nl_built = true;
+    if (n_pairs_out) *n_pairs_out = np;
+    return MDN_OK;
+}
+
+int mdn_nl_pairs(mdn_ctx* ctx, int32_t* out, int64_t cap, int64_t* n_pairs) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (n_pairs) *n_pairs = ctx->n_pairs;
+    if (cap < ctx->n_pairs) return fail(ctx, MDN_ERR_CAPACITY, "need room for %ld pairs", ctx->n_pairs);
+    if (ctx->n_pairs > 0)
+        HIPCHK(hipMemcpy(out, ctx->pairs.p, sizeof(int) * 2 * ctx->n_pairs, hipMemcpyDeviceToHost));
+    return MDN_OK;
+}
+
+int mdn_boundary_list(mdn_ctx* ctx, int32_t* out_beads, int64_t cap, int32_t* out_nneighbors,
+                      int64_t* n_interactions) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (n_interactions) *n_interactions = ctx->n_bint;
+    if (cap < ctx->n_bint) return fail(ctx, MDN_ERR_CAPACITY, "need room for %d interactions", ctx->n_bint);
+    if (ctx->n_bint > 0 && out_beads)
+        HIPCHK(hipMemcpy(out_beads, ctx->bnd_b.p, sizeof(int) * ctx->n_bint, hipMemcpyDeviceToHost));
+    if (out_nneighbors)
+        for (size_t e = 0; e < ctx->bnd_nneighbors.size(); e++) out_nneighbors[e] = ctx->bnd_nneighbors[e];
+    return MDN_OK;
+}
+
+int mdn_energy(mdn_ctx* ctx, const double* coord, mdn_energy_report* out) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!out) return fail(ctx, MDN_ERR_ARG, "null report");
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * 3 * ctx->n_beads, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    if ((st = energy_report(ctx, x))) return st;
+    fill_report(ctx->h_eout, out);
+    return MDN_OK;
+}
+
+static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force, int32_t n_var) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!force || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    if ((st = eval_slots(ctx, x))) return st;
+    long lo = 0, hi = ctx->sys.slot0[NFF];
+    const double* base = nullptr;
+    if (ff >= 0) {
+        lo = ctx->sys.slot0[ff];
+        hi = ctx->sys.slot0[ff + 1];
+        HIPCHK(hipMemcpyAsync(ctx->scratch_f.p, force, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+        base = ctx->scratch_f.p;
+    }
+    if ((st = gather(ctx, lo, hi, base, ctx->f.p))) return st;
+    HIPCHK(hipMemcpyAsync(force, ctx->f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+int mdn_forces(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var) {
+    return forces_impl(ctx, -1, coord, force, n_var);
+}
+
+int mdn_ff_forces(mdn_ctx* ctx, int32_t ff, const double* coord, double* force, int32_t n_var) {
+    if (ff < 0 || ff >= NFF) return fail(ctx, MDN_ERR_ARG, "force field index %d", ff);
+    return forces_impl(ctx, ff, coord, force, n_var);
+}
+
+int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor_out) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (linker_out && ctx->sys.n[FF_LINKER])
+        HIPCHK(hipMemcpy(linker_out, ctx->lnk_sf.p, sizeof(double) * ctx->sys.n[FF_LINKER], hipMemcpyDeviceToHost));
+    if (motor_out && ctx->sys.n[FF_MOTOR])
+        HIPCHK(hipMemcpy(motor_out, ctx->mot_sf.p, sizeof(double) * ctx->sys.n[FF_MOTOR], hipMemcpyDeviceToHost));
+    return MDN_OK;
+}
+
+int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var) {
+    return fail(ctx, MDN_ERR_UNSUPPORTED, "mixed precision not built in this library version");
+}
+
+// ----------------------------------------------------------------------------------------
+// CGMethod::minimize on the device
+// ----------------------------------------------------------------------------------------
+static void launch_slot(mdn_ctx* ctx, CgBufs B) {
+    const DevSys& s = ctx->sys;
+    const int all = std::max(1, s.blk[NFF + 1]);
+    const int beads = bead_blocks(ctx);
+    hipLaunchKernelGGL(k_cg_ls, dim3(all), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_move, dim3(all), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_gather, dim3(beads), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_dir, dim3(beads), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
+}
+
+int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* force_out, mdn_min_result* res) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!p || !coord || !res) return fail(ctx, MDN_ERR_ARG, "null argument");
+    if (p->precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "only FP64 minimization in this version");
+    HIPCHK(hipSetDevice(ctx->device));
+    memset(res, 0, sizeof(*res));
+    const int nv = 3 * ctx->n_beads;
+    // startMinimization + initial energy / forces (CGMethod.hpp:876-896)
+    ctx->cur = 0;
+    HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = energy_report(ctx, ctx->x[0].p))) return st;
+    fill_report(ctx->h_eout, &res->energies_before);
+    if ((st = eval_slots(ctx, ctx->x[0].p))) return st;
+    if ((st = gather(ctx, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p))) return st;
+    std::vector<double> hf(nv);
+    HIPCHK(hipMemcpyAsync(hf.data(), ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double mx = 0.0, ff = 0.0;
+    for (int i = 0; i < nv; i++) {
+        mx = rmax(mx, std::fabs(hf[i]));
+        ff += hf[i] * hf[i];
+    }
+    CgState& S = *ctx->h_cg;
+    memset(&S, 0, sizeof(S));
+    S.method = p->method;
+    S.try_recover = p->try_recover;
+    S.force_call_limit = std::max(p->force_call_limit, 3 * ctx->n_beads);
+    S.energy_call_limit = p->energy_call_limit;
+    S.max_dist = p->max_dist;
+    S.lambda_max = p->lambda_max;
+    S.grad_tol = p->grad_tol;
+    S.energy_rel_tol = p->energy_rel_tol;
+    S.num_energy_call = 1;
+    S.num_force_call = 1;
+    S.grad_inf_norm = mx;
+    S.energy_rel_change = INFINITY;
+    S.prev_energy = res->energies_before.total;
+    S.minimum_e = INFINITY;
+    S.ls_K = 2;
+    S.fdotf = ff;
+    res->force_call_limit = S.force_call_limit;
+    res->energy_call_limit = S.energy_call_limit;
+    res->grad_tol = S.grad_tol;
+    res->energy_rel_tol = S.energy_rel_tol;
+    if (!std::isfinite(mx)) return fail(ctx, MDN_ERR_NONFINITE, "maxF is not finite at the start");
+    // resetCGParams before the loop: d = F, lastFDotF = F.F, F_prev = F
+    HIPCHK(hipMemcpyAsync(ctx->d.p, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->fp.p, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+    S.last_fdotf = ff;
+    S.phase = PH_LS;
+    S.parity = 0;
+    S.ls_lambda = rmin(S.lambda_max, S.max_dist / S.grad_inf_norm);
+    const bool limit0 = (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
+    S.done = (S.grad_inf_norm < S.grad_tol || S.energy_rel_change < S.energy_rel_tol || limit0) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->ticket.p, 0, sizeof(unsigned) * 8, ctx->stream));
+
+    CgBufs B;
+    B.x[0] = ctx->x[0].p; B.x[1] = ctx->x[1].p;
+    B.f = ctx->f.p; B.fp = ctx->fp.p; B.d = ctx->d.p; B.xmin = ctx->xmin.p;
+    B.slots = ctx->slots.p; B.partial = ctx->partial.p; B.partial_i = ctx->partial_i.p; B.bad_count = ctx->bad_count.p;
+
+    // Slots are launched in batches; the host polls the done flag once per batch, keeping one
+    // batch in flight ahead of the poll.
+    const int BATCH = 16;
+    const long max_slots = 8L * ((long)S.force_call_limit + 16) + 64;
+    long launched = 0;
+    bool done = S.done != 0;
+    hipEvent_t ev[2];
+    HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int* h_done = ctx->h_int + 4;
+    h_done[0] = h_done[1] = 0;
+    int k = 0;
+    while (!done) {
+        for (int i = 0; i < BATCH; i++) launch_slot(ctx, B);
+        HIPCHK(hipGetLastError());
+        launched += BATCH;
+        HIPCHK(hipMemcpyAsync(&h_done[k], &ctx->cg.p->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipEventRecord(ev[k], ctx->stream));
+        // wait for the previous batch's flag while this batch runs
+        int prev = k ^ 1;
+        if (launched > BATCH) {
+            HIPCHK(hipEventSynchronize(ev[prev]));
+            if (h_done[prev]) done = true;
+        }
+        k ^= 1;
+        if (launched > max_slots) break;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
+    if (!S.done) return fail(ctx, MDN_ERR_STATE, "minimizer did not terminate within %ld slots", launched);
+    ctx->cur = S.parity;
+    res->error_flags = S.error_flags;
+    res->num_energy_call = S.num_energy_call;
+    res->num_force_call = S.num_force_call;
+    res->num_iter = S.num_iter;
+    res->num_safe_mode = S.num_safe_mode;
+    res->num_recovery = S.num_recovery;
+    res->grad_inf_norm = S.grad_inf_norm;
+    res->energy_rel_change = S.energy_rel_change;
+    if (S.status == 9)
+        return fail(ctx, MDN_ERR_UNSUPPORTED,
+                    "an excluded-volume pair needed the numerical force fallback during the minimization");
+    if (S.status != 0) return fail(ctx, S.status, "minimization stopped with status %d", S.status);
+    const bool limit = (S.energy_call_limit != 0 && S.num_energy_call > S.energy_call_limit) ||
+                       (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
+    res->call_limit_exceeded = limit;
+    res->converged_grad = S.grad_inf_norm < S.grad_tol;
+    res->converged_energy = S.energy_rel_change < S.energy_rel_tol;
+    double* xres = ctx->x[ctx->cur].p;
+    if (S.error_flags == 0) {
+        if (p->report_on_call_limit && limit && S.have_min_e) {   // copybackupcoordinates
+            HIPCHK(hipMemcpyAsync(xres, ctx->xmin.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+            res->restored_min_energy = 1;
+        }
+        if ((st = energy_report(ctx, xres))) return st;
+        fill_report(ctx->h_eout, &res->energies_after);
+        res->num_energy_call += 1;
+    }
+    HIPCHK(hipMemcpyAsync(coord, xres, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    if (force_out) HIPCHK(hipMemcpyAsync(force_out, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (S.error_flags & MDN_MIN_ERR_LINESEARCH) return fail(ctx, MDN_ERR_LINESEARCH, "line search failed (recovery disabled)");
+    return MDN_OK;
+}
+
+// ----------------------------------------------------------------------------------------
+// Benchmark helpers: device-resident coordinates, HIP events on the context stream
+// ----------------------------------------------------------------------------------------
+int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval, double* kernel_ms) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "precision %d", precision);
+    HIPCHK(hipSetDevice(ctx->device));
+    const DevSys& s = ctx->sys;
+    const double* x = ctx->x[ctx->cur].p;
+    // warm-up (and fallback check) once
+    if ((st = eval_slots(ctx, x))) return st;
+    hipEvent_t e0, e1, k0, k1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&k0)); HIPCHK(hipEventCreate(&k1));
+    // whole evaluation: terms + gather
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iters; i++) {
+        hipLaunchKernelGGL(k_forces_terms, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
+                           ctx->slots.p, ctx->bad_count.p, ctx->bad_list.p);
+        hipLaunchKernelGGL(k_gather_plain, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s,
+                           (const D3*)ctx->slots.p, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
+    }
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    // dominant kernel alone
+    HIPCHK(hipEventRecord(k0, ctx->stream));
+    for (int i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_forces_terms, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
+                           ctx->slots.p, ctx->bad_count.p, ctx->bad_list.p);
+    HIPCHK(hipEventRecord(k1, ctx->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(k1));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, e0, e1));
+    HIPCHK(hipEventElapsedTime(&b, k0, k1));
+    if (ms_per_eval) *ms_per_eval = a / iters;
+    if (kernel_ms) *kernel_ms = b / iters;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(k0); (void)hipEventDestroy(k1);
+    return MDN_OK;
+}
+
+int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "precision %d", precision);
+    HIPCHK(hipSetDevice(ctx->device));
+    const DevSys& s = ctx->sys;
+    const double* x = ctx->x[ctx->cur].p;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iters; i++)
+        hipLaunchKernelGGL(k_energy_report, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
+                           ctx->partial.p, ctx->partial_i.p, ctx->ticket.p + 7, ctx->eout.p);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventSynchronize(e1));
+    float a = 0;
+    HIPCHK(hipEventElapsedTime(&a, e0, e1));
+    if (ms_per_eval) *ms_per_eval = a / iters;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    return MDN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+"""Python host mirror of the reference's mechanical-equilibration interfaces over the C ABI.
+
+``ForceFieldManager`` mirrors medyan::ForceFieldManager (ForceFieldManager.h:64-123):
+``vectorize_all_force_fields`` / ``compute_energy`` / ``compute_individual_energies`` /
+``compute_forces`` (zero-fill then accumulate), plus per-force-field accumulation
+(ForceField::computeForces). ``CGMethod.minimize`` mirrors CGMethod::minimize
+(CGMethod.hpp:1746-1759) as driven by MController::run (MController.h:108-211).
+Everything runs in libmdn_mech.so on the GPU; there is no CPU fallback: a missing library or
+device raises.
+
+Citations are relative to medyan-5.4.0/ in the reference tree.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+dp = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+
+EXPORTED = [
+    "mdn_abi_version", "mdn_ctx_create", "mdn_ctx_destroy", "mdn_last_error", "mdn_system_upload",
+    "mdn_system_counts", "mdn_nl_build", "mdn_nl_pairs", "mdn_boundary_list", "mdn_energy",
+    "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
+    "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
+]
+
+
+class MdnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.status_name(code)}: {msg}")
+        self.code = code
+
+
+def lib_path():
+    return os.path.join(_HERE, "libmdn_mech.so")
+
+
+def lib():
+    """Load libmdn_mech.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(path)
+        L.mdn_abi_version.restype = C.c_int
+        L.mdn_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.mdn_ctx_destroy.argtypes = [C.c_void_p]
+        L.mdn_last_error.argtypes = [C.c_void_p]
+        L.mdn_last_error.restype = C.c_char_p
+        L.mdn_system_upload.argtypes = [C.c_void_p, C.POINTER(abi.SystemDesc), dp]
+        L.mdn_system_counts.argtypes = [C.c_void_p, i64p]
+        L.mdn_nl_build.argtypes = [C.c_void_p, dp, i64p]
+        L.mdn_nl_pairs.argtypes = [C.c_void_p, i32p, C.c_int64, i64p]
+        L.mdn_boundary_list.argtypes = [C.c_void_p, i32p, C.c_int64, i32p, i64p]
+        L.mdn_energy.argtypes = [C.c_void_p, dp, C.POINTER(abi.EnergyReport)]
+        L.mdn_forces.argtypes = [C.c_void_p, dp, dp, C.c_int32]
+        L.mdn_ff_forces.argtypes = [C.c_void_p, C.c_int32, dp, dp, C.c_int32]
+        L.mdn_crosslink_stretch_forces.argtypes = [C.c_void_p, dp, dp]
+        L.mdn_minimize.argtypes = [C.c_void_p, C.POINTER(abi.CgParams), dp, dp, C.POINTER(abi.MinResult)]
+        L.mdn_bench_forces.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, dp]
+        L.mdn_bench_energy.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
+        L.mdn_set_coords.argtypes = [C.c_void_p, dp]
+        L.mdn_get_coords.argtypes = [C.c_void_p, dp]
+        L.mdn_forces_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int32]
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class ForceFieldManager:
+    """A device context holding one vectorized system (one HIP stream, not thread-safe)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        self._h = C.c_void_p()
+        st = self._L.mdn_ctx_create(device, C.byref(self._h))
+        if st != 0:
+            raise MdnError(st, f"mdn_ctx_create(device={device}) failed (is a GPU visible?)")
+        self.n_beads = 0
+
+    def _chk(self, st):
+        if st != 0:
+            raise MdnError(st, self._L.mdn_last_error(self._h).decode())
+
+    # ForceFieldManager::vectorizeAllForceFields (bonded + crosslink tables)
+    def vectorize_all_force_fields(self, desc, coord):
+        ptr = desc if isinstance(desc, C._Pointer) else C.pointer(desc)
+        c = _f64(coord)
+        self._chk(self._L.mdn_system_upload(self._h, ptr, c.ctypes.data_as(dp)))
+        self.n_beads = ptr.contents.n_beads
+        return self
+
+    upload = vectorize_all_force_fields
+
+    def counts(self):
+        out = (C.c_int64 * 8)()
+        self._chk(self._L.mdn_system_counts(self._h, out))
+        keys = ["n_beads", "n_cylinders"] + list(abi.FF_NAMES)
+        return dict(zip(keys, list(out)))
+
+    # SubSystem::resetNeighborLists -> HybridCylinderCylinderNL::reset + BoundaryCylinderNL::reset
+    def reset_neighbor_lists(self, coord=None):
+        n = C.c_int64()
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_nl_build(self._h, cp, C.byref(n)))
+        return n.value
+
+    def neighbor_pairs(self):
+        n = C.c_int64()
+        self._L.mdn_nl_pairs(self._h, None, 0, C.byref(n))
+        out = np.zeros(2 * max(1, n.value), np.int32)
+        self._chk(self._L.mdn_nl_pairs(self._h, out.ctypes.data_as(i32p), n.value, C.byref(n)))
+        return out[: 2 * n.value].reshape(-1, 2)
+
+    def boundary_interactions(self, n_elem):
+        n = C.c_int64()
+        self._L.mdn_boundary_list(self._h, None, 0, None, C.byref(n))
+        beads = np.zeros(max(1, n.value), np.int32)
+        nn = np.zeros(max(1, n_elem), np.int32)
+        self._chk(self._L.mdn_boundary_list(self._h, beads.ctypes.data_as(i32p), n.value,
+                                            nn.ctypes.data_as(i32p), C.byref(n)))
+        return beads[: n.value], nn[:n_elem]
+
+    def compute_individual_energies(self, coord=None):
+        """EnergyReport per force field (ForceFieldManager::computeIndividualEnergies)."""
+        rep = abi.EnergyReport()
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_energy(self._h, cp, C.byref(rep)))
+        return rep
+
+    def compute_energy(self, coord=None):
+        """ForceFieldManager::computeEnergy: +inf when a force field reports <= -1."""
+        rep = self.compute_individual_energies(coord)
+        tot = 0.0
+        for e in rep.ff:
+            if e <= -1:
+                return float("inf")
+            tot += e
+        return tot
+
+    def compute_forces(self, coord=None):
+        """ForceFieldManager::computeForces: zero-filled, every force field accumulated."""
+        f = np.zeros(3 * self.n_beads)
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_forces(self._h, cp, f.ctypes.data_as(dp), f.size))
+        return f
+
+    def ff_compute_forces(self, ff, coord=None, force=None):
+        """ForceField::computeForces of force field ``ff``: accumulates into ``force``."""
+        f = np.zeros(3 * self.n_beads) if force is None else force
+        assert f.dtype == np.float64 and f.flags.c_contiguous
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_ff_forces(self._h, ff, cp, f.ctypes.data_as(dp), f.size))
+        return f
+
+    def stretch_forces(self, n_linkers, n_motors):
+        lo, mo = np.zeros(max(1, n_linkers)), np.zeros(max(1, n_motors))
+        self._chk(self._L.mdn_crosslink_stretch_forces(self._h, lo.ctypes.data_as(dp), mo.ctypes.data_as(dp)))
+        return lo[:n_linkers], mo[:n_motors]
+
+    def set_coords(self, coord):
+        self._chk(self._L.mdn_set_coords(self._h, _f64(coord).ctypes.data_as(dp)))
+
+    def get_coords(self):
+        x = np.zeros(3 * self.n_beads)
+        self._chk(self._L.mdn_get_coords(self._h, x.ctypes.data_as(dp)))
+        return x
+
+    def minimize(self, params, coord):
+        """CGMethod::minimize from ``coord`` with the neighbor lists frozen.
+        Returns (coord_out, force_out, MinResult)."""
+        x = _f64(coord).copy()
+        f = np.zeros_like(x)
+        res = abi.MinResult()
+        self._chk(self._L.mdn_minimize(self._h, C.byref(params), x.ctypes.data_as(dp),
+                                       f.ctypes.data_as(dp), C.byref(res)))
+        return x, f, res
+
+    def bench_forces(self, iters, precision=abi.MDN_PREC_FP64):
+        a, b = C.c_double(), C.c_double()
+        self._chk(self._L.mdn_bench_forces(self._h, precision, iters, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def bench_energy(self, iters, precision=abi.MDN_PREC_FP64):
+        a = C.c_double()
+        self._chk(self._L.mdn_bench_energy(self._h, precision, iters, C.byref(a)))
+        return a.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mdn_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,108 @@
+"""GPU parity: libmdn_mech.so (gfx950) against the CPU oracle on the same seeded inputs.
+
+Bars:
+  neighbor lists (excluded-volume pairs, boundary interactions): identical, element for element;
+  stretching / bending / linker / motor forces: bit-identical in FP64 (same expression order,
+    no FMA contraction, per-bead summation in the reference's order);
+  excluded-volume and boundary forces: |dF| <= 1e-12 * max|F| (libm atan / exp ulps differ);
+  energies: relative 1e-12 (reduction order differs);
+  minimization: both converge by the reference criterion, final energies within 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+FF_EXACT = (abi.MDN_FF_STRETCH, abi.MDN_FF_BEND, abi.MDN_FF_LINKER, abi.MDN_FF_MOTOR)
+
+
+def _pair(gpu, cfg, **kw):
+    net = synth.Network.config(cfg, **kw)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    npairs = g.reset_neighbor_lists(net.coord)
+    o = of.OracleSystem(net.desc)
+    o.build_nl(net.coord)
+    return net, g, o, npairs
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_neighbor_lists_identical(gpu, cfg):
+    net, g, o, npairs = _pair(gpu, cfg)
+    print(f"config {cfg}: {npairs} pairs", flush=True)
+    assert npairs == o.n_pairs
+    np.testing.assert_array_equal(g.neighbor_pairs(), o.pairs())
+    nel = net.desc.n_boundary
+    beads, nn = g.boundary_interactions(nel)
+    np.testing.assert_array_equal(beads, o.boundary_list())
+    np.testing.assert_array_equal(nn, o.boundary_nneighbors(nel))
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_forces_per_ff(gpu, cfg):
+    net, g, o, _ = _pair(gpu, cfg)
+    x = net.coord
+    for ff in range(abi.MDN_NUM_FF):
+        fg = g.ff_compute_forces(ff, x)
+        fo = o.ff_forces(ff, x)
+        if ff in FF_EXACT:
+            np.testing.assert_array_equal(fg, fo, err_msg=abi.FF_NAMES[ff])
+        else:
+            scale = max(1.0, float(np.abs(fo).max()))
+            assert float(np.abs(fg - fo).max()) <= 1e-12 * scale, abi.FF_NAMES[ff]
+    ft, fto = g.compute_forces(x), o.forces(x)
+    scale = float(np.abs(fto).max())
+    assert float(np.abs(ft - fto).max()) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_energies(gpu, cfg):
+    net, g, o, _ = _pair(gpu, cfg)
+    rep = g.compute_individual_energies(net.coord)
+    _, repo = o.energy(net.coord)
+    for ff in range(abi.MDN_NUM_FF):
+        assert rep.ff[ff] == pytest.approx(repo.ff[ff], rel=1e-12, abs=1e-300), abi.FF_NAMES[ff]
+    assert rep.total == pytest.approx(repo.total, rel=1e-12)
+
+
+@pytest.mark.parametrize("ncalls", [8, 40, 120])
+def test_minimize_trajectory_prefix(gpu, ncalls):
+    """The first iterations follow the oracle's trajectory: stop both after `ncalls` energy
+    evaluations (energyCallLimit) and compare coordinates. Reduction order differs (tree vs
+    sequential), so agreement is to rounding, which a non-convex landscape amplifies slowly."""
+    net, g, o, _ = _pair(gpu, 0)
+    p = abi.default_cg_params(energy_call_limit=ncalls)
+    xg, fg, rg = g.minimize(p, net.coord)
+    st, xo, fo, ro = o.minimize(p, net.coord)
+    print(ncalls, "gpu", rg.num_iter, rg.num_energy_call, rg.num_force_call, rg.grad_inf_norm,
+          "oracle", ro.num_iter, ro.num_energy_call, ro.num_force_call, ro.grad_inf_norm, flush=True)
+    print("max |dx|", float(np.abs(xg - xo).max()), flush=True)
+    assert (rg.num_iter, rg.num_energy_call, rg.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+    np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
+
+
+def test_minimize_config0(gpu):
+    """Full minimization: both paths converge by the reference criterion (max |F| < 1 pN or
+    |dE/E| < 1e-9); the GPU's final state is checked by the oracle (its forces there satisfy the
+    criterion) rather than against the oracle's final state, because the two trajectories end in
+    different local minima once rounding flips a line-search decision."""
+    net, g, o, _ = _pair(gpu, 0)
+    p = abi.default_cg_params()
+    xg, fg, rg = g.minimize(p, net.coord)
+    st, xo, fo, ro = o.minimize(p, net.coord)
+    assert st == 0
+    print("gpu", rg.as_dict(), flush=True)
+    print("oracle", ro.as_dict(), flush=True)
+    assert rg.success() and rg.converged()
+    assert rg.energies_before.total == pytest.approx(ro.energies_before.total, rel=1e-12)
+    assert rg.energies_after.total < 0.2 * rg.energies_before.total
+    f_at_xg = o.forces(xg)
+    np.testing.assert_allclose(fg, f_at_xg, rtol=0, atol=1e-9 * max(1.0, float(np.abs(fg).max())))
+    if rg.converged_grad:
+        assert float(np.abs(f_at_xg).max()) < p.grad_tol
+    e_at_xg, _ = o.energy(xg)
+    assert rg.energies_after.total == pytest.approx(e_at_xg, rel=1e-12)
+    assert 0.5 * ro.num_iter <= rg.num_iter <= 2.0 * ro.num_iter
