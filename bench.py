@@ -112,6 +112,8 @@ def main():
         x, f, res = ffm.minimize(p, net.coord)
         tmin = time.perf_counter() - t1
         extra = {"minimizations_per_s": 1.0 / tmin, "minimization_s": tmin,
+                 "us_per_cg_iteration": 1e6 * tmin / max(1, res.num_iter),
+                 "launch_stats": ffm.last_min_stats(),
                  "cg_iterations": res.num_iter, "energy_calls": res.num_energy_call,
                  "force_calls": res.num_force_call,
                  "criterion": "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit"),

@@ -219,6 +219,10 @@ int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor
 int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* params, double* coord, double* force_out,
                  mdn_min_result* res);
 
+/* Launch statistics of the last mdn_minimize on this context: out[0] line-search launches
+ * that did work, out[1] reserved, out[2] force gathers, out[3] CG iterations. */
+int mdn_last_min_stats(mdn_ctx* ctx, int64_t* out);
+
 /* ---- device-resident variants (benchmarking, zero host traffic) ---------------------- */
 
 /* Time `iters` force evaluations (mdn_forces semantics) on the context's current device

@@ -1,23 +1,27 @@
 // mdn_kernels.h -- gfx950 kernels of the mechanical-equilibration path (FP64 path).
 //
 // Layout in HBM (all device buffers owned by the context):
-//   x[3*n_beads], F, F_prev, d (search direction), x_min: the reference's flat AoS coord
-//   vectors (DofSerializer.hpp:64-187), double.
-//   Term tables, structure of arrays per force field (bead indices int32, parameters double).
-//   slots[S]: one D3 per (interaction, bead-of-interaction); slot index order = force-field
-//   order x term order x bead order, i.e. the reference's += order into `force`.
-//   Incidence CSR per bead (static part: stretching/bending/linker/motor, built at upload;
-//   dynamic part: excluded volume/boundary, built after each neighbor-list rebuild), each bead's
-//   slots ascending, so an owner-computes gather reproduces the reference's per-bead
-//   summation order exactly and needs no atomics.
+//   x[2][3*n_beads] (current / next coordinates), F, F_prev, d[2] (search direction, current
+//   / next), x_min[2] (minimum-energy backup, committed / candidate): the reference's flat AoS
+//   coord vectors (DofSerializer.hpp:64-187), double.
+//   Term tables: structure of arrays per force field (bead indices int32, parameters double).
+//   slots[S]: one D3 per (interaction, bead of the interaction), laid out bead-sorted: bead b
+//   owns slots [bstart[b], bstart[b+1]) in ascending order of (force field, interaction, bead of
+//   the interaction), which is the order the reference adds into `force`. Interaction kernels
+//   write through pos[] (slot id -> bead-sorted position); the gather is one contiguous,
+//   independent read per bead, reproduces the reference's per-bead summation order exactly and
+//   needs no atomics.
 //
-// Grid layout of the term / bead kernels: blocks [blk[ff], blk[ff+1]) hold force field ff's
-// terms (each FF range padded to whole blocks, so a block belongs to one FF and per-FF energies
-// come out of block partials without mixing), blocks [blk[6], blk[7]) hold the beads.
+// Grid layout of the interaction kernels: blocks [blk[ff], blk[ff+1]) hold force field ff's
+// interactions (each range padded to whole blocks, so a block belongs to one force field), then
+// blocks [blk[6], blk[7]) hold the beads. The line-search kernel repeats the interaction blocks
+// once per trial step length.
 //
-// Reductions are deterministic: block tree reduction, block partials in fixed slots, and the
-// last block to finish (ticket counter + device-scope fences) reduces the partials in block
-// order and runs the scalar control logic of CGMethod::minimize.
+// Reductions are deterministic: block tree reduction, block partials in fixed slots, then a
+// one-block finalize kernel reduces the partials with all its threads in a fixed order and runs
+// the scalar control logic of CGMethod::minimize on an LDS copy of the state. (A last-block
+// election would need a device-scope fence per block, which on gfx950 writes back the L2 each
+// time; one kernel boundary is far cheaper.)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,8 +33,22 @@ namespace mdn {
 
 constexpr int BLOCK = 256;
 constexpr int NFF = 6;
-constexpr int MAXK = 8;          // line-search trials evaluated per launch
 constexpr int MAXELEM = 8;       // boundary elements held in the kernel argument
+constexpr int NTRIAL = 2;        // line-search step lengths evaluated per launch (lambda, lambda/2)
+constexpr int PSTRIDE = 4;       // partial slots per block
+constexpr int GB = 64;           // beads per gather block
+constexpr int MOVE_GRID = 256;   // grid of the (rarely active) move kernel
+constexpr int DEFER_CAP = 256;   // excluded-volume energies deferred to the Simpson pass per launch
+
+// An excluded-volume pair whose closed-form energy is non-finite (or < -1): the reference then
+// integrates numerically (energyN). The pass runs block-parallel in the finalize kernel.
+struct EnDefer {
+    int code;      // trial / report slot the energy belongs to
+    int t;         // pair index
+    double ks;     // kRepScaled
+    double c[12];  // the four bead positions
+    double U;      // result
+};
 
 enum { FF_STRETCH = 0, FF_BEND, FF_LINKER, FF_MOTOR, FF_EXVOL, FF_BOUNDARY };
 
@@ -38,7 +56,7 @@ struct DevSys {
     int n_beads;
     int n[NFF];                 // interactions per force field
     int blk[NFF + 2];           // first block of each FF range, beads, end
-    long slot0[NFF + 1];        // first slot of each force field, total
+    long slot0[NFF + 1];        // first slot id of each force field, total
     // stretching
     const int* str_b;  const double* str_k; const double* str_eql;
     // bending
@@ -54,33 +72,41 @@ struct DevSys {
     const int* bnd_b;  const int* bnd_e; const double* bnd_krep; const double* bnd_slen;
     int n_elem;
     BElem elem[MAXELEM];
-    // incidence CSR (static + dynamic), slot indices
-    const int* inc_s_start; const int* inc_s;
-    const int* inc_d_start; const int* inc_d;
+    // bead-sorted slot layout
+    const int* pos;        // [S] slot id -> position
+    const int* bstart;     // [n_beads + 1]
+    const int* sorted_id;  // [S] position -> slot id
+    // deferred Simpson energies
+    EnDefer* defer;
+    int* defer_n;
 };
 
 // ---------------------------------------------------------------------------------------
 // CG device state (CGMethod::minimize locals + MinimizationResult)
-enum { PH_LS = 0, PH_MOVE = 1, PH_GATHERED = 2, PH_RESET = 3, PH_DONE = 4 };
+enum { PH_LS = 0, PH_MOVE = 1, PH_GATHER = 2 };
 
 struct CgState {
     // parameters
     int method, try_recover, force_call_limit, energy_call_limit;
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
     // control
-    int phase, parity, done, status;
+    int phase, parity, dpar, bkpar, done, status;
     int num_iter, num_energy_call, num_force_call, num_safe_mode, num_recovery;
     unsigned error_flags;
     int safe_mode, recovery_move;
-    double grad_inf_norm, energy_rel_change, prev_energy, last_fdotf, fdotf, beta;
+    double grad_inf_norm, energy_rel_change, prev_energy, last_fdotf, fdotf;
+    // pending shiftSearchDir(beta): d <- F + beta d; dir_reset marks resetCGParams
+    int dir_pending, dir_reset;
+    double dir_beta;
     // line search
-    int ls_active, ls_K, ecur_valid, ls_trials;
+    int ls_active, ecur_valid, ls_trials, spec_ok;
     double ls_lambda, ls_cur, ls_slope, ls_reduced, lambda, new_energy, rec_step;
     // minimum-energy backup (CGMethod.hpp:794-814)
     double total_energy, minimum_e;
-    int have_min_e, do_backup;
-    // ticket counters of the last-block reductions
-    unsigned ticket[8];
+    int have_min_e;
+    // launches that did work (statistics)
+    int n_ls_launch, n_move_launch, n_gather_launch;
+    int reserved[2];
 };
 
 __device__ inline int ff_of_block(const DevSys& s, int b) {
@@ -103,153 +129,199 @@ __device__ inline double wave_min(double v) {
     for (int o = 32; o > 0; o >>= 1) v = rmin(v, __shfl_down(v, o, 64));
     return v;
 }
-// op: 0 sum, 1 max, 2 min. Result valid in thread 0.
-template <int OP>
-__device__ inline double block_reduce(double v, double* lds) {
-    v = OP == 0 ? wave_sum(v) : OP == 1 ? wave_max(v) : wave_min(v);
+// op: 0 sum, 1 max, 2 min, for NV values at once. Results valid in thread 0.
+template <int NV>
+__device__ inline void block_reduce_n(double* v, const int* op, double* lds /* >= NV * BLOCK/64 */) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+        v[i] = op[i] == 0 ? wave_sum(v[i]) : op[i] == 1 ? wave_max(v[i]) : wave_min(v[i]);
     __syncthreads();
-    if (lane == 0) lds[w] = v;
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; i++) lds[i * (BLOCK / 64) + w] = v[i];
     __syncthreads();
-    double r = 0;
-    if (threadIdx.x == 0) {
-        r = lds[0];
-        for (int i = 1; i < BLOCK / 64; i++)
-            r = OP == 0 ? r + lds[i] : OP == 1 ? rmax(r, lds[i]) : rmin(r, lds[i]);
-    }
-    return r;
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            double r = lds[i * (BLOCK / 64)];
+            for (int q = 1; q < BLOCK / 64; q++) {
+                double u = lds[i * (BLOCK / 64) + q];
+                r = op[i] == 0 ? r + u : op[i] == 1 ? rmax(r, u) : rmin(r, u);
+            }
+            v[i] = r;
+        }
 }
 
-// Last-block election. Every block must call it after writing its partials.
-__device__ inline bool last_block(unsigned* ticket) {
-    __shared__ int am_last;
-    __threadfence();
+// Stage the whole CG state through LDS: one parallel round trip instead of a chain of
+// dependent global accesses by the thread that runs the scalar logic.
+template <typename S>
+__device__ inline void state_load(S* lds, const S* g) {
+    const int n = sizeof(S) / sizeof(int);
+    const int* src = reinterpret_cast<const int*>(g);
+    int* dst = reinterpret_cast<int*>(lds);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(ticket, 1u);
-        am_last = (t == gridDim.x - 1);
-    }
+}
+template <typename S>
+__device__ inline void state_store(S* g, const S* lds) {
     __syncthreads();
-    if (am_last) {
-        __threadfence();
-        if (threadIdx.x == 0) *ticket = 0u;
-    }
-    return am_last != 0;
+    const int n = sizeof(S) / sizeof(int);
+    const int* src = reinterpret_cast<const int*>(lds);
+    int* dst = reinterpret_cast<int*>(g);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
-// coherent (L2-bypassing) load of a partial written by another block
-__device__ inline double ld_partial(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline int ld_partial_i(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// ---- positions x + lambda * d_eff, with d_eff = d or the pending F + beta d ------------
+struct Pos {
+    const double* __restrict__ x;
+    const double* __restrict__ d;
+    const double* __restrict__ f;
+    double beta, lam;
+    int mode;   // 0: x; 1: x + lam d; 2: x + lam (F + beta d)
+};
+
+__device__ inline D3 pos_of(const Pos& p, int b) {
+    const double* x = p.x + 3 * b;
+    if (p.mode == 0) return D3{x[0], x[1], x[2]};
+    const double* d = p.d + 3 * b;
+    if (p.mode == 1) return D3{x[0] + p.lam * d[0], x[1] + p.lam * d[1], x[2] + p.lam * d[2]};
+    const double* f = p.f + 3 * b;
+    // shiftSearchDir (CGMethod.cpp:356-360) then calcCoordLineSearch (:43-46)
+    double d0 = f[0] + p.beta * d[0], d1 = f[1] + p.beta * d[1], d2 = f[2] + p.beta * d[2];
+    return D3{x[0] + p.lam * d0, x[1] + p.lam * d1, x[2] + p.lam * d2};
 }
 
-// ---------------------------------------------------------------------------------------
-// Energy of term t of force field ff at positions x + lambda*d. *bad is set for an excluded
-// volume pair that stays non-finite (the reference's -1 sentinel).
-__device__ inline double term_energy(const DevSys& s, int ff, int t, const double* __restrict__ x,
-                                     const double* __restrict__ d, double lam, int* bad) {
+__device__ inline void put_slot(const DevSys& s, D3* __restrict__ slots, long id, D3 v) {
+    slots[s.pos[id]] = v;
+}
+
+// Energy (and, when slots != null, forces into the bead-sorted slots) of interaction t of
+// force field ff at positions p. *bad: excluded-volume energy non-finite after the Simpson
+// fallback (the reference's -1 sentinel). *fbad: excluded-volume analytic forces non-finite
+// (the reference switches to forceN).
+__device__ inline void push_defer(const DevSys& s, int code, int t, D3 c1, D3 c2, D3 c3, D3 c4) {
+    const int k = atomicAdd(s.defer_n, 1);
+    if (k >= DEFER_CAP) return;   // overflow is reported by the finalize kernel
+    EnDefer& e = s.defer[k];
+    e.code = code;
+    e.t = t;
+    e.ks = s.ev_ks[t];
+    const D3 cs[4] = {c1, c2, c3, c4};
+    for (int q = 0; q < 4; q++) { e.c[3 * q] = cs[q].x; e.c[3 * q + 1] = cs[q].y; e.c[3 * q + 2] = cs[q].z; }
+}
+
+// code: which energy the value belongs to (recorded with a deferred Simpson pair), or -1 when
+// only the forces are wanted.
+__device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, D3* __restrict__ slots,
+                                 int code, int* fbad) {
+    D3 out[4];
     switch (ff) {
     case FF_STRETCH: {
-        D3 c1 = ld3_at(x, d, lam, s.str_b[2 * t]), c2 = ld3_at(x, d, lam, s.str_b[2 * t + 1]);
+        D3 c1 = pos_of(p, s.str_b[2 * t]), c2 = pos_of(p, s.str_b[2 * t + 1]);
+        if (slots) {
+            stretch_forces(c1, c2, s.str_k[t], s.str_eql[t], out);
+            const long id = s.slot0[FF_STRETCH] + 2L * t;
+            put_slot(s, slots, id, out[0]); put_slot(s, slots, id + 1, out[1]);
+        }
         return stretch_energy(c1, c2, s.str_k[t], s.str_eql[t]);
     }
     case FF_BEND: {
-        D3 c1 = ld3_at(x, d, lam, s.bend_b[3 * t]), c2 = ld3_at(x, d, lam, s.bend_b[3 * t + 1]),
-           c3 = ld3_at(x, d, lam, s.bend_b[3 * t + 2]);
+        D3 c1 = pos_of(p, s.bend_b[3 * t]), c2 = pos_of(p, s.bend_b[3 * t + 1]), c3 = pos_of(p, s.bend_b[3 * t + 2]);
+        if (slots) {
+            bend_forces(c1, c2, c3, s.bend_k[t], s.bend_eqt[t], out);
+            const long id = s.slot0[FF_BEND] + 3L * t;
+            put_slot(s, slots, id, out[0]); put_slot(s, slots, id + 1, out[1]); put_slot(s, slots, id + 2, out[2]);
+        }
         return bend_energy(c1, c2, c3, s.bend_k[t], s.bend_eqt[t]);
     }
     case FF_LINKER:
     case FF_MOTOR: {
-        const int* b = ff == FF_LINKER ? s.lnk_b : s.mot_b;
-        D3 c1 = ld3_at(x, d, lam, b[4 * t]), c2 = ld3_at(x, d, lam, b[4 * t + 1]),
-           c3 = ld3_at(x, d, lam, b[4 * t + 2]), c4 = ld3_at(x, d, lam, b[4 * t + 3]);
-        if (ff == FF_LINKER)
-            return xlink_energy(c1, c2, c3, c4, s.lnk_k[t], s.lnk_eql[t], s.lnk_p1[t], s.lnk_p2[t]);
-        return xlink_energy(c1, c2, c3, c4, s.mot_k[t], s.mot_eql[t], s.mot_p1[t], s.mot_p2[t]);
-    }
-    case FF_EXVOL: {
-        D3 c1 = ld3_at(x, d, lam, s.ev_b[4 * t]), c2 = ld3_at(x, d, lam, s.ev_b[4 * t + 1]),
-           c3 = ld3_at(x, d, lam, s.ev_b[4 * t + 2]), c4 = ld3_at(x, d, lam, s.ev_b[4 * t + 3]);
-        return exvol_energy(c1, c2, c3, c4, s.ev_ks[t], bad);
-    }
-    default: {
-        D3 c = ld3_at(x, d, lam, s.bnd_b[t]);
-        return boundary_energy(s.elem[s.bnd_e[t]], c, s.bnd_krep[t], s.bnd_slen[t]);
-    }
-    }
-}
-
-// Forces of term t into its slots. Returns false for an excluded-volume pair whose analytic
-// forces are non-finite (the reference then integrates numerically, forceN).
-__device__ inline bool term_forces(const DevSys& s, int ff, int t, const double* __restrict__ x,
-                                   const double* __restrict__ d, double lam, D3* __restrict__ slots) {
-    D3 out[4];
-    switch (ff) {
-    case FF_STRETCH: {
-        D3 c1 = ld3_at(x, d, lam, s.str_b[2 * t]), c2 = ld3_at(x, d, lam, s.str_b[2 * t + 1]);
-        stretch_forces(c1, c2, s.str_k[t], s.str_eql[t], out);
-        D3* o = slots + s.slot0[FF_STRETCH] + 2 * (long)t;
-        o[0] = out[0]; o[1] = out[1];
-        return true;
-    }
-    case FF_BEND: {
-        D3 c1 = ld3_at(x, d, lam, s.bend_b[3 * t]), c2 = ld3_at(x, d, lam, s.bend_b[3 * t + 1]),
-           c3 = ld3_at(x, d, lam, s.bend_b[3 * t + 2]);
-        bend_forces(c1, c2, c3, s.bend_k[t], s.bend_eqt[t], out);
-        D3* o = slots + s.slot0[FF_BEND] + 3 * (long)t;
-        o[0] = out[0]; o[1] = out[1]; o[2] = out[2];
-        return true;
-    }
-    case FF_LINKER:
-    case FF_MOTOR: {
-        const int* b = ff == FF_LINKER ? s.lnk_b : s.mot_b;
-        D3 c1 = ld3_at(x, d, lam, b[4 * t]), c2 = ld3_at(x, d, lam, b[4 * t + 1]),
-           c3 = ld3_at(x, d, lam, b[4 * t + 2]), c4 = ld3_at(x, d, lam, b[4 * t + 3]);
-        double sf;
-        if (ff == FF_LINKER) {
-            sf = xlink_forces(c1, c2, c3, c4, s.lnk_k[t], s.lnk_eql[t], s.lnk_p1[t], s.lnk_p2[t], out);
-            s.lnk_sf[t] = sf;
-        } else {
-            sf = xlink_forces(c1, c2, c3, c4, s.mot_k[t], s.mot_eql[t], s.mot_p1[t], s.mot_p2[t], out);
-            s.mot_sf[t] = sf;
+        const bool lk = ff == FF_LINKER;
+        const int* b = lk ? s.lnk_b : s.mot_b;
+        const double k = lk ? s.lnk_k[t] : s.mot_k[t], eql = lk ? s.lnk_eql[t] : s.mot_eql[t];
+        const double p1 = lk ? s.lnk_p1[t] : s.mot_p1[t], p2 = lk ? s.lnk_p2[t] : s.mot_p2[t];
+        D3 c1 = pos_of(p, b[4 * t]), c2 = pos_of(p, b[4 * t + 1]), c3 = pos_of(p, b[4 * t + 2]), c4 = pos_of(p, b[4 * t + 3]);
+        if (slots) {
+            double sf = xlink_forces(c1, c2, c3, c4, k, eql, p1, p2, out);
+            (lk ? s.lnk_sf : s.mot_sf)[t] = sf;
+            const long id = s.slot0[ff] + 4L * t;
+            for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
         }
-        D3* o = slots + s.slot0[ff] + 4 * (long)t;
-        o[0] = out[0]; o[1] = out[1]; o[2] = out[2]; o[3] = out[3];
-        return true;
+        return xlink_energy(c1, c2, c3, c4, k, eql, p1, p2);
     }
     case FF_EXVOL: {
-        D3 c1 = ld3_at(x, d, lam, s.ev_b[4 * t]), c2 = ld3_at(x, d, lam, s.ev_b[4 * t + 1]),
-           c3 = ld3_at(x, d, lam, s.ev_b[4 * t + 2]), c4 = ld3_at(x, d, lam, s.ev_b[4 * t + 3]);
-        bool ok = exvol_forces(c1, c2, c3, c4, s.ev_ks[t], out);
-        D3* o = slots + s.slot0[FF_EXVOL] + 4 * (long)t;
-        if (!ok) out[0] = out[1] = out[2] = out[3] = D3{0.0, 0.0, 0.0};
-        o[0] = out[0]; o[1] = out[1]; o[2] = out[2]; o[3] = out[3];
-        return ok;
+        D3 c1 = pos_of(p, s.ev_b[4 * t]), c2 = pos_of(p, s.ev_b[4 * t + 1]), c3 = pos_of(p, s.ev_b[4 * t + 2]),
+           c4 = pos_of(p, s.ev_b[4 * t + 3]);
+        int deferred = 0;
+        double U = 0.0;
+        if (slots) {
+            bool ok = true;
+            if (code < 0) ok = exvol_forces(c1, c2, c3, c4, s.ev_ks[t], out);
+            else U = exvol_energy_forces(c1, c2, c3, c4, s.ev_ks[t], &deferred, &ok, out);
+            if (!ok) {
+                out[0] = out[1] = out[2] = out[3] = D3{0.0, 0.0, 0.0};
+                *fbad = 1;
+            }
+            const long id = s.slot0[FF_EXVOL] + 4L * t;
+            for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
+        } else {
+            U = exvol_energy(c1, c2, c3, c4, s.ev_ks[t], &deferred);
+        }
+        if (deferred && code >= 0) push_defer(s, code, t, c1, c2, c3, c4);
+        return U;
     }
     default: {
-        D3 c = ld3_at(x, d, lam, s.bnd_b[t]);
-        slots[s.slot0[FF_BOUNDARY] + t] = boundary_force(s.elem[s.bnd_e[t]], c, s.bnd_krep[t], s.bnd_slen[t]);
-        return true;
+        D3 c = pos_of(p, s.bnd_b[t]);
+        const BElem& e = s.elem[s.bnd_e[t]];
+        if (slots) put_slot(s, slots, s.slot0[FF_BOUNDARY] + t, boundary_force(e, c, s.bnd_krep[t], s.bnd_slen[t]));
+        return boundary_energy(e, c, s.bnd_krep[t], s.bnd_slen[t]);
     }
     }
 }
 
-// Sum of bead b's slots with slot index in [lo, hi), in ascending slot order, starting from
-// `acc` (the reference's `+=` into force).
-__device__ inline D3 gather_bead(const DevSys& s, int b, const D3* __restrict__ slots, long lo, long hi, D3 acc) {
-    for (int q = s.inc_s_start[b]; q < s.inc_s_start[b + 1]; q++) {
-        long sl = s.inc_s[q];
-        if (sl < lo || sl >= hi) continue;
-        D3 v = slots[sl];
-        acc.x += v.x; acc.y += v.y; acc.z += v.z;
-    }
-    for (int q = s.inc_d_start[b]; q < s.inc_d_start[b + 1]; q++) {
-        long sl = s.inc_d[q];
-        if (sl < lo || sl >= hi) continue;
-        D3 v = slots[sl];
-        acc.x += v.x; acc.y += v.y; acc.z += v.z;
+// Gather of GB beads per block: the beads own the contiguous slot range
+// [bstart[b0], bstart[b0 + GB]); all 256 threads stream it into LDS with coalesced, unrolled
+// loads (GUNROLL in flight per thread), then the first wave adds each bead's slots in ascending
+// order (the reference's `+=` order). Only slots whose id is in [lo, hi) are added when `all` is
+// false. Returns the sum for threads < GB.
+constexpr int GCHUNK = 1536;                 // slots per LDS chunk (36 KiB)
+constexpr int GUNROLL = 3 * GCHUNK / BLOCK;  // 18 doubles per thread per chunk
+
+__device__ inline D3 gather_block(const DevSys& s, const D3* __restrict__ slots, long lo, long hi, bool all,
+                                  D3 acc, double* lds /* [3 * GCHUNK] */) {
+    const int b0 = blockIdx.x * GB;
+    const int b = b0 + threadIdx.x;
+    const int bend = min(b0 + GB, s.n_beads);
+    const int r0 = s.bstart[b0], r1 = s.bstart[bend];
+    const bool mine = threadIdx.x < GB && b < s.n_beads;
+    const int my0 = mine ? s.bstart[b] : 0, my1 = mine ? s.bstart[b + 1] : 0;
+    const double* __restrict__ src = reinterpret_cast<const double*>(slots);
+    for (int c0 = r0; c0 < r1; c0 += GCHUNK) {
+        const int c1 = min(c0 + GCHUNK, r1);
+        const int n = 3 * (c1 - c0);
+        double v[GUNROLL];
+#pragma unroll
+        for (int j = 0; j < GUNROLL; j++) {
+            const int i = threadIdx.x + j * BLOCK;
+            v[j] = i < n ? src[3L * c0 + i] : 0.0;
+        }
+        __syncthreads();   // the previous chunk is consumed
+#pragma unroll
+        for (int j = 0; j < GUNROLL; j++) {
+            const int i = threadIdx.x + j * BLOCK;
+            if (i < n) lds[i] = v[j];
+        }
+        __syncthreads();
+        const int q0 = max(my0, c0), q1 = min(my1, c1);
+        for (int q = q0; q < q1; q++) {
+            if (!all) {
+                long id = s.sorted_id[q];
+                if (id < lo || id >= hi) continue;
+            }
+            const double* w = lds + 3 * (q - c0);
+            acc.x += w[0]; acc.y += w[1]; acc.z += w[2];
+        }
     }
     return acc;
 }
@@ -258,61 +330,135 @@ __device__ inline D3 gather_bead(const DevSys& s, int b, const D3* __restrict__ 
 // Plain (non-CG) kernels
 // =======================================================================================
 
-// Slots of every term at x (forces of ForceFieldManager::computeForces before the gather).
-// bad_count / bad_list collect excluded-volume pairs that need the numerical fallback.
+// Slots of every interaction at x (ForceFieldManager::computeForces before the gather).
+// Excluded-volume pairs needing the numerical fallback are listed in bad_list.
 __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* __restrict__ x,
                                                         D3* __restrict__ slots, int* bad_count, int* bad_list) {
     const int ff = ff_of_block(s, blockIdx.x);
     if (ff >= NFF) return;
     const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
     if (t >= s.n[ff]) return;
-    if (!term_forces(s, ff, t, x, nullptr, 0.0, slots)) {
-        int k = atomicAdd(bad_count, 1);
-        bad_list[k] = t;
+    Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
+    D3 out[4];
+    // forces only (no energy): same arithmetic as term_ef's force half
+    switch (ff) {
+    case FF_EXVOL: {
+        D3 c1 = pos_of(p, s.ev_b[4 * t]), c2 = pos_of(p, s.ev_b[4 * t + 1]), c3 = pos_of(p, s.ev_b[4 * t + 2]),
+           c4 = pos_of(p, s.ev_b[4 * t + 3]);
+        bool ok = exvol_forces(c1, c2, c3, c4, s.ev_ks[t], out);
+        if (!ok) {
+            out[0] = out[1] = out[2] = out[3] = D3{0.0, 0.0, 0.0};
+            int k = atomicAdd(bad_count, 1);
+            bad_list[k] = t;
+        }
+        const long id = s.slot0[FF_EXVOL] + 4L * t;
+        for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
+        return;
+    }
+    default: {
+        int fbad = 0;
+        (void)term_ef(s, ff, t, p, slots, -1, &fbad);   // the energy half is dead code here
+        return;
+    }
     }
 }
 
-// force[b] = base[b] + sum of b's slots within [lo, hi) (base may be null: zero-fill).
+// force[b] = base[b] + sum of b's slots with id in [lo, hi) (base may be null: zero-fill).
+// Grid: bead blocks only.
 __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, const D3* __restrict__ slots, long lo, long hi,
                                                         const double* __restrict__ base, double* __restrict__ f) {
-    const int b = blockIdx.x * BLOCK + threadIdx.x;   // grid: bead blocks only
-    if (b >= s.n_beads) return;
-    D3 acc = base ? ld3(base, b) : D3{0.0, 0.0, 0.0};
-    acc = gather_bead(s, b, slots, lo, hi, acc);
+    __shared__ double lds[3 * GCHUNK];
+    const int b = blockIdx.x * GB + threadIdx.x;
+    const bool mine = threadIdx.x < GB && b < s.n_beads;
+    D3 acc = (base && mine) ? D3{base[3 * b], base[3 * b + 1], base[3 * b + 2]} : D3{0.0, 0.0, 0.0};
+    acc = gather_block(s, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, lds);
+    if (!mine) return;
     f[3 * b] = acc.x; f[3 * b + 1] = acc.y; f[3 * b + 2] = acc.z;
 }
 
-// Per-FF energies at x: out[0..5] per force field (exvol = -1 on the sentinel), out[6] =
-// computeIndividualEnergies total (plain sum), out[7] = computeEnergy total (+inf on sentinel).
+// Block-parallel Simpson energies (energyN) of the pairs deferred by the preceding kernel.
+// Thread 0 receives, per code, the sum of the energies in (code, pair index) order and a flag for
+// a pair that stays non-finite (the reference's -1 sentinel); the deferred list is reset.
+// Returns the number of deferred pairs (DEFER_CAP exceeded when larger).
+__device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int* badc, int ncode) {
+    __shared__ int nsh;
+    __syncthreads();
+    if (threadIdx.x == 0) nsh = *s.defer_n;
+    __syncthreads();
+    const int n = nsh;
+    const int m = min(n, DEFER_CAP);
+    const int op1[1] = {0};
+    for (int k = 0; k < m; k++) {
+        EnDefer& e = s.defer[k];
+        const D3 c1{e.c[0], e.c[1], e.c[2]}, c2{e.c[3], e.c[4], e.c[5]}, c3{e.c[6], e.c[7], e.c[8]},
+            c4{e.c[9], e.c[10], e.c[11]};
+        double r = exvol_energyN_share(c1, c2, c3, c4, threadIdx.x, BLOCK);
+        block_reduce_n<1>(&r, op1, lds);
+        if (threadIdx.x == 0) e.U = exvol_energyN_finish(c1, c2, c3, c4, e.ks, r);
+    }
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < ncode; c++) {
+            sum[c] = 0.0;
+            badc[c] = 0;
+            int last_t = -1;
+            for (;;) {
+                int best = -1;
+                for (int k = 0; k < m; k++)
+                    if (s.defer[k].code == c && s.defer[k].t > last_t && (best < 0 || s.defer[k].t < s.defer[best].t))
+                        best = k;
+                if (best < 0) break;
+                const double U = s.defer[best].U;
+                if (energy_bad(U)) badc[c] = 1;
+                else sum[c] += U;
+                last_t = s.defer[best].t;
+            }
+        }
+        *s.defer_n = 0;
+    }
+    return n;
+}
+
+// Per-FF energies at x, block partials. Grid: interaction blocks only.
 __global__ void __launch_bounds__(BLOCK) k_energy_report(DevSys s, const double* __restrict__ x,
-                                                         double* __restrict__ partial, int* __restrict__ partial_bad,
-                                                         unsigned* ticket, double* __restrict__ out) {
+                                                         double* __restrict__ partial) {
     __shared__ double lds[BLOCK / 64];
     const int ff = ff_of_block(s, blockIdx.x);
-    double e = 0.0;
-    int bad = 0;
+    double v = 0.0;
     if (ff < NFF) {
         const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
-        if (t < s.n[ff]) e = term_energy(s, ff, t, x, nullptr, 0.0, &bad);
+        int fbad = 0;
+        Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
+        if (t < s.n[ff]) v = term_ef(s, ff, t, p, nullptr, 0, &fbad);
     }
-    double bsum = block_reduce<0>(e, lds);
-    double bbad = block_reduce<1>((double)bad, lds);
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = bsum;
-        partial_bad[blockIdx.x] = bbad > 0.0;
+    const int op1[1] = {0};
+    block_reduce_n<1>(&v, op1, lds);
+    if (threadIdx.x == 0) partial[PSTRIDE * blockIdx.x] = v;
+}
+
+// out[0..5] per force field (excluded volume = -1 on the sentinel), out[6] =
+// computeIndividualEnergies total (plain sum), out[7] = computeEnergy total (+inf on a sentinel),
+// out[8] = deferred Simpson pairs beyond capacity. One block.
+__global__ void __launch_bounds__(BLOCK) k_energy_report_fin(DevSys s, const double* __restrict__ partial,
+                                                             double* __restrict__ out) {
+    __shared__ double lds[NFF * (BLOCK / 64)];
+    double acc[NFF];
+    int opn[NFF];
+#pragma unroll
+    for (int f2 = 0; f2 < NFF; f2++) {
+        acc[f2] = 0.0;
+        opn[f2] = 0;
+        for (int b = s.blk[f2] + threadIdx.x; b < s.blk[f2 + 1]; b += BLOCK) acc[f2] += partial[PSTRIDE * b];
     }
-    if (!last_block(ticket)) return;
+    block_reduce_n<NFF>(acc, opn, lds);
+    double dsum[1];
+    int dbad[1];
+    const int n = run_defers(s, lds, dsum, dbad, 1);
     if (threadIdx.x == 0) {
+        acc[FF_EXVOL] += dsum[0];
         double tot = 0.0, tot_s = 0.0;
         bool failed = false;
         for (int f2 = 0; f2 < NFF; f2++) {
-            double ef = 0.0;
-            int bd = 0;
-            for (int b = s.blk[f2]; b < s.blk[f2 + 1]; b++) {
-                ef += ld_partial(&partial[b]);
-                bd |= ld_partial_i(&partial_bad[b]);
-            }
-            if (bd) ef = -1.0;
+            double ef = (f2 == FF_EXVOL && dbad[0]) ? -1.0 : acc[f2];
             out[f2] = ef;
             tot += ef;
             if (!failed) {
@@ -322,12 +468,13 @@ __global__ void __launch_bounds__(BLOCK) k_energy_report(DevSys s, const double*
         }
         out[6] = tot;
         out[7] = failed ? INFINITY : tot_s;
+        out[8] = n > DEFER_CAP ? (double)n : 0.0;
     }
 }
 
 // Numerical excluded-volume force fallback (forceN, CylinderExclVolRepulsion.cpp:984-1254)
 // for one pair: block b handles rows i = b, b + grid, ... of the 10^4 x 10^4 Simpson grid
-// and writes 6 row-sum partials per block; k_exvol_forceN_finish adds them in block order.
+// and writes 6 partials per block; k_exvol_forceN_finish adds them in block order.
 __device__ inline void exvol_force_integrand(double a, double b, double c, double d, double e, double F,
                                              double s, double t, double* out) {
     double r_2 = c + 2 * s * e + s * s * a - 2 * t * F - 2 * s * t * d + t * t * b;
@@ -350,7 +497,7 @@ __device__ inline PairGeom exvol_pair_geom(const DevSys& s, const double* x, int
 
 __global__ void __launch_bounds__(BLOCK) k_exvol_forceN_rows(DevSys s, const double* __restrict__ x, int t,
                                                              int movebeads, double* __restrict__ partial) {
-    __shared__ double lds[BLOCK / 64];
+    __shared__ double lds[6 * (BLOCK / 64)];
     PairGeom g = exvol_pair_geom(s, x, t, movebeads);
     double a = scalar_product(g.c1, g.c2, g.c1, g.c2), b = scalar_product(g.c3, g.c4, g.c3, g.c4);
     double c = scalar_product(g.c3, g.c1, g.c3, g.c1), d = scalar_product(g.c1, g.c2, g.c3, g.c4);
@@ -358,7 +505,7 @@ __global__ void __launch_bounds__(BLOCK) k_exvol_forceN_rows(DevSys s, const dou
     const int N = 10000;
     const double delta = 1.0 / ((double)N);
     double acc[6] = {0, 0, 0, 0, 0, 0}, in[6];
-    // T2 / T3 / T4 rows: row i has s = (i+1)*delta (the loop advances s before use)
+    // rows i = 0..N with s = (i+1)*delta (the reference's loop advances s before use)
     for (int i = blockIdx.x; i <= N; i += gridDim.x) {
         const double sv = (i + 1) * delta;
         const double fs = (i % 2 == 0) ? 2.0 : 4.0;
@@ -382,10 +529,10 @@ __global__ void __launch_bounds__(BLOCK) k_exvol_forceN_rows(DevSys s, const dou
             }
         }
     }
-    for (int k = 0; k < 6; k++) {
-        double r = block_reduce<0>(acc[k], lds);
-        if (threadIdx.x == 0) partial[6 * blockIdx.x + k] = r;
-    }
+    const int op6[6] = {0, 0, 0, 0, 0, 0};
+    block_reduce_n<6>(acc, op6, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) partial[6 * blockIdx.x + k] = acc[k];
 }
 
 __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, int t, int movebeads, int nblk,
@@ -422,332 +569,397 @@ __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, in
     }
     *ok_out = ok ? 1 : 0;
     if (ok) {
-        D3* o = slots + s.slot0[FF_EXVOL] + 4 * (long)t;
-        for (int q = 0; q < 4; q++) o[q] = D3{f[q][0], f[q][1], f[q][2]};
+        const long id = s.slot0[FF_EXVOL] + 4L * t;
+        for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, D3{f[q][0], f[q][1], f[q][2]});
     }
 }
 
 // =======================================================================================
-// CG kernels: one "slot" of the device-driven minimizer is
-//     k_cg_ls -> k_cg_move -> k_cg_gather -> k_cg_dir
-// Each kernel reads the device state first and returns at once when its phase is not due.
+// CG kernels. One "slot" of the device-driven minimizer is
+//     k_cg_ls -> k_cg_move -> k_cg_gather
+// Each kernel reads the device state first and returns at once when it has nothing to do.
+// A typical CG iteration is one k_cg_ls (trial energies at lambda and lambda/2, forces at
+// lambda speculatively, the pending direction update applied on the fly) and one k_cg_gather;
+// k_cg_move only runs when the accepted step is not the first trial, or for the recovery move.
 // =======================================================================================
 
 struct CgBufs {
     double* x[2];       // current / next coordinates (parity)
+    double* d[2];       // current / next search direction (dpar)
+    double* xmin[2];    // committed / candidate minimum-energy backup (bkpar)
     double* f;          // force
     double* fp;         // previous force
-    double* d;          // search direction
-    double* xmin;       // minimum-energy backup
     D3* slots;
-    double* partial;    // [blocks * 16]
-    int* partial_i;     // [blocks * 16]
+    double* partial;    // [blocks * PSTRIDE]
     int* bad_count;     // excluded-volume pairs needing the numerical fallback in a CG move
 };
 
-// ---- line-search trials: E(x + lambda_k d) for k = 0..K (k = 0 only when E(x) unknown) ----
-__global__ void __launch_bounds__(BLOCK) k_cg_ls(DevSys s, CgState* __restrict__ st, CgBufs B) {
-    __shared__ double lds[BLOCK / 64];
-    if (st->done || st->phase != PH_LS) return;
-    const int K = st->ls_K;
-    const bool first = st->ls_active == 0;
-    const bool need_e0 = first && !st->ecur_valid;
-    const double lam0 = st->ls_lambda;
-    const double* __restrict__ x = B.x[st->parity];
-    const int ff = ff_of_block(s, blockIdx.x);
-    double e[MAXK + 1];
-    int bad[MAXK + 1];
-#pragma unroll
-    for (int k = 0; k <= MAXK; k++) { e[k] = 0.0; bad[k] = 0; }
-    double slope = 0.0, step = INFINITY;
-    if (ff < NFF) {
-        const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
-        if (t < s.n[ff]) {
-            if (need_e0) e[0] = term_energy(s, ff, t, x, nullptr, 0.0, &bad[0]);
-            double lam = lam0;
-            for (int k = 1; k <= K; k++) {
-                e[k] = term_energy(s, ff, t, x, B.d, lam, &bad[k]);
-                lam *= 0.5;
-            }
-        }
-    } else if (first) {
-        const int b = (blockIdx.x - s.blk[NFF]) * BLOCK + threadIdx.x;
-        if (b < s.n_beads) {
-            for (int q = 0; q < 3; q++) {
-                double dv = B.d[3 * b + q], fv = B.f[3 * b + q];
-                slope += dv * fv;
-                // moveAlongDirectionLimitingBeadMove_ step (MController.h:293-305)
-                double v = st->max_dist / fabs(dv);
-                step = rmin(step, v);
-            }
-        }
-    }
-    double* P = B.partial + (long)blockIdx.x * 16;
-    int* PI = B.partial_i + (long)blockIdx.x * 16;
-    for (int k = 0; k <= K; k++) {
-        double r = block_reduce<0>(e[k], lds);
-        double rb = block_reduce<1>((double)bad[k], lds);
-        if (threadIdx.x == 0) { P[k] = r; PI[k] = rb > 0.0; }
-    }
-    if (first) {
-        double r = block_reduce<0>(slope, lds);
-        double m = block_reduce<2>(step, lds);
-        if (threadIdx.x == 0) { P[MAXK + 1] = r; P[MAXK + 2] = m; }
-    }
-    if (!last_block(&st->ticket[0])) return;
-    if (threadIdx.x != 0) return;
+__device__ inline bool call_limit(const CgState* st) {
+    return (st->energy_call_limit != 0 && st->num_energy_call > st->energy_call_limit) ||
+           (st->force_call_limit != 0 && st->num_force_call > st->force_call_limit);
+}
+__device__ inline bool converged(const CgState* st) {
+    return st->grad_inf_norm < st->grad_tol || st->energy_rel_change < st->energy_rel_tol;
+}
 
-    // ---- final reductions in block order, per force field then total (computeEnergy) ----
-    double E[MAXK + 1];
-    for (int k = 0; k <= K; k++) {
-        double tot = 0.0;
-        bool failed = false;
-        for (int f2 = 0; f2 < NFF && !failed; f2++) {
-            double ef = 0.0;
-            int bd = 0;
-            for (int b = s.blk[f2]; b < s.blk[f2 + 1]; b++) {
-                ef += ld_partial(&B.partial[(long)b * 16 + k]);
-                bd |= ld_partial_i(&B.partial_i[(long)b * 16 + k]);
+// ---- line search: trial energies (and speculative forces at the first trial) ----------
+// Grid: NTRIAL copies of the interaction blocks, then the bead blocks. Writes block partials;
+// k_cg_ls_fin decides.
+__global__ void __launch_bounds__(BLOCK) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    if (st->done || st->phase != PH_LS) return;
+    const bool first = st->ls_active == 0;
+    const bool pend = first && st->dir_pending;
+    const int T = s.blk[NFF];
+    const double lam0 = st->ls_lambda;
+    const double beta = st->dir_beta;
+    const bool need_e0 = first && !st->ecur_valid;
+    const int par = st->parity, dpar = st->dpar;
+    const double* __restrict__ xc = B.x[par];
+    const double* __restrict__ dc = B.d[dpar];
+    const bool term_block = (int)blockIdx.x < NTRIAL * T;
+    // block partials: interaction blocks (E(trial), E(x)); bead blocks (d.F, recovery step)
+    double v[2] = {0.0, term_block ? 0.0 : INFINITY};
+    if (term_block) {
+        const int trial = blockIdx.x / T;
+        const int blk = blockIdx.x - trial * T;
+        const int ff = ff_of_block(s, blk);
+        const int t = (blk - s.blk[ff]) * BLOCK + threadIdx.x;
+        int fbad = 0;
+        if (t < s.n[ff]) {
+            Pos p{xc, dc, B.f, beta, trial == 0 ? lam0 : lam0 * 0.5, pend ? 2 : 1};
+            v[0] = term_ef(s, ff, t, p, trial == 0 ? B.slots : nullptr, trial, &fbad);
+            // E(x) rides with the energy-only trial (lower register pressure than the force trial)
+            if (trial == 1 && need_e0) {
+                Pos p0{xc, nullptr, nullptr, 0.0, 0.0, 0};
+                v[1] = term_ef(s, ff, t, p0, nullptr, 2, &fbad);
             }
-            if (bd) ef = -1.0;
-            if (ef <= -1) failed = true;
-            else tot += ef;
+            if (fbad) atomicAdd(B.bad_count, 1);
         }
-        E[k] = failed ? INFINITY : tot;
+    } else {
+        const int b = (blockIdx.x - NTRIAL * T) * BLOCK + threadIdx.x;
+        if (b < s.n_beads) {
+            const double maxd = st->max_dist;
+            double* xn = B.x[par ^ 1];
+            double* dn = B.d[dpar ^ 1];
+            for (int q = 0; q < 3; q++) {
+                const double fv = B.f[3 * b + q];
+                const double de = pend ? fv + beta * dc[3 * b + q] : dc[3 * b + q];
+                xn[3 * b + q] = xc[3 * b + q] + lam0 * de;   // speculative x_next (first trial)
+                if (first) {
+                    v[0] += de * fv;                          // d.F
+                    v[1] = rmin(v[1], maxd / fabs(de));       // MController.h:293-305
+                    if (pend) {
+                        dn[3 * b + q] = de;                   // shiftSearchDir materialized
+                        B.fp[3 * b + q] = fv;                 // forcePrev = force
+                    }
+                }
+            }
+        }
     }
-    double lambda = lam0;
+    const int op2t[2] = {0, 0}, op2b[2] = {0, 2};
+    block_reduce_n<2>(v, term_block ? op2t : op2b, lds);
+    if (threadIdx.x == 0) {
+        B.partial[(long)PSTRIDE * blockIdx.x] = v[0];
+        B.partial[(long)PSTRIDE * blockIdx.x + 1] = v[1];
+    }
+}
+
+__device__ inline bool call_limit_s(const CgState& S) {
+    return (S.energy_call_limit != 0 && S.num_energy_call > S.energy_call_limit) ||
+           (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
+}
+__device__ inline bool converged_s(const CgState& S) {
+    return S.grad_inf_norm < S.grad_tol || S.energy_rel_change < S.energy_rel_tol;
+}
+
+// Scalar logic after the trial energies: backtrackingLineSearch (CGMethod.hpp:238-373) over the
+// evaluated step lengths, and the failure handling of CGMethod::minimize (:1337-1367).
+// acc = {E(lambda), E(lambda/2), bad, bad2, E(x), bad0, d.F, recovery step}
+__device__ inline void ls_control(CgState& S, const double* acc, int* bad_count) {
+    const bool first = S.ls_active == 0;
+    const double E1 = acc[2] > 0.0 ? INFINITY : acc[0];   // computeEnergy: -1 sentinel -> +inf
+    const double E2 = acc[3] > 0.0 ? INFINITY : acc[1];
+    const double lam0 = S.ls_lambda;
+    S.n_ls_launch += 1;
     if (first) {
-        double slope_t = 0.0, step_t = INFINITY;
-        if (s.blk[NFF + 1] > s.blk[NFF]) {
-            for (int b = s.blk[NFF]; b < s.blk[NFF + 1]; b++) {
-                slope_t += ld_partial(&B.partial[(long)b * 16 + MAXK + 1]);
-                step_t = rmin(step_t, ld_partial(&B.partial[(long)b * 16 + MAXK + 2]));
+        if (S.dir_pending) {
+            if (!S.dir_reset && acc[6] <= 0) {
+                // d.F <= 0 after shiftSearchDir(beta): resetCGParams (CGMethod.hpp:1505-1507);
+                // redo this line-search launch with d = F
+                S.dir_beta = 0.0;
+                S.dir_reset = 1;
+                *bad_count = 0;
+                return;
             }
+            S.dpar ^= 1;
+            S.dir_pending = 0;
+            if (S.dir_reset) S.last_fdotf = S.fdotf;   // lastFDotF = F.F
+            S.dir_reset = 0;
         }
-        st->num_iter += 1;           // numIter++ at the top of the loop (CGMethod.hpp:1297)
-        st->rec_step = step_t;
-        st->ls_trials = 0;
-        if (st->grad_inf_norm == 0.0) {
-            // backtrackingLineSearch returns lambda = 0 at once when maxForce == 0 (:252-256)
-        } else {
-            st->ls_cur = need_e0 ? E[0] : st->ls_cur;
-            st->num_energy_call += 1;
-            st->ls_slope = slope_t;
-            st->ls_reduced = (st->safe_mode ? 0.0 : 0.4) * slope_t;
-            st->ls_active = 1;
+        S.num_iter += 1;             // numIter++ at the top of the loop (CGMethod.hpp:1297)
+        S.rec_step = acc[7];
+        S.ls_trials = 0;
+        if (S.grad_inf_norm != 0.0) {   // maxForce == 0 returns lambda = 0 at once (:252-256)
+            if (!S.ecur_valid) S.ls_cur = acc[5] > 0.0 ? INFINITY : acc[4];
+            S.num_energy_call += 1;
+            S.ls_slope = acc[6];
+            S.ls_reduced = (S.safe_mode ? 0.0 : 0.4) * acc[6];
+            S.ls_active = 1;
         }
     }
-    // ---- backtrackingLineSearch loop (CGMethod.hpp:297-361) over the evaluated trials ----
     bool conv = false;
-    double energyLambda = 0.0;
-    if (first && st->grad_inf_norm == 0.0) {
+    int acc_trial = -1;
+    double lambda = lam0, energyLambda = 0.0;
+    if (first && S.grad_inf_norm == 0.0) {
         conv = true;
         lambda = 0.0;
-        energyLambda = 0.0;
     }
-    for (int k = 1; k <= K && !conv; k++) {
-        energyLambda = E[k];
-        st->num_energy_call += 1;
-        st->ls_trials += 1;
-        const double ideal = -st->ls_reduced * lambda;
+    for (int k = 0; k < NTRIAL && !conv; k++) {
+        energyLambda = k == 0 ? E1 : E2;
+        S.num_energy_call += 1;
+        S.ls_trials += 1;
+        const double ideal = -S.ls_reduced * lambda;
         if (lambda < 0 || ideal > 0) { conv = true; lambda = 0.0; }
-        const double change = energyLambda - st->ls_cur;
-        if (change <= ideal) conv = true;
+        const double change = energyLambda - S.ls_cur;
+        if (change <= ideal) { conv = true; acc_trial = k; }
         else lambda *= 0.5;
         if (lambda <= 0.0 || lambda <= 1e-8) {
             conv = true;
+            acc_trial = -1;
             lambda = 0.0;
-            energyLambda = st->ls_cur;
+            energyLambda = S.ls_cur;
         }
     }
-    if (!conv) {                 // keep halving in the next slot
-        st->ls_lambda = lambda;
-        st->ls_K = MAXK;
+    const bool spec = *bad_count == 0;
+    *bad_count = 0;
+    if (!conv) {                 // keep halving in the next launch
+        S.ls_lambda = lambda;
         return;
     }
-    const double rec_step = st->rec_step;
-    st->total_energy = energyLambda;   // TotalEnergy = energyLambda (CGMethod.hpp:368)
-    st->ls_active = 0;
-    {
-        int k2 = st->ls_trials + 1;
-        st->ls_K = k2 < 2 ? 2 : (k2 > MAXK ? MAXK : k2);
-    }
+    S.total_energy = energyLambda;   // TotalEnergy = energyLambda (CGMethod.hpp:368)
+    S.ls_active = 0;
     if (lambda != 0.0) {
-        st->lambda = lambda;
-        st->new_energy = energyLambda;
-        st->recovery_move = 0;
-        st->phase = PH_MOVE;
+        S.lambda = lambda;
+        S.new_energy = energyLambda;
+        S.recovery_move = 0;
+        // accepted at the first trial with usable speculative forces: gather directly
+        S.phase = (acc_trial == 0 && spec) ? PH_GATHER : PH_MOVE;
         return;
     }
-    // ---- line-search failure (CGMethod.hpp:1337-1367) ----
-    if (st->try_recover) {
-        if (st->safe_mode) {
-            if (rec_step == INFINITY) {   // MController.h:312-316 throws
-                st->status = 7;           // MDN_ERR_RECOVERY
-                st->done = 1;
+    if (S.try_recover) {
+        if (S.safe_mode) {
+            if (S.rec_step == INFINITY) {   // MController.h:312-316 throws
+                S.status = 7;               // MDN_ERR_RECOVERY
+                S.done = 1;
                 return;
             }
-            st->lambda = rec_step;
-            st->recovery_move = 1;
-            st->phase = PH_MOVE;
-        } else {
-            st->safe_mode = 1;
-            st->num_safe_mode += 1;
-            st->phase = PH_RESET;
+            S.lambda = S.rec_step;
+            S.recovery_move = 1;
+            S.phase = PH_MOVE;
+            return;
         }
-    } else {
-        st->error_flags |= 1u;
-        st->done = 1;
+        S.safe_mode = 1;
+        S.num_safe_mode += 1;
+        S.dir_pending = 1;             // resetCGParams before the next line search
+        S.dir_beta = 0.0;
+        S.dir_reset = 1;
+        if (call_limit_s(S) || converged_s(S)) S.done = 1;
+        return;
     }
-    // loop condition after an iteration that did not move
-    if (!st->done && st->phase == PH_RESET) {
-        bool limit = (st->energy_call_limit != 0 && st->num_energy_call > st->energy_call_limit) ||
-                     (st->force_call_limit != 0 && st->num_force_call > st->force_call_limit);
-        bool convd = st->grad_inf_norm < st->grad_tol || st->energy_rel_change < st->energy_rel_tol;
-        if (limit || convd) st->done = 1;
-    }
+    S.error_flags |= 1u;
+    S.done = 1;
 }
 
-// ---- move + term forces at x + lambda d; writes x_next ----
-__global__ void __launch_bounds__(BLOCK) k_cg_move(DevSys s, CgState* __restrict__ st, CgBufs B) {
+// One block: reduce the line-search partials in a fixed order (8 independent loads in flight per
+// thread), add the deferred Simpson energies, and run ls_control.
+__global__ void __launch_bounds__(BLOCK) k_cg_ls_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
+    __shared__ CgState S;
+    __shared__ double lds[8 * (BLOCK / 64)];
+    state_load(&S, st);
+    if (S.done || S.phase != PH_LS) return;
+    const int T = s.blk[NFF];
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
+    const int op8[8] = {0, 0, 1, 1, 0, 1, 0, 2};   // E1, E2, bad1, bad2, E0, bad0, d.F, step
+    for (int base = 0; base < nblk; base += 8 * BLOCK) {
+        double p0[8], p1[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bk = base + threadIdx.x + j * BLOCK;
+            p0[j] = bk < nblk ? B.partial[(long)PSTRIDE * bk] : 0.0;
+            p1[j] = bk < nblk ? B.partial[(long)PSTRIDE * bk + 1] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bk = base + threadIdx.x + j * BLOCK;
+            if (bk >= nblk) continue;
+            if (bk < NTRIAL * T) {
+                const int trial = bk / T;
+                acc[trial] += p0[j];
+                if (trial == 1) acc[4] += p1[j];
+            } else {
+                acc[6] += p0[j];
+                acc[7] = rmin(acc[7], p1[j]);
+            }
+        }
+    }
+    block_reduce_n<8>(acc, op8, lds);
+    double dsum[3];
+    int dbad[3];
+    const int n = run_defers(s, lds, dsum, dbad, 3);
+    if (threadIdx.x == 0) {
+        acc[0] += dsum[0]; acc[2] = dbad[0];
+        acc[1] += dsum[1]; acc[3] = dbad[1];
+        acc[4] += dsum[2]; acc[5] = dbad[2];
+        if (n > DEFER_CAP) {
+            S.status = 9;
+            S.done = 1;
+        } else {
+            ls_control(S, acc, B.bad_count);
+        }
+    }
+    state_store(st, &S);
+}
+
+// ---- forces and x_next at the accepted (non-first) step or the recovery step ----------
+// Small grid (MOVE_GRID blocks) striding over all interactions and beads: the kernel is a no-op
+// in most slots, so its empty launch must be cheap.
+__global__ void __launch_bounds__(BLOCK) k_cg_move(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     if (st->done || st->phase != PH_MOVE) return;
     const double lam = st->lambda;
-    const double* __restrict__ x = B.x[st->parity];
-    const int ff = ff_of_block(s, blockIdx.x);
-    if (ff < NFF) {
-        const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
-        if (t < s.n[ff])
-            if (!term_forces(s, ff, t, x, B.d, lam, B.slots)) atomicAdd(B.bad_count, 1);
-    } else {
-        const int b = (blockIdx.x - s.blk[NFF]) * BLOCK + threadIdx.x;
-        if (b < s.n_beads) {
-            double* xn = B.x[st->parity ^ 1];
-            for (int q = 0; q < 3; q++) xn[3 * b + q] = x[3 * b + q] + lam * B.d[3 * b + q];
+    const int par = st->parity;
+    const double* __restrict__ x = B.x[par];
+    const double* __restrict__ d = B.d[st->dpar];
+    const Pos p{x, d, nullptr, 0.0, lam, 1};
+    const long nterm = (long)s.blk[NFF] * BLOCK, ntot = nterm + s.n_beads;
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < ntot; i += (long)gridDim.x * BLOCK) {
+        if (i < nterm) {
+            const int blk = (int)(i / BLOCK);
+            const int ff = ff_of_block(s, blk);
+            const int t = (blk - s.blk[ff]) * BLOCK + (int)(i % BLOCK);
+            if (t < s.n[ff]) {
+                int fbad = 0;
+                (void)term_ef(s, ff, t, p, B.slots, -1, &fbad);
+                if (fbad) atomicAdd(B.bad_count, 1);
+            }
+        } else {
+            const int b = (int)(i - nterm);
+            double* xn = B.x[par ^ 1];
+            // moveAlongSearchDir (CGMethod.cpp:350-354) / the recovery move (MController.h:318-321)
+            for (int q = 0; q < 3; q++) xn[3 * b + q] = x[3 * b + q] + lam * d[3 * b + q];
         }
     }
 }
 
-// ---- gather F; reductions F.F, F.F_prev, max|F|; or materialize a CG reset ----
-__global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, CgState* __restrict__ st, CgBufs B) {
-    __shared__ double lds[BLOCK / 64];
+// ---- gather F; F.F, F.F_prev, max|F| partials -----------------------------------------
+// Grid: bead blocks only.
+__global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __restrict__ st, CgBufs B) {
+    __shared__ double lds[3 * GCHUNK];
     if (st->done) return;
-    const int phase = st->phase;
-    if (phase != PH_MOVE && phase != PH_RESET) return;
-    const int b = blockIdx.x * BLOCK + threadIdx.x;   // grid: bead blocks only
-    if (phase == PH_RESET) {
-        // resetCGParams (CGMethod.hpp:963-972): d = F (shiftSearchDir(0)), F_prev = F
-        if (b < s.n_beads)
-            for (int q = 0; q < 3; q++) {
-                double fv = B.f[3 * b + q];
-                B.d[3 * b + q] = fv + 0 * B.d[3 * b + q];
-                B.fp[3 * b + q] = fv;
-            }
-        if (!last_block(&st->ticket[1])) return;
-        if (threadIdx.x == 0) {
-            st->last_fdotf = st->fdotf;
-            st->phase = PH_LS;
-            st->ls_lambda = rmin(st->lambda_max, st->max_dist / st->grad_inf_norm);
-        }
-        return;
-    }
-    double ff = 0.0, fpd = 0.0, mx = 0.0;
+    if (st->phase != PH_MOVE && st->phase != PH_GATHER) return;
+    const int b = blockIdx.x * GB + threadIdx.x;
+    // copycoordsifminimumE candidate (CGMethod.hpp:1564-1566): the energy and iteration parts of
+    // the condition are known now, the force part after the reduction
+    const bool cand = st->num_iter > st->force_call_limit / 2 && st->total_energy <= st->minimum_e;
+    const int par = st->parity, bk = st->bkpar;
+    D3 acc = gather_block(s, B.slots, 0, 0, true, D3{0.0, 0.0, 0.0}, lds);
+    if (threadIdx.x >= GB) return;   // one wave finishes
+    double v[3] = {0.0, 0.0, 0.0};
     if (b < s.n_beads) {
-        D3 acc = gather_bead(s, b, B.slots, 0, s.slot0[NFF], D3{0.0, 0.0, 0.0});
-        double fv[3] = {acc.x, acc.y, acc.z};
+        const double fv[3] = {acc.x, acc.y, acc.z};
+        const double* xn = B.x[par ^ 1];
         for (int q = 0; q < 3; q++) {
             B.f[3 * b + q] = fv[q];
-            ff += fv[q] * fv[q];
-            fpd += fv[q] * B.fp[3 * b + q];
-            mx = rmax(mx, fabs(fv[q]));
+            v[0] += fv[q] * fv[q];
+            v[1] += fv[q] * B.fp[3 * b + q];
+            v[2] = rmax(v[2], fabs(fv[q]));
+            if (cand) B.xmin[bk ^ 1][3 * b + q] = xn[3 * b + q];
         }
     }
-    double r0 = block_reduce<0>(ff, lds), r1 = block_reduce<0>(fpd, lds), r2 = block_reduce<1>(mx, lds);
-    double* P = B.partial + (long)blockIdx.x * 16;
-    if (threadIdx.x == 0) { P[0] = r0; P[1] = r1; P[2] = r2; }
-    if (!last_block(&st->ticket[2])) return;
-    if (threadIdx.x != 0) return;
-    double FF = 0.0, FP = 0.0, MX = 0.0;
-    for (int k = 0; k < (int)gridDim.x; k++) {
-        FF += ld_partial(&B.partial[(long)k * 16 + 0]);
-        FP += ld_partial(&B.partial[(long)k * 16 + 1]);
-        MX = rmax(MX, ld_partial(&B.partial[(long)k * 16 + 2]));
-    }
-    st->parity ^= 1;
-    st->num_force_call += 1;
-    st->grad_inf_norm = MX;
-    st->fdotf = FF;
-    if (!isfinite(MX)) {        // maxF error (CGMethod.hpp:202-212)
-        st->status = 4;
-        st->done = 1;
-        return;
-    }
-    if (*B.bad_count != 0) {    // numerical fallback needed: handled by the host
-        st->status = 9;
-        st->done = 1;
-        return;
-    }
-    if (st->recovery_move) {
-        st->num_recovery += 1;
-        st->recovery_move = 0;
-        st->safe_mode = 0;
-        st->ecur_valid = 0;
-        st->phase = PH_RESET;
-        bool limit = (st->energy_call_limit != 0 && st->num_energy_call > st->energy_call_limit) ||
-                     (st->force_call_limit != 0 && st->num_force_call > st->force_call_limit);
-        bool convd = st->grad_inf_norm < st->grad_tol || st->energy_rel_change < st->energy_rel_tol;
-        if (limit || convd) st->done = 1;
-        return;
-    }
-    st->energy_rel_change = fabs(st->new_energy - st->prev_energy) / fabs(st->prev_energy);
-    st->prev_energy = st->new_energy;
-    st->ls_cur = st->new_energy;      // E(x_new) = accepted trial energy, bit for bit
-    st->ecur_valid = 1;
-    double beta = 0.0;
-    if (st->method == 1) beta = FF / st->last_fdotf;
-    else if (st->method == 2) beta = rmax(0.0, (FF - FP) / st->last_fdotf);
-    st->beta = beta;
-    // copycoordsifminimumE condition (CGMethod.hpp:1564-1566)
-    st->do_backup = (MX <= 10 * st->grad_tol && st->num_iter > st->force_call_limit / 2 &&
-                     st->total_energy <= st->minimum_e) ? 1 : 0;
-    if (st->do_backup) {
-        st->minimum_e = st->total_energy;
-        st->have_min_e = 1;
-    }
-    st->phase = PH_GATHERED;
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+    v[2] = wave_max(v[2]);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) B.partial[(long)PSTRIDE * blockIdx.x + q] = v[q];
 }
 
-// ---- d = F + beta d, F_prev = F, d.F check ----
-__global__ void __launch_bounds__(BLOCK) k_cg_dir(DevSys s, CgState* __restrict__ st, CgBufs B) {
-    __shared__ double lds[BLOCK / 64];
-    if (st->done || st->phase != PH_GATHERED) return;
-    const int b = blockIdx.x * BLOCK + threadIdx.x;   // grid: bead blocks only
-    const double beta = st->beta;
-    const bool pr = st->method == 2;
-    double df = 0.0;
-    if (b < s.n_beads) {
-        const double* xc = B.x[st->parity];
-        for (int q = 0; q < 3; q++) {
-            double fv = B.f[3 * b + q];
-            double dv = fv + beta * B.d[3 * b + q];
-            B.d[3 * b + q] = dv;
-            df += dv * fv;
-            if (pr) B.fp[3 * b + q] = fv;
-            if (st->do_backup) B.xmin[3 * b + q] = xc[3 * b + q];
+// Scalar logic after a force evaluation: maxF, energyRelChange, beta, the minimum-energy
+// backup and the loop condition (CGMethod.hpp:1406-1576).
+__device__ inline void gather_control(CgState& S, const double* acc, int* bad_count) {
+    const double FF = acc[0], FP = acc[1], MX = acc[2];
+    const bool cand = S.num_iter > S.force_call_limit / 2 && S.total_energy <= S.minimum_e;
+    S.n_gather_launch += 1;
+    S.parity ^= 1;
+    S.num_force_call += 1;
+    S.grad_inf_norm = MX;
+    S.fdotf = FF;
+    if (!isfinite(MX)) {        // maxF error (CGMethod.hpp:202-212)
+        S.status = 4;
+        S.done = 1;
+        return;
+    }
+    if (*bad_count != 0) {      // numerical force fallback needed
+        S.status = 9;
+        S.done = 1;
+        return;
+    }
+    S.phase = PH_LS;
+    S.ls_active = 0;
+    S.ls_lambda = rmin(S.lambda_max, S.max_dist / MX);
+    if (S.recovery_move) {
+        // after the recovery move: forces, maxF, resetCGParams, safe mode off (:1342-1348)
+        S.num_recovery += 1;
+        S.recovery_move = 0;
+        S.safe_mode = 0;
+        S.ecur_valid = 0;
+        S.dir_pending = 1;
+        S.dir_beta = 0.0;
+        S.dir_reset = 1;
+    } else {
+        S.energy_rel_change = fabs(S.new_energy - S.prev_energy) / fabs(S.prev_energy);
+        S.prev_energy = S.new_energy;
+        S.ls_cur = S.new_energy;      // E(x_new) = the accepted trial energy, bit for bit
+        S.ecur_valid = 1;
+        double beta = 0.0;
+        if (S.method == 1) beta = FF / S.last_fdotf;
+        else if (S.method == 2) beta = rmax(0.0, (FF - FP) / S.last_fdotf);
+        S.dir_pending = 1;
+        S.dir_beta = beta;
+        S.dir_reset = 0;
+        if (cand && MX <= 10 * S.grad_tol) {   // commit the candidate backup
+            S.bkpar ^= 1;
+            S.minimum_e = S.total_energy;
+            S.have_min_e = 1;
+        }
+        if (S.method != 0) S.last_fdotf = FF;   // lastFDotF = newGrad (CGMethod.hpp:1570-1572)
+    }
+    if (call_limit_s(S) || converged_s(S)) S.done = 1;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cg_gather_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
+    __shared__ CgState S;
+    __shared__ double lds[3 * (BLOCK / 64)];
+    state_load(&S, st);
+    if (S.done || (S.phase != PH_MOVE && S.phase != PH_GATHER)) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int op3[3] = {0, 0, 1};
+    for (int base = 0; base < nblk; base += 4 * BLOCK) {
+        double p[4][3];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int bk = base + threadIdx.x + j * BLOCK;
+            for (int q = 0; q < 3; q++) p[j][q] = bk < nblk ? B.partial[(long)PSTRIDE * bk + q] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[0] += p[j][0];
+            acc[1] += p[j][1];
+            acc[2] = rmax(acc[2], p[j][2]);
         }
     }
-    double r = block_reduce<0>(df, lds);
-    if (threadIdx.x == 0) B.partial[(long)blockIdx.x * 16 + 3] = r;
-    if (!last_block(&st->ticket[3])) return;
-    if (threadIdx.x != 0) return;
-    double DF = 0.0;
-    for (int k = 0; k < (int)gridDim.x; k++) DF += ld_partial(&B.partial[(long)k * 16 + 3]);
-    st->last_fdotf = st->fdotf;   // lastFDotF = newGrad (== F.F also after a reset)
-    st->phase = (DF <= 0) ? PH_RESET : PH_LS;
-    st->ls_lambda = rmin(st->lambda_max, st->max_dist / st->grad_inf_norm);
-    bool limit = (st->energy_call_limit != 0 && st->num_energy_call > st->energy_call_limit) ||
-                 (st->force_call_limit != 0 && st->num_force_call > st->force_call_limit);
-    bool convd = st->grad_inf_norm < st->grad_tol || st->energy_rel_change < st->energy_rel_tol;
-    if (limit || convd) st->done = 1;
+    block_reduce_n<3>(acc, op3, lds);
+    if (threadIdx.x == 0) gather_control(S, acc, B.bad_count);
+    state_store(st, &S);
 }
 
 // =======================================================================================
@@ -898,7 +1110,7 @@ __global__ void k_bnd_emit(NLDev n, DevSys s, const int* __restrict__ cnt, const
     bnd_b[o] = n.cyl_b2[c]; bnd_e[o] = e; krep[o] = krep_e[e]; slen[o] = slen_e[e];
 }
 
-// dynamic incidence CSR: per bead, the excluded-volume and boundary slots, ascending
+// dynamic incidence lists: per bead, the excluded-volume and boundary slot ids, ascending
 __global__ void k_inc_count(DevSys s, int* __restrict__ cnt) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long nev = 4L * s.n[FF_EXVOL], nb = s.n[FF_BOUNDARY];
@@ -924,6 +1136,32 @@ __global__ void k_inc_sort(int n_beads, const int* __restrict__ start, int* __re
         int v = list[i], j = i - 1;
         while (j >= lo && list[j] > v) { list[j + 1] = list[j]; j--; }
         list[j + 1] = v;
+    }
+}
+// per-bead slot count of the merged (static + dynamic) layout
+__global__ void k_inc_total(int n_beads, const int* __restrict__ s_start, const int* __restrict__ d_start,
+                            int* __restrict__ cnt) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_beads) return;
+    cnt[b] = (s_start[b + 1] - s_start[b]) + (d_start[b + 1] - d_start[b]);
+}
+// bead-sorted layout: static slots then dynamic slots of each bead (static ids are smaller, so
+// the bead's range stays ascending); sorted_id[p] = slot id at p, pos[slot id] = p
+__global__ void k_inc_merge(int n_beads, const int* __restrict__ s_start, const int* __restrict__ s_list,
+                            const int* __restrict__ d_start, const int* __restrict__ d_list,
+                            const int* __restrict__ bstart, int* __restrict__ sorted_id, int* __restrict__ pos) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_beads) return;
+    int p = bstart[b];
+    for (int q = s_start[b]; q < s_start[b + 1]; q++, p++) {
+        int id = s_list[q];
+        sorted_id[p] = id;
+        pos[id] = p;
+    }
+    for (int q = d_start[b]; q < d_start[b + 1]; q++, p++) {
+        int id = d_list[q];
+        sorted_id[p] = id;
+        pos[id] = p;
     }
 }
 

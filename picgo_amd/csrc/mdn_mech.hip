@@ -69,7 +69,7 @@ struct mdn_ctx {
     DBuf<int> str_b, bend_b, lnk_b, mot_b, ev_b, bnd_b, bnd_e;
     DBuf<double> str_k, str_eql, bend_k, bend_eqt, lnk_k, lnk_eql, lnk_p1, lnk_p2, lnk_sf;
     DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
-    DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor;
+    DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
     // cylinder tables for the NL
     DBuf<int> d_cyl_b1, d_cyl_b2, d_cyl_fil, d_cyl_pos, d_cyl_type;
     DBuf<char> d_cyl_minus;
@@ -84,14 +84,16 @@ struct mdn_ctx {
     std::vector<int> bnd_nneighbors;
 
     // state vectors
-    DBuf<double> x[2], f, fp, d, xmin, scratch_f, ext;
+    DBuf<double> x[2], f, fp, d[2], xmin[2], scratch_f, ext;
     DBuf<D3> slots;
     DBuf<double> partial;
-    DBuf<int> partial_i, bad_count, bad_list;
+    DBuf<int> bad_count, bad_list;
     DBuf<unsigned> ticket;
     DBuf<double> eout;
     DBuf<CgState> cg;
     DBuf<double> fallback_partial;
+    DBuf<EnDefer> defer;
+    DBuf<int> defer_n;
     DBuf<int> fallback_ok;
     CgState* h_cg = nullptr;   // pinned
     double* h_eout = nullptr;  // pinned
@@ -102,7 +104,8 @@ struct mdn_ctx {
 
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
-    int graph_slots = 0;
+    long sys_gen = 0, graph_gen = -1;
+    CgState last_cg{};
 };
 
 namespace {
@@ -151,6 +154,7 @@ void layout(mdn_ctx* ctx) {
 }
 
 int bead_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, BLOCK)); }
+int gather_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, GB)); }
 
 void refresh_ptrs(mdn_ctx* ctx) {
     DevSys& s = ctx->sys;
@@ -162,8 +166,8 @@ void refresh_ptrs(mdn_ctx* ctx) {
     s.mot_p2 = ctx->mot_p2.p; s.mot_sf = ctx->mot_sf.p;
     s.ev_b = ctx->ev_b.p; s.ev_ks = ctx->ev_ks.p;
     s.bnd_b = ctx->bnd_b.p; s.bnd_e = ctx->bnd_e.p; s.bnd_krep = ctx->bnd_krep.p; s.bnd_slen = ctx->bnd_slen.p;
-    s.inc_s_start = ctx->inc_s_start.p; s.inc_s = ctx->inc_s.p;
-    s.inc_d_start = ctx->inc_d_start.p; s.inc_d = ctx->inc_d.p;
+    s.pos = ctx->pos.p; s.bstart = ctx->bstart.p; s.sorted_id = ctx->sorted_id.p;
+    s.defer = ctx->defer.p; s.defer_n = ctx->defer_n.p;
     s.n_elem = (int)ctx->elems.size();
     for (int e = 0; e < s.n_elem; e++) {
         s.elem[e].kind = ctx->elems[e].kind;
@@ -175,8 +179,10 @@ void refresh_ptrs(mdn_ctx* ctx) {
 int ensure_eval_buffers(mdn_ctx* ctx) {
     const DevSys& s = ctx->sys;
     HIPCHK(ctx->slots.ensure((size_t)s.slot0[NFF]));
-    HIPCHK(ctx->partial.ensure((size_t)(s.blk[NFF + 1] + 1) * 16));
-    HIPCHK(ctx->partial_i.ensure((size_t)(s.blk[NFF + 1] + 1) * 16));
+    const long nls = NTRIAL * (long)s.blk[NFF] + (s.blk[NFF + 1] - s.blk[NFF]);
+    HIPCHK(ctx->partial.ensure((size_t)(std::max(nls, (long)gather_blocks(ctx)) + 1) * PSTRIDE));
+    HIPCHK(ctx->pos.ensure((size_t)std::max(1L, s.slot0[NFF])));
+    HIPCHK(ctx->sorted_id.ensure((size_t)std::max(1L, s.slot0[NFF])));
     HIPCHK(ctx->bad_list.ensure((size_t)std::max(1, s.n[FF_EXVOL])));
     return MDN_OK;
 }
@@ -262,7 +268,7 @@ int eval_slots(mdn_ctx* ctx, const double* x) {
 }
 
 int gather(mdn_ctx* ctx, long lo, long hi, const double* base, double* out) {
-    hipLaunchKernelGGL(k_gather_plain, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+    hipLaunchKernelGGL(k_gather_plain, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
                        (const D3*)ctx->slots.p, lo, hi, base, out);
     HIPCHK(hipGetLastError());
     return MDN_OK;
@@ -272,11 +278,15 @@ int gather(mdn_ctx* ctx, long lo, long hi, const double* base, double* out) {
 int energy_report(mdn_ctx* ctx, const double* x) {
     const DevSys& s = ctx->sys;
     int grid = std::max(1, s.blk[NFF]);
-    hipLaunchKernelGGL(k_energy_report, dim3(grid), dim3(BLOCK), 0, ctx->stream, s, x, ctx->partial.p,
-                       ctx->partial_i.p, ctx->ticket.p + 7, ctx->eout.p);
+    hipLaunchKernelGGL(k_energy_report, dim3(grid), dim3(BLOCK), 0, ctx->stream, s, x, ctx->partial.p);
+    hipLaunchKernelGGL(k_energy_report_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, (const double*)ctx->partial.p,
+                       ctx->eout.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ctx->h_eout, ctx->eout.p, sizeof(double) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_eout, ctx->eout.p, sizeof(double) * 9, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_eout[8] != 0.0)
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "%.0f excluded-volume pairs need the Simpson energy (capacity %d)",
+                    ctx->h_eout[8], DEFER_CAP);
     return MDN_OK;
 }
 
@@ -309,18 +319,20 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
         return MDN_ERR_HIP;
     }
     if (hipHostMalloc(&ctx->h_cg, sizeof(CgState)) != hipSuccess ||
-        hipHostMalloc(&ctx->h_eout, sizeof(double) * 8) != hipSuccess ||
+        hipHostMalloc(&ctx->h_eout, sizeof(double) * 16) != hipSuccess ||
         hipHostMalloc(&ctx->h_int, sizeof(int) * 16) != hipSuccess) {
         delete ctx;
         return MDN_ERR_HIP;
     }
-    if (ctx->ticket.ensure(8) != hipSuccess || ctx->eout.ensure(8) != hipSuccess ||
+    if (ctx->ticket.ensure(8) != hipSuccess || ctx->eout.ensure(16) != hipSuccess ||
         ctx->bad_count.ensure(1) != hipSuccess || ctx->cg.ensure(1) != hipSuccess ||
-        ctx->nl_err.ensure(1) != hipSuccess) {
+        ctx->nl_err.ensure(1) != hipSuccess || ctx->defer.ensure(DEFER_CAP) != hipSuccess ||
+        ctx->defer_n.ensure(1) != hipSuccess) {
         delete ctx;
         return MDN_ERR_HIP;
     }
     (void)hipMemset(ctx->ticket.p, 0, sizeof(unsigned) * 8);
+    (void)hipMemset(ctx->defer_n.p, 0, sizeof(int));
     *out = ctx;
     return MDN_OK;
 }
@@ -336,19 +348,23 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                        &ctx->inc_cursor, &ctx->d_cyl_b1, &ctx->d_cyl_b2, &ctx->d_cyl_fil, &ctx->d_cyl_pos,
                        &ctx->d_cyl_type, &ctx->bin, &ctx->bin_count, &ctx->bin_start, &ctx->bin_cursor,
                        &ctx->members, &ctx->nl_count, &ctx->nl_off, &ctx->pairs, &ctx->bnd_cnt, &ctx->bnd_off,
-                       &ctx->nl_err, &ctx->partial_i, &ctx->bad_count, &ctx->bad_list, &ctx->fallback_ok};
+                       &ctx->nl_err, &ctx->bad_count, &ctx->bad_list, &ctx->fallback_ok, &ctx->bstart,
+                       &ctx->sorted_id, &ctx->pos};
     for (auto* b : ib) b->release();
     DBuf<double>* db[] = {&ctx->str_k, &ctx->str_eql, &ctx->bend_k, &ctx->bend_eqt, &ctx->lnk_k, &ctx->lnk_eql,
                           &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
                           &ctx->mot_p2, &ctx->mot_sf, &ctx->ev_ks, &ctx->bnd_krep, &ctx->bnd_slen,
                           &ctx->d_cyl_kexvol, &ctx->d_cyl_eql, &ctx->d_krep_e, &ctx->d_slen_e, &ctx->mid,
-                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d, &ctx->xmin, &ctx->scratch_f,
+                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
+                          &ctx->xmin[1], &ctx->scratch_f,
                           &ctx->ext, &ctx->partial, &ctx->eout, &ctx->fallback_partial};
     for (auto* b : db) b->release();
     ctx->d_cyl_minus.release();
     ctx->cub_tmp.release();
     ctx->slots.release();
     ctx->ticket.release();
+    ctx->defer.release();
+    ctx->defer_n.release();
     ctx->cg.release();
     if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
     if (ctx->h_eout) (void)hipHostFree(ctx->h_eout);
@@ -498,7 +514,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     }
     // state vectors
     size_t nv = 3 * (size_t)ctx->n_beads;
-    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d, &ctx->xmin, &ctx->scratch_f, &ctx->ext})
+    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0], &ctx->xmin[1],
+                    &ctx->scratch_f, &ctx->ext})
         HIPCHK(b->ensure(nv));
     ctx->cur = 0;
     HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
@@ -647,8 +664,18 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
         hipLaunchKernelGGL(k_inc_sort, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
                            ctx->inc_d_start.p, ctx->inc_d.p);
     }
+    // bead-sorted slot layout (static then dynamic slots of each bead)
+    HIPCHK(ctx->bstart.ensure(ctx->n_beads + 1));
+    hipLaunchKernelGGL(k_inc_total, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
+                       ctx->inc_s_start.p, ctx->inc_d_start.p, ctx->inc_cnt.p);
+    HIPCHK(hipMemsetAsync(ctx->inc_cnt.p + ctx->n_beads, 0, sizeof(int), ctx->stream));
+    if ((st = exclusive_scan(ctx, ctx->inc_cnt.p, ctx->bstart.p, ctx->n_beads + 1))) return st;
+    hipLaunchKernelGGL(k_inc_merge, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
+                       ctx->inc_s_start.p, ctx->inc_s.p, ctx->inc_d_start.p, ctx->inc_d.p, ctx->bstart.p,
+                       ctx->sorted_id.p, ctx->pos.p);
     HIPCHK(hipGetLastError());
     refresh_ptrs(ctx);
+    ctx->sys_gen++;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nl_built = true;
     if (n_pairs_out) *n_pairs_out = np;
@@ -744,14 +771,45 @@ int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n
 // ----------------------------------------------------------------------------------------
 // CGMethod::minimize on the device
 // ----------------------------------------------------------------------------------------
-static void launch_slot(mdn_ctx* ctx, CgBufs B) {
+static CgBufs cg_bufs(mdn_ctx* ctx) {
+    CgBufs B;
+    B.x[0] = ctx->x[0].p; B.x[1] = ctx->x[1].p;
+    B.d[0] = ctx->d[0].p; B.d[1] = ctx->d[1].p;
+    B.xmin[0] = ctx->xmin[0].p; B.xmin[1] = ctx->xmin[1].p;
+    B.f = ctx->f.p; B.fp = ctx->fp.p;
+    B.slots = ctx->slots.p; B.partial = ctx->partial.p; B.bad_count = ctx->bad_count.p;
+    return B;
+}
+
+static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
     const DevSys& s = ctx->sys;
-    const int all = std::max(1, s.blk[NFF + 1]);
-    const int beads = bead_blocks(ctx);
-    hipLaunchKernelGGL(k_cg_ls, dim3(all), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_move, dim3(all), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_gather, dim3(beads), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_dir, dim3(beads), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B);
+    const int T = s.blk[NFF], beads = bead_blocks(ctx);
+    const int nls = NTRIAL * T + beads;
+    hipLaunchKernelGGL(k_cg_ls, dim3(nls), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_ls_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B, nls);
+    const int gbk = gather_blocks(ctx);
+    hipLaunchKernelGGL(k_cg_move, dim3(MOVE_GRID), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_gather, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B, gbk);
+}
+
+// A batch of minimizer slots captured once per system layout and replayed as a hipGraph; the
+// kernels take every decision from the device state, so replays need no host work.
+static const int CG_BATCH = 16;
+
+static int cg_graph(mdn_ctx* ctx) {
+    if (ctx->graph_exec && ctx->graph_gen == ctx->sys_gen) return MDN_OK;
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    CgBufs B = cg_bufs(ctx);
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < CG_BATCH; i++) launch_slot(ctx, B);
+    HIPCHK(hipStreamEndCapture(ctx->stream, &ctx->graph));
+    HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    ctx->graph_gen = ctx->sys_gen;
+    return MDN_OK;
 }
 
 int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* force_out, mdn_min_result* res) {
@@ -759,6 +817,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     if (st) return st;
     if (!p || !coord || !res) return fail(ctx, MDN_ERR_ARG, "null argument");
     if (p->precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "only FP64 minimization in this version");
+    if (p->method < 0 || p->method > 2) return fail(ctx, MDN_ERR_ARG, "unknown descent method %d", p->method);
     HIPCHK(hipSetDevice(ctx->device));
     memset(res, 0, sizeof(*res));
     const int nv = 3 * ctx->n_beads;
@@ -793,63 +852,54 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     S.energy_rel_change = INFINITY;
     S.prev_energy = res->energies_before.total;
     S.minimum_e = INFINITY;
-    S.ls_K = 2;
     S.fdotf = ff;
+    // resetCGParams before the loop (d = F + 0 d, lastFDotF = F.F, F_prev = F), applied by the
+    // first line-search launch
+    S.dir_pending = 1;
+    S.dir_beta = 0.0;
+    S.dir_reset = 1;
+    S.phase = PH_LS;
+    S.ls_lambda = rmin(S.lambda_max, S.max_dist / S.grad_inf_norm);
     res->force_call_limit = S.force_call_limit;
     res->energy_call_limit = S.energy_call_limit;
     res->grad_tol = S.grad_tol;
     res->energy_rel_tol = S.energy_rel_tol;
     if (!std::isfinite(mx)) return fail(ctx, MDN_ERR_NONFINITE, "maxF is not finite at the start");
-    // resetCGParams before the loop: d = F, lastFDotF = F.F, F_prev = F
-    HIPCHK(hipMemcpyAsync(ctx->d.p, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->fp.p, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
-    S.last_fdotf = ff;
-    S.phase = PH_LS;
-    S.parity = 0;
-    S.ls_lambda = rmin(S.lambda_max, S.max_dist / S.grad_inf_norm);
     const bool limit0 = (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
     S.done = (S.grad_inf_norm < S.grad_tol || S.energy_rel_change < S.energy_rel_tol || limit0) ? 1 : 0;
+    HIPCHK(hipMemsetAsync(ctx->d[0].p, 0, sizeof(double) * nv, ctx->stream));   // searchDir.assign(numDof, 0)
     HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->ticket.p, 0, sizeof(unsigned) * 8, ctx->stream));
+    if ((st = cg_graph(ctx))) return st;
 
-    CgBufs B;
-    B.x[0] = ctx->x[0].p; B.x[1] = ctx->x[1].p;
-    B.f = ctx->f.p; B.fp = ctx->fp.p; B.d = ctx->d.p; B.xmin = ctx->xmin.p;
-    B.slots = ctx->slots.p; B.partial = ctx->partial.p; B.partial_i = ctx->partial_i.p; B.bad_count = ctx->bad_count.p;
-
-    // Slots are launched in batches; the host polls the done flag once per batch, keeping one
-    // batch in flight ahead of the poll.
-    const int BATCH = 16;
-    const long max_slots = 8L * ((long)S.force_call_limit + 16) + 64;
+    // Batches are replayed back to back; the host polls the done flag of batch i while batch
+    // i+1 runs, so the device never idles waiting for the host.
+    const long max_batches = (4L * ((long)S.force_call_limit + 16)) / CG_BATCH + 8;
     long launched = 0;
     bool done = S.done != 0;
+    int* h_done = ctx->h_int + 4;
+    h_done[0] = h_done[1] = 0;
     hipEvent_t ev[2];
     HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    int* h_done = ctx->h_int + 4;
-    h_done[0] = h_done[1] = 0;
     int k = 0;
-    while (!done) {
-        for (int i = 0; i < BATCH; i++) launch_slot(ctx, B);
-        HIPCHK(hipGetLastError());
-        launched += BATCH;
+    while (!done && launched < max_batches) {
+        HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        launched++;
         HIPCHK(hipMemcpyAsync(&h_done[k], &ctx->cg.p->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipEventRecord(ev[k], ctx->stream));
-        // wait for the previous batch's flag while this batch runs
-        int prev = k ^ 1;
-        if (launched > BATCH) {
-            HIPCHK(hipEventSynchronize(ev[prev]));
-            if (h_done[prev]) done = true;
+        if (launched > 1) {
+            HIPCHK(hipEventSynchronize(ev[k ^ 1]));
+            if (h_done[k ^ 1]) done = true;
         }
         k ^= 1;
-        if (launched > max_slots) break;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
-    if (!S.done) return fail(ctx, MDN_ERR_STATE, "minimizer did not terminate within %ld slots", launched);
+    ctx->last_cg = S;
+    if (!S.done) return fail(ctx, MDN_ERR_STATE, "minimizer did not terminate within %ld batches", launched);
     ctx->cur = S.parity;
     res->error_flags = S.error_flags;
     res->num_energy_call = S.num_energy_call;
@@ -871,7 +921,8 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     double* xres = ctx->x[ctx->cur].p;
     if (S.error_flags == 0) {
         if (p->report_on_call_limit && limit && S.have_min_e) {   // copybackupcoordinates
-            HIPCHK(hipMemcpyAsync(xres, ctx->xmin.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(xres, ctx->xmin[S.bkpar].p, sizeof(double) * nv, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
             res->restored_min_energy = 1;
         }
         if ((st = energy_report(ctx, xres))) return st;
@@ -882,6 +933,17 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     if (force_out) HIPCHK(hipMemcpyAsync(force_out, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (S.error_flags & MDN_MIN_ERR_LINESEARCH) return fail(ctx, MDN_ERR_LINESEARCH, "line search failed (recovery disabled)");
+    return MDN_OK;
+}
+
+// Launch statistics of the last minimization: [line-search launches that did work, move
+// launches, gather launches, graph batches].
+int mdn_last_min_stats(mdn_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return MDN_ERR_ARG;
+    out[0] = ctx->last_cg.n_ls_launch;
+    out[1] = ctx->last_cg.n_move_launch;
+    out[2] = ctx->last_cg.n_gather_launch;
+    out[3] = ctx->last_cg.num_iter;
     return MDN_OK;
 }
 
@@ -904,7 +966,7 @@ int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
     for (int i = 0; i < iters; i++) {
         hipLaunchKernelGGL(k_forces_terms, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
                            ctx->slots.p, ctx->bad_count.p, ctx->bad_list.p);
-        hipLaunchKernelGGL(k_gather_plain, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s,
+        hipLaunchKernelGGL(k_gather_plain, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s,
                            (const D3*)ctx->slots.p, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
     }
     HIPCHK(hipEventRecord(e1, ctx->stream));
@@ -935,9 +997,12 @@ int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    for (int i = 0; i < iters; i++)
+    for (int i = 0; i < iters; i++) {
         hipLaunchKernelGGL(k_energy_report, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
-                           ctx->partial.p, ctx->partial_i.p, ctx->ticket.p + 7, ctx->eout.p);
+                           ctx->partial.p);
+        hipLaunchKernelGGL(k_energy_report_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s,
+                           (const double*)ctx->partial.p, ctx->eout.p);
+    }
     HIPCHK(hipEventRecord(e1, ctx->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(e1));

@@ -225,51 +225,54 @@ __device__ inline double exvol_energy_closed(D3 c1, D3 c2, D3 c3, D3 c4, double 
 }
 
 // energyN: 400x400 Simpson fallback with the reference's sample placement
-// (CylinderExclVolRepulsion.cpp:884-982).
+// (CylinderExclVolRepulsion.cpp:884-982): the loop advances s (t) before use, so row i samples
+// s = (i+1)/400; rows 1..399 carry the edge terms, row 0 the t-edges, rows 1..399 x cols 1..399
+// the interior. Block-parallel: every thread sums a strided share, then a block tree.
 __device__ inline double exvol_integrand(double a, double b, double c, double d, double e, double F,
                                          double s, double t) {
     double r_2 = c + 2 * s * e + s * s * a - 2 * t * F - 2 * s * t * d + t * t * b;
     return (1.0 / (r_2 * r_2));
 }
-__device__ __noinline__ double exvol_energyN(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled) {
+// returns this thread's share of (T2 + T3 + T4); T1 is added by the caller
+__device__ inline double exvol_energyN_share(D3 c1, D3 c2, D3 c3, D3 c4, int tid, int nthreads) {
     double a = scalar_product(c1, c2, c1, c2), b = scalar_product(c3, c4, c3, c4);
     double c = scalar_product(c3, c1, c3, c1), d = scalar_product(c1, c2, c3, c4);
     double e = scalar_product(c1, c2, c3, c1), F = scalar_product(c3, c4, c3, c1);
-    const int N1 = 400, N2 = 400;
-    double deltas = 1.0 / ((double)N1), deltat = 1.0 / ((double)N2);
-    double T1 = exvol_integrand(a, b, c, d, e, F, 0.0, 0.0) + exvol_integrand(a, b, c, d, e, F, 0.0, 1.0) +
-                exvol_integrand(a, b, c, d, e, F, 1.0, 0.0) + exvol_integrand(a, b, c, d, e, F, 1.0, 1.0);
-    double T2 = 0.0, T3 = 0.0, T4 = 0.0, s = 0.0, t;
-    for (int i = 0; i <= N1; i++) {
-        t = 0.0;
-        s += deltas;
-        double fs = (i % 2 == 0) ? 2.0 : 4.0;
-        if (i > 0 && i < N1)
-            T2 += fs * (exvol_integrand(a, b, c, d, e, F, s, 0.0) + exvol_integrand(a, b, c, d, e, F, s, 1.0));
-        for (int j = 0; j <= N2; j++) {
-            t += deltat;
-            double ft = (j % 2 == 0) ? 2.0 : 4.0;
-            if (j > 0 && j < N2) {
-                if (i == 0)
-                    T3 += ft * (exvol_integrand(a, b, c, d, e, F, 0.0, t) + exvol_integrand(a, b, c, d, e, F, 1.0, t));
-                else if (i < N1)
-                    T4 += fs * ft * exvol_integrand(a, b, c, d, e, F, s, t);
-            }
+    const int N = 400;
+    const double delta = 1.0 / ((double)N);
+    double acc = 0.0;
+    for (int k = tid; k < (N + 1) * (N + 1); k += nthreads) {
+        const int i = k / (N + 1), j = k % (N + 1);
+        const double sv = (i + 1) * delta, tv = (j + 1) * delta;
+        const double fs = (i % 2 == 0) ? 2.0 : 4.0, ft = (j % 2 == 0) ? 2.0 : 4.0;
+        if (j == 0 && i > 0 && i < N)
+            acc += fs * (exvol_integrand(a, b, c, d, e, F, sv, 0.0) + exvol_integrand(a, b, c, d, e, F, sv, 1.0));
+        if (j > 0 && j < N) {
+            if (i == 0)
+                acc += ft * (exvol_integrand(a, b, c, d, e, F, 0.0, tv) + exvol_integrand(a, b, c, d, e, F, 1.0, tv));
+            else if (i < N)
+                acc += fs * ft * exvol_integrand(a, b, c, d, e, F, sv, tv);
         }
     }
-    return kRepScaled * (deltas * deltat / 9.0) * (T1 + T2 + T3 + T4);
+    return acc;
+}
+__device__ inline double exvol_energyN_finish(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, double rest) {
+    double a = scalar_product(c1, c2, c1, c2), b = scalar_product(c3, c4, c3, c4);
+    double c = scalar_product(c3, c1, c3, c1), d = scalar_product(c1, c2, c3, c4);
+    double e = scalar_product(c1, c2, c3, c1), F = scalar_product(c3, c4, c3, c1);
+    const double delta = 1.0 / 400.0;
+    double T1 = exvol_integrand(a, b, c, d, e, F, 0.0, 0.0) + exvol_integrand(a, b, c, d, e, F, 0.0, 1.0) +
+                exvol_integrand(a, b, c, d, e, F, 1.0, 0.0) + exvol_integrand(a, b, c, d, e, F, 1.0, 1.0);
+    return kRepScaled * (delta * delta / 9.0) * (T1 + rest);
 }
 
-// pair energy with the fallback; sets *bad when the pair stays non-finite (the reference's
-// whole-force-field -1 sentinel)
-__device__ inline double exvol_energy(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, int* bad) {
+// Closed-form pair energy; *deferred is set when it is non-finite or < -1 and the Simpson
+// fallback must be evaluated (by a block-parallel pass).
+__device__ inline double exvol_energy(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, int* deferred) {
     double U = exvol_energy_closed(c1, c2, c3, c4, kRepScaled);
     if (energy_bad(U)) {
-        U = exvol_energyN(c1, c2, c3, c4, kRepScaled);
-        if (energy_bad(U)) {
-            *bad = 1;
-            U = 0.0;
-        }
+        *deferred = 1;
+        U = 0.0;
     }
     return U;
 }
@@ -279,6 +282,10 @@ __device__ inline bool bad_component(double v) {
     return a == INFINITY || isnan(a) || isinf(a) || a > 1e15;
 }
 
+// Analytic pair forces (CylinderExclVolRepulsion.cpp:610-858) on the 4 beads, from the pair
+// invariants t of the (possibly shifted) geometry c1, c2, c3, c4.
+__device__ inline bool exvol_forces_from(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, const ExvTerms& t, D3* out);
+
 // Analytic pair forces (CylinderExclVolRepulsion.cpp:610-858) on the 4 beads. Returns false
 // when a component is non-finite or > 1e15, in which case the reference switches to the
 // 10^4 x 10^4 Simpson forceN (flagged for the fallback pass).
@@ -287,6 +294,34 @@ __device__ inline bool exvol_forces(D3 c1, D3 c2in, D3 c3, D3 c4, double kRepSca
     if (are_in_plane(c1, c2, c3, c4)) c2 = move_out_of_plane(c1, c2, c3, 2, 0.01);
     ExvTerms t;
     exvol_terms(c1, c2, c3, c4, t);
+    return exvol_forces_from(c1, c2, c3, c4, kRepScaled, t, out);
+}
+
+// Energy and forces of one pair: the invariants are shared when the coplanar shift does not
+// apply (the energy never shifts, CylinderExclVolRepulsion.cpp:353-435 vs :610-630).
+// *deferred: the closed-form energy needs the Simpson fallback.
+__device__ inline double exvol_energy_forces(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, int* deferred,
+                                             bool* fok, D3* out) {
+    ExvTerms t;
+    exvol_terms(c1, c2, c3, c4, t);
+    double U = 0.5 * kRepScaled / t.JJ *
+               (t.CC / t.AA * t.ATG1 + t.GG / t.EE * t.ATG2 + t.DD / t.BB * t.ATG3 + t.HH / t.FF * t.ATG4);
+    if (are_in_plane(c1, c2, c3, c4)) {
+        D3 c2s = move_out_of_plane(c1, c2, c3, 2, 0.01);
+        ExvTerms ts;
+        exvol_terms(c1, c2s, c3, c4, ts);
+        *fok = exvol_forces_from(c1, c2s, c3, c4, kRepScaled, ts, out);
+    } else {
+        *fok = exvol_forces_from(c1, c2, c3, c4, kRepScaled, t, out);
+    }
+    if (energy_bad(U)) {
+        *deferred = 1;
+        U = 0.0;
+    }
+    return U;
+}
+
+__device__ inline bool exvol_forces_from(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, const ExvTerms& t, D3* out) {
     const double a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
     const double AA = t.AA, BB = t.BB, CC = t.CC, DD = t.DD, EE = t.EE, FF = t.FF, GG = t.GG,
                  HH = t.HH, JJ = t.JJ;

@@ -29,6 +29,7 @@ EXPORTED = [
     "mdn_system_counts", "mdn_nl_build", "mdn_nl_pairs", "mdn_boundary_list", "mdn_energy",
     "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
+    "mdn_last_min_stats",
 ]
 
 
@@ -70,6 +71,7 @@ def lib():
         L.mdn_set_coords.argtypes = [C.c_void_p, dp]
         L.mdn_get_coords.argtypes = [C.c_void_p, dp]
         L.mdn_forces_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int32]
+        L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
         _lib = L
     return _lib
 
@@ -186,6 +188,11 @@ class ForceFieldManager:
         self._chk(self._L.mdn_minimize(self._h, C.byref(params), x.ctypes.data_as(dp),
                                        f.ctypes.data_as(dp), C.byref(res)))
         return x, f, res
+
+    def last_min_stats(self):
+        out = (C.c_int64 * 4)()
+        self._chk(self._L.mdn_last_min_stats(self._h, out))
+        return {"ls_launches": out[0], "gathers": out[2], "iterations": out[3]}
 
     def bench_forces(self, iters, precision=abi.MDN_PREC_FP64):
         a, b = C.c_double(), C.c_double()
