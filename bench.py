@@ -80,7 +80,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-min", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run 20 device copies of 64 MiB (FETCH_SIZE / WRITE_SIZE calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(20):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.numel() * 8, "launches": 20}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)

@@ -1,0 +1,108 @@
+"""The C-ABI libraries load on a CPU-only host and export every function their headers declare."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from picgo_amd import abi, mech, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|void|const\s+\w+\s*\*|\w+\s*\*)\s*(mdn_\w+)\s*\(", text, flags=re.M)))
+
+
+def exported(lib_path):
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+@pytest.mark.parametrize("header,lib", [("mdn_mech.h", "libmdn_mech.so"), ("mdn_synth.h", "libmdn_synth.so")])
+def test_every_declared_symbol_is_exported(header, lib):
+    names = declared(header)
+    assert len(names) >= 5
+    syms = exported(os.path.join(ROOT, "picgo_amd", lib))
+    missing = [n for n in names if n not in syms]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header():
+    assert set(mech.EXPORTED) == set(declared("mdn_mech.h"))
+
+
+def test_abi_version_and_struct_sizes():
+    L = mech.lib()
+    assert L.mdn_abi_version() == 1
+    # layouts are fixed by the header; sizes checked against the C compiler's view
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "mdn_mech.h"
+#include "mdn_synth.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(mdn_crosslink), sizeof(mdn_boundary_element),
+         sizeof(mdn_system_desc), sizeof(mdn_energy_report), sizeof(mdn_cg_params),
+         sizeof(mdn_min_result), sizeof(mdn_synth_params));
+  return 0;
+}
+"""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        cpath, exe = os.path.join(d, "s.c"), os.path.join(d, "s")
+        open(cpath, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, cpath], check=True)
+        sizes = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    py = [C.sizeof(t) for t in (abi.Crosslink, abi.BoundaryElement, abi.SystemDesc, abi.EnergyReport,
+                                abi.CgParams, abi.MinResult, abi.SynthParams)]
+    assert sizes == py
+
+
+def test_context_without_gpu_fails_cleanly():
+    """On a host without a visible GPU the library reports MDN_ERR_HIP instead of crashing."""
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is visible")
+    except Exception:
+        pass
+    h = C.c_void_p()
+    st = mech.lib().mdn_ctx_create(0, C.byref(h))
+    assert st == 2 and not h
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """The product path has no CPU fallback: without the HIP library it raises."""
+    monkeypatch.setattr(mech, "_lib", None)
+    monkeypatch.setattr(mech, "lib_path", lambda: str(tmp_path / "libmdn_mech.so"))
+    with pytest.raises(RuntimeError, match="not built"):
+        mech.lib()
+
+
+def test_synth_deterministic_and_inside_boundary():
+    a = synth.Network.config(0)
+    b = synth.Network.config(0)
+    assert (a.coord == b.coord).all()
+    s = a.summary()
+    assert s == {"n_beads": 550, "n_filaments": 50, "n_cylinders": 500, "n_linkers": 20, "n_motors": 0,
+                 "n_boundary_elements": 6}
+    x = a.coord.reshape(-1, 3)
+    assert x.min() >= 50.0 and x.max() <= 1950.0
+    seg = x.reshape(50, 11, 3)
+    L = (((seg[:, 1:] - seg[:, :-1]) ** 2).sum(-1)) ** 0.5
+    assert abs(L - 108.0).max() < 1e-9
+    c = synth.Network.config(1)
+    assert c.summary()["n_cylinders"] == 40000 and c.summary()["n_linkers"] == 8000
+    assert c.summary()["n_motors"] == 4000
+
+
+def test_synth_sphere_config():
+    net = synth.Network.config(4, n_fil=200, n_linkers=100, n_motors=50)
+    x = net.coord.reshape(-1, 3)
+    r = ((x - 10000.0) ** 2).sum(1) ** 0.5
+    assert r.max() <= 10000.0 - 25.0
+    assert net.desc.n_boundary == 1 and net.desc.boundary[0].kind == abi.MDN_BND_SPHERE
