@@ -1,0 +1,147 @@
+// mdn_mech.hpp -- header-only C++17 adapters over the C ABI with the reference's class shapes,
+// for dropping the gfx950 path into MEDYAN's MController (src/Controller/MController.h:108-211).
+//
+//   mdn::ForceFieldManager  ~ medyan::ForceFieldManager (ForceFieldManager.h:64-123):
+//       vectorizeAllForceFields, computeEnergy (+inf sentinel), computeIndividualEnergies,
+//       computeForces (zero-fill + accumulate), plus resetNeighborLists.
+//   mdn::CGMethod           ~ medyan::CGMethod::minimize (CGMethod.hpp:1746-1759) with the
+//       ConjugateGradientParams fields (CGMethod.hpp:44-63).
+//
+// Errors raise mdn::Error (the reference throws std::runtime_error on the same conditions).
+#pragma once
+
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mdn_mech.h"
+
+namespace mdn {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct EnergyReport {
+    double total = 0;
+    std::vector<std::pair<std::string, double>> individual;
+};
+
+inline const char* ff_name(int ff) {
+    static const char* names[MDN_NUM_FF] = {"FilamentStretching", "FilamentBending", "LinkerStretching",
+                                            "MotorGhostStretching", "CylinderExclVolume",
+                                            "BoundaryCylinderRepulsion"};
+    return (ff >= 0 && ff < MDN_NUM_FF) ? names[ff] : "?";
+}
+
+class ForceFieldManager {
+public:
+    explicit ForceFieldManager(int device = 0) {
+        int st = mdn_ctx_create(device, &ctx_);
+        if (st != MDN_OK) throw Error(st, "mdn_ctx_create failed (no HIP device?)");
+    }
+    ~ForceFieldManager() { mdn_ctx_destroy(ctx_); }
+    ForceFieldManager(const ForceFieldManager&) = delete;
+    ForceFieldManager& operator=(const ForceFieldManager&) = delete;
+
+    // serializeDof + vectorizeAllForceFields for the bonded and crosslink force fields
+    void vectorizeAllForceFields(const mdn_system_desc& sys, const std::vector<double>& coord) {
+        check(mdn_system_upload(ctx_, &sys, coord.data()));
+        nvar_ = 3 * sys.n_beads;
+    }
+    // SubSystem::resetNeighborLists (hybrid cylinder NL + boundary NL) and the excluded-volume /
+    // boundary vectorization; returns the number of excluded-volume pairs
+    long resetNeighborLists(const std::vector<double>& coord) {
+        int64_t n = 0;
+        check(mdn_nl_build(ctx_, coord.data(), &n));
+        return (long)n;
+    }
+    EnergyReport computeIndividualEnergies(const double* coord) {
+        mdn_energy_report r{};
+        check(mdn_energy(ctx_, coord, &r));
+        EnergyReport out;
+        out.total = r.total;
+        for (int ff = 0; ff < MDN_NUM_FF; ff++) out.individual.emplace_back(ff_name(ff), r.ff[ff]);
+        return out;
+    }
+    double computeEnergy(const double* coord) {
+        mdn_energy_report r{};
+        check(mdn_energy(ctx_, coord, &r));
+        double e = 0;
+        for (double v : r.ff) {
+            if (v <= -1) return std::numeric_limits<double>::infinity();
+            e += v;
+        }
+        return e;
+    }
+    void computeForces(const double* coord, double* force, int numVar) {
+        check(mdn_forces(ctx_, coord, force, numVar));
+    }
+    // ForceField::computeForces of one force field (accumulates)
+    void computeForces(int ff, const double* coord, double* force, int numVar) {
+        check(mdn_ff_forces(ctx_, ff, coord, force, numVar));
+    }
+    mdn_ctx* ctx() const { return ctx_; }
+    int numVar() const { return nvar_; }
+
+private:
+    void check(int st) const {
+        if (st != MDN_OK) throw Error(st, mdn_last_error(ctx_));
+    }
+    mdn_ctx* ctx_ = nullptr;
+    int nvar_ = 0;
+};
+
+// ConjugateGradientParams (CGMethod.hpp:44-63) with MController's values (MController.cpp:34-63)
+struct ConjugateGradientParams {
+    int descentSearchMethod = MDN_CG_POLAK_RIBIERE;
+    double maxDist = 1.0;
+    double lambdaMax = 0.01;
+    double gradTol = 1.0;
+    double energyRelTol = 1e-9;
+    int energyCallLimit = 0;
+    int forceCallLimit = 10000;
+    bool reportOnCallLimit = true;
+    bool tryToRecoverInLineSearchError = true;
+    int precision = MDN_PREC_FP64;
+
+    mdn_cg_params c() const {
+        mdn_cg_params p{};
+        p.method = descentSearchMethod;
+        p.try_recover = tryToRecoverInLineSearchError;
+        p.max_dist = maxDist;
+        p.lambda_max = lambdaMax;
+        p.grad_tol = gradTol;
+        p.energy_rel_tol = energyRelTol;
+        p.force_call_limit = forceCallLimit;
+        p.energy_call_limit = energyCallLimit;
+        p.report_on_call_limit = reportOnCallLimit;
+        p.precision = precision;
+        return p;
+    }
+};
+
+struct MinimizationResult : mdn_min_result {
+    bool success() const { return error_flags == 0; }
+    bool converged() const { return converged_grad || converged_energy; }
+};
+
+class CGMethod {
+public:
+    // minimize from coord (in/out); forces of the final state into *outForce. The neighbor lists
+    // of ffm stay frozen during the minimization, as in the reference.
+    MinimizationResult minimize(ForceFieldManager& ffm, const ConjugateGradientParams& p, std::vector<double>& coord,
+                                std::vector<double>* outForce) {
+        MinimizationResult res{};
+        mdn_cg_params cp = p.c();
+        if (outForce) outForce->assign(coord.size(), 0.0);
+        int st = mdn_minimize(ffm.ctx(), &cp, coord.data(), outForce ? outForce->data() : nullptr, &res);
+        if (st == MDN_ERR_LINESEARCH) return res;   // MController runs its diagnosis and throws
+        if (st != MDN_OK) throw Error(st, mdn_last_error(ffm.ctx()));
+        return res;
+    }
+};
+
+}  // namespace mdn
