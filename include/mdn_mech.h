@@ -238,9 +238,19 @@ int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
 int mdn_set_coords(mdn_ctx* ctx, const double* coord);
 int mdn_get_coords(mdn_ctx* ctx, double* coord);
 
-/* Mixed-precision force evaluation (FP32 force arithmetic on FP64-differenced bond vectors,
- * FP64 accumulation); same semantics as mdn_forces. */
+/* Mixed precision (MDN_PREC_MIXED): the stretching, bending, linker, motor and boundary energies
+ * and forces are computed in FP32 from bond vectors and coordinate differences formed in FP64
+ * (absolute coordinates never enter FP32 arithmetic); excluded volume stays FP64 (its closed form
+ * is a 0/0 limit for near-coplanar pairs); per-interaction forces are stored in FP32; per-bead
+ * force sums, energy sums, dot products and all minimizer decisions stay FP64. No reference
+ * counterpart: the reference's `floatingpoint` type switches the whole program to float
+ * (src/utility.h:38-43), which this replaces for the force path.
+ * mdn_forces_mixed: same semantics as mdn_forces (ForceFieldManager::computeForces,
+ * ForceFieldManager.cpp:242-290); mdn_energy_mixed: as mdn_energy (:209-227).
+ * mdn_minimize runs in mixed precision when params->precision == MDN_PREC_MIXED (its reported
+ * energies_before / energies_after stay FP64). */
 int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
+int mdn_energy_mixed(mdn_ctx* ctx, const double* coord, mdn_energy_report* out);
 
 #ifdef __cplusplus
 }

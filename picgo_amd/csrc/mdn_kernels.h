@@ -1,16 +1,24 @@
-// mdn_kernels.h -- gfx950 kernels of the mechanical-equilibration path (FP64 path).
+// mdn_kernels.h -- gfx950 kernels of the mechanical-equilibration path.
+//
+// The interaction kernels are templates over the arithmetic type R of the per-interaction math
+// (double: the FP64 path; float: the mixed path). Coordinates, search directions, forces after
+// the gather, every reduction and all CG control scalars are FP64 in both instantiations; the
+// slots hold R.
 //
 // Layout in HBM (all device buffers owned by the context):
 //   x[2][3*n_beads] (current / next coordinates), F, F_prev, d[2] (search direction, current
 //   / next), x_min[2] (minimum-energy backup, committed / candidate): the reference's flat AoS
 //   coord vectors (DofSerializer.hpp:64-187), double.
 //   Term tables: structure of arrays per force field (bead indices int32, parameters double).
-//   slots[S]: one D3 per (interaction, bead of the interaction), laid out bead-sorted: bead b
-//   owns slots [bstart[b], bstart[b+1]) in ascending order of (force field, interaction, bead of
-//   the interaction), which is the order the reference adds into `force`. Interaction kernels
-//   write through pos[] (slot id -> bead-sorted position); the gather is one contiguous,
-//   independent read per bead, reproduces the reference's per-bead summation order exactly and
-//   needs no atomics.
+//   slots[S]: one 3-vector of R (24 B in FP64, 12 B in the mixed path) per (interaction, bead
+//   of the interaction) of the linkers, motors, excluded volume and boundary, laid out
+//   bead-sorted: bead b owns slots [bstart[b], bstart[b+1]) in ascending order of (force field,
+//   interaction, bead of the interaction), which is the order the reference adds into `force`.
+//   Interaction kernels write through pos[] (slot id -> bead-sorted position). The filament
+//   stretching and bending forces, which come first in that order, go to bead-major pieces
+//   piece[5][n_beads] instead (coalesced writes: consecutive cylinders have consecutive beads);
+//   the gather adds a bead's pieces, then streams its contiguous slot range, so it reproduces
+//   the reference's per-bead summation order exactly and needs no atomics.
 //
 // Grid layout of the interaction kernels: blocks [blk[ff], blk[ff+1]) hold force field ff's
 // interactions (each range padded to whole blocks, so a block belongs to one force field), then
@@ -72,7 +80,12 @@ struct DevSys {
     const int* bnd_b;  const int* bnd_e; const double* bnd_krep; const double* bnd_slen;
     int n_elem;
     BElem elem[MAXELEM];
-    // bead-sorted slot layout
+    // filament bonded forces, bead-major: piece[k][bead] (3 values of the evaluation's arithmetic
+    // type) for k = 0: the stretching term with the bead as bead 2, 1: as bead 1, 2..4: the bending
+    // term with the bead as bead 3, 2, 1; pmask[bead] bit k marks the pieces the bead has
+    void* piece;
+    const unsigned char* pmask;
+    // bead-sorted slot layout (linkers, motors, excluded volume, boundary)
     const int* pos;        // [S] slot id -> position
     const int* bstart;     // [n_beads + 1]
     const int* sorted_id;  // [S] position -> slot id
@@ -192,8 +205,18 @@ __device__ inline D3 pos_of(const Pos& p, int b) {
     return D3{x[0] + p.lam * d0, x[1] + p.lam * d1, x[2] + p.lam * d2};
 }
 
-__device__ inline void put_slot(const DevSys& s, D3* __restrict__ slots, long id, D3 v) {
-    slots[s.pos[id]] = v;
+// Slots hold the per-(interaction, bead) forces in the arithmetic type R of the evaluation:
+// 3 values per slot, bead-sorted position p at [3p, 3p + 3). Pieces: 3 values at
+// [3 (k n_beads + bead), + 3).
+template <typename R>
+__device__ inline void put_piece(const DevSys& s, int k, int bead, D3 v) {
+    R* q = static_cast<R*>(s.piece) + 3L * ((long)k * s.n_beads + bead);
+    q[0] = (R)v.x; q[1] = (R)v.y; q[2] = (R)v.z;
+}
+template <typename R>
+__device__ inline void put_slot(const DevSys& s, R* __restrict__ slots, long id, D3 v) {
+    R* q = slots + 3L * s.pos[id];
+    q[0] = (R)v.x; q[1] = (R)v.y; q[2] = (R)v.z;
 }
 
 // Energy (and, when slots != null, forces into the bead-sorted slots) of interaction t of
@@ -212,28 +235,37 @@ __device__ inline void push_defer(const DevSys& s, int code, int t, D3 c1, D3 c2
 }
 
 // code: which energy the value belongs to (recorded with a deferred Simpson pair), or -1 when
-// only the forces are wanted.
-__device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, D3* __restrict__ slots,
-                                 int code, int* fbad) {
+// only the forces are wanted. *bad_t receives the excluded-volume pair whose forces need the
+// numerical fallback. Excluded volume is evaluated in FP64 in both instantiations: its closed
+// form divides a bracket that vanishes like the squared triple product of the pair by that
+// square, so FP32 rounding is amplified like 1/tau^2 for near-coplanar pairs (tau the normalized
+// triple product), and FP32 gains nothing on gfx950 for this latency-bound term.
+template <typename R>
+__device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R* __restrict__ slots,
+                                 int code, int* fbad, int* bad_t = nullptr) {
     D3 out[4];
     switch (ff) {
+    // filament stretching / bending: forces into the bead-major pieces
     case FF_STRETCH: {
-        D3 c1 = pos_of(p, s.str_b[2 * t]), c2 = pos_of(p, s.str_b[2 * t + 1]);
+        const int b1 = s.str_b[2 * t], b2 = s.str_b[2 * t + 1];
+        D3 c1 = pos_of(p, b1), c2 = pos_of(p, b2);
         if (slots) {
-            stretch_forces(c1, c2, s.str_k[t], s.str_eql[t], out);
-            const long id = s.slot0[FF_STRETCH] + 2L * t;
-            put_slot(s, slots, id, out[0]); put_slot(s, slots, id + 1, out[1]);
+            stretch_forces<R>(c1, c2, s.str_k[t], s.str_eql[t], out);
+            put_piece<R>(s, 1, b1, out[0]);
+            put_piece<R>(s, 0, b2, out[1]);
         }
-        return stretch_energy(c1, c2, s.str_k[t], s.str_eql[t]);
+        return (double)stretch_energy<R>(c1, c2, s.str_k[t], s.str_eql[t]);
     }
     case FF_BEND: {
-        D3 c1 = pos_of(p, s.bend_b[3 * t]), c2 = pos_of(p, s.bend_b[3 * t + 1]), c3 = pos_of(p, s.bend_b[3 * t + 2]);
+        const int b1 = s.bend_b[3 * t], b2 = s.bend_b[3 * t + 1], b3 = s.bend_b[3 * t + 2];
+        D3 c1 = pos_of(p, b1), c2 = pos_of(p, b2), c3 = pos_of(p, b3);
         if (slots) {
-            bend_forces(c1, c2, c3, s.bend_k[t], s.bend_eqt[t], out);
-            const long id = s.slot0[FF_BEND] + 3L * t;
-            put_slot(s, slots, id, out[0]); put_slot(s, slots, id + 1, out[1]); put_slot(s, slots, id + 2, out[2]);
+            bend_forces<R>(c1, c2, c3, s.bend_k[t], s.bend_eqt[t], out);
+            put_piece<R>(s, 4, b1, out[0]);
+            put_piece<R>(s, 3, b2, out[1]);
+            put_piece<R>(s, 2, b3, out[2]);
         }
-        return bend_energy(c1, c2, c3, s.bend_k[t], s.bend_eqt[t]);
+        return (double)bend_energy<R>(c1, c2, c3, s.bend_k[t], s.bend_eqt[t]);
     }
     case FF_LINKER:
     case FF_MOTOR: {
@@ -243,75 +275,109 @@ __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, D
         const double p1 = lk ? s.lnk_p1[t] : s.mot_p1[t], p2 = lk ? s.lnk_p2[t] : s.mot_p2[t];
         D3 c1 = pos_of(p, b[4 * t]), c2 = pos_of(p, b[4 * t + 1]), c3 = pos_of(p, b[4 * t + 2]), c4 = pos_of(p, b[4 * t + 3]);
         if (slots) {
-            double sf = xlink_forces(c1, c2, c3, c4, k, eql, p1, p2, out);
+            double sf = xlink_forces<R>(c1, c2, c3, c4, k, eql, p1, p2, out);
             (lk ? s.lnk_sf : s.mot_sf)[t] = sf;
             const long id = s.slot0[ff] + 4L * t;
             for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
         }
-        return xlink_energy(c1, c2, c3, c4, k, eql, p1, p2);
+        return (double)xlink_energy<R>(c1, c2, c3, c4, k, eql, p1, p2);
     }
     case FF_EXVOL: {
-        D3 c1 = pos_of(p, s.ev_b[4 * t]), c2 = pos_of(p, s.ev_b[4 * t + 1]), c3 = pos_of(p, s.ev_b[4 * t + 2]),
-           c4 = pos_of(p, s.ev_b[4 * t + 3]);
+        const int tp = t;
+        D3 c1 = pos_of(p, s.ev_b[4 * tp]), c2 = pos_of(p, s.ev_b[4 * tp + 1]), c3 = pos_of(p, s.ev_b[4 * tp + 2]),
+           c4 = pos_of(p, s.ev_b[4 * tp + 3]);
+        const long id = s.slot0[FF_EXVOL] + 4L * tp;
+        // FP64, the reference's expression order
         int deferred = 0;
         double U = 0.0;
         if (slots) {
             bool ok = true;
-            if (code < 0) ok = exvol_forces(c1, c2, c3, c4, s.ev_ks[t], out);
-            else U = exvol_energy_forces(c1, c2, c3, c4, s.ev_ks[t], &deferred, &ok, out);
+            if (code < 0) ok = exvol_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], out);
+            else U = exvol_energy_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
             if (!ok) {
                 out[0] = out[1] = out[2] = out[3] = D3{0.0, 0.0, 0.0};
                 *fbad = 1;
+                if (bad_t) *bad_t = tp;
             }
-            const long id = s.slot0[FF_EXVOL] + 4L * t;
             for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
         } else {
-            U = exvol_energy(c1, c2, c3, c4, s.ev_ks[t], &deferred);
+            U = exvol_energy<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred);
         }
-        if (deferred && code >= 0) push_defer(s, code, t, c1, c2, c3, c4);
+        if (deferred && code >= 0) push_defer(s, code, tp, c1, c2, c3, c4);
         return U;
     }
     default: {
         D3 c = pos_of(p, s.bnd_b[t]);
         const BElem& e = s.elem[s.bnd_e[t]];
-        if (slots) put_slot(s, slots, s.slot0[FF_BOUNDARY] + t, boundary_force(e, c, s.bnd_krep[t], s.bnd_slen[t]));
-        return boundary_energy(e, c, s.bnd_krep[t], s.bnd_slen[t]);
+        if (slots) put_slot(s, slots, s.slot0[FF_BOUNDARY] + t, boundary_force<R>(e, c, s.bnd_krep[t], s.bnd_slen[t]));
+        return (double)boundary_energy<R>(e, c, s.bnd_krep[t], s.bnd_slen[t]);
     }
     }
 }
 
-// Gather of GB beads per block: the beads own the contiguous slot range
-// [bstart[b0], bstart[b0 + GB]); all 256 threads stream it into LDS with coalesced, unrolled
-// loads (GUNROLL in flight per thread), then the first wave adds each bead's slots in ascending
-// order (the reference's `+=` order). Only slots whose id is in [lo, hi) are added when `all` is
-// false. Returns the sum for threads < GB.
-constexpr int GCHUNK = 1536;                 // slots per LDS chunk (36 KiB)
-constexpr int GUNROLL = 3 * GCHUNK / BLOCK;  // 18 doubles per thread per chunk
+// Gather of GB beads per block, in the reference's per-bead `+=` order:
+//  1. the bead's filament bonded pieces (pmask bits; bmask bit 0 selects the stretching pieces
+//     0-1, bit 1 the bending pieces 2-4): independent coalesced loads;
+//  2. the bead's slots (linkers, motors, excluded volume, boundary), which own the contiguous
+//     range [bstart[b0], bstart[b0 + GB]): all 256 threads stream it into LDS with coalesced,
+//     unrolled loads (the first chunk's loads issued before the pieces are added), then the first
+//     wave adds each bead's slots in ascending order.
+// Only slots whose id is in [lo, hi) are added when `all` is false. The first wave accumulates in
+// FP64 and returns the sum for threads < GB.
+constexpr int GCHUNK_BYTES = 12288;   // LDS bytes per chunk (512 FP64 slots, 1024 FP32 slots)
+template <typename R> constexpr int gchunk() { return GCHUNK_BYTES / (3 * (int)sizeof(R)); }
+template <typename R> constexpr int gunroll() { return 3 * gchunk<R>() / BLOCK; }
+constexpr int NBOND = 5;              // bonded pieces per bead
 
-__device__ inline D3 gather_block(const DevSys& s, const D3* __restrict__ slots, long lo, long hi, bool all,
-                                  D3 acc, double* lds /* [3 * GCHUNK] */) {
+// The slot buffer is padded by GCHUNK_PAD values past its last slot and the piece arrays cover
+// every bead, so the loads below are unconditional (the results are selected, not branched on).
+constexpr long GCHUNK_PAD = GCHUNK_BYTES / 4;   // values (>= 3 * gchunk<R>() for R = float, double)
+
+template <typename R>
+__device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restrict__ slots, long lo, long hi,
+                                  bool all, D3 acc, double* lds_raw /* GCHUNK_BYTES */) {
+    constexpr int CH = gchunk<R>(), UN = gunroll<R>();
+    static_assert(3 * CH <= GCHUNK_PAD, "slot padding");
+    R* lds = reinterpret_cast<R*>(lds_raw);
     const int b0 = blockIdx.x * GB;
     const int b = b0 + threadIdx.x;
     const int bend = min(b0 + GB, s.n_beads);
-    const int r0 = s.bstart[b0], r1 = s.bstart[bend];
     const bool mine = threadIdx.x < GB && b < s.n_beads;
+    const int bc = mine ? b : b0;   // an in-range bead for the unconditional loads
+    // bonded pieces in flight first
+    const unsigned m = (mine && bmask) ? s.pmask[bc] & ((bmask & 1 ? 0x3u : 0u) | (bmask & 2 ? 0x1cu : 0u)) : 0u;
+    R pv[NBOND][3] = {};
+    if (threadIdx.x < GB) {   // wave 0 (uniform branch)
+        const R* pc = static_cast<const R*>(s.piece);
+#pragma unroll
+        for (int k = 0; k < NBOND; k++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) pv[k][q] = pc[3L * ((long)k * s.n_beads + bc) + q];
+    }
+    // then the first slot chunk
+    const int r0 = s.bstart[b0], r1 = s.bstart[bend];
     const int my0 = mine ? s.bstart[b] : 0, my1 = mine ? s.bstart[b + 1] : 0;
-    const double* __restrict__ src = reinterpret_cast<const double*>(slots);
-    for (int c0 = r0; c0 < r1; c0 += GCHUNK) {
-        const int c1 = min(c0 + GCHUNK, r1);
+    int c0 = r0;
+    R v[UN];
+    {
+        const int n = 3 * (min(c0 + CH, r1) - c0);   // block-uniform: whole rows of loads are skipped
+#pragma unroll
+        for (int j = 0; j < UN; j++)
+            if (j * BLOCK < n) v[j] = slots[3L * c0 + threadIdx.x + j * BLOCK];
+    }
+#pragma unroll
+    for (int k = 0; k < NBOND; k++) {
+        const bool on = (m >> k) & 1;
+        acc.x = on ? acc.x + (double)pv[k][0] : acc.x;
+        acc.y = on ? acc.y + (double)pv[k][1] : acc.y;
+        acc.z = on ? acc.z + (double)pv[k][2] : acc.z;
+    }
+    while (c0 < r1) {   // uniform over the block
+        const int c1 = min(c0 + CH, r1);
         const int n = 3 * (c1 - c0);
-        double v[GUNROLL];
 #pragma unroll
-        for (int j = 0; j < GUNROLL; j++) {
-            const int i = threadIdx.x + j * BLOCK;
-            v[j] = i < n ? src[3L * c0 + i] : 0.0;
-        }
-        __syncthreads();   // the previous chunk is consumed
-#pragma unroll
-        for (int j = 0; j < GUNROLL; j++) {
-            const int i = threadIdx.x + j * BLOCK;
-            if (i < n) lds[i] = v[j];
-        }
+        for (int j = 0; j < UN; j++)
+            if (j * BLOCK < n) lds[threadIdx.x + j * BLOCK] = v[j];
         __syncthreads();
         const int q0 = max(my0, c0), q1 = min(my1, c1);
         for (int q = q0; q < q1; q++) {
@@ -319,9 +385,17 @@ __device__ inline D3 gather_block(const DevSys& s, const D3* __restrict__ slots,
                 long id = s.sorted_id[q];
                 if (id < lo || id >= hi) continue;
             }
-            const double* w = lds + 3 * (q - c0);
-            acc.x += w[0]; acc.y += w[1]; acc.z += w[2];
+            const R* w = lds + 3 * (q - c0);
+            acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
         }
+        c0 = c1;
+        if (c0 < r1) {
+            const int n2 = 3 * (min(c0 + CH, r1) - c0);
+#pragma unroll
+            for (int j = 0; j < UN; j++)
+                if (j * BLOCK < n2) v[j] = slots[3L * c0 + threadIdx.x + j * BLOCK];
+        }
+        __syncthreads();   // the chunk is consumed before the next one is stored
     }
     return acc;
 }
@@ -332,46 +406,33 @@ __device__ inline D3 gather_block(const DevSys& s, const D3* __restrict__ slots,
 
 // Slots of every interaction at x (ForceFieldManager::computeForces before the gather).
 // Excluded-volume pairs needing the numerical fallback are listed in bad_list.
+// Grid: the interaction blocks.
+template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* __restrict__ x,
-                                                        D3* __restrict__ slots, int* bad_count, int* bad_list) {
+                                                        R* __restrict__ slots, int* bad_count, int* bad_list) {
     const int ff = ff_of_block(s, blockIdx.x);
     if (ff >= NFF) return;
     const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
     if (t >= s.n[ff]) return;
     Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
-    D3 out[4];
-    // forces only (no energy): same arithmetic as term_ef's force half
-    switch (ff) {
-    case FF_EXVOL: {
-        D3 c1 = pos_of(p, s.ev_b[4 * t]), c2 = pos_of(p, s.ev_b[4 * t + 1]), c3 = pos_of(p, s.ev_b[4 * t + 2]),
-           c4 = pos_of(p, s.ev_b[4 * t + 3]);
-        bool ok = exvol_forces(c1, c2, c3, c4, s.ev_ks[t], out);
-        if (!ok) {
-            out[0] = out[1] = out[2] = out[3] = D3{0.0, 0.0, 0.0};
-            int k = atomicAdd(bad_count, 1);
-            bad_list[k] = t;
-        }
-        const long id = s.slot0[FF_EXVOL] + 4L * t;
-        for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
-        return;
-    }
-    default: {
-        int fbad = 0;
-        (void)term_ef(s, ff, t, p, slots, -1, &fbad);   // the energy half is dead code here
-        return;
-    }
+    int fbad = 0, bad_t = -1;
+    (void)term_ef<R>(s, ff, t, p, slots, -1, &fbad, &bad_t);   // forces only: the energy half is dead code
+    if (fbad) {
+        const int k = atomicAdd(bad_count, 1);
+        bad_list[k] = bad_t;
     }
 }
 
-// force[b] = base[b] + sum of b's slots with id in [lo, hi) (base may be null: zero-fill).
-// Grid: bead blocks only.
-__global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, const D3* __restrict__ slots, long lo, long hi,
-                                                        const double* __restrict__ base, double* __restrict__ f) {
-    __shared__ double lds[3 * GCHUNK];
+// force[b] = base[b] + the bonded pieces selected by bmask + b's slots with id in [lo, hi)
+// (base may be null: zero-fill). Grid: gather blocks (GB beads each).
+template <typename R>
+__global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, const R* __restrict__ slots, long lo,
+                                                        long hi, const double* __restrict__ base, double* __restrict__ f) {
+    __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     const int b = blockIdx.x * GB + threadIdx.x;
     const bool mine = threadIdx.x < GB && b < s.n_beads;
     D3 acc = (base && mine) ? D3{base[3 * b], base[3 * b + 1], base[3 * b + 2]} : D3{0.0, 0.0, 0.0};
-    acc = gather_block(s, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, lds);
+    acc = gather_block<R>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L);
     if (!mine) return;
     f[3 * b] = acc.x; f[3 * b + 1] = acc.y; f[3 * b + 2] = acc.z;
 }
@@ -419,6 +480,7 @@ __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int*
 }
 
 // Per-FF energies at x, block partials. Grid: interaction blocks only.
+template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_energy_report(DevSys s, const double* __restrict__ x,
                                                          double* __restrict__ partial) {
     __shared__ double lds[BLOCK / 64];
@@ -428,7 +490,7 @@ __global__ void __launch_bounds__(BLOCK) k_energy_report(DevSys s, const double*
         const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
         int fbad = 0;
         Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
-        if (t < s.n[ff]) v = term_ef(s, ff, t, p, nullptr, 0, &fbad);
+        if (t < s.n[ff]) v = term_ef<R>(s, ff, t, p, (R*)nullptr, 0, &fbad);
     }
     const int op1[1] = {0};
     block_reduce_n<1>(&v, op1, lds);
@@ -499,9 +561,8 @@ __global__ void __launch_bounds__(BLOCK) k_exvol_forceN_rows(DevSys s, const dou
                                                              int movebeads, double* __restrict__ partial) {
     __shared__ double lds[6 * (BLOCK / 64)];
     PairGeom g = exvol_pair_geom(s, x, t, movebeads);
-    double a = scalar_product(g.c1, g.c2, g.c1, g.c2), b = scalar_product(g.c3, g.c4, g.c3, g.c4);
-    double c = scalar_product(g.c3, g.c1, g.c3, g.c1), d = scalar_product(g.c1, g.c2, g.c3, g.c4);
-    double e = scalar_product(g.c1, g.c2, g.c3, g.c1), F = scalar_product(g.c3, g.c4, g.c3, g.c1);
+    const SixInv v6 = six_inv(g.c1, g.c2, g.c3, g.c4);
+    const double a = v6.a, b = v6.b, c = v6.c, d = v6.d, e = v6.e, F = v6.F;
     const int N = 10000;
     const double delta = 1.0 / ((double)N);
     double acc[6] = {0, 0, 0, 0, 0, 0}, in[6];
@@ -535,13 +596,13 @@ __global__ void __launch_bounds__(BLOCK) k_exvol_forceN_rows(DevSys s, const dou
         for (int k = 0; k < 6; k++) partial[6 * blockIdx.x + k] = acc[k];
 }
 
+template <typename R>
 __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, int t, int movebeads, int nblk,
-                                      const double* __restrict__ partial, D3* __restrict__ slots, int* ok_out) {
+                                      const double* __restrict__ partial, R* __restrict__ slots, int* ok_out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     PairGeom g = exvol_pair_geom(s, x, t, movebeads);
-    double a = scalar_product(g.c1, g.c2, g.c1, g.c2), b = scalar_product(g.c3, g.c4, g.c3, g.c4);
-    double c = scalar_product(g.c3, g.c1, g.c3, g.c1), d = scalar_product(g.c1, g.c2, g.c3, g.c4);
-    double e = scalar_product(g.c1, g.c2, g.c3, g.c1), F = scalar_product(g.c3, g.c4, g.c3, g.c1);
+    const SixInv v6 = six_inv(g.c1, g.c2, g.c3, g.c4);
+    const double a = v6.a, b = v6.b, c = v6.c, d = v6.d, e = v6.e, F = v6.F;
     double T1[6] = {0, 0, 0, 0, 0, 0}, in[6], rest[6] = {0, 0, 0, 0, 0, 0};
     const double lim[2] = {0.0, 1.0};
     for (int p = 0; p < 2; p++)
@@ -565,7 +626,7 @@ __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, in
         f[1][k] = 4.0 * ks * (-I6[1] * vC[k] - I6[2] * vA[k] + I6[3] * vB[k]);
         f[2][k] = 4.0 * ks * (I6[0] * vC[k] + I6[1] * vA[k] - I6[4] * (vB[k] + vC[k]) - I6[3] * vA[k] + I6[5] * vB[k]);
         f[3][k] = 4.0 * ks * (I6[4] * vC[k] + I6[3] * vA[k] - I6[5] * vB[k]);
-        for (int q = 0; q < 4; q++) ok = ok && !bad_component(f[q][k]);
+        for (int q = 0; q < 4; q++) ok = ok && !bad_component<double>(f[q][k]);
     }
     *ok_out = ok ? 1 : 0;
     if (ok) {
@@ -589,7 +650,7 @@ struct CgBufs {
     double* xmin[2];    // committed / candidate minimum-energy backup (bkpar)
     double* f;          // force
     double* fp;         // previous force
-    D3* slots;
+    void* slots;        // 3 values of the evaluation's arithmetic type per slot
     double* partial;    // [blocks * PSTRIDE]
     int* bad_count;     // excluded-volume pairs needing the numerical fallback in a CG move
 };
@@ -605,7 +666,9 @@ __device__ inline bool converged(const CgState* st) {
 // ---- line search: trial energies (and speculative forces at the first trial) ----------
 // Grid: NTRIAL copies of the interaction blocks, then the bead blocks. Writes block partials;
 // k_cg_ls_fin decides.
-__global__ void __launch_bounds__(BLOCK) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
+// 3 waves per SIMD (at most 168 VGPRs, a few spilled) measured faster than the unconstrained 2.
+template <typename R>
+__global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double lds[2 * (BLOCK / 64)];
     if (st->done || st->phase != PH_LS) return;
     const bool first = st->ls_active == 0;
@@ -628,11 +691,11 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ls(DevSys s, const CgState* __rest
         int fbad = 0;
         if (t < s.n[ff]) {
             Pos p{xc, dc, B.f, beta, trial == 0 ? lam0 : lam0 * 0.5, pend ? 2 : 1};
-            v[0] = term_ef(s, ff, t, p, trial == 0 ? B.slots : nullptr, trial, &fbad);
+            v[0] = term_ef<R>(s, ff, t, p, trial == 0 ? static_cast<R*>(B.slots) : nullptr, trial, &fbad);
             // E(x) rides with the energy-only trial (lower register pressure than the force trial)
             if (trial == 1 && need_e0) {
                 Pos p0{xc, nullptr, nullptr, 0.0, 0.0, 0};
-                v[1] = term_ef(s, ff, t, p0, nullptr, 2, &fbad);
+                v[1] = term_ef<R>(s, ff, t, p0, (R*)nullptr, 2, &fbad);
             }
             if (fbad) atomicAdd(B.bad_count, 1);
         }
@@ -824,6 +887,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ls_fin(DevSys s, CgState* __restri
 // ---- forces and x_next at the accepted (non-first) step or the recovery step ----------
 // Small grid (MOVE_GRID blocks) striding over all interactions and beads: the kernel is a no-op
 // in most slots, so its empty launch must be cheap.
+template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_cg_move(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     if (st->done || st->phase != PH_MOVE) return;
     const double lam = st->lambda;
@@ -839,7 +903,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_move(DevSys s, const CgState* __re
             const int t = (blk - s.blk[ff]) * BLOCK + (int)(i % BLOCK);
             if (t < s.n[ff]) {
                 int fbad = 0;
-                (void)term_ef(s, ff, t, p, B.slots, -1, &fbad);
+                (void)term_ef<R>(s, ff, t, p, static_cast<R*>(B.slots), -1, &fbad);
                 if (fbad) atomicAdd(B.bad_count, 1);
             }
         } else {
@@ -853,8 +917,9 @@ __global__ void __launch_bounds__(BLOCK) k_cg_move(DevSys s, const CgState* __re
 
 // ---- gather F; F.F, F.F_prev, max|F| partials -----------------------------------------
 // Grid: bead blocks only.
+template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __restrict__ st, CgBufs B) {
-    __shared__ double lds[3 * GCHUNK];
+    __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     if (st->done) return;
     if (st->phase != PH_MOVE && st->phase != PH_GATHER) return;
     const int b = blockIdx.x * GB + threadIdx.x;
@@ -862,7 +927,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     // the condition are known now, the force part after the reduction
     const bool cand = st->num_iter > st->force_call_limit / 2 && st->total_energy <= st->minimum_e;
     const int par = st->parity, bk = st->bkpar;
-    D3 acc = gather_block(s, B.slots, 0, 0, true, D3{0.0, 0.0, 0.0}, lds);
+    D3 acc = gather_block<R>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
     if (threadIdx.x >= GB) return;   // one wave finishes
     double v[3] = {0.0, 0.0, 0.0};
     if (b < s.n_beads) {

@@ -21,7 +21,8 @@ using namespace mdn;
 
 namespace {
 
-constexpr int SLOTS_PER_TERM[NFF] = {2, 3, 4, 4, 4, 1};
+// stored slots per interaction (filament stretching / bending forces are computed by the gather)
+constexpr int SLOTS_PER_TERM[NFF] = {0, 0, 4, 4, 4, 1};
 
 template <typename T>
 struct DBuf {
@@ -70,6 +71,8 @@ struct mdn_ctx {
     DBuf<double> str_k, str_eql, bend_k, bend_eqt, lnk_k, lnk_eql, lnk_p1, lnk_p2, lnk_sf;
     DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
     DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
+    DBuf<unsigned char> pmask;
+    DBuf<double> piece;   // NBOND * 3 * n_beads values (FP64; the mixed path uses the first half as float)
     // cylinder tables for the NL
     DBuf<int> d_cyl_b1, d_cyl_b2, d_cyl_fil, d_cyl_pos, d_cyl_type;
     DBuf<char> d_cyl_minus;
@@ -85,7 +88,7 @@ struct mdn_ctx {
 
     // state vectors
     DBuf<double> x[2], f, fp, d[2], xmin[2], scratch_f, ext;
-    DBuf<D3> slots;
+    DBuf<double> slots;   // 3 values per slot; the mixed path uses the first half as float
     DBuf<double> partial;
     DBuf<int> bad_count, bad_list;
     DBuf<unsigned> ticket;
@@ -102,9 +105,9 @@ struct mdn_ctx {
     DevSys sys{};
     int cur = 0;   // which x buffer holds the current coordinates
 
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    long sys_gen = 0, graph_gen = -1;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // per precision (MDN_PREC_*)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    long sys_gen = 0, graph_gen[2] = {-1, -1};
     CgState last_cg{};
 };
 
@@ -166,6 +169,7 @@ void refresh_ptrs(mdn_ctx* ctx) {
     s.mot_p2 = ctx->mot_p2.p; s.mot_sf = ctx->mot_sf.p;
     s.ev_b = ctx->ev_b.p; s.ev_ks = ctx->ev_ks.p;
     s.bnd_b = ctx->bnd_b.p; s.bnd_e = ctx->bnd_e.p; s.bnd_krep = ctx->bnd_krep.p; s.bnd_slen = ctx->bnd_slen.p;
+    s.piece = (void*)ctx->piece.p; s.pmask = ctx->pmask.p;
     s.pos = ctx->pos.p; s.bstart = ctx->bstart.p; s.sorted_id = ctx->sorted_id.p;
     s.defer = ctx->defer.p; s.defer_n = ctx->defer_n.p;
     s.n_elem = (int)ctx->elems.size();
@@ -178,7 +182,8 @@ void refresh_ptrs(mdn_ctx* ctx) {
 // Size the per-evaluation buffers (slots, partials) for the current layout.
 int ensure_eval_buffers(mdn_ctx* ctx) {
     const DevSys& s = ctx->sys;
-    HIPCHK(ctx->slots.ensure((size_t)s.slot0[NFF]));
+    // padded by one gather chunk: the gather's chunk loads are unconditional
+    HIPCHK(ctx->slots.ensure(3 * (size_t)std::max(1L, s.slot0[NFF]) + GCHUNK_PAD));
     const long nls = NTRIAL * (long)s.blk[NFF] + (s.blk[NFF + 1] - s.blk[NFF]);
     HIPCHK(ctx->partial.ensure((size_t)(std::max(nls, (long)gather_blocks(ctx)) + 1) * PSTRIDE));
     HIPCHK(ctx->pos.ensure((size_t)std::max(1L, s.slot0[NFF])));
@@ -229,13 +234,15 @@ int set_coords_dev(mdn_ctx* ctx, const double* coord) {
     return MDN_OK;
 }
 
-// Term forces + numerical fallback for non-finite excluded-volume pairs, into ctx->slots, at
-// device coordinates x.
-int eval_slots(mdn_ctx* ctx, const double* x) {
+// Term forces + numerical fallback for non-finite excluded-volume pairs, into ctx->slots (in
+// the arithmetic type R), at device coordinates x.
+template <typename R>
+int eval_slots_t(mdn_ctx* ctx, const double* x) {
     const DevSys& s = ctx->sys;
+    R* slots = reinterpret_cast<R*>(ctx->slots.p);
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
     if (s.blk[NFF] > 0)
-        hipLaunchKernelGGL(k_forces_terms, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x, ctx->slots.p,
+        hipLaunchKernelGGL(k_forces_terms<R>, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x, slots,
                            ctx->bad_count.p, ctx->bad_list.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -249,13 +256,14 @@ int eval_slots(mdn_ctx* ctx, const double* x) {
     HIPCHK(ctx->fallback_partial.ensure(6 * G));
     HIPCHK(ctx->fallback_ok.ensure(1));
     for (int t : bad) {
-        // forceN, retried once with beads 1 and 2 moved out of plane (:1198-1204)
+        // forceN (FP64 in both paths), retried once with beads 1 and 2 moved out of plane
+        // (CylinderExclVolRepulsion.cpp:1198-1204)
         int ok = 0;
         for (int mv = 0; mv < 2 && !ok; mv++) {
             hipLaunchKernelGGL(k_exvol_forceN_rows, dim3(G), dim3(BLOCK), 0, ctx->stream, s, x, t, mv,
                                ctx->fallback_partial.p);
-            hipLaunchKernelGGL(k_exvol_forceN_finish, dim3(1), dim3(64), 0, ctx->stream, s, x, t, mv, G,
-                               ctx->fallback_partial.p, ctx->slots.p, ctx->fallback_ok.p);
+            hipLaunchKernelGGL(k_exvol_forceN_finish<R>, dim3(1), dim3(64), 0, ctx->stream, s, x, t, mv, G,
+                               (const double*)ctx->fallback_partial.p, slots, ctx->fallback_ok.p);
             HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->fallback_ok.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
             ok = ctx->h_int[0];
@@ -266,21 +274,37 @@ int eval_slots(mdn_ctx* ctx, const double* x) {
     }
     return MDN_OK;
 }
+int eval_slots(mdn_ctx* ctx, const double* x, int prec) {
+    return prec == MDN_PREC_MIXED ? eval_slots_t<float>(ctx, x) : eval_slots_t<double>(ctx, x);
+}
 
-int gather(mdn_ctx* ctx, long lo, long hi, const double* base, double* out) {
-    hipLaunchKernelGGL(k_gather_plain, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                       (const D3*)ctx->slots.p, lo, hi, base, out);
+// force = base + the bonded pieces selected by bmask (bit 0 stretching, bit 1 bending) + the
+// slots with id in [lo, hi)
+int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec) {
+    if (prec == MDN_PREC_MIXED)
+        hipLaunchKernelGGL(k_gather_plain<float>, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                           bmask, (const float*)ctx->slots.p, lo, hi, base, out);
+    else
+        hipLaunchKernelGGL(k_gather_plain<double>, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                           bmask, (const double*)ctx->slots.p, lo, hi, base, out);
     HIPCHK(hipGetLastError());
     return MDN_OK;
 }
 
-// Per-FF energies at device coordinates x into ctx->h_eout[0..7] (synchronous).
-int energy_report(mdn_ctx* ctx, const double* x) {
+void launch_energy(mdn_ctx* ctx, const double* x, int prec) {
     const DevSys& s = ctx->sys;
     int grid = std::max(1, s.blk[NFF]);
-    hipLaunchKernelGGL(k_energy_report, dim3(grid), dim3(BLOCK), 0, ctx->stream, s, x, ctx->partial.p);
+    if (prec == MDN_PREC_MIXED)
+        hipLaunchKernelGGL(k_energy_report<float>, dim3(grid), dim3(BLOCK), 0, ctx->stream, s, x, ctx->partial.p);
+    else
+        hipLaunchKernelGGL(k_energy_report<double>, dim3(grid), dim3(BLOCK), 0, ctx->stream, s, x, ctx->partial.p);
     hipLaunchKernelGGL(k_energy_report_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, (const double*)ctx->partial.p,
                        ctx->eout.p);
+}
+
+// Per-FF energies at device coordinates x into ctx->h_eout[0..7] (synchronous).
+int energy_report(mdn_ctx* ctx, const double* x, int prec) {
+    launch_energy(ctx, x, prec);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_eout, ctx->eout.p, sizeof(double) * 9, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -341,8 +365,10 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     if (!ctx) return MDN_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    for (int q = 0; q < 2; q++) {
+        if (ctx->graph_exec[q]) (void)hipGraphExecDestroy(ctx->graph_exec[q]);
+        if (ctx->graph[q]) (void)hipGraphDestroy(ctx->graph[q]);
+    }
     DBuf<int>* ib[] = {&ctx->str_b, &ctx->bend_b, &ctx->lnk_b, &ctx->mot_b, &ctx->ev_b, &ctx->bnd_b, &ctx->bnd_e,
                        &ctx->inc_s_start, &ctx->inc_s, &ctx->inc_d_start, &ctx->inc_d, &ctx->inc_cnt,
                        &ctx->inc_cursor, &ctx->d_cyl_b1, &ctx->d_cyl_b2, &ctx->d_cyl_fil, &ctx->d_cyl_pos,
@@ -360,6 +386,8 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                           &ctx->ext, &ctx->partial, &ctx->eout, &ctx->fallback_partial};
     for (auto* b : db) b->release();
     ctx->d_cyl_minus.release();
+    ctx->pmask.release();
+    ctx->piece.release();
     ctx->cub_tmp.release();
     ctx->slots.release();
     ctx->ticket.release();
@@ -468,12 +496,25 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     s.n[FF_BOUNDARY] = 0;
     layout(ctx);
 
-    // static incidence CSR: per bead, slots of stretching / bending / linker / motor ascending
+    // per bead, which bonded pieces it has, in the reference's per-bead order: stretching term
+    // with the bead as bead 2, as bead 1; bending term as bead 3, 2, 1
+    std::vector<unsigned char> pmask(ctx->n_beads, 0);
+    {
+        auto put = [&](int b, int k) {
+            unsigned char& m = pmask[b];
+            if (m & (1u << k)) return false;   // a bead listed twice in the filament tables
+            m |= (unsigned char)(1u << k);
+            return true;
+        };
+        bool ok = true;
+        for (int t = 0; t < n_cyl; t++) ok = put(str_b[2 * t + 1], 0) && put(str_b[2 * t], 1) && ok;
+        for (int t = 0; t < n_bend; t++)
+            ok = put(bend_b[3 * t + 2], 2) && put(bend_b[3 * t + 1], 3) && put(bend_b[3 * t], 4) && ok;
+        if (!ok) return fail(ctx, MDN_ERR_ARG, "a bead appears more than once in the filament bead lists");
+    }
+    // static incidence CSR: per bead, slots of linkers / motors ascending
     std::vector<int> cnt(ctx->n_beads + 1, 0);
     auto each_static = [&](auto&& fn) {
-        for (int t = 0; t < n_cyl; t++) { fn(str_b[2 * t], s.slot0[0] + 2L * t); fn(str_b[2 * t + 1], s.slot0[0] + 2L * t + 1); }
-        for (int t = 0; t < n_bend; t++)
-            for (int q = 0; q < 3; q++) fn(bend_b[3 * t + q], s.slot0[1] + 3L * t + q);
         for (int w = 0; w < 2; w++)
             for (size_t t = 0; t < xk[w].size(); t++)
                 for (int q = 0; q < 4; q++) fn(xb[w][4 * t + q], s.slot0[2 + w] + 4L * t + q);
@@ -496,6 +537,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     HIPCHK(upload(ctx, ctx->mot_p1, xp1[1])); HIPCHK(upload(ctx, ctx->mot_p2, xp2[1]));
     HIPCHK(ctx->mot_sf.ensure(std::max<size_t>(1, xk[1].size())));
     HIPCHK(upload(ctx, ctx->inc_s_start, cnt)); HIPCHK(upload(ctx, ctx->inc_s, inc));
+    HIPCHK(upload(ctx, ctx->pmask, pmask));
+    HIPCHK(ctx->piece.ensure(NBOND * 3 * (size_t)ctx->n_beads));
     // cylinder tables for the neighbor lists
     HIPCHK(upload(ctx, ctx->d_cyl_b1, ctx->cyl_b1)); HIPCHK(upload(ctx, ctx->d_cyl_b2, ctx->cyl_b2));
     HIPCHK(upload(ctx, ctx->d_cyl_fil, ctx->cyl_fil)); HIPCHK(upload(ctx, ctx->d_cyl_pos, ctx->cyl_pos));
@@ -705,7 +748,7 @@ int mdn_boundary_list(mdn_ctx* ctx, int32_t* out_beads, int64_t cap, int32_t* ou
     return MDN_OK;
 }
 
-int mdn_energy(mdn_ctx* ctx, const double* coord, mdn_energy_report* out) {
+static int energy_impl(mdn_ctx* ctx, const double* coord, mdn_energy_report* out, int prec) {
     int st = need_system(ctx, true);
     if (st) return st;
     if (!out) return fail(ctx, MDN_ERR_ARG, "null report");
@@ -715,12 +758,20 @@ int mdn_energy(mdn_ctx* ctx, const double* coord, mdn_energy_report* out) {
         HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * 3 * ctx->n_beads, hipMemcpyHostToDevice, ctx->stream));
         x = ctx->ext.p;
     }
-    if ((st = energy_report(ctx, x))) return st;
+    if ((st = energy_report(ctx, x, prec))) return st;
     fill_report(ctx->h_eout, out);
     return MDN_OK;
 }
 
-static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force, int32_t n_var) {
+int mdn_energy(mdn_ctx* ctx, const double* coord, mdn_energy_report* out) {
+    return energy_impl(ctx, coord, out, MDN_PREC_FP64);
+}
+
+int mdn_energy_mixed(mdn_ctx* ctx, const double* coord, mdn_energy_report* out) {
+    return energy_impl(ctx, coord, out, MDN_PREC_MIXED);
+}
+
+static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force, int32_t n_var, int prec) {
     int st = need_system(ctx, true);
     if (st) return st;
     if (!force || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
@@ -730,7 +781,7 @@ static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force,
         HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
         x = ctx->ext.p;
     }
-    if ((st = eval_slots(ctx, x))) return st;
+    if ((st = eval_slots(ctx, x, prec))) return st;
     long lo = 0, hi = ctx->sys.slot0[NFF];
     const double* base = nullptr;
     if (ff >= 0) {
@@ -739,19 +790,20 @@ static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force,
         HIPCHK(hipMemcpyAsync(ctx->scratch_f.p, force, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
         base = ctx->scratch_f.p;
     }
-    if ((st = gather(ctx, lo, hi, base, ctx->f.p))) return st;
+    const int bmask = ff < 0 ? 3 : ff == FF_STRETCH ? 1 : ff == FF_BEND ? 2 : 0;
+    if ((st = gather(ctx, bmask, lo, hi, base, ctx->f.p, prec))) return st;
     HIPCHK(hipMemcpyAsync(force, ctx->f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
 }
 
 int mdn_forces(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var) {
-    return forces_impl(ctx, -1, coord, force, n_var);
+    return forces_impl(ctx, -1, coord, force, n_var, MDN_PREC_FP64);
 }
 
 int mdn_ff_forces(mdn_ctx* ctx, int32_t ff, const double* coord, double* force, int32_t n_var) {
     if (ff < 0 || ff >= NFF) return fail(ctx, MDN_ERR_ARG, "force field index %d", ff);
-    return forces_impl(ctx, ff, coord, force, n_var);
+    return forces_impl(ctx, ff, coord, force, n_var, MDN_PREC_FP64);
 }
 
 int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor_out) {
@@ -765,7 +817,7 @@ int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor
 }
 
 int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var) {
-    return fail(ctx, MDN_ERR_UNSUPPORTED, "mixed precision not built in this library version");
+    return forces_impl(ctx, -1, coord, force, n_var, MDN_PREC_MIXED);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -777,38 +829,44 @@ static CgBufs cg_bufs(mdn_ctx* ctx) {
     B.d[0] = ctx->d[0].p; B.d[1] = ctx->d[1].p;
     B.xmin[0] = ctx->xmin[0].p; B.xmin[1] = ctx->xmin[1].p;
     B.f = ctx->f.p; B.fp = ctx->fp.p;
-    B.slots = ctx->slots.p; B.partial = ctx->partial.p; B.bad_count = ctx->bad_count.p;
+    B.slots = (void*)ctx->slots.p; B.partial = ctx->partial.p; B.bad_count = ctx->bad_count.p;
     return B;
 }
 
+extern "C++" {
+template <typename R>
 static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
     const DevSys& s = ctx->sys;
     const int T = s.blk[NFF], beads = bead_blocks(ctx);
     const int nls = NTRIAL * T + beads;
-    hipLaunchKernelGGL(k_cg_ls, dim3(nls), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_ls<R>, dim3(nls), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_ls_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B, nls);
     const int gbk = gather_blocks(ctx);
-    hipLaunchKernelGGL(k_cg_move, dim3(MOVE_GRID), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_gather, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_move<R>, dim3(MOVE_GRID), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_gather<R>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B, gbk);
 }
+}  // extern "C++"
 
 // A batch of minimizer slots captured once per system layout and replayed as a hipGraph; the
 // kernels take every decision from the device state, so replays need no host work.
 static const int CG_BATCH = 16;
 
-static int cg_graph(mdn_ctx* ctx) {
-    if (ctx->graph_exec && ctx->graph_gen == ctx->sys_gen) return MDN_OK;
-    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph_exec = nullptr;
-    ctx->graph = nullptr;
+static int cg_graph(mdn_ctx* ctx, int prec) {
+    if (ctx->graph_exec[prec] && ctx->graph_gen[prec] == ctx->sys_gen) return MDN_OK;
+    if (ctx->graph_exec[prec]) (void)hipGraphExecDestroy(ctx->graph_exec[prec]);
+    if (ctx->graph[prec]) (void)hipGraphDestroy(ctx->graph[prec]);
+    ctx->graph_exec[prec] = nullptr;
+    ctx->graph[prec] = nullptr;
     CgBufs B = cg_bufs(ctx);
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < CG_BATCH; i++) launch_slot(ctx, B);
-    HIPCHK(hipStreamEndCapture(ctx->stream, &ctx->graph));
-    HIPCHK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
-    ctx->graph_gen = ctx->sys_gen;
+    for (int i = 0; i < CG_BATCH; i++) {
+        if (prec == MDN_PREC_MIXED) launch_slot<float>(ctx, B);
+        else launch_slot<double>(ctx, B);
+    }
+    HIPCHK(hipStreamEndCapture(ctx->stream, &ctx->graph[prec]));
+    HIPCHK(hipGraphInstantiate(&ctx->graph_exec[prec], ctx->graph[prec], nullptr, nullptr, 0));
+    ctx->graph_gen[prec] = ctx->sys_gen;
     return MDN_OK;
 }
 
@@ -816,7 +874,9 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     int st = need_system(ctx, true);
     if (st) return st;
     if (!p || !coord || !res) return fail(ctx, MDN_ERR_ARG, "null argument");
-    if (p->precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "only FP64 minimization in this version");
+    if (p->precision != MDN_PREC_FP64 && p->precision != MDN_PREC_MIXED)
+        return fail(ctx, MDN_ERR_ARG, "unknown precision %d", p->precision);
+    const int prec = p->precision;
     if (p->method < 0 || p->method > 2) return fail(ctx, MDN_ERR_ARG, "unknown descent method %d", p->method);
     HIPCHK(hipSetDevice(ctx->device));
     memset(res, 0, sizeof(*res));
@@ -824,10 +884,17 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     // startMinimization + initial energy / forces (CGMethod.hpp:876-896)
     ctx->cur = 0;
     HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
-    if ((st = energy_report(ctx, ctx->x[0].p))) return st;
+    // the reported energies are FP64 in both precisions; the minimizer's own energies and forces
+    // are evaluated in the requested precision
+    if ((st = energy_report(ctx, ctx->x[0].p, MDN_PREC_FP64))) return st;
     fill_report(ctx->h_eout, &res->energies_before);
-    if ((st = eval_slots(ctx, ctx->x[0].p))) return st;
-    if ((st = gather(ctx, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p))) return st;
+    double e_start = res->energies_before.total;
+    if (prec == MDN_PREC_MIXED) {
+        if ((st = energy_report(ctx, ctx->x[0].p, prec))) return st;
+        e_start = ctx->h_eout[6];
+    }
+    if ((st = eval_slots(ctx, ctx->x[0].p, prec))) return st;
+    if ((st = gather(ctx, 3, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p, prec))) return st;
     std::vector<double> hf(nv);
     HIPCHK(hipMemcpyAsync(hf.data(), ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -850,7 +917,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     S.num_force_call = 1;
     S.grad_inf_norm = mx;
     S.energy_rel_change = INFINITY;
-    S.prev_energy = res->energies_before.total;
+    S.prev_energy = e_start;
     S.minimum_e = INFINITY;
     S.fdotf = ff;
     // resetCGParams before the loop (d = F + 0 d, lastFDotF = F.F, F_prev = F), applied by the
@@ -870,7 +937,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     HIPCHK(hipMemsetAsync(ctx->d[0].p, 0, sizeof(double) * nv, ctx->stream));   // searchDir.assign(numDof, 0)
     HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    if ((st = cg_graph(ctx))) return st;
+    if ((st = cg_graph(ctx, prec))) return st;
 
     // Batches are replayed back to back; the host polls the done flag of batch i while batch
     // i+1 runs, so the device never idles waiting for the host.
@@ -884,7 +951,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int k = 0;
     while (!done && launched < max_batches) {
-        HIPCHK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+        HIPCHK(hipGraphLaunch(ctx->graph_exec[prec], ctx->stream));
         launched++;
         HIPCHK(hipMemcpyAsync(&h_done[k], &ctx->cg.p->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipEventRecord(ev[k], ctx->stream));
@@ -925,7 +992,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
                                   ctx->stream));
             res->restored_min_energy = 1;
         }
-        if ((st = energy_report(ctx, xres))) return st;
+        if ((st = energy_report(ctx, xres, MDN_PREC_FP64))) return st;
         fill_report(ctx->h_eout, &res->energies_after);
         res->num_energy_call += 1;
     }
@@ -950,31 +1017,32 @@ int mdn_last_min_stats(mdn_ctx* ctx, int64_t* out) {
 // ----------------------------------------------------------------------------------------
 // Benchmark helpers: device-resident coordinates, HIP events on the context stream
 // ----------------------------------------------------------------------------------------
-int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval, double* kernel_ms) {
-    int st = need_system(ctx, true);
-    if (st) return st;
-    if (precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "precision %d", precision);
-    HIPCHK(hipSetDevice(ctx->device));
+extern "C++" {
+template <typename R>
+static int bench_forces_t(mdn_ctx* ctx, int32_t iters, double* ms_per_eval, double* kernel_ms) {
     const DevSys& s = ctx->sys;
     const double* x = ctx->x[ctx->cur].p;
+    R* slots = reinterpret_cast<R*>(ctx->slots.p);
+    const int nb = std::max(1, s.blk[NFF]);
+    int st;
     // warm-up (and fallback check) once
-    if ((st = eval_slots(ctx, x))) return st;
+    if ((st = eval_slots_t<R>(ctx, x))) return st;
     hipEvent_t e0, e1, k0, k1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&k0)); HIPCHK(hipEventCreate(&k1));
     // whole evaluation: terms + gather
     HIPCHK(hipEventRecord(e0, ctx->stream));
     for (int i = 0; i < iters; i++) {
-        hipLaunchKernelGGL(k_forces_terms, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
-                           ctx->slots.p, ctx->bad_count.p, ctx->bad_list.p);
-        hipLaunchKernelGGL(k_gather_plain, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s,
-                           (const D3*)ctx->slots.p, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
+        hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
+                           ctx->bad_count.p, ctx->bad_list.p);
+        hipLaunchKernelGGL(k_gather_plain<R>, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s, 3,
+                           (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
     }
     HIPCHK(hipEventRecord(e1, ctx->stream));
     // dominant kernel alone
     HIPCHK(hipEventRecord(k0, ctx->stream));
     for (int i = 0; i < iters; i++)
-        hipLaunchKernelGGL(k_forces_terms, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
-                           ctx->slots.p, ctx->bad_count.p, ctx->bad_list.p);
+        hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
+                           ctx->bad_count.p, ctx->bad_list.p);
     HIPCHK(hipEventRecord(k1, ctx->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(k1));
@@ -986,23 +1054,31 @@ int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(k0); (void)hipEventDestroy(k1);
     return MDN_OK;
 }
+}  // extern "C++"
+
+int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval, double* kernel_ms) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (precision != MDN_PREC_FP64 && precision != MDN_PREC_MIXED)
+        return fail(ctx, MDN_ERR_ARG, "unknown precision %d", precision);
+    if (iters <= 0) return fail(ctx, MDN_ERR_ARG, "iters must be positive");
+    HIPCHK(hipSetDevice(ctx->device));
+    return precision == MDN_PREC_MIXED ? bench_forces_t<float>(ctx, iters, ms_per_eval, kernel_ms)
+                                       : bench_forces_t<double>(ctx, iters, ms_per_eval, kernel_ms);
+}
 
 int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval) {
     int st = need_system(ctx, true);
     if (st) return st;
-    if (precision != MDN_PREC_FP64) return fail(ctx, MDN_ERR_UNSUPPORTED, "precision %d", precision);
+    if (precision != MDN_PREC_FP64 && precision != MDN_PREC_MIXED)
+        return fail(ctx, MDN_ERR_ARG, "unknown precision %d", precision);
+    if (iters <= 0) return fail(ctx, MDN_ERR_ARG, "iters must be positive");
     HIPCHK(hipSetDevice(ctx->device));
-    const DevSys& s = ctx->sys;
     const double* x = ctx->x[ctx->cur].p;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    for (int i = 0; i < iters; i++) {
-        hipLaunchKernelGGL(k_energy_report, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s, x,
-                           ctx->partial.p);
-        hipLaunchKernelGGL(k_energy_report_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s,
-                           (const double*)ctx->partial.p, ctx->eout.p);
-    }
+    for (int i = 0; i < iters; i++) launch_energy(ctx, x, precision);
     HIPCHK(hipEventRecord(e1, ctx->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(e1));

@@ -29,7 +29,7 @@ EXPORTED = [
     "mdn_system_counts", "mdn_nl_build", "mdn_nl_pairs", "mdn_boundary_list", "mdn_energy",
     "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
-    "mdn_last_min_stats",
+    "mdn_last_min_stats", "mdn_energy_mixed",
 ]
 
 
@@ -71,6 +71,7 @@ def lib():
         L.mdn_set_coords.argtypes = [C.c_void_p, dp]
         L.mdn_get_coords.argtypes = [C.c_void_p, dp]
         L.mdn_forces_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int32]
+        L.mdn_energy_mixed.argtypes = [C.c_void_p, dp, C.POINTER(abi.EnergyReport)]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
         _lib = L
     return _lib
@@ -134,16 +135,17 @@ class ForceFieldManager:
                                             nn.ctypes.data_as(i32p), C.byref(n)))
         return beads[: n.value], nn[:n_elem]
 
-    def compute_individual_energies(self, coord=None):
+    def compute_individual_energies(self, coord=None, precision=abi.MDN_PREC_FP64):
         """EnergyReport per force field (ForceFieldManager::computeIndividualEnergies)."""
         rep = abi.EnergyReport()
         cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
-        self._chk(self._L.mdn_energy(self._h, cp, C.byref(rep)))
+        fn = self._L.mdn_energy_mixed if precision == abi.MDN_PREC_MIXED else self._L.mdn_energy
+        self._chk(fn(self._h, cp, C.byref(rep)))
         return rep
 
-    def compute_energy(self, coord=None):
+    def compute_energy(self, coord=None, precision=abi.MDN_PREC_FP64):
         """ForceFieldManager::computeEnergy: +inf when a force field reports <= -1."""
-        rep = self.compute_individual_energies(coord)
+        rep = self.compute_individual_energies(coord, precision)
         tot = 0.0
         for e in rep.ff:
             if e <= -1:
@@ -151,11 +153,12 @@ class ForceFieldManager:
             tot += e
         return tot
 
-    def compute_forces(self, coord=None):
+    def compute_forces(self, coord=None, precision=abi.MDN_PREC_FP64):
         """ForceFieldManager::computeForces: zero-filled, every force field accumulated."""
         f = np.zeros(3 * self.n_beads)
         cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
-        self._chk(self._L.mdn_forces(self._h, cp, f.ctypes.data_as(dp), f.size))
+        fn = self._L.mdn_forces_mixed if precision == abi.MDN_PREC_MIXED else self._L.mdn_forces
+        self._chk(fn(self._h, cp, f.ctypes.data_as(dp), f.size))
         return f
 
     def ff_compute_forces(self, ff, coord=None, force=None):
