@@ -238,10 +238,12 @@ int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
 int mdn_set_coords(mdn_ctx* ctx, const double* coord);
 int mdn_get_coords(mdn_ctx* ctx, double* coord);
 
-/* Mixed precision (MDN_PREC_MIXED): the stretching, bending, linker, motor and boundary energies
- * and forces are computed in FP32 from bond vectors and coordinate differences formed in FP64
- * (absolute coordinates never enter FP32 arithmetic); excluded volume stays FP64 (its closed form
- * is a 0/0 limit for near-coplanar pairs); per-interaction forces are stored in FP32; per-bead
+/* Mixed precision (MDN_PREC_MIXED): the bending and boundary terms and the force components of
+ * the harmonic springs (stretching, linkers, motors) are computed in FP32 from bond vectors and
+ * coordinate differences formed in FP64 (absolute coordinates never enter FP32 arithmetic; spring
+ * lengths and extensions L - L0 stay FP64, as L0 reaches micrometres for crosslinks); excluded
+ * volume stays FP64 (its closed form is a 0/0 limit for near-coplanar pairs); per-interaction
+ * forces are stored in FP32; per-bead
  * force sums, energy sums, dot products and all minimizer decisions stay FP64. No reference
  * counterpart: the reference's `floatingpoint` type switches the whole program to float
  * (src/utility.h:38-43), which this replaces for the force path.
@@ -251,6 +253,29 @@ int mdn_get_coords(mdn_ctx* ctx, double* coord);
  * energies_before / energies_after stay FP64). */
 int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
 int mdn_energy_mixed(mdn_ctx* ctx, const double* coord, mdn_energy_report* out);
+
+/* ---- batched minimization ------------------------------------------------------------- */
+
+/* CGMethod::minimize (CGMethod.hpp:1746-1759) of n independent systems in one persistent kernel:
+ * each workgroup pulls a system from a device queue and minimizes it to completion (same
+ * semantics as mdn_minimize per system; only the rounding order of the block reductions
+ * differs), then pulls the next. ctxs[i] must have its system uploaded and its neighbor lists
+ * built, all on one device; ctxs[0] is the lead context (its stream runs the batch and it keeps
+ * the tuned schedule). coords[i] (3 * n_beads of system i) is the start and receives the result;
+ * forces_out may be null, or forces_out[i] may be null; results[i] is filled for every system.
+ * The schedule (workgroup size 256 or 512, given or largest-first order) is chosen on the first
+ * call for a given (n, precision, total beads) by timing a short run of each candidate.
+ * Returns MDN_OK when every system succeeded, else the status of the first failing system (its
+ * context's mdn_last_error and the lead's carry the message). No reference counterpart for the
+ * batching itself: the reference minimizes one system per call. */
+int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* params,
+                         double* const* coords, double* const* forces_out, mdn_min_result* results);
+
+/* The schedule of the last batched minimization on a lead context: out3 = [workgroup size,
+ * largest-first order (1) or given order (0), tuned in that call (1) or reused (0)]; tune_ms4
+ * (may be null) = the tuning times of the candidates (256 given, 256 sorted, 512 given,
+ * 512 sorted) in ms. */
+int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms4);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,333 @@
+// mdn_batched.h -- persistent batched minimizer: many independent systems minimized by one
+// kernel launch. Each workgroup pulls a system from a device queue (in the order the host gives,
+// e.g. largest first) and runs CGMethod::minimize on it to completion with workgroup barriers
+// only; then it pulls the next one. Every system keeps its own tables and vectors (the device
+// buffers of its context, described by a DevSys and CgBufs record).
+//
+// The work of one CG "slot" is the same as in the multi-kernel minimizer (k_cg_ls, k_cg_ls_fin,
+// k_cg_move, k_cg_gather, k_cg_gather_fin) and the scalar control is the same code
+// (ls_control, gather_control), so a system follows the reference's CGMethod::minimize
+// (CGMethod.hpp:826-1743) exactly as the single-system path does; only the order of the
+// block-level energy and dot-product reductions differs.
+//
+// Citations are relative to medyan-5.4.0/ in the reference tree.
+#pragma once
+
+#include "mdn_kernels.h"
+
+namespace mdn {
+
+struct BatchItem {
+    DevSys s;         // the system's tables
+    CgBufs B;         // its CG vectors (B.x[0] holds the start coordinates)
+    CgState* st;      // out: final CG state
+    double* report;   // out: [0..8] FP64 energy report at the start, [9..17] at the result
+};
+
+struct BatchParams {
+    int method, try_recover, force_call_limit, energy_call_limit, report_on_call_limit;
+    int limit_override;   // > 0: force-call limit of every system (schedule tuning runs)
+    double max_dist, lambda_max, grad_tol, energy_rel_tol;
+};
+
+// Per-force-field energies at positions p summed over this thread's interactions; code tags
+// pairs deferred to the Simpson pass.
+template <typename R, int NT>
+__device__ inline void blk_energies(const DevSys& s, const Pos& p, int code, double* e) {
+    for (int ff = 0; ff < NFF; ff++) {
+        double a = 0.0;
+        for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
+            int fb = 0;
+            a += term_ef<R>(s, ff, t, p, (R*)nullptr, code, &fb);
+        }
+        e[ff] = a;
+    }
+}
+
+// The energy report of k_energy_report / k_energy_report_fin at x (arithmetic R) into
+// rep[0..8] (thread 0).
+template <typename R, int NT>
+__device__ inline void blk_report(const DevSys& s, const double* x, double* rep, double* red) {
+    double acc[NFF];
+    const Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
+    blk_energies<R, NT>(s, p, 0, acc);
+    const int opn[NFF] = {0, 0, 0, 0, 0, 0};
+    block_reduce_n<NFF, NT>(acc, opn, red);
+    double dsum[1];
+    int dbad[1];
+    const int n = run_defers<NT>(s, red, dsum, dbad, 1);
+    if (threadIdx.x == 0) {
+        acc[FF_EXVOL] += dsum[0];
+        double tot = 0.0, tot_s = 0.0;
+        bool failed = false;
+        for (int f2 = 0; f2 < NFF; f2++) {
+            const double ef = (f2 == FF_EXVOL && dbad[0]) ? -1.0 : acc[f2];
+            rep[f2] = ef;
+            tot += ef;
+            if (!failed) {
+                if (ef <= -1) failed = true;
+                else tot_s += ef;
+            }
+        }
+        rep[6] = tot;
+        rep[7] = failed ? INFINITY : tot_s;
+        rep[8] = n > DEFER_CAP ? (double)n : 0.0;
+    }
+    __syncthreads();
+}
+
+// force[b] = the bead's bonded pieces + its slots (the order of gather_block), with the partial
+// dot products F.F, F.F_prev (fp may be null) and max|F| of this thread's beads, and the
+// minimum-energy backup candidate xmin_dst = xn when cand.
+template <typename R, int NT>
+__device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, double* __restrict__ f,
+                                  const double* __restrict__ fp, bool cand, const double* __restrict__ xn,
+                                  double* __restrict__ xmin_dst, double* v) {
+    const R* pc = static_cast<const R*>(s.piece);
+    for (int b = threadIdx.x; b < s.n_beads; b += NT) {
+        D3 acc{0.0, 0.0, 0.0};
+        const unsigned m = s.pmask[b];
+#pragma unroll
+        for (int k = 0; k < NBOND; k++)
+            if ((m >> k) & 1) {
+                const R* q = pc + 3L * ((long)k * s.n_beads + b);
+                acc.x += (double)q[0]; acc.y += (double)q[1]; acc.z += (double)q[2];
+            }
+        const int q1 = s.bstart[b + 1];
+        for (int q = s.bstart[b]; q < q1; q++) {
+            const R* w = slots + 3L * q;
+            acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
+        }
+        const double fv[3] = {acc.x, acc.y, acc.z};
+        for (int c = 0; c < 3; c++) {
+            f[3 * b + c] = fv[c];
+            v[0] += fv[c] * fv[c];
+            if (fp) v[1] += fv[c] * fp[3 * b + c];
+            v[2] = rmax(v[2], fabs(fv[c]));
+            if (cand) xmin_dst[3 * b + c] = xn[3 * b + c];
+        }
+    }
+}
+
+// The line-search slot: k_cg_ls (trial energies at lambda and lambda/2 with speculative forces
+// at lambda, E(x) when needed, the bead work) and k_cg_ls_fin (reductions, Simpson energies,
+// ls_control).
+template <typename R, int NT>
+__device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, double* red, int& sh_bad) {
+    const bool first = S.ls_active == 0;
+    const bool pend = first && S.dir_pending;
+    const double lam0 = S.ls_lambda, beta = S.dir_beta;
+    const bool need_e0 = first && !S.ecur_valid;
+    const int par = S.parity, dpar = S.dpar;
+    const double* __restrict__ xc = B.x[par];
+    const double* __restrict__ dc = B.d[dpar];
+    R* slots = static_cast<R*>(B.slots);
+    const Pos p1{xc, dc, B.f, beta, lam0, pend ? 2 : 1};
+    const Pos p2{xc, dc, B.f, beta, lam0 * 0.5, pend ? 2 : 1};
+    const Pos pz{xc, nullptr, nullptr, 0.0, 0.0, 0};
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
+    int fbad = 0;
+    // one pass per evaluation keeps a single interaction's working set live at a time
+    for (int ff = 0; ff < NFF; ff++)
+        for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R>(s, ff, t, p1, slots, 0, &fbad);
+    if (fbad) atomicOr(&sh_bad, 1);
+    for (int ff = 0; ff < NFF; ff++)
+        for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
+            int fb2 = 0;
+            acc[1] += term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fb2);
+        }
+    if (need_e0)
+        for (int ff = 0; ff < NFF; ff++)
+            for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
+                int fb2 = 0;
+                acc[4] += term_ef<R>(s, ff, t, pz, (R*)nullptr, 2, &fb2);
+            }
+    const double maxd = S.max_dist;
+    double* __restrict__ xn = B.x[par ^ 1];
+    double* __restrict__ dn = B.d[dpar ^ 1];
+    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) {
+        const double fv = B.f[i];
+        const double de = pend ? fv + beta * dc[i] : dc[i];
+        xn[i] = xc[i] + lam0 * de;   // speculative x_next
+        if (first) {
+            acc[6] += de * fv;                            // d.F
+            acc[7] = rmin(acc[7], maxd / fabs(de));       // MController.h:293-305
+            if (pend) {
+                dn[i] = de;                               // shiftSearchDir materialized
+                B.fp[i] = fv;                             // forcePrev = force
+            }
+        }
+    }
+    const int op8[8] = {0, 0, 1, 1, 0, 1, 0, 2};
+    block_reduce_n<8, NT>(acc, op8, red);
+    double dsum[3];
+    int dbad[3];
+    const int n = run_defers<NT>(s, red, dsum, dbad, 3);
+    if (threadIdx.x == 0) {
+        acc[0] += dsum[0]; acc[2] = dbad[0];
+        acc[1] += dsum[1]; acc[3] = dbad[1];
+        acc[4] += dsum[2]; acc[5] = dbad[2];
+        if (n > DEFER_CAP) {
+            S.status = 9;
+            S.done = 1;
+        } else {
+            ls_control(S, acc, &sh_bad);
+        }
+    }
+    __syncthreads();
+}
+
+// The move slot (k_cg_move): forces and x_next at the accepted step (or the recovery step).
+template <typename R, int NT>
+__device__ inline void blk_move(const DevSys& s, const CgBufs& B, const CgState& S, int& sh_bad) {
+    const double lam = S.lambda;
+    const double* __restrict__ x = B.x[S.parity];
+    const double* __restrict__ d = B.d[S.dpar];
+    const Pos p{x, d, nullptr, 0.0, lam, 1};
+    R* slots = static_cast<R*>(B.slots);
+    int fbad = 0;
+    for (int ff = 0; ff < NFF; ff++)
+        for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p, slots, -1, &fbad);
+    if (fbad) atomicOr(&sh_bad, 1);
+    double* __restrict__ xn = B.x[S.parity ^ 1];
+    // moveAlongSearchDir (CGMethod.cpp:350-354) / the recovery move (MController.h:318-321)
+    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) xn[i] = x[i] + lam * d[i];
+    __syncthreads();
+}
+
+// The gather slot (k_cg_gather, k_cg_gather_fin).
+template <typename R, int NT>
+__device__ inline void blk_gather_step(const DevSys& s, const CgBufs& B, CgState& S, double* red, int& sh_bad) {
+    // copycoordsifminimumE candidate (CGMethod.hpp:1564-1566)
+    const bool cand = S.num_iter > S.force_call_limit / 2 && S.total_energy <= S.minimum_e;
+    const int par = S.parity, bk = S.bkpar;
+    double v[3] = {0.0, 0.0, 0.0};
+    blk_gather<R, NT>(s, static_cast<const R*>(B.slots), B.f, B.fp, cand, B.x[par ^ 1], B.xmin[bk ^ 1], v);
+    const int op3[3] = {0, 0, 1};
+    block_reduce_n<3, NT>(v, op3, red);
+    if (threadIdx.x == 0) gather_control(S, v, &sh_bad);
+    __syncthreads();
+}
+
+// CGMethod::minimize of one system by one workgroup: the set-up of mdn_minimize
+// (CGMethod.hpp:876-896: energies, forces, maxF; resetCGParams applied by the first line
+// search), the CG slots until done, the minimum-energy restore on a call limit
+// (CGMethod.hpp:1579-1590) and the final energy report.
+template <typename R, int NT>
+__device__ inline void blk_minimize(const BatchItem& it, const BatchParams& P, CgState& S, double* red,
+                                    int& sh_bad) {
+    const DevSys& s = it.s;
+    const CgBufs& B = it.B;
+    const int nb = s.n_beads;
+    __shared__ double e_start;
+    // energies before: the FP64 report; the minimizer's own start energy in R arithmetic
+    blk_report<double, NT>(s, B.x[0], it.report, red);
+    if constexpr (sizeof(R) == sizeof(double)) {
+        if (threadIdx.x == 0) e_start = it.report[6];
+    } else {
+        blk_report<R, NT>(s, B.x[0], it.report + 18, red);   // scratch report
+        if (threadIdx.x == 0) e_start = it.report[18 + 6];
+    }
+    // forces at the start, F.F and max|F|; searchDir.assign(numDof, 0)
+    if (threadIdx.x == 0) sh_bad = 0;
+    __syncthreads();
+    {
+        const Pos p0{B.x[0], nullptr, nullptr, 0.0, 0.0, 0};
+        R* slots = static_cast<R*>(B.slots);
+        int fbad = 0;
+        for (int ff = 0; ff < NFF; ff++)
+            for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p0, slots, -1, &fbad);
+        if (fbad) atomicOr(&sh_bad, 1);
+        for (int i = threadIdx.x; i < 3 * nb; i += NT) B.d[0][i] = 0.0;
+    }
+    __syncthreads();
+    double v[3] = {0.0, 0.0, 0.0};
+    blk_gather<R, NT>(s, static_cast<const R*>(B.slots), B.f, nullptr, false, nullptr, nullptr, v);
+    const int op3[3] = {0, 0, 1};
+    block_reduce_n<3, NT>(v, op3, red);
+    if (threadIdx.x == 0) {
+        S = CgState{};
+        S.method = P.method;
+        S.try_recover = P.try_recover;
+        S.force_call_limit = P.limit_override > 0 ? P.limit_override : max(P.force_call_limit, 3 * nb);
+        S.energy_call_limit = P.energy_call_limit;
+        S.max_dist = P.max_dist;
+        S.lambda_max = P.lambda_max;
+        S.grad_tol = P.grad_tol;
+        S.energy_rel_tol = P.energy_rel_tol;
+        S.num_energy_call = 1;
+        S.num_force_call = 1;
+        S.grad_inf_norm = v[2];
+        S.energy_rel_change = INFINITY;
+        S.prev_energy = e_start;
+        S.minimum_e = INFINITY;
+        S.fdotf = v[0];
+        S.dir_pending = 1;
+        S.dir_beta = 0.0;
+        S.dir_reset = 1;
+        S.phase = PH_LS;
+        S.ls_lambda = rmin(S.lambda_max, S.max_dist / S.grad_inf_norm);
+        if (sh_bad) {
+            S.status = 9;   // MDN_ERR_UNSUPPORTED: an excluded-volume pair needs the numerical force fallback
+            S.done = 1;
+        } else if (!isfinite(v[2])) {
+            S.status = 4;   // MDN_ERR_NONFINITE: maxF not finite (CGMethod.hpp:202-212)
+            S.done = 1;
+        } else {
+            const bool limit0 = S.force_call_limit != 0 && S.num_force_call > S.force_call_limit;
+            S.done = (S.grad_inf_norm < S.grad_tol || S.energy_rel_change < S.energy_rel_tol || limit0) ? 1 : 0;
+        }
+        sh_bad = 0;
+    }
+    __syncthreads();
+    // CG slots; every slot either advances the call counters or ends the line search, so the
+    // bound below is never reached by a correct state machine (it guards the exit)
+    const long max_slots = 4L * ((long)S.force_call_limit + 16) + 128;
+    for (long k = 0; k < max_slots && !S.done; k++) {
+        if (S.phase == PH_LS) blk_ls<R, NT>(s, B, S, red, sh_bad);
+        if (!S.done && S.phase == PH_MOVE) blk_move<R, NT>(s, B, S, sh_bad);
+        if (!S.done && (S.phase == PH_MOVE || S.phase == PH_GATHER)) blk_gather_step<R, NT>(s, B, S, red, sh_bad);
+    }
+    if (threadIdx.x == 0) {
+        if (!S.done) {
+            S.status = 3;   // MDN_ERR_STATE: did not terminate
+            S.done = 1;
+        }
+        const bool limit = call_limit_s(S);
+        S.reserved[0] = (S.status == 0 && S.error_flags == 0 && P.report_on_call_limit && limit && S.have_min_e) ? 1 : 0;
+    }
+    __syncthreads();
+    if (S.reserved[0]) {   // copybackupcoordinates
+        double* xr = B.x[S.parity];
+        const double* xb = B.xmin[S.bkpar];
+        for (int i = threadIdx.x; i < 3 * nb; i += NT) xr[i] = xb[i];
+        __syncthreads();
+    }
+    if (S.status == 0 && S.error_flags == 0) blk_report<double, NT>(s, B.x[S.parity], it.report + 9, red);
+    if (threadIdx.x == 0) *it.st = S;
+    __syncthreads();   // S is reused by the next system
+}
+
+// Persistent kernel: workgroups pull systems order[k], k = 0, 1, ... from the queue counter until
+// it passes n (every workgroup then returns).
+template <typename R, int NT>
+__global__ void __launch_bounds__(NT) k_cg_batched(const BatchItem* __restrict__ items, const int* __restrict__ order,
+                                                   int n, int* __restrict__ queue, BatchParams P) {
+    __shared__ CgState S;
+    __shared__ double red[8 * (NT / 64)];
+    __shared__ int sh_bad, sh_k;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_k = atomicAdd(queue, 1);
+        __syncthreads();
+        const int k = sh_k;
+        if (k >= n) return;
+        blk_minimize<R, NT>(items[order[k]], P, S, red, sh_bad);
+    }
+}
+
+__global__ void k_fill_int(int* p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+}  // namespace mdn

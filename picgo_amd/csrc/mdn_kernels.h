@@ -142,9 +142,11 @@ __device__ inline double wave_min(double v) {
     for (int o = 32; o > 0; o >>= 1) v = rmin(v, __shfl_down(v, o, 64));
     return v;
 }
-// op: 0 sum, 1 max, 2 min, for NV values at once. Results valid in thread 0.
-template <int NV>
-__device__ inline void block_reduce_n(double* v, const int* op, double* lds /* >= NV * BLOCK/64 */) {
+// op: 0 sum, 1 max, 2 min, for NV values at once over a block of NTH threads. Results valid in
+// thread 0.
+template <int NV, int NTH = BLOCK>
+__device__ inline void block_reduce_n(double* v, const int* op, double* lds /* >= NV * NTH/64 */) {
+    constexpr int NW = NTH / 64;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int i = 0; i < NV; i++)
@@ -152,14 +154,14 @@ __device__ inline void block_reduce_n(double* v, const int* op, double* lds /* >
     __syncthreads();
     if (lane == 0)
 #pragma unroll
-        for (int i = 0; i < NV; i++) lds[i * (BLOCK / 64) + w] = v[i];
+        for (int i = 0; i < NV; i++) lds[i * NW + w] = v[i];
     __syncthreads();
     if (threadIdx.x == 0)
 #pragma unroll
         for (int i = 0; i < NV; i++) {
-            double r = lds[i * (BLOCK / 64)];
-            for (int q = 1; q < BLOCK / 64; q++) {
-                double u = lds[i * (BLOCK / 64) + q];
+            double r = lds[i * NW];
+            for (int q = 1; q < NW; q++) {
+                double u = lds[i * NW + q];
                 r = op[i] == 0 ? r + u : op[i] == 1 ? rmax(r, u) : rmin(r, u);
             }
             v[i] = r;
@@ -440,11 +442,14 @@ __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, con
 // Block-parallel Simpson energies (energyN) of the pairs deferred by the preceding kernel.
 // Thread 0 receives, per code, the sum of the energies in (code, pair index) order and a flag for
 // a pair that stays non-finite (the reference's -1 sentinel); the deferred list is reset.
-// Returns the number of deferred pairs (DEFER_CAP exceeded when larger).
+// Returns the number of deferred pairs (DEFER_CAP exceeded when larger). The count is read and
+// reset with atomics: in the batched minimizer the pairs were pushed by other waves of the same
+// workgroup within the kernel.
+template <int NTH = BLOCK>
 __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int* badc, int ncode) {
     __shared__ int nsh;
     __syncthreads();
-    if (threadIdx.x == 0) nsh = *s.defer_n;
+    if (threadIdx.x == 0) nsh = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     const int n = nsh;
     const int m = min(n, DEFER_CAP);
@@ -453,8 +458,8 @@ __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int*
         EnDefer& e = s.defer[k];
         const D3 c1{e.c[0], e.c[1], e.c[2]}, c2{e.c[3], e.c[4], e.c[5]}, c3{e.c[6], e.c[7], e.c[8]},
             c4{e.c[9], e.c[10], e.c[11]};
-        double r = exvol_energyN_share(c1, c2, c3, c4, threadIdx.x, BLOCK);
-        block_reduce_n<1>(&r, op1, lds);
+        double r = exvol_energyN_share(c1, c2, c3, c4, threadIdx.x, NTH);
+        block_reduce_n<1, NTH>(&r, op1, lds);
         if (threadIdx.x == 0) e.U = exvol_energyN_finish(c1, c2, c3, c4, e.ks, r);
     }
     if (threadIdx.x == 0) {
@@ -474,7 +479,7 @@ __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int*
                 last_t = s.defer[best].t;
             }
         }
-        *s.defer_n = 0;
+        __hip_atomic_store(s.defer_n, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return n;
 }
@@ -827,6 +832,9 @@ __device__ inline void ls_control(CgState& S, const double* acc, int* bad_count)
         S.dir_pending = 1;             // resetCGParams before the next line search
         S.dir_beta = 0.0;
         S.dir_reset = 1;
+        // the next backtracking search starts again from lambda0 = min(lambdaMax, maxDist / maxF)
+        // (CGMethod.hpp:258-262); maxF is unchanged, no force evaluation happened
+        S.ls_lambda = rmin(S.lambda_max, S.max_dist / S.grad_inf_norm);
         if (call_limit_s(S) || converged_s(S)) S.done = 1;
         return;
     }

@@ -16,6 +16,7 @@
 
 #include "../../include/mdn_mech.h"
 #include "mdn_kernels.h"
+#include "mdn_batched.h"
 
 using namespace mdn;
 
@@ -105,6 +106,13 @@ struct mdn_ctx {
     DevSys sys{};
     int cur = 0;   // which x buffer holds the current coordinates
 
+    // batched minimizer (the lead context of a batch owns these)
+    DBuf<BatchItem> batch_items;
+    DBuf<int> batch_order, batch_queue;
+    long batch_key = -1;          // (systems, precision, beads) of the tuned schedule
+    int batch_nt = 256, batch_sorted = 1, batch_tuned = 0;
+    double batch_tune_ms[4] = {0, 0, 0, 0};
+
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // per precision (MDN_PREC_*)
     hipGraph_t graph[2] = {nullptr, nullptr};
     long sys_gen = 0, graph_gen[2] = {-1, -1};
@@ -132,6 +140,19 @@ int fail(mdn_ctx* c, int code, const char* fmt, ...) {
             return fail(ctx, MDN_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
                         __FILE__, __LINE__);                                                    \
     } while (0)
+
+#define HIPCHK_CTX(c, call)                                                                     \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((c), MDN_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                                    \
+    } while (0)
+
+bool call_limit_host(const CgState& S) {
+    return (S.energy_call_limit != 0 && S.num_energy_call > S.energy_call_limit) ||
+           (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
+}
 
 template <typename T>
 hipError_t upload(mdn_ctx* ctx, DBuf<T>& b, const std::vector<T>& v) {
@@ -343,12 +364,12 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
         return MDN_ERR_HIP;
     }
     if (hipHostMalloc(&ctx->h_cg, sizeof(CgState)) != hipSuccess ||
-        hipHostMalloc(&ctx->h_eout, sizeof(double) * 16) != hipSuccess ||
+        hipHostMalloc(&ctx->h_eout, sizeof(double) * 32) != hipSuccess ||
         hipHostMalloc(&ctx->h_int, sizeof(int) * 16) != hipSuccess) {
         delete ctx;
         return MDN_ERR_HIP;
     }
-    if (ctx->ticket.ensure(8) != hipSuccess || ctx->eout.ensure(16) != hipSuccess ||
+    if (ctx->ticket.ensure(8) != hipSuccess || ctx->eout.ensure(32) != hipSuccess ||
         ctx->bad_count.ensure(1) != hipSuccess || ctx->cg.ensure(1) != hipSuccess ||
         ctx->nl_err.ensure(1) != hipSuccess || ctx->defer.ensure(DEFER_CAP) != hipSuccess ||
         ctx->defer_n.ensure(1) != hipSuccess) {
@@ -393,6 +414,9 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     ctx->ticket.release();
     ctx->defer.release();
     ctx->defer_n.release();
+    ctx->batch_items.release();
+    ctx->batch_order.release();
+    ctx->batch_queue.release();
     ctx->cg.release();
     if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
     if (ctx->h_eout) (void)hipHostFree(ctx->h_eout);
@@ -1000,6 +1024,202 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     if (force_out) HIPCHK(hipMemcpyAsync(force_out, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (S.error_flags & MDN_MIN_ERR_LINESEARCH) return fail(ctx, MDN_ERR_LINESEARCH, "line search failed (recovery disabled)");
+    return MDN_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------------------
+// Batched minimizer: many independent systems, one persistent kernel
+// ----------------------------------------------------------------------------------------
+static void fill_result(const CgState& S, const double* rep_before, const double* rep_after, bool have_after,
+                        mdn_min_result* res) {
+    memset(res, 0, sizeof(*res));
+    fill_report(rep_before, &res->energies_before);
+    if (have_after) fill_report(rep_after, &res->energies_after);
+    res->error_flags = S.error_flags;
+    res->num_energy_call = S.num_energy_call + (have_after ? 1 : 0);
+    res->num_force_call = S.num_force_call;
+    res->force_call_limit = S.force_call_limit;
+    res->energy_call_limit = S.energy_call_limit;
+    res->num_iter = S.num_iter;
+    res->num_safe_mode = S.num_safe_mode;
+    res->num_recovery = S.num_recovery;
+    res->grad_inf_norm = S.grad_inf_norm;
+    res->grad_tol = S.grad_tol;
+    res->energy_rel_change = S.energy_rel_change;
+    res->energy_rel_tol = S.energy_rel_tol;
+    res->call_limit_exceeded = call_limit_host(S);
+    res->converged_grad = S.grad_inf_norm < S.grad_tol;
+    res->converged_energy = S.energy_rel_change < S.energy_rel_tol;
+    res->restored_min_energy = S.reserved[0];
+}
+
+template <typename R, int NT>
+static int launch_batched(mdn_ctx* L, int n, const BatchParams& P) {
+    auto kern = k_cg_batched<R, NT>;
+    int per_cu = 0, n_cu = 0;
+    HIPCHK_CTX(L, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
+    HIPCHK_CTX(L, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, L->device));
+    const int grid = std::max(1, std::min(n, std::max(1, per_cu) * std::max(1, n_cu)));
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, L->stream, L->batch_queue.p, 0);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, L->stream, (const BatchItem*)L->batch_items.p,
+                       (const int*)L->batch_order.p, n, L->batch_queue.p, P);
+    HIPCHK_CTX(L, hipGetLastError());
+    return MDN_OK;
+}
+
+static int run_batched(mdn_ctx* L, int n, const BatchParams& P, int prec, int nt) {
+    if (prec == MDN_PREC_MIXED) return nt == 512 ? launch_batched<float, 512>(L, n, P) : launch_batched<float, 256>(L, n, P);
+    return nt == 512 ? launch_batched<double, 512>(L, n, P) : launch_batched<double, 256>(L, n, P);
+}
+
+extern "C" {
+
+int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p, double* const* coords,
+                         double* const* forces_out, mdn_min_result* results) {
+    if (!ctxs || n <= 0 || !ctxs[0]) return MDN_ERR_ARG;
+    mdn_ctx* L = ctxs[0];
+    if (!p || !coords || !results) return fail(L, MDN_ERR_ARG, "null argument");
+    if (p->precision != MDN_PREC_FP64 && p->precision != MDN_PREC_MIXED)
+        return fail(L, MDN_ERR_ARG, "unknown precision %d", p->precision);
+    if (p->method < 0 || p->method > 2) return fail(L, MDN_ERR_ARG, "unknown descent method %d", p->method);
+    long beads = 0;
+    for (int i = 0; i < n; i++) {
+        mdn_ctx* c = ctxs[i];
+        if (!c || !coords[i]) return fail(L, MDN_ERR_ARG, "system %d: null context or coordinates", i);
+        if (c->device != L->device) return fail(L, MDN_ERR_ARG, "system %d is on another device", i);
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == c) return fail(L, MDN_ERR_ARG, "system %d repeats context %d", i, j);
+        int st = need_system(c, true);
+        if (st) return fail(L, st, "system %d: %s", i, c->err.c_str());
+        beads += c->n_beads;
+    }
+    HIPCHK_CTX(L, hipSetDevice(L->device));
+    // every context's own work is complete before the lead stream reads its buffers
+    for (int i = 1; i < n; i++) HIPCHK_CTX(L, hipStreamSynchronize(ctxs[i]->stream));
+    std::vector<BatchItem> items(n);
+    std::vector<long> size(n);
+    for (int i = 0; i < n; i++) {
+        mdn_ctx* c = ctxs[i];
+        c->cur = 0;
+        HIPCHK_CTX(L, hipMemcpyAsync(c->x[0].p, coords[i], sizeof(double) * 3 * c->n_beads, hipMemcpyHostToDevice,
+                                     L->stream));
+        items[i].s = c->sys;
+        items[i].B = cg_bufs(c);
+        items[i].st = c->cg.p;
+        items[i].report = c->eout.p;
+        size[i] = c->n_beads;
+        for (int ff = 0; ff < NFF; ff++) size[i] += c->sys.n[ff];
+    }
+    BatchParams P{};
+    P.method = p->method;
+    P.try_recover = p->try_recover;
+    P.force_call_limit = p->force_call_limit;
+    P.energy_call_limit = p->energy_call_limit;
+    P.report_on_call_limit = p->report_on_call_limit;
+    P.limit_override = 0;
+    P.max_dist = p->max_dist;
+    P.lambda_max = p->lambda_max;
+    P.grad_tol = p->grad_tol;
+    P.energy_rel_tol = p->energy_rel_tol;
+    HIPCHK_CTX(L, L->batch_items.ensure(n));
+    HIPCHK_CTX(L, L->batch_order.ensure(n));
+    HIPCHK_CTX(L, L->batch_queue.ensure(1));
+    HIPCHK_CTX(L, hipMemcpyAsync(L->batch_items.p, items.data(), sizeof(BatchItem) * n, hipMemcpyHostToDevice,
+                                 L->stream));
+    std::vector<int> order_given(n), order_sorted(n);
+    for (int i = 0; i < n; i++) order_given[i] = order_sorted[i] = i;
+    std::stable_sort(order_sorted.begin(), order_sorted.end(), [&](int a, int b) { return size[a] > size[b]; });
+    int st;
+    // schedule: workgroup size x system order, chosen once per (systems, precision, beads) by
+    // timing a short run of each candidate (force-call limit 20) on this batch
+    const long key = ((long)n * 2 + p->precision) * 1000003L + beads;
+    if (L->batch_key != key) {
+        BatchParams T = P;
+        T.limit_override = 20;
+        hipEvent_t e0, e1;
+        HIPCHK_CTX(L, hipEventCreate(&e0));
+        HIPCHK_CTX(L, hipEventCreate(&e1));
+        float best = 1e30f;
+        const int nts[2] = {256, 512};
+        for (int c = 0; c < 4; c++) {
+            const int nt = nts[c >> 1], sorted = c & 1;
+            HIPCHK_CTX(L, hipMemcpyAsync(L->batch_order.p, sorted ? order_sorted.data() : order_given.data(),
+                                         sizeof(int) * n, hipMemcpyHostToDevice, L->stream));
+            HIPCHK_CTX(L, hipEventRecord(e0, L->stream));
+            if ((st = run_batched(L, n, T, p->precision, nt))) return st;
+            HIPCHK_CTX(L, hipEventRecord(e1, L->stream));
+            HIPCHK_CTX(L, hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK_CTX(L, hipEventElapsedTime(&ms, e0, e1));
+            L->batch_tune_ms[c] = ms;
+            if (ms < best) {
+                best = ms;
+                L->batch_nt = nt;
+                L->batch_sorted = sorted;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        L->batch_key = key;
+        L->batch_tuned = 1;
+        // the tuning runs moved the coordinates: start again from the caller's
+        for (int i = 0; i < n; i++)
+            HIPCHK_CTX(L, hipMemcpyAsync(ctxs[i]->x[0].p, coords[i], sizeof(double) * 3 * ctxs[i]->n_beads,
+                                         hipMemcpyHostToDevice, L->stream));
+    } else {
+        L->batch_tuned = 0;
+    }
+    HIPCHK_CTX(L, hipMemcpyAsync(L->batch_order.p, L->batch_sorted ? order_sorted.data() : order_given.data(),
+                                 sizeof(int) * n, hipMemcpyHostToDevice, L->stream));
+    if ((st = run_batched(L, n, P, p->precision, L->batch_nt))) return st;
+    HIPCHK_CTX(L, hipStreamSynchronize(L->stream));
+    // results
+    int first_err = MDN_OK;
+    std::vector<double> rep(27);
+    for (int i = 0; i < n; i++) {
+        mdn_ctx* c = ctxs[i];
+        CgState S;
+        HIPCHK_CTX(L, hipMemcpy(&S, c->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
+        HIPCHK_CTX(L, hipMemcpy(rep.data(), c->eout.p, sizeof(double) * 27, hipMemcpyDeviceToHost));
+        c->last_cg = S;
+        c->cur = S.parity;
+        const bool ok = S.status == 0 && S.error_flags == 0;
+        fill_result(S, rep.data(), rep.data() + 9, ok, &results[i]);
+        const int nv = 3 * c->n_beads;
+        HIPCHK_CTX(L, hipMemcpy(coords[i], c->x[c->cur].p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+        if (forces_out && forces_out[i]) HIPCHK_CTX(L, hipMemcpy(forces_out[i], c->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+        int code = MDN_OK;
+        if (S.status != 0) {
+            code = S.status;
+            fail(c, code, S.status == 9 ? "an excluded-volume pair needed the numerical force fallback"
+                                        : "minimization stopped with status %d", S.status);
+        } else if (ok && rep[9 + 8] != 0.0) {
+            code = MDN_ERR_UNSUPPORTED;
+            fail(c, code, "excluded-volume pairs beyond the Simpson-energy capacity");
+        } else if (S.error_flags & MDN_MIN_ERR_LINESEARCH) {
+            code = MDN_ERR_LINESEARCH;
+            fail(c, code, "line search failed (recovery disabled)");
+        }
+        if (code != MDN_OK && first_err == MDN_OK) {
+            first_err = code;
+            if (c != L) fail(L, code, "system %d: %s", i, c->err.c_str());
+        }
+    }
+    return first_err;
+}
+
+// The schedule of the last batched minimization on a lead context: [workgroup size,
+// largest-first order (1) or given order (0), tuned in that call (1) or reused (0)], and the
+// tuning times (ms) of the four candidates (256 given, 256 sorted, 512 given, 512 sorted).
+int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms4) {
+    if (!ctx || !out3) return MDN_ERR_ARG;
+    out3[0] = ctx->batch_nt;
+    out3[1] = ctx->batch_sorted;
+    out3[2] = ctx->batch_tuned;
+    if (tune_ms4)
+        for (int c = 0; c < 4; c++) tune_ms4[c] = ctx->batch_tune_ms[c];
     return MDN_OK;
 }
 

@@ -9,7 +9,8 @@
 //                evaluate excluded volume with R = double in both paths). Absolute coordinates
 //                stay FP64; every bond vector / coordinate difference is formed in FP64 and only
 //                then rounded to FP32, so the FP32 arithmetic never sees 10^3-10^4 nm absolute
-//                positions. Distances to boundary planes are formed in FP64 for the same reason.
+//                positions. Distances to boundary planes, and the lengths and extensions of the
+//                harmonic springs, are formed in FP64 for the same reason.
 // Both instantiations return energies and forces widened to double; accumulation is FP64.
 //
 // Citations are relative to medyan-5.4.0/ in the reference tree.
@@ -42,20 +43,23 @@ template <typename R> MDN_HD inline R dot(V3<R> a, V3<R> b) { return a.x * b.x +
 template <typename R> MDN_HD inline D3 widen(V3<R> a) { return D3{(double)a.x, (double)a.y, (double)a.z}; }
 
 // ---------------- FilamentStretchingHarmonic (FilamentStretchingHarmonic.cpp:185-248) -------
+// The harmonic springs (filament stretching, linkers, motors): the length L and the extension
+// L - L0 are formed in FP64 in both paths (L0 reaches micrometres for crosslinks, where an FP32
+// L - L0 would lose the force to cancellation); the mixed path rounds the tension
+// k (L - L0) / L to FP32 and forms the force components in FP32.
 // twoPointDistance (MathFunctions.h:294-297) of r = c2 - c1
-template <typename R> MDN_HD inline R stretch_energy(D3 c1, D3 c2, double kd, double eqld) {
-    const R k = (R)kd, eql = (R)eqld;
-    V3<R> r = dif<R>(c2, c1);
-    R dist = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) - eql;
-    return (R)0.5 * k * dist * dist;
+template <typename R> MDN_HD inline R stretch_energy(D3 c1, D3 c2, double k, double eql) {
+    V3<double> r = dif<double>(c2, c1);
+    double dist = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) - eql;
+    return (R)(0.5 * k * dist * dist);
 }
 // out[0] -> bead 1, out[1] -> bead 2
-template <typename R> MDN_HD inline void stretch_forces(D3 c1, D3 c2, double kd, double eqld, D3* out) {
-    const R k = (R)kd, eql = (R)eqld;
-    V3<R> r = dif<R>(c2, c1);
-    R dist = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
-    R invL = 1 / dist;
-    R f0 = k * (dist - eql) * invL;
+template <typename R> MDN_HD inline void stretch_forces(D3 c1, D3 c2, double k, double eql, D3* out) {
+    V3<double> r64 = dif<double>(c2, c1);
+    double dist = sqrt(r64.x * r64.x + r64.y * r64.y + r64.z * r64.z);
+    double invL = 1 / dist;
+    const R f0 = (R)(k * (dist - eql) * invL);
+    const V3<R> r{(R)r64.x, (R)r64.y, (R)r64.z};
     R Fx = -f0 * r.x, Fy = -f0 * r.y, Fz = -f0 * r.z;
     out[0] = D3{(double)-Fx, (double)-Fy, (double)-Fz};
     out[1] = D3{(double)Fx, (double)Fy, (double)Fz};
@@ -122,29 +126,31 @@ MDN_HD inline D3 mid_point(D3 v1, D3 v2, double alpha) {
     return D3{(v1.x * beta + alpha * v2.x), (v1.y * beta + alpha * v2.y), (v1.z * beta + alpha * v2.z)};
 }
 template <typename R>
-MDN_HD inline R xlink_energy(D3 c1, D3 c2, D3 c3, D3 c4, double kd, double eqld, double p1, double p2) {
+MDN_HD inline R xlink_energy(D3 c1, D3 c2, D3 c3, D3 c4, double k, double eql, double p1, double p2) {
     D3 v1 = mid_point(c1, c2, p1);
     D3 v2 = mid_point(c3, c4, p2);
-    V3<R> l = dif<R>(v2, v1);
-    R dist = sqrt(l.x * l.x + l.y * l.y + l.z * l.z) - (R)eqld;
-    return (R)0.5 * (R)kd * dist * dist;
+    V3<double> l = dif<double>(v2, v1);
+    double dist = sqrt(l.x * l.x + l.y * l.y + l.z * l.z) - eql;
+    return (R)(0.5 * k * dist * dist);
 }
 template <typename R>
-MDN_HD inline double xlink_forces(D3 c1, D3 c2, D3 c3, D3 c4, double kd, double eqld, double p1d, double p2d,
+MDN_HD inline double xlink_forces(D3 c1, D3 c2, D3 c3, D3 c4, double k, double eql, double p1d, double p2d,
                                       D3* out) {
     D3 v1 = mid_point(c1, c2, p1d);
     D3 v2 = mid_point(c3, c4, p2d);
-    V3<R> l = dif<R>(v2, v1);
-    R dist = sqrt(l.x * l.x + l.y * l.y + l.z * l.z);
-    R invL = 1 / dist;
-    R f0 = (R)kd * (dist - (R)eqld) * invL;
+    V3<double> l64 = dif<double>(v2, v1);
+    double dist = sqrt(l64.x * l64.x + l64.y * l64.y + l64.z * l64.z);
+    double invL = 1 / dist;
+    double f064 = k * (dist - eql) * invL;
+    const R f0 = (R)f064;
+    const V3<R> l{(R)l64.x, (R)l64.y, (R)l64.z};
     R Flx = -f0 * l.x, Fly = -f0 * l.y, Flz = -f0 * l.z;
     const R p1 = (R)p1d, p2 = (R)p2d;
     out[0] = D3{(double)(-Flx * (1 - p1)), (double)(-Fly * (1 - p1)), (double)(-Flz * (1 - p1))};
     out[1] = D3{(double)(-Flx * (p1)), (double)(-Fly * (p1)), (double)(-Flz * (p1))};
     out[2] = D3{(double)(Flx * (1 - p2)), (double)(Fly * (1 - p2)), (double)(Flz * (1 - p2))};
     out[3] = D3{(double)(Flx * (p2)), (double)(Fly * (p2)), (double)(Flz * (p2))};
-    return (double)(f0 / invL);   // stretchforce
+    return f064 / invL;   // stretchforce
 }
 
 // ---------------- CylinderExclVolRepulsion (CylinderExclVolRepulsion.cpp:330-882) -----------

@@ -29,7 +29,7 @@ EXPORTED = [
     "mdn_system_counts", "mdn_nl_build", "mdn_nl_pairs", "mdn_boundary_list", "mdn_energy",
     "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
-    "mdn_last_min_stats", "mdn_energy_mixed",
+    "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
 ]
 
 
@@ -72,6 +72,9 @@ def lib():
         L.mdn_get_coords.argtypes = [C.c_void_p, dp]
         L.mdn_forces_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int32]
         L.mdn_energy_mixed.argtypes = [C.c_void_p, dp, C.POINTER(abi.EnergyReport)]
+        L.mdn_minimize_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(abi.CgParams),
+                                           C.POINTER(dp), C.POINTER(dp), C.POINTER(abi.MinResult)]
+        L.mdn_last_batch_schedule.argtypes = [C.c_void_p, i32p, dp]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
         _lib = L
     return _lib
@@ -217,3 +220,32 @@ class ForceFieldManager:
             self.close()
         except Exception:
             pass
+
+
+def minimize_batched(systems, params, coords, want_forces=True):
+    """CGMethod::minimize of many independent systems in one persistent kernel
+    (mdn_minimize_batched). ``systems``: ForceFieldManager objects with uploaded systems and
+    built neighbor lists (the first one is the batch's lead context); ``coords``: one start
+    vector per system. Returns (list of coordinates, list of forces or None, list of MinResult)."""
+    L = lib()
+    n = len(systems)
+    xs = [_f64(c).copy() for c in coords]
+    fs = [np.zeros_like(x) for x in xs] if want_forces else None
+    ctxs = (C.c_void_p * n)(*[s._h.value for s in systems])
+    xp = (dp * n)(*[x.ctypes.data_as(dp) for x in xs])
+    fp = (dp * n)(*[f.ctypes.data_as(dp) for f in fs]) if want_forces else None
+    res = (abi.MinResult * n)()
+    st = L.mdn_minimize_batched(ctxs, n, C.byref(params), xp, fp, res)
+    if st != 0:
+        raise MdnError(st, L.mdn_last_error(systems[0]._h).decode())
+    return xs, fs, list(res)
+
+
+def last_batch_schedule(lead):
+    """(workgroup size, largest-first order, tuned in the last call, tuning times in ms)."""
+    out = (C.c_int32 * 3)()
+    ms = (C.c_double * 4)()
+    st = lib().mdn_last_batch_schedule(lead._h, out, ms)
+    if st != 0:
+        raise MdnError(st, "mdn_last_batch_schedule")
+    return out[0], out[1], out[2], list(ms)

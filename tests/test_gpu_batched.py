@@ -1,0 +1,97 @@
+"""GPU: the persistent batched minimizer (mdn_minimize_batched) on many independent networks.
+
+Bars:
+  trajectory prefix: with an energy-call limit, every system of the batch follows the oracle's
+    CGMethod::minimize trajectory (same iteration / energy-call / force-call counts, coordinates
+    within rtol 1e-8: only the rounding order of the reductions differs), through the
+    safe-mode entries and recovery moves these dense networks start with;
+  full batch: every system stops by the reference criterion without an error; the reported
+    max|F| is the FP64 max|F| at the returned coordinates (FP64 batch: exactly; mixed batch:
+    within 1e-3 pN), the reported final energy is the FP64 energy there, and it does not exceed
+    the start energy;
+  dense networks (excluded volume, safe mode, recovery moves): the same checks.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi, mech, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(gpu, n, dense=False, first=0):
+    nets = [synth.Network.config(3, index=first + i, dense=dense) for i in range(n)]
+    ffms = []
+    for net in nets:
+        g = gpu()
+        g.upload(net.desc, net.coord)
+        g.reset_neighbor_lists(net.coord)
+        ffms.append(g)
+    return nets, ffms
+
+
+@pytest.mark.parametrize("ncalls", [10, 22])
+def test_batched_trajectory_prefix_vs_oracle(gpu, ncalls):
+    nets, ffms = _batch(gpu, 6, dense=True)
+    p = abi.default_cg_params(energy_call_limit=ncalls)
+    xs, fs, res = mech.minimize_batched(ffms, p, [n.coord for n in nets])
+    for i, net in enumerate(nets):
+        o = of.OracleSystem(net.desc)
+        o.build_nl(net.coord)
+        st, xo, fo, ro = o.minimize(p, net.coord)
+        rb = res[i]
+        print(f"system {i}: beads {net.n_beads} batched {rb.num_iter}/{rb.num_energy_call}/{rb.num_force_call} "
+              f"oracle {ro.num_iter}/{ro.num_energy_call}/{ro.num_force_call} "
+              f"max|dx| {float(np.abs(xs[i] - xo).max()):.3e}", flush=True)
+        assert (rb.num_iter, rb.num_energy_call, rb.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+        np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_batched_config3_converges(gpu, prec):
+    nets, ffms = _batch(gpu, 64)
+    p = abi.default_cg_params(precision=prec)
+    xs, fs, res = mech.minimize_batched(ffms, p, [n.coord for n in nets])
+    lead_sched = mech.last_batch_schedule(ffms[0])
+    print(f"precision {prec}: schedule {lead_sched}, iterations max {max(r.num_iter for r in res)}", flush=True)
+    bar = 0.0 if prec == abi.MDN_PREC_FP64 else 1e-3
+    for i, (g, x, f, r) in enumerate(zip(ffms, xs, fs, res)):
+        assert r.error_flags == 0 and not r.call_limit_exceeded, i
+        assert r.converged_grad or r.converged_energy, i
+        fx = g.compute_forces(x)
+        assert abs(float(np.abs(fx).max()) - r.grad_inf_norm) <= bar, i
+        assert float(np.abs(f - fx).max()) <= max(bar, 1e-9 * float(np.abs(fx).max())), i
+        assert r.energies_after.total == pytest.approx(g.compute_energy(x), rel=1e-12), i
+        assert r.energies_after.total <= r.energies_before.total * (1 + 1e-12), i
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_batched_dense_converges(gpu, prec):
+    """Dense networks (excluded-volume pairs, safe-mode entries and recovery moves at the start):
+    every system converges by the reference criterion and its result is consistent under FP64
+    re-evaluation."""
+    nets, ffms = _batch(gpu, 16, dense=True)
+    p = abi.default_cg_params(precision=prec)
+    xs, fs, res = mech.minimize_batched(ffms, p, [n.coord for n in nets])
+    bar = 0.0 if prec == abi.MDN_PREC_FP64 else 1e-3
+    print(f"precision {prec}: iterations {[r.num_iter for r in res]}, safe modes {sum(r.num_safe_mode for r in res)}",
+          flush=True)
+    for i, (g, x, r) in enumerate(zip(ffms, xs, res)):
+        assert r.error_flags == 0 and not r.call_limit_exceeded, i
+        assert r.converged_grad or r.converged_energy, i
+        fx = g.compute_forces(x)
+        assert abs(float(np.abs(fx).max()) - r.grad_inf_norm) <= bar + 1e-12 * float(np.abs(fx).max()), i
+        assert r.energies_after.total == pytest.approx(g.compute_energy(x), rel=1e-12), i
+        assert r.energies_after.total < r.energies_before.total, i
+
+
+def test_batched_rejects_bad_input(gpu):
+    nets, ffms = _batch(gpu, 2)
+    p = abi.default_cg_params()
+    with pytest.raises(mech.MdnError):
+        mech.minimize_batched([ffms[0], ffms[0]], p, [nets[0].coord, nets[0].coord])   # repeated context
+    g = gpu()
+    g.upload(nets[1].desc, nets[1].coord)   # no neighbor lists yet
+    with pytest.raises(mech.MdnError):
+        mech.minimize_batched([ffms[0], g], p, [nets[0].coord, nets[1].coord])

@@ -106,3 +106,27 @@ def test_minimize_config0(gpu):
     e_at_xg, _ = o.energy(xg)
     assert rg.energies_after.total == pytest.approx(e_at_xg, rel=1e-12)
     assert 0.5 * ro.num_iter <= rg.num_iter <= 2.0 * ro.num_iter
+
+
+@pytest.mark.parametrize("ncalls", [13, 22, 40])
+def test_minimize_prefix_through_safe_mode(gpu, ncalls):
+    """A dense network whose first iterations fail the line search: safe mode, the recovery
+    move, safe mode again. The next backtracking search after a safe-mode entry starts again
+    from lambda0 = min(lambdaMax, maxDist / maxF) (CGMethod.hpp:258-259), not from the halved
+    lambda of the failed search; the counts and coordinates follow the oracle through it."""
+    net = synth.Network.config(3, index=4, dense=True)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    o = of.OracleSystem(net.desc)
+    o.build_nl(net.coord)
+    p = abi.default_cg_params(energy_call_limit=ncalls)
+    xg, fg, rg = g.minimize(p, net.coord)
+    st, xo, fo, ro = o.minimize(p, net.coord)
+    print(ncalls, "gpu", rg.num_iter, rg.num_energy_call, rg.num_force_call, rg.num_safe_mode, rg.num_recovery,
+          "oracle", ro.num_iter, ro.num_energy_call, ro.num_force_call, ro.num_safe_mode, ro.num_recovery,
+          "max|dx|", float(np.abs(xg - xo).max()), flush=True)
+    assert ro.num_safe_mode >= 3 and ro.num_recovery >= 2
+    assert (rg.num_iter, rg.num_energy_call, rg.num_force_call, rg.num_safe_mode, rg.num_recovery) == \
+        (ro.num_iter, ro.num_energy_call, ro.num_force_call, ro.num_safe_mode, ro.num_recovery)
+    np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
