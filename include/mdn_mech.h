@@ -254,6 +254,29 @@ int mdn_get_coords(mdn_ctx* ctx, double* coord);
 int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
 int mdn_energy_mixed(mdn_ctx* ctx, const double* coord, mdn_energy_report* out);
 
+/* ---- differentiation ---------------------------------------------------------------------- */
+
+/* dE/dtheta at coord (null: the current device coordinates) in FP64, for the per-cylinder
+ * parameters theta = (stretching k [n_cyl], rest length L0 [n_cyl], bending k [n_cyl]); out
+ * holds 3 * n_cylinders values in that order. L0 enters the stretching terms
+ * (FilamentStretchingHarmonic.cpp:185-199) and kRepScaled = krep L0_i L0_j of every excluded-
+ * volume pair with the cylinder (CylinderExclVolume.cpp:73-138); a cylinder's bending k is the k
+ * of the bending term whose second cylinder it is (FilamentBending.cpp:29-60; 0 when there is
+ * none). No reference counterpart (the reference has no parameter derivatives). */
+int mdn_param_grad(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out);
+
+/* Hessian-vector product H v = -(dF/dx) v at coord (null: the current device coordinates) in
+ * FP64, by forward-mode dual arithmetic through the same force kernels (the derivative of the
+ * implemented forces, including the coplanar shift of CylinderExclVolRepulsion.cpp:610-630).
+ * The reference computes a dense finite-difference Hessian instead (ForceFieldManager.cpp:
+ * 354-500). MDN_ERR_UNSUPPORTED when an excluded-volume pair needs the numerical force fallback. */
+int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var);
+
+/* Replace the per-cylinder parameters (any pointer may be null: unchanged): stretching k and
+ * rest length L0 (the excluded-volume kRepScaled of the current neighbor list is rebuilt) per
+ * cylinder, bending k per cylinder (applied to the bending term whose second cylinder it is). */
+int mdn_set_params(mdn_ctx* ctx, const double* kstr, const double* eql, const double* kbend);
+
 /* ---- batched minimization ------------------------------------------------------------- */
 
 /* CGMethod::minimize (CGMethod.hpp:1746-1759) of n independent systems in one persistent kernel:

@@ -1036,6 +1036,133 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather_fin(DevSys s, CgState* __re
 }
 
 // =======================================================================================
+// Tangent (Hessian-vector product) and parameter-gradient kernels
+// =======================================================================================
+__device__ inline DualD3 pos_dual(const double* __restrict__ x, const double* __restrict__ v, int b) {
+    return DualD3{ld3(x, b), ld3(v, b)};
+}
+
+// d/de of interaction t's forces at x + e v (forward-mode dual arithmetic, FP64) into the
+// pieces / slots; their gather is (dF/dx) v = -H v. *fbad: an excluded-volume pair whose
+// analytic force is non-finite (the reference would switch to forceN, which has no tangent here).
+__device__ inline void term_tangent(const DevSys& s, int ff, int t, const double* __restrict__ x,
+                                    const double* __restrict__ v, double* __restrict__ slots, int* fbad) {
+    DualD3 out[4];
+    switch (ff) {
+    case FF_STRETCH: {
+        const int b1 = s.str_b[2 * t], b2 = s.str_b[2 * t + 1];
+        stretch_forces<DualD>(pos_dual(x, v, b1), pos_dual(x, v, b2), s.str_k[t], s.str_eql[t], out);
+        put_piece<double>(s, 1, b1, out[0].d);
+        put_piece<double>(s, 0, b2, out[1].d);
+        return;
+    }
+    case FF_BEND: {
+        const int b1 = s.bend_b[3 * t], b2 = s.bend_b[3 * t + 1], b3 = s.bend_b[3 * t + 2];
+        bend_forces<DualD>(pos_dual(x, v, b1), pos_dual(x, v, b2), pos_dual(x, v, b3), s.bend_k[t], s.bend_eqt[t], out);
+        put_piece<double>(s, 4, b1, out[0].d);
+        put_piece<double>(s, 3, b2, out[1].d);
+        put_piece<double>(s, 2, b3, out[2].d);
+        return;
+    }
+    case FF_LINKER:
+    case FF_MOTOR: {
+        const bool lk = ff == FF_LINKER;
+        const int* b = lk ? s.lnk_b : s.mot_b;
+        const double k = lk ? s.lnk_k[t] : s.mot_k[t], eql = lk ? s.lnk_eql[t] : s.mot_eql[t];
+        const double p1 = lk ? s.lnk_p1[t] : s.mot_p1[t], p2 = lk ? s.lnk_p2[t] : s.mot_p2[t];
+        (void)xlink_forces<DualD>(pos_dual(x, v, b[4 * t]), pos_dual(x, v, b[4 * t + 1]), pos_dual(x, v, b[4 * t + 2]),
+                                  pos_dual(x, v, b[4 * t + 3]), k, eql, p1, p2, out);
+        const long id = s.slot0[ff] + 4L * t;
+        for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q].d);
+        return;
+    }
+    case FF_EXVOL: {
+        const bool ok = exvol_forces<DualD>(pos_dual(x, v, s.ev_b[4 * t]), pos_dual(x, v, s.ev_b[4 * t + 1]),
+                                            pos_dual(x, v, s.ev_b[4 * t + 2]), pos_dual(x, v, s.ev_b[4 * t + 3]),
+                                            s.ev_ks[t], out);
+        const long id = s.slot0[FF_EXVOL] + 4L * t;
+        for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, ok ? out[q].d : D3{0.0, 0.0, 0.0});
+        if (!ok) *fbad = 1;
+        return;
+    }
+    default: {
+        const BElem& e = s.elem[s.bnd_e[t]];
+        put_slot(s, slots, s.slot0[FF_BOUNDARY] + t,
+                 boundary_force_dual(e, pos_dual(x, v, s.bnd_b[t]), s.bnd_krep[t], s.bnd_slen[t]).d);
+        return;
+    }
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_tangent_terms(DevSys s, const double* __restrict__ x,
+                                                         const double* __restrict__ v, double* __restrict__ slots,
+                                                         int* bad_count) {
+    const int ff = ff_of_block(s, blockIdx.x);
+    if (ff >= NFF) return;
+    const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+    if (t >= s.n[ff]) return;
+    int fbad = 0;
+    term_tangent(s, ff, t, x, v, slots, &fbad);
+    if (fbad) atomicAdd(bad_count, 1);
+}
+
+// Per-pair excluded-volume energies (closed form, FP64); NaN marks a pair whose closed form is
+// non-finite or < -1 (the reference then integrates numerically: k_ev_pair_energyN).
+__global__ void k_ev_pair_energy(DevSys s, const double* __restrict__ x, double* __restrict__ U) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= s.n[FF_EXVOL]) return;
+    int deferred = 0;
+    const double u = exvol_energy<double>(ld3(x, s.ev_b[4 * t]), ld3(x, s.ev_b[4 * t + 1]), ld3(x, s.ev_b[4 * t + 2]),
+                                          ld3(x, s.ev_b[4 * t + 3]), s.ev_ks[t], &deferred);
+    U[t] = deferred ? NAN : u;
+}
+// The Simpson energy (energyN) of pair t into U[t] (-1 when it stays non-finite). One block.
+__global__ void __launch_bounds__(BLOCK) k_ev_pair_energyN(DevSys s, const double* __restrict__ x, int t,
+                                                           double* __restrict__ U) {
+    __shared__ double lds[BLOCK / 64];
+    const D3 c1 = ld3(x, s.ev_b[4 * t]), c2 = ld3(x, s.ev_b[4 * t + 1]), c3 = ld3(x, s.ev_b[4 * t + 2]),
+             c4 = ld3(x, s.ev_b[4 * t + 3]);
+    double r = exvol_energyN_share(c1, c2, c3, c4, threadIdx.x, BLOCK);
+    const int op1[1] = {0};
+    block_reduce_n<1>(&r, op1, lds);
+    if (threadIdx.x == 0) {
+        const double u = exvol_energyN_finish(c1, c2, c3, c4, s.ev_ks[t], r);
+        U[t] = energy_bad(u) ? -1.0 : u;
+    }
+}
+
+// dE/dtheta per cylinder c, theta = (stretching k, rest length L0, bending k):
+//   stretching term c (FilamentStretchingHarmonic.cpp:185-199): dE/dk = (L - L0)^2 / 2,
+//     dE/dL0 = -k (L - L0);
+//   excluded volume (kRepScaled = krep L0_i L0_j, CylinderExclVolume.cpp:73-138): every pair with
+//     cylinder c adds U / L0_c to dE/dL0_c; those pairs are the ones in the incidence list of c's
+//     first bead with the bead as the pair's bead 1 or bead 3 (ascending pair order);
+//   bending term whose second cylinder is c (its k, FilamentBending.cpp:29-60): dE/dk = U / k.
+// out[c], out[n_cyl + c], out[2 n_cyl + c].
+__global__ void k_param_grad(DevSys s, int n_cyl, const int* __restrict__ cyl_b1, const int* __restrict__ cyl_bend,
+                             const double* __restrict__ x, const double* __restrict__ U, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cyl) return;
+    const V3<double> r = dif<double>(ld3(x, s.str_b[2 * c + 1]), ld3(x, s.str_b[2 * c]));
+    const double dl = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) - s.str_eql[c];
+    out[c] = 0.5 * dl * dl;
+    double gl = -s.str_k[c] * dl;
+    const int b = cyl_b1[c];
+    const long e0 = s.slot0[FF_EXVOL], e1 = s.slot0[FF_EXVOL + 1];
+    for (int q = s.bstart[b]; q < s.bstart[b + 1]; q++) {
+        const long id = s.sorted_id[q];
+        if (id < e0 || id >= e1) continue;
+        const long loc = id - e0;
+        if ((loc & 1) == 0) gl += U[loc >> 2] / s.str_eql[c];   // bead 1 or bead 3 of the pair
+    }
+    out[n_cyl + c] = gl;
+    const int tb = cyl_bend[c];
+    out[2 * n_cyl + c] = tb >= 0 ? bend_energy<double>(ld3(x, s.bend_b[3 * tb]), ld3(x, s.bend_b[3 * tb + 1]),
+                                                       ld3(x, s.bend_b[3 * tb + 2]), 1.0, s.bend_eqt[tb])
+                                 : 0.0;
+}
+
+// =======================================================================================
 // Neighbor lists
 // =======================================================================================
 struct NLDev {

@@ -73,6 +73,11 @@ struct mdn_ctx {
     DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
     DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
     DBuf<unsigned char> pmask;
+    // parameter gradients / tangents: per cylinder the bending term it is the second cylinder of
+    // (-1: none), per bending term its cylinder (host), per-pair excluded-volume energies, a vector
+    DBuf<int> d_cyl_bend;
+    std::vector<int> bend_cyl;
+    DBuf<double> ev_U, grad_out, tan_v;
     DBuf<double> piece;   // NBOND * 3 * n_beads values (FP64; the mixed path uses the first half as float)
     // cylinder tables for the NL
     DBuf<int> d_cyl_b1, d_cyl_b2, d_cyl_fil, d_cyl_pos, d_cyl_type;
@@ -408,6 +413,10 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     for (auto* b : db) b->release();
     ctx->d_cyl_minus.release();
     ctx->pmask.release();
+    ctx->d_cyl_bend.release();
+    ctx->ev_U.release();
+    ctx->grad_out.release();
+    ctx->tan_v.release();
     ctx->piece.release();
     ctx->cub_tmp.release();
     ctx->slots.release();
@@ -467,6 +476,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     int n_bend = n_cyl - d->n_filaments;
     std::vector<int> bend_b(3 * (size_t)std::max(0, n_bend));
     std::vector<double> bend_k(std::max(0, n_bend)), bend_eqt(std::max(0, n_bend));
+    std::vector<int> cyl_bend(std::max(1, n_cyl), -1);
+    ctx->bend_cyl.assign(std::max(0, n_bend), 0);
     {
         int i = 0;
         c = 0;
@@ -479,6 +490,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
                 bend_b[3 * i + 2] = ctx->cyl_b2[cc];
                 bend_k[i] = d->cyl_kbend[cc];
                 bend_eqt[i] = d->cyl_eqtheta[cc];
+                ctx->bend_cyl[i] = cc;
+                cyl_bend[cc] = i;
             }
             c += nb - 1;
         }
@@ -562,6 +575,7 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     HIPCHK(ctx->mot_sf.ensure(std::max<size_t>(1, xk[1].size())));
     HIPCHK(upload(ctx, ctx->inc_s_start, cnt)); HIPCHK(upload(ctx, ctx->inc_s, inc));
     HIPCHK(upload(ctx, ctx->pmask, pmask));
+    HIPCHK(upload(ctx, ctx->d_cyl_bend, cyl_bend));
     HIPCHK(ctx->piece.ensure(NBOND * 3 * (size_t)ctx->n_beads));
     // cylinder tables for the neighbor lists
     HIPCHK(upload(ctx, ctx->d_cyl_b1, ctx->cyl_b1)); HIPCHK(upload(ctx, ctx->d_cyl_b2, ctx->cyl_b2));
@@ -837,6 +851,103 @@ int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor
         HIPCHK(hipMemcpy(linker_out, ctx->lnk_sf.p, sizeof(double) * ctx->sys.n[FF_LINKER], hipMemcpyDeviceToHost));
     if (motor_out && ctx->sys.n[FF_MOTOR])
         HIPCHK(hipMemcpy(motor_out, ctx->mot_sf.p, sizeof(double) * ctx->sys.n[FF_MOTOR], hipMemcpyDeviceToHost));
+    return MDN_OK;
+}
+
+int mdn_param_grad(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!out || n_out != 3L * ctx->n_cyl) return fail(ctx, MDN_ERR_ARG, "n_out must be 3*n_cylinders = %d", 3 * ctx->n_cyl);
+    HIPCHK(hipSetDevice(ctx->device));
+    const DevSys& s = ctx->sys;
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * 3 * ctx->n_beads, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    const int np = s.n[FF_EXVOL];
+    HIPCHK(ctx->ev_U.ensure(std::max(1, np)));
+    HIPCHK(ctx->grad_out.ensure(3 * (size_t)std::max(1, ctx->n_cyl)));
+    if (np > 0) {
+        hipLaunchKernelGGL(k_ev_pair_energy, dim3(ceil_div(np, BLOCK)), dim3(BLOCK), 0, ctx->stream, s, x, ctx->ev_U.p);
+        HIPCHK(hipGetLastError());
+        std::vector<double> U(np);
+        HIPCHK(hipMemcpyAsync(U.data(), ctx->ev_U.p, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::vector<int> redo;
+        for (int t = 0; t < np; t++)
+            if (U[t] != U[t]) redo.push_back(t);
+        for (int t : redo)
+            hipLaunchKernelGGL(k_ev_pair_energyN, dim3(1), dim3(BLOCK), 0, ctx->stream, s, x, t, ctx->ev_U.p);
+        if (!redo.empty()) {
+            for (int t : redo)
+                HIPCHK(hipMemcpyAsync(&U[t], ctx->ev_U.p + t, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            for (int t : redo)
+                if (U[t] <= -1.0)
+                    return fail(ctx, MDN_ERR_NONFINITE, "excluded-volume pair %d: energy non-finite after the Simpson fallback", t);
+        }
+    }
+    if (ctx->n_cyl > 0)
+        hipLaunchKernelGGL(k_param_grad, dim3(ceil_div(ctx->n_cyl, BLOCK)), dim3(BLOCK), 0, ctx->stream, s, ctx->n_cyl,
+                           (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x, (const double*)ctx->ev_U.p,
+                           ctx->grad_out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, ctx->grad_out.p, sizeof(double) * 3 * ctx->n_cyl, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!v || !out || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    HIPCHK(hipSetDevice(ctx->device));
+    const DevSys& s = ctx->sys;
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    HIPCHK(ctx->tan_v.ensure(n_var));
+    HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+    if (s.blk[NFF] > 0)
+        hipLaunchKernelGGL(k_tangent_terms, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
+                           (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
+    HIPCHK(hipGetLastError());
+    if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64))) return st;
+    HIPCHK(hipMemcpyAsync(out, ctx->scratch_f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_int[0] != 0)
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "%d excluded-volume pairs need the numerical force fallback (no tangent)",
+                    ctx->h_int[0]);
+    for (int i = 0; i < n_var; i++) out[i] = -out[i];   // H v = -(dF/dx) v
+    return MDN_OK;
+}
+
+int mdn_set_params(mdn_ctx* ctx, const double* kstr, const double* eql, const double* kbend) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int nc = ctx->n_cyl;
+    if (kstr) HIPCHK(hipMemcpyAsync(ctx->str_k.p, kstr, sizeof(double) * nc, hipMemcpyHostToDevice, ctx->stream));
+    if (eql) {
+        for (int c = 0; c < nc; c++) ctx->cyl_eql[c] = eql[c];
+        HIPCHK(upload(ctx, ctx->str_eql, ctx->cyl_eql));
+        HIPCHK(upload(ctx, ctx->d_cyl_eql, ctx->cyl_eql));
+        if (ctx->nl_built && ctx->n_pairs > 0)   // kRepScaled = krep L0_i L0_j
+            hipLaunchKernelGGL(k_ev_vectorize, dim3(ceil_div(ctx->n_pairs, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                               nl_view(ctx), ctx->n_pairs, (const int*)ctx->pairs.p, ctx->ev_b.p, ctx->ev_ks.p);
+    }
+    if (kbend) {
+        std::vector<double> bk(ctx->bend_cyl.size());
+        for (size_t i = 0; i < bk.size(); i++) bk[i] = kbend[ctx->bend_cyl[i]];
+        HIPCHK(upload(ctx, ctx->bend_k, bk));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
 }
 

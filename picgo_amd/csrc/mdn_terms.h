@@ -42,6 +42,61 @@ template <typename R> MDN_HD inline V3<R> dif(D3 a, D3 b) {
 template <typename R> MDN_HD inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <typename R> MDN_HD inline D3 widen(V3<R> a) { return D3{(double)a.x, (double)a.y, (double)a.z}; }
 
+// ---- forward-mode dual numbers (Hessian-vector products) -------------------------------
+// Dual<T> carries a value and its directional derivative; the term functions instantiated with
+// R = Dual<double> on dual positions (x, v) return the forces and their derivative along v.
+// Comparisons (branches such as clamps and the coplanar test) use the value.
+template <typename T> struct Dual {
+    T v, d;
+    MDN_HD Dual() : v(0), d(0) {}
+    MDN_HD Dual(T x) : v(x), d(0) {}
+    MDN_HD Dual(T x, T dx) : v(x), d(dx) {}
+    MDN_HD friend Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+    MDN_HD friend Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+    MDN_HD friend Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+    MDN_HD friend Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+    MDN_HD friend Dual operator/(Dual a, Dual b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+    MDN_HD friend Dual operator+(Dual a, T b) { return Dual(a.v + b, a.d); }
+    MDN_HD friend Dual operator+(T a, Dual b) { return Dual(a + b.v, b.d); }
+    MDN_HD friend Dual operator-(Dual a, T b) { return Dual(a.v - b, a.d); }
+    MDN_HD friend Dual operator-(T a, Dual b) { return Dual(a - b.v, -b.d); }
+    MDN_HD friend Dual operator*(Dual a, T b) { return Dual(a.v * b, a.d * b); }
+    MDN_HD friend Dual operator*(T a, Dual b) { return Dual(a * b.v, a * b.d); }
+    MDN_HD friend Dual operator/(Dual a, T b) { return Dual(a.v / b, a.d / b); }
+    MDN_HD friend Dual operator/(T a, Dual b) { return Dual(a / b.v, -a * b.d / (b.v * b.v)); }
+    MDN_HD friend bool operator<(Dual a, Dual b) { return a.v < b.v; }
+    MDN_HD friend bool operator>(Dual a, Dual b) { return a.v > b.v; }
+    MDN_HD friend bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+    MDN_HD friend bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+    MDN_HD friend bool operator==(Dual a, Dual b) { return a.v == b.v; }
+    MDN_HD friend bool operator!=(Dual a, Dual b) { return a.v != b.v; }
+    MDN_HD friend Dual sqrt(Dual a) {
+        const T r = sqrt(a.v);
+        return Dual(r, a.d / (2 * r));
+    }
+    MDN_HD friend Dual atan(Dual a) { return Dual(atan(a.v), a.d / (1 + a.v * a.v)); }
+    MDN_HD friend Dual exp(Dual a) {
+        const T e = exp(a.v);
+        return Dual(e, a.d * e);
+    }
+    MDN_HD friend Dual fabs(Dual a) { return a.v < 0 ? -a : a; }
+};
+typedef Dual<double> DualD;
+
+// a position and its tangent
+struct DualD3 { D3 v, d; };
+template <typename R> MDN_HD inline V3<R> dif(DualD3 a, DualD3 b) {
+    return V3<R>{R(a.v.x - b.v.x, a.d.x - b.d.x), R(a.v.y - b.v.y, a.d.y - b.d.y), R(a.v.z - b.v.z, a.d.z - b.d.z)};
+}
+// the FP64 companion type of an arithmetic type (spring lengths, boundary distances)
+template <typename R> struct Wide { typedef double T; };
+template <> struct Wide<DualD> { typedef DualD T; };
+
+// force outputs: D3 for the value paths, DualD3 (value + derivative) for the tangent path
+MDN_HD inline void put3(D3& o, double x, double y, double z) { o = D3{x, y, z}; }
+MDN_HD inline void put3(D3& o, float x, float y, float z) { o = D3{(double)x, (double)y, (double)z}; }
+MDN_HD inline void put3(DualD3& o, DualD x, DualD y, DualD z) { o = DualD3{D3{x.v, y.v, z.v}, D3{x.d, y.d, z.d}}; }
+
 // ---------------- FilamentStretchingHarmonic (FilamentStretchingHarmonic.cpp:185-248) -------
 // The harmonic springs (filament stretching, linkers, motors): the length L and the extension
 // L - L0 are formed in FP64 in both paths (L0 reaches micrometres for crosslinks, where an FP32
@@ -54,15 +109,17 @@ template <typename R> MDN_HD inline R stretch_energy(D3 c1, D3 c2, double k, dou
     return (R)(0.5 * k * dist * dist);
 }
 // out[0] -> bead 1, out[1] -> bead 2
-template <typename R> MDN_HD inline void stretch_forces(D3 c1, D3 c2, double k, double eql, D3* out) {
-    V3<double> r64 = dif<double>(c2, c1);
-    double dist = sqrt(r64.x * r64.x + r64.y * r64.y + r64.z * r64.z);
-    double invL = 1 / dist;
+template <typename R, typename P, typename O>
+MDN_HD inline void stretch_forces(P c1, P c2, double k, double eql, O* out) {
+    typedef typename Wide<R>::T W;
+    V3<W> r64 = dif<W>(c2, c1);
+    W dist = sqrt(r64.x * r64.x + r64.y * r64.y + r64.z * r64.z);
+    W invL = 1 / dist;
     const R f0 = (R)(k * (dist - eql) * invL);
     const V3<R> r{(R)r64.x, (R)r64.y, (R)r64.z};
     R Fx = -f0 * r.x, Fy = -f0 * r.y, Fz = -f0 * r.z;
-    out[0] = D3{(double)-Fx, (double)-Fy, (double)-Fz};
-    out[1] = D3{(double)Fx, (double)Fy, (double)Fz};
+    put3(out[0], -Fx, -Fy, -Fz);
+    put3(out[1], Fx, Fy, Fz);
 }
 
 // ---------------- FilamentBendingCosine (FilamentBendingCosine.cpp:199-345) -----------------
@@ -85,7 +142,8 @@ template <typename R> MDN_HD inline R bend_energy(D3 c1, D3 c2, D3 c3, double kd
     R cosAminusB = cosA * cos(eqt) + sinA * sin(eqt);
     return kbend * (1 - cosAminusB);
 }
-template <typename R> MDN_HD inline void bend_forces(D3 c1, D3 c2, D3 c3, double kd, double eqtd, D3* out) {
+template <typename R, typename P, typename O>
+MDN_HD inline void bend_forces(P c1, P c2, P c3, double kd, double eqtd, O* out) {
     const R kbend = (R)kd;
     V3<R> r1 = dif<R>(c2, c1), r2 = dif<R>(c3, c2);
     R L1sq = dot(r1, r1);
@@ -99,13 +157,13 @@ template <typename R> MDN_HD inline void bend_forces(D3 c1, D3 c2, D3 c3, double
     if (are_equal_zero(eqtd)) {
         k = kbend;
     } else {
-        const R eqt = (R)eqtd;
+        const R ce = (R)cos(eqtd), se = (R)sin(eqtd);   // constants of the term
         if (fabs(fabs(x) - (R)1.0) < (R)0.001) x = (R)0.999 * x;
         if (x < (R)-1.0) x = (R)-1.0;
         else if (x > (R)1.0) x = (R)1.0;
         R cosp = x;
         R sinp = sqrt(rmax((1 - cosp * cosp), (R)0.0));
-        R sinpminusq = sinp * cos(eqt) - cosp * sin(eqt);
+        R sinpminusq = sinp * ce - cosp * se;
         k = kbend * sinpminusq / sinp;
     }
     R Fr1x = k * (r2.x * A - r1.x * B);
@@ -114,9 +172,9 @@ template <typename R> MDN_HD inline void bend_forces(D3 c1, D3 c2, D3 c3, double
     R Fr2x = k * (r1.x * A - r2.x * C);
     R Fr2y = k * (r1.y * A - r2.y * C);
     R Fr2z = k * (r1.z * A - r2.z * C);
-    out[0] = D3{(double)-Fr1x, (double)-Fr1y, (double)-Fr1z};
-    out[1] = D3{(double)(Fr1x - Fr2x), (double)(Fr1y - Fr2y), (double)(Fr1z - Fr2z)};
-    out[2] = D3{(double)Fr2x, (double)Fr2y, (double)Fr2z};
+    put3(out[0], -Fr1x, -Fr1y, -Fr1z);
+    put3(out[1], Fr1x - Fr2x, Fr1y - Fr2y, Fr1z - Fr2z);
+    put3(out[2], Fr2x, Fr2y, Fr2z);
 }
 
 // ---------------- Linker / MotorGhost stretching (LinkerStretchingHarmonic.cpp:191-291) -----
@@ -124,6 +182,9 @@ template <typename R> MDN_HD inline void bend_forces(D3 c1, D3 c2, D3 c3, double
 MDN_HD inline D3 mid_point(D3 v1, D3 v2, double alpha) {
     double beta = 1 - alpha;
     return D3{(v1.x * beta + alpha * v2.x), (v1.y * beta + alpha * v2.y), (v1.z * beta + alpha * v2.z)};
+}
+MDN_HD inline DualD3 mid_point(DualD3 v1, DualD3 v2, double alpha) {
+    return DualD3{mid_point(v1.v, v2.v, alpha), mid_point(v1.d, v2.d, alpha)};
 }
 template <typename R>
 MDN_HD inline R xlink_energy(D3 c1, D3 c2, D3 c3, D3 c4, double k, double eql, double p1, double p2) {
@@ -133,23 +194,24 @@ MDN_HD inline R xlink_energy(D3 c1, D3 c2, D3 c3, D3 c4, double k, double eql, d
     double dist = sqrt(l.x * l.x + l.y * l.y + l.z * l.z) - eql;
     return (R)(0.5 * k * dist * dist);
 }
-template <typename R>
-MDN_HD inline double xlink_forces(D3 c1, D3 c2, D3 c3, D3 c4, double k, double eql, double p1d, double p2d,
-                                      D3* out) {
-    D3 v1 = mid_point(c1, c2, p1d);
-    D3 v2 = mid_point(c3, c4, p2d);
-    V3<double> l64 = dif<double>(v2, v1);
-    double dist = sqrt(l64.x * l64.x + l64.y * l64.y + l64.z * l64.z);
-    double invL = 1 / dist;
-    double f064 = k * (dist - eql) * invL;
+template <typename R, typename P, typename O>
+MDN_HD inline typename Wide<R>::T xlink_forces(P c1, P c2, P c3, P c4, double k, double eql, double p1d, double p2d,
+                                               O* out) {
+    typedef typename Wide<R>::T W;
+    P v1 = mid_point(c1, c2, p1d);
+    P v2 = mid_point(c3, c4, p2d);
+    V3<W> l64 = dif<W>(v2, v1);
+    W dist = sqrt(l64.x * l64.x + l64.y * l64.y + l64.z * l64.z);
+    W invL = 1 / dist;
+    W f064 = k * (dist - eql) * invL;
     const R f0 = (R)f064;
     const V3<R> l{(R)l64.x, (R)l64.y, (R)l64.z};
     R Flx = -f0 * l.x, Fly = -f0 * l.y, Flz = -f0 * l.z;
     const R p1 = (R)p1d, p2 = (R)p2d;
-    out[0] = D3{(double)(-Flx * (1 - p1)), (double)(-Fly * (1 - p1)), (double)(-Flz * (1 - p1))};
-    out[1] = D3{(double)(-Flx * (p1)), (double)(-Fly * (p1)), (double)(-Flz * (p1))};
-    out[2] = D3{(double)(Flx * (1 - p2)), (double)(Fly * (1 - p2)), (double)(Flz * (1 - p2))};
-    out[3] = D3{(double)(Flx * (p2)), (double)(Fly * (p2)), (double)(Flz * (p2))};
+    put3(out[0], -Flx * (1 - p1), -Fly * (1 - p1), -Flz * (1 - p1));
+    put3(out[1], -Flx * (p1), -Fly * (p1), -Flz * (p1));
+    put3(out[2], Flx * (1 - p2), Fly * (1 - p2), Flz * (1 - p2));
+    put3(out[3], Flx * (p2), Fly * (p2), Flz * (p2));
     return f064 / invL;   // stretchforce
 }
 
@@ -187,6 +249,21 @@ MDN_HD inline D3 move_out_of_plane(D3 p1, D3 p2, D3 p3, int i, double d) {
     return D3{(p.x + n[0] * d), (p.y + n[1] * d), (p.z + n[2] * d)};
 }
 
+// the coplanar test on the values; the shift of bead 2 with its derivative
+MDN_HD inline bool are_in_plane(DualD3 p1, DualD3 p2, DualD3 p3, DualD3 p4) {
+    return are_in_plane(p1.v, p2.v, p3.v, p4.v);
+}
+MDN_HD inline DualD3 move_out_of_plane(DualD3 p1, DualD3 p2, DualD3 p3, int i, double d) {
+    const V3<DualD> v1 = dif<DualD>(p2, p1), v2 = dif<DualD>(p3, p1);
+    V3<DualD> n{v1.y * v2.z - v1.z * v2.y, v1.z * v2.x - v1.x * v2.z, v1.x * v2.y - v1.y * v2.x};
+    const DualD norm = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    n = V3<DualD>{n.x / norm, n.y / norm, n.z / norm};
+    const DualD3 p = (i == 1) ? p1 : p2;
+    const DualD px = DualD(p.v.x, p.d.x) + n.x * d, py = DualD(p.v.y, p.d.y) + n.y * d,
+                pz = DualD(p.v.z, p.d.z) + n.z * d;
+    return DualD3{D3{px.v, py.v, pz.v}, D3{px.d, py.d, pz.d}};
+}
+
 template <typename R> struct ExvTerms {
     R a, b, c, d, e, F, g, h, I;
     R AA, BB, CC, DD, EE, FF, GG, HH, JJ;
@@ -196,7 +273,7 @@ template <typename R> struct ExvTerms {
 
 // The pair invariants (CylinderExclVolRepulsion.cpp:353-397) from five FP64 difference vectors:
 // scalarProduct(p1,p2,p3,p4) = (p2 - p1).(p4 - p3), c1 - c2 = -(c2 - c1) exactly, etc.
-template <typename R> MDN_HD inline void exvol_terms(D3 c1, D3 c2, D3 c3, D3 c4, ExvTerms<R>& t) {
+template <typename R, typename P> MDN_HD inline void exvol_terms(P c1, P c2, P c3, P c4, ExvTerms<R>& t) {
     const V3<R> u1 = dif<R>(c2, c1), u2 = dif<R>(c4, c3), u3 = dif<R>(c1, c3);
     const V3<R> w14 = dif<R>(c4, c1), w32 = dif<R>(c2, c3);
     t.u1 = u1; t.u2 = u2; t.u3 = u3;
@@ -313,8 +390,8 @@ template <typename R> MDN_HD inline bool bad_component(R v) {
 // Analytic pair forces (CylinderExclVolRepulsion.cpp:610-858) on the 4 beads, from the
 // invariants of the (possibly shifted) geometry. Returns false when a component is non-finite or
 // > 1e15, in which case the reference switches to the 10^4 x 10^4 Simpson forceN.
-template <typename R>
-MDN_HD inline bool exvol_forces_from(const ExvTerms<R>& t, R kRepScaled, D3* out) {
+template <typename R, typename O>
+MDN_HD inline bool exvol_forces_from(const ExvTerms<R>& t, R kRepScaled, O* out) {
     const R a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
     const R AA = t.AA, BB = t.BB, CC = t.CC, DD = t.DD, EE = t.EE, FF = t.FF, GG = t.GG, HH = t.HH, JJ = t.JJ;
     const R ATG1 = t.ATG1, ATG2 = t.ATG2, ATG3 = t.ATG3, ATG4 = t.ATG4;
@@ -403,15 +480,15 @@ MDN_HD inline bool exvol_forces_from(const ExvTerms<R>& t, R kRepScaled, D3* out
             f[k] = -(dv[bi][0] * u1[k] + dv[bi][1] * u2[k] + dv[bi][2] * u3[k]);
             ok = ok && !bad_component<R>(f[k]);
         }
-        out[bi] = D3{(double)f[0], (double)f[1], (double)f[2]};
+        put3(out[bi], f[0], f[1], f[2]);
     }
     return ok;
 }
 
 // Forces only: the coplanar shift of c2 (CylinderExclVolRepulsion.cpp:610-630) applies.
-template <typename R>
-MDN_HD inline bool exvol_forces(D3 c1, D3 c2in, D3 c3, D3 c4, double kRepScaled, D3* out) {
-    D3 c2 = c2in;
+template <typename R, typename P, typename O>
+MDN_HD inline bool exvol_forces(P c1, P c2in, P c3, P c4, double kRepScaled, O* out) {
+    P c2 = c2in;
     if (are_in_plane(c1, c2, c3, c4)) c2 = move_out_of_plane(c1, c2, c3, 2, 0.01);
     ExvTerms<R> t;
     exvol_terms<R>(c1, c2, c3, c4, t);
@@ -474,6 +551,33 @@ template <typename R> MDN_HD inline D3 boundary_force(const BElem& e, D3 x, doub
     R Rr = -r / (R)slen;
     R f0 = (R)krep * exp(Rr) / (R)slen;
     return D3{(double)(f0 * n.x), (double)(f0 * n.y), (double)(f0 * n.z)};
+}
+
+// the tangent path: distance, normal and force with their derivatives along the tangent
+MDN_HD inline DualD boundary_distance(const BElem& e, DualD3 x) {
+    if (e.kind == 0) {
+        const double a = e.p[0], b = e.p[1], c = e.p[2], d = e.p[3];
+        const double nrm = sqrt(a * a + b * b + c * c);
+        return DualD((a * x.v.x + b * x.v.y + c * x.v.z + d) / nrm, (a * x.d.x + b * x.d.y + c * x.d.z) / nrm);
+    }
+    const V3<DualD> r = dif<DualD>(x, DualD3{D3{e.p[0], e.p[1], e.p[2]}, D3{0.0, 0.0, 0.0}});
+    return e.p[3] - sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+}
+MDN_HD inline DualD3 boundary_force_dual(const BElem& e, DualD3 x, double krep, double slen) {
+    const DualD r = boundary_distance(e, x);
+    V3<DualD> n;
+    if (e.kind == 0) {
+        n = V3<DualD>{DualD(e.p[0]), DualD(e.p[1]), DualD(e.p[2])};
+    } else {
+        const V3<DualD> v = dif<DualD>(DualD3{D3{e.p[0], e.p[1], e.p[2]}, D3{0.0, 0.0, 0.0}}, x);
+        const DualD invD = 1 / sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        n = V3<DualD>{invD * v.x, invD * v.y, invD * v.z};
+    }
+    const DualD Rr = -r / slen;
+    const DualD f0 = krep * exp(Rr) / slen;
+    DualD3 o;
+    put3(o, f0 * n.x, f0 * n.y, f0 * n.z);
+    return o;
 }
 
 }  // namespace mdn

@@ -30,6 +30,7 @@ EXPORTED = [
     "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
+    "mdn_param_grad", "mdn_hvp", "mdn_set_params",
 ]
 
 
@@ -75,6 +76,9 @@ def lib():
         L.mdn_minimize_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(abi.CgParams),
                                            C.POINTER(dp), C.POINTER(dp), C.POINTER(abi.MinResult)]
         L.mdn_last_batch_schedule.argtypes = [C.c_void_p, i32p, dp]
+        L.mdn_param_grad.argtypes = [C.c_void_p, dp, dp, C.c_int64]
+        L.mdn_hvp.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
+        L.mdn_set_params.argtypes = [C.c_void_p, dp, dp, dp]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
         _lib = L
     return _lib
@@ -199,6 +203,29 @@ class ForceFieldManager:
         out = (C.c_int64 * 4)()
         self._chk(self._L.mdn_last_min_stats(self._h, out))
         return {"ls_launches": out[0], "gathers": out[2], "iterations": out[3]}
+
+    def param_grad(self, coord=None):
+        """dE/dtheta for theta = (stretching k, rest length L0, bending k) per cylinder; returns
+        an array of shape (3, n_cylinders) (mdn_param_grad)."""
+        nc = self.counts()["n_cylinders"]
+        out = np.zeros(3 * nc)
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_param_grad(self._h, cp, out.ctypes.data_as(dp), out.size))
+        return out.reshape(3, nc)
+
+    def hvp(self, v, coord=None):
+        """Hessian-vector product H v = -(dF/dx) v (mdn_hvp)."""
+        vv = _f64(v)
+        out = np.zeros(3 * self.n_beads)
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_hvp(self._h, cp, vv.ctypes.data_as(dp), out.ctypes.data_as(dp), out.size))
+        return out
+
+    def set_params(self, kstr=None, eql=None, kbend=None):
+        """Replace per-cylinder stretching k, rest length and bending k (mdn_set_params)."""
+        arrs = [None if a is None else _f64(a) for a in (kstr, eql, kbend)]
+        ptrs = [None if a is None else a.ctypes.data_as(dp) for a in arrs]
+        self._chk(self._L.mdn_set_params(self._h, *ptrs))
 
     def bench_forces(self, iters, precision=abi.MDN_PREC_FP64):
         a, b = C.c_double(), C.c_double()

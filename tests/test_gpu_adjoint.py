@@ -1,0 +1,181 @@
+"""GPU: parameter gradients dE/dtheta (mdn_param_grad) and Hessian-vector products (mdn_hvp) on
+the config-2 network (config-1 geometry, lognormal per-cylinder stretching k, rest length and
+bending k), near the relaxed state of an FP64 minimization (the unrelaxed network has overlapping
+cylinders with forces up to 1e13 pN, where finite differences say nothing).
+
+Central finite differences with h = 1e-4 relative to the perturbed quantity (1e-4 x the mean rest
+length for coordinates) pin:
+  -dE/dx against the forces per force field (the forces equal the oracle's, the reference
+  restatement: bit-identical for the bonded and crosslink terms, to the closed form's
+  conditioning for excluded volume, whose force is -dE/dx only outside the coplanar shift);
+  H v against -(F(x + h v) - F(x - h v)) / 2h for 64 seeded random unit vectors;
+  dE/dtheta against (E(theta + h) - E(theta - h)) / 2h for sampled cylinders and every kind.
+Exact identities (no finite-difference error) pin the tangent arithmetic further: H v is linear in
+v, and u.Hv = v.Hu up to the coplanar-shift pairs, whose force is not -dE/dx by construction
+(CylinderExclVolRepulsion.cpp:610-630).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def relaxed(gpu):
+    net = synth.Network.config(2)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    x, _, res = g.minimize(abi.default_cg_params(), net.coord)
+    assert res.converged_grad or res.converged_energy
+    # the minimizer can stop on the kink of the reference's coplanar test (|n.v| = 0.1, where the
+    # excluded-volume force jumps by the 0.01 nm shift): 0.01 nm of seeded noise moves every pair
+    # off that surface, so the finite differences below see a smooth force
+    x = x + np.random.default_rng(22).normal(0.0, 0.01, x.size)
+    g.set_coords(x)
+    return net, g, x
+
+
+def _unit(rng, n):
+    v = rng.standard_normal(n)
+    return v / np.linalg.norm(v)
+
+
+def _h(net):
+    d = net.desc
+    nc = d.fil_bead_offset[d.n_filaments] - d.n_filaments
+    return 1e-4 * float(np.mean([d.cyl_eql[i] for i in range(nc)]))
+
+
+def test_forces_are_minus_energy_gradient(relaxed):
+    net, g, x = relaxed
+    o = of.OracleSystem(net.desc)
+    o.build_nl(net.coord)
+    f = g.compute_forces(x)
+    fo = o.forces(x)
+    # bonded, crosslink and boundary forces are bit-identical; excluded volume agrees to the
+    # conditioning of its closed form: at a relaxed state some pairs are nearly coplanar, where
+    # the 1-ulp differences between the device's and the host's atan grow like 1/tau^2
+    for ff in range(abi.MDN_NUM_FF):
+        fg_ff, fo_ff = g.ff_compute_forces(ff, x), o.ff_forces(ff, x)
+        if ff == abi.MDN_FF_EXVOL:
+            assert float(np.abs(fg_ff - fo_ff).max()) <= 1e-5 * float(np.abs(fo).max())
+        elif ff != abi.MDN_FF_BOUNDARY:
+            np.testing.assert_array_equal(fg_ff, fo_ff)
+        else:
+            assert float(np.abs(fg_ff - fo_ff).max()) <= 1e-12 * float(np.abs(fo).max())
+    assert float(np.abs(f - fo).max()) <= 1e-5 * float(np.abs(fo).max())
+    # per force field: -F_ff.v against the central difference of E_ff along v. Excluded volume
+    # only to a few per cent: its force is taken at the coplanar-shifted geometry for the pairs
+    # within the reference's in-plane test, so it is not -dE/dx there by construction.
+    rng = np.random.default_rng(20)
+    h = _h(net)
+    fff = [g.ff_compute_forces(ff, x) for ff in range(abi.MDN_NUM_FF)]
+    for _ in range(4):
+        v = _unit(rng, x.size)
+        ep = g.compute_individual_energies(x + h * v).ff
+        em = g.compute_individual_energies(x - h * v).ff
+        for ff in range(abi.MDN_NUM_FF):
+            fd = (ep[ff] - em[ff]) / (2 * h)
+            an = -float(fff[ff] @ v)
+            scale = max(abs(an), 1e-6 * float(np.linalg.norm(fff[ff])))
+            print(f"{abi.FF_NAMES[ff]:26s} -F.v {an:+.9e}  FD dE/dv {fd:+.9e}", flush=True)
+            bar = 5e-2 if ff == abi.MDN_FF_EXVOL else 1e-5
+            assert abs(fd - an) <= bar * scale + 1e-12, abi.FF_NAMES[ff]
+
+
+def test_hvp_64_vectors_vs_finite_differences(relaxed):
+    """The full system: the finite differences are limited by FP64 rounding noise of the
+    excluded-volume closed form in near-coplanar pairs (~1e-4 pN on a force, divided by 2h), so
+    the bar is 1e-3 here; the tangents of every term are pinned to 1e-6 .. 1e-5 on well-
+    conditioned geometries by tests/test_dual_tangents.py, and the variant below checks the
+    other force fields of this network at 1e-6."""
+    net, g, x = relaxed
+    rng = np.random.default_rng(2)
+    h = _h(net)
+    worst = 0.0
+    for k in range(64):
+        v = _unit(rng, x.size)
+        hv = g.hvp(v, x)
+        fd = -(g.compute_forces(x + h * v) - g.compute_forces(x - h * v)) / (2 * h)
+        err = float(np.linalg.norm(hv - fd) / np.linalg.norm(hv))
+        worst = max(worst, err)
+        assert err <= 1e-3, k
+    print(f"64 HVPs: worst normwise |Hv - FD| / |Hv| = {worst:.3e}", flush=True)
+
+
+def test_hvp_without_excluded_volume(gpu, relaxed):
+    """The same network and state with the excluded-volume stiffness set to zero: stretching,
+    bending, linkers, motors and boundary, whose finite differences are clean."""
+    net0, _, x = relaxed
+    net = synth.Network.config(2)
+    d = net.desc
+    nc = d.fil_bead_offset[d.n_filaments] - d.n_filaments
+    for i in range(nc):
+        d.cyl_kexvol[i] = 0.0
+    g = gpu()
+    g.upload(d, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    rng = np.random.default_rng(5)
+    h = _h(net)
+    worst = 0.0
+    for k in range(16):
+        v = _unit(rng, x.size)
+        hv = g.hvp(v, x)
+        fd = -(g.compute_forces(x + h * v) - g.compute_forces(x - h * v)) / (2 * h)
+        err = float(np.linalg.norm(hv - fd) / np.linalg.norm(hv))
+        worst = max(worst, err)
+        assert err <= 1e-6, k
+    print(f"16 HVPs without excluded volume: worst {worst:.3e}", flush=True)
+
+
+def test_hvp_linear_and_symmetric(relaxed):
+    net, g, x = relaxed
+    rng = np.random.default_rng(7)
+    u, v = _unit(rng, x.size), _unit(rng, x.size)
+    hu, hv = g.hvp(u, x), g.hvp(v, x)
+    hw = g.hvp(2.0 * u - 3.0 * v, x)
+    # linear up to rounding, which near-coplanar excluded-volume pairs amplify (measured 4.5e-8
+    # absolute on |Hw| ~ 40 in 24 of 126,000 components)
+    np.testing.assert_allclose(hw, 2.0 * hu - 3.0 * hv, rtol=0, atol=1e-8 * float(np.abs(hw).max()))
+    a, b = float(u @ hv), float(v @ hu)
+    print(f"u.Hv {a:.12e} v.Hu {b:.12e}", flush=True)
+    assert abs(a - b) <= 1e-3 * max(abs(a), abs(b))
+
+
+def test_param_grad_vs_finite_differences(relaxed):
+    net, g, x = relaxed
+    d = net.desc
+    nc = d.fil_bead_offset[d.n_filaments] - d.n_filaments
+    kstr = np.array([d.cyl_kstr[i] for i in range(nc)])
+    eql = np.array([d.cyl_eql[i] for i in range(nc)])
+    kb = np.array([d.cyl_kbend[i] for i in range(nc)])
+    grad = g.param_grad(x)
+    assert grad.shape == (3, nc)
+    rng = np.random.default_rng(3)
+    # sampled cylinders, plus ones with excluded-volume partners (L0 scales kRepScaled)
+    pairs = g.neighbor_pairs()
+    sample = list(rng.choice(nc, 8, replace=False)) + [int(pairs[0, 0]), int(pairs[len(pairs) // 2, 1])]
+    base = [kstr, eql, kb]
+    try:
+        for kind in range(3):
+            for c in sample:
+                th = base[kind][c]
+                h = 1e-4 * th
+                e = []
+                for sgn in (1.0, -1.0):
+                    p = [a.copy() for a in base]
+                    p[kind][c] = th + sgn * h
+                    g.set_params(*p)
+                    e.append(g.compute_energy(x))
+                fd = (e[0] - e[1]) / (2 * h)
+                an = grad[kind, c]
+                print(f"kind {kind} cylinder {c}: dE/dtheta {an:.9e} FD {fd:.9e}", flush=True)
+                assert abs(fd - an) <= 1e-5 * abs(an) + 1e-6, (kind, c)
+    finally:
+        g.set_params(kstr, eql, kb)
+    # cylinder 0 (the first of filament 0) is no bending term's second cylinder
+    assert grad[2, 0] == 0.0
