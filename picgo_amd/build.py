@@ -2,6 +2,7 @@
 
     libmdn_mech.so   the gfx950 HIP kernels + C ABI (include/mdn_mech.h)
     libmdn_synth.so  the seeded synthetic-network generator (include/mdn_synth.h)
+    _torch_mech.so   libtorch autograd functions over the C ABI (csrc/torch_mech.cpp)
 
 Usage: python -m picgo_amd.build
 """
@@ -55,8 +56,30 @@ def build_mech(verbose=False):
     return out
 
 
+def build_torch_ext(verbose=False):
+    """The libtorch extension: host C++ only (it drives the GPU through libmdn_mech.so)."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+    out = os.path.join(HERE, "_torch_mech.so")
+    tmp = out + ".tmp"
+    tlib = os.path.join(os.path.dirname(torch.__file__), "lib")
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-w",
+           f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DTORCH_EXTENSION_NAME=_torch_mech", "-DTORCH_API_INCLUDE_EXTENSION_H"]
+    for inc in ce.include_paths() + [sysconfig.get_paths()["include"]]:
+        cmd += ["-I", inc]
+    cmd += ["-o", tmp, os.path.join(CSRC, "torch_mech.cpp"),
+            f"-L{tlib}", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-Wl,-rpath,{tlib}",
+            f"-L{HERE}", "-lmdn_mech", "-Wl,-rpath,$ORIGIN"]
+    _run(cmd, verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(verbose=False):
-    return [build_synth(verbose), build_mech(verbose)]
+    return [build_synth(verbose), build_mech(verbose), build_torch_ext(verbose)]
 
 
 if __name__ == "__main__":
