@@ -75,6 +75,103 @@ def load_traffic(kernel):
     return None
 
 
+STRESS = "config4: 20000 filaments x 20 cylinders (420000 beads), 80000 linkers, 40000 motors, 10 um sphere"
+BATCH = "config3: 256 networks, 20-400 filaments x 5-30 cylinders, 2 linkers + 1 motor per filament"
+
+
+def run_stress(args):
+    """Successive minimizations of the config-4 network: before each one the beads get seeded
+    N(0, 5 nm) noise and the neighbor lists are rebuilt; the same noise sequence for both
+    precisions. Everything between the first perturbation and the last result is timed."""
+    net = synth.Network.config(4)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    ffm.reset_neighbor_lists(net.coord)
+    out = {}
+    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+        p = abi.default_cg_params(precision=prec)
+        ffm.minimize(abi.default_cg_params(precision=prec, energy_call_limit=20), net.coord)   # warm
+        rng = np.random.default_rng(4)
+        x = net.coord.copy()
+        iters, crit, t_nl, pairs = [], {"grad": 0, "energy": 0, "limit": 0}, 0.0, []
+        t0 = time.perf_counter()
+        for k in range(args.stress_count):
+            x = x + rng.normal(0.0, 5.0, x.size)
+            t1 = time.perf_counter()
+            pairs.append(ffm.reset_neighbor_lists(x))
+            t_nl += time.perf_counter() - t1
+            x, f, res = ffm.minimize(p, x)
+            iters.append(res.num_iter)
+            crit["grad" if res.converged_grad else ("energy" if res.converged_energy else "limit")] += 1
+            if k % 10 == 0:
+                print(f"# stress {name}: minimization {k} {res.num_iter} iterations, "
+                      f"{time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+        total = time.perf_counter() - t0
+        out[name] = {"minimizations_per_s": args.stress_count / total, "total_s": total,
+                     "nl_rebuild_ms_mean": 1e3 * t_nl / args.stress_count, "cg_iterations_total": int(sum(iters)),
+                     "cg_iterations_first": iters[0], "cg_iterations_median": float(np.median(iters)),
+                     "us_per_cg_iteration": 1e6 * (total - t_nl) / max(1, sum(iters)), "criteria": crit,
+                     "exvol_pairs_last": int(pairs[-1]), "final_energy_fp64": res.energies_after.total}
+    line = {
+        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
+        "value": out["mixed"]["minimizations_per_s"], "unit": "minimizations/s",
+        "n_gpus": 1, "steps": args.stress_count, "warmup": 1,
+        "ms_per_step": 1e3 * out["mixed"]["total_s"] / args.stress_count,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 term math + f64 excluded volume / accumulation",
+        "data": "synthetic (seeded generator, config 4; seeded N(0, 5 nm) perturbations)",
+        "config": {"workload": STRESS, "n_beads": net.n_beads, "minimizations": args.stress_count},
+        "mixed": out["mixed"], "fp64": out["fp64"],
+        "mixed_vs_fp64_minimizations": out["mixed"]["minimizations_per_s"] / out["fp64"]["minimizations_per_s"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_batched(args):
+    """The config-3 batch in one persistent kernel (mixed and FP64), against the same systems
+    minimized one after another by the single-system minimizer (FP64)."""
+    nets = [synth.Network.config(3, index=i) for i in range(args.batch_count)]
+    ffms = []
+    for n in nets:
+        g = mech.ForceFieldManager(0)
+        g.upload(n.desc, n.coord)
+        g.reset_neighbor_lists(n.coord)
+        ffms.append(g)
+    coords = [n.coord for n in nets]
+    out = {}
+    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+        p = abi.default_cg_params(precision=prec)
+        mech.minimize_batched(ffms, p, coords)   # schedule tuning + warm-up
+        t0 = time.perf_counter()
+        xs, fs, res = mech.minimize_batched(ffms, p, coords)
+        dt = time.perf_counter() - t0
+        nt, srt, _, tune = mech.last_batch_schedule(ffms[0])
+        out[name] = {"minimizations_per_s": len(nets) / dt, "batch_s": dt,
+                     "converged": sum(1 for r in res if r.converged_grad or r.converged_energy),
+                     "cg_iterations_max": max(r.num_iter for r in res),
+                     "cg_iterations_total": int(sum(r.num_iter for r in res)),
+                     "schedule": {"workgroup": nt, "largest_first": srt, "tuning_ms": tune}}
+        print(f"# batched {name}: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    p = abi.default_cg_params()
+    t0 = time.perf_counter()
+    for g, c in zip(ffms, coords):
+        g.minimize(p, c)
+    dseq = time.perf_counter() - t0
+    out["sequential_fp64"] = {"minimizations_per_s": len(nets) / dseq, "total_s": dseq}
+    line = {
+        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
+        "value": out["mixed"]["minimizations_per_s"], "unit": "minimizations/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * out["mixed"]["batch_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 term math + f64 excluded volume / accumulation",
+        "data": "synthetic (seeded generator, config 3)",
+        "config": {"workload": BATCH, "systems": len(nets), "n_beads_total": int(sum(n.n_beads for n in nets))},
+        "batched_mixed": out["mixed"], "batched_fp64": out["fp64"], "sequential_fp64": out["sequential_fp64"],
+        "batched_mixed_vs_sequential_fp64": out["mixed"]["minimizations_per_s"] / out["sequential_fp64"]["minimizations_per_s"],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +180,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-min", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=["config1", "stress", "batched"], default="config1",
+                    help="config1: the metric's single network (default); stress: 20,000 x 20 filaments in a "
+                         "10 um sphere, successive minimizations with 5 nm noise and a neighbor-list rebuild "
+                         "each; batched: 256 uneven networks in one persistent kernel")
+    ap.add_argument("--stress-count", type=int, default=100)
+    ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run 20 device copies of 64 MiB (FETCH_SIZE / WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -99,6 +202,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "stress":
+        return run_stress(args)
+    if args.workload == "batched":
+        return run_batched(args)
 
     net = synth.Network.config(1)
     ffm = mech.ForceFieldManager(0)

@@ -97,6 +97,9 @@ struct DevSys {
 // ---------------------------------------------------------------------------------------
 // CG device state (CGMethod::minimize locals + MinimizationResult)
 enum { PH_LS = 0, PH_MOVE = 1, PH_GATHER = 2 };
+// CgState.status beyond the MDN_ERR_* codes: the minimizer paused for the numerical
+// excluded-volume force fallback of the current step
+constexpr int ST_NEED_FORCEN = 100;
 
 struct CgState {
     // parameters
@@ -959,6 +962,13 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
 // Scalar logic after a force evaluation: maxF, energyRelChange, beta, the minimum-energy
 // backup and the loop condition (CGMethod.hpp:1406-1576).
 __device__ inline void gather_control(CgState& S, const double* acc, int* bad_count) {
+    if (*bad_count != 0) {
+        // an excluded-volume pair needs the numerical force fallback (forceN): pause before any
+        // state change; the host completes this step's forces and re-runs the gather
+        S.status = ST_NEED_FORCEN;
+        S.done = 1;
+        return;
+    }
     const double FF = acc[0], FP = acc[1], MX = acc[2];
     const bool cand = S.num_iter > S.force_call_limit / 2 && S.total_energy <= S.minimum_e;
     S.n_gather_launch += 1;
@@ -968,11 +978,6 @@ __device__ inline void gather_control(CgState& S, const double* acc, int* bad_co
     S.fdotf = FF;
     if (!isfinite(MX)) {        // maxF error (CGMethod.hpp:202-212)
         S.status = 4;
-        S.done = 1;
-        return;
-    }
-    if (*bad_count != 0) {      // numerical force fallback needed
-        S.status = 9;
         S.done = 1;
         return;
     }

@@ -1078,28 +1078,56 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     // i+1 runs, so the device never idles waiting for the host.
     const long max_batches = (4L * ((long)S.force_call_limit + 16)) / CG_BATCH + 8;
     long launched = 0;
-    bool done = S.done != 0;
-    int* h_done = ctx->h_int + 4;
-    h_done[0] = h_done[1] = 0;
     hipEvent_t ev[2];
     HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    int k = 0;
-    while (!done && launched < max_batches) {
-        HIPCHK(hipGraphLaunch(ctx->graph_exec[prec], ctx->stream));
-        launched++;
-        HIPCHK(hipMemcpyAsync(&h_done[k], &ctx->cg.p->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipEventRecord(ev[k], ctx->stream));
-        if (launched > 1) {
-            HIPCHK(hipEventSynchronize(ev[k ^ 1]));
-            if (h_done[k ^ 1]) done = true;
+    int n_forcen = 0;
+    for (;;) {
+        bool done = S.done != 0;
+        int* h_done = ctx->h_int + 4;
+        h_done[0] = h_done[1] = 0;
+        int k = 0;
+        long here = 0;
+        while (!done && launched < max_batches) {
+            HIPCHK(hipGraphLaunch(ctx->graph_exec[prec], ctx->stream));
+            launched++;
+            here++;
+            HIPCHK(hipMemcpyAsync(&h_done[k], &ctx->cg.p->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipEventRecord(ev[k], ctx->stream));
+            if (here > 1) {
+                HIPCHK(hipEventSynchronize(ev[k ^ 1]));
+                if (h_done[k ^ 1]) done = true;
+            }
+            k ^= 1;
         }
-        k ^= 1;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
+        if (!(S.done && S.status == ST_NEED_FORCEN)) break;
+        // The step's forces need the numerical excluded-volume fallback (forceN,
+        // CylinderExclVolRepulsion.cpp:984-1254): recompute them at the new coordinates with it
+        // (same positions, so every other slot is bit-identical), then run this step's gather
+        // again and resume.
+        n_forcen++;
+        if ((st = eval_slots(ctx, ctx->x[S.parity ^ 1].p, prec))) return st;
+        HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+        S.status = 0;
+        S.done = 0;
+        S.reserved[1] = n_forcen;
+        HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
+        const CgBufs B = cg_bufs(ctx);
+        const int gbk = gather_blocks(ctx);
+        if (prec == MDN_PREC_MIXED)
+            hipLaunchKernelGGL(k_cg_gather<float>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                               (const CgState*)ctx->cg.p, B);
+        else
+            hipLaunchKernelGGL(k_cg_gather<double>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                               (const CgState*)ctx->cg.p, B);
+        hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, ctx->sys, ctx->cg.p, B, gbk);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
-    HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
     ctx->last_cg = S;
     if (!S.done) return fail(ctx, MDN_ERR_STATE, "minimizer did not terminate within %ld batches", launched);
     ctx->cur = S.parity;
@@ -1112,8 +1140,8 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     res->grad_inf_norm = S.grad_inf_norm;
     res->energy_rel_change = S.energy_rel_change;
     if (S.status == 9)
-        return fail(ctx, MDN_ERR_UNSUPPORTED,
-                    "an excluded-volume pair needed the numerical force fallback during the minimization");
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "more excluded-volume pairs need the Simpson energy than the "
+                    "minimizer's capacity (%d per line-search launch)", DEFER_CAP);
     if (S.status != 0) return fail(ctx, S.status, "minimization stopped with status %d", S.status);
     const bool limit = (S.energy_call_limit != 0 && S.num_energy_call > S.energy_call_limit) ||
                        (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
@@ -1303,9 +1331,11 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
         if (forces_out && forces_out[i]) HIPCHK_CTX(L, hipMemcpy(forces_out[i], c->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost));
         int code = MDN_OK;
         if (S.status != 0) {
-            code = S.status;
-            fail(c, code, S.status == 9 ? "an excluded-volume pair needed the numerical force fallback"
-                                        : "minimization stopped with status %d", S.status);
+            code = (S.status == ST_NEED_FORCEN || S.status == 9) ? MDN_ERR_UNSUPPORTED : S.status;
+            fail(c, code, S.status == ST_NEED_FORCEN ? "an excluded-volume pair needed the numerical force fallback, "
+                                                       "which the batched minimizer does not run"
+                          : S.status == 9 ? "more excluded-volume pairs need the Simpson energy than the capacity"
+                                          : "minimization stopped with status %d", S.status);
         } else if (ok && rep[9 + 8] != 0.0) {
             code = MDN_ERR_UNSUPPORTED;
             fail(c, code, "excluded-volume pairs beyond the Simpson-energy capacity");
