@@ -109,9 +109,9 @@ __device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, 
     }
 }
 
-// The line-search slot: k_cg_ls (trial energies at lambda and lambda/2 with speculative forces
-// at lambda, E(x) when needed, the bead work) and k_cg_ls_fin (reductions, Simpson energies,
-// ls_control).
+// The line-search slot: k_cg_ls (the trial energy at lambda with speculative forces, at lambda/2
+// too after a rejection, E(x) when needed, the bead work) and k_cg_ls_fin (reductions, Simpson
+// energies, ls_control).
 template <typename R, int NT>
 __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, double* red, int& sh_bad) {
     const bool first = S.ls_active == 0;
@@ -131,11 +131,12 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     for (int ff = 0; ff < NFF; ff++)
         for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R>(s, ff, t, p1, slots, 0, &fbad);
     if (fbad) atomicOr(&sh_bad, 1);
-    for (int ff = 0; ff < NFF; ff++)
-        for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
-            int fb2 = 0;
-            acc[1] += term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fb2);
-        }
+    if (!first)   // the first launch of a search evaluates lambda0 alone (see ls_control)
+        for (int ff = 0; ff < NFF; ff++)
+            for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
+                int fb2 = 0;
+                acc[1] += term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fb2);
+            }
     if (need_e0)
         for (int ff = 0; ff < NFF; ff++)
             for (int t = threadIdx.x; t < s.n[ff]; t += NT) {

@@ -40,6 +40,7 @@
 namespace mdn {
 
 constexpr int BLOCK = 256;
+constexpr int FIN = 1024;        // threads of the one-block finalize kernels
 constexpr int NFF = 6;
 constexpr int MAXELEM = 8;       // boundary elements held in the kernel argument
 constexpr int NTRIAL = 2;        // line-search step lengths evaluated per launch (lambda, lambda/2)
@@ -647,8 +648,9 @@ __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, in
 // CG kernels. One "slot" of the device-driven minimizer is
 //     k_cg_ls -> k_cg_move -> k_cg_gather
 // Each kernel reads the device state first and returns at once when it has nothing to do.
-// A typical CG iteration is one k_cg_ls (trial energies at lambda and lambda/2, forces at
-// lambda speculatively, the pending direction update applied on the fly) and one k_cg_gather;
+// A typical CG iteration is one k_cg_ls (the trial energy at lambda0 with its forces,
+// speculatively; after a rejection the next two halvings per launch; the pending direction
+// update applied on the fly) and one k_cg_gather;
 // k_cg_move only runs when the accepted step is not the first trial, or for the recovery move.
 // =======================================================================================
 
@@ -685,6 +687,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     const double lam0 = st->ls_lambda;
     const double beta = st->dir_beta;
     const bool need_e0 = first && !st->ecur_valid;
+    const int ntrial = first ? 1 : NTRIAL;   // see ls_control
     const int par = st->parity, dpar = st->dpar;
     const double* __restrict__ xc = B.x[par];
     const double* __restrict__ dc = B.d[dpar];
@@ -693,13 +696,21 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     double v[2] = {0.0, term_block ? 0.0 : INFINITY};
     if (term_block) {
         const int trial = blockIdx.x / T;
+        if (trial >= ntrial && !(trial == 1 && need_e0)) {   // nothing to evaluate: zero partials
+            if (threadIdx.x == 0) {
+                B.partial[(long)PSTRIDE * blockIdx.x] = 0.0;
+                B.partial[(long)PSTRIDE * blockIdx.x + 1] = 0.0;
+            }
+            return;
+        }
         const int blk = blockIdx.x - trial * T;
         const int ff = ff_of_block(s, blk);
         const int t = (blk - s.blk[ff]) * BLOCK + threadIdx.x;
         int fbad = 0;
         if (t < s.n[ff]) {
             Pos p{xc, dc, B.f, beta, trial == 0 ? lam0 : lam0 * 0.5, pend ? 2 : 1};
-            v[0] = term_ef<R>(s, ff, t, p, trial == 0 ? static_cast<R*>(B.slots) : nullptr, trial, &fbad);
+            if (trial < ntrial)
+                v[0] = term_ef<R>(s, ff, t, p, trial == 0 ? static_cast<R*>(B.slots) : nullptr, trial, &fbad);
             // E(x) rides with the energy-only trial (lower register pressure than the force trial)
             if (trial == 1 && need_e0) {
                 Pos p0{xc, nullptr, nullptr, 0.0, 0.0, 0};
@@ -786,7 +797,11 @@ __device__ inline void ls_control(CgState& S, const double* acc, int* bad_count)
         conv = true;
         lambda = 0.0;
     }
-    for (int k = 0; k < NTRIAL && !conv; k++) {
+    // The first launch of a search evaluates lambda0 alone (usually accepted: its forces are
+    // the speculative ones); after a rejection each launch evaluates the next NTRIAL halvings.
+    // The trial sequence is the reference's either way.
+    const int ntrial = first ? 1 : NTRIAL;
+    for (int k = 0; k < ntrial && !conv; k++) {
         energyLambda = k == 0 ? E1 : E2;
         S.num_energy_call += 1;
         S.ls_trials += 1;
@@ -847,25 +862,25 @@ __device__ inline void ls_control(CgState& S, const double* acc, int* bad_count)
 
 // One block: reduce the line-search partials in a fixed order (8 independent loads in flight per
 // thread), add the deferred Simpson energies, and run ls_control.
-__global__ void __launch_bounds__(BLOCK) k_cg_ls_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
+__global__ void __launch_bounds__(FIN) k_cg_ls_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
     __shared__ CgState S;
-    __shared__ double lds[8 * (BLOCK / 64)];
+    __shared__ double lds[8 * (FIN / 64)];
     state_load(&S, st);
     if (S.done || S.phase != PH_LS) return;
     const int T = s.blk[NFF];
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
     const int op8[8] = {0, 0, 1, 1, 0, 1, 0, 2};   // E1, E2, bad1, bad2, E0, bad0, d.F, step
-    for (int base = 0; base < nblk; base += 8 * BLOCK) {
+    for (int base = 0; base < nblk; base += 8 * FIN) {
         double p0[8], p1[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int bk = base + threadIdx.x + j * BLOCK;
+            const int bk = base + threadIdx.x + j * FIN;
             p0[j] = bk < nblk ? B.partial[(long)PSTRIDE * bk] : 0.0;
             p1[j] = bk < nblk ? B.partial[(long)PSTRIDE * bk + 1] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int bk = base + threadIdx.x + j * BLOCK;
+            const int bk = base + threadIdx.x + j * FIN;
             if (bk >= nblk) continue;
             if (bk < NTRIAL * T) {
                 const int trial = bk / T;
@@ -877,10 +892,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_ls_fin(DevSys s, CgState* __restri
             }
         }
     }
-    block_reduce_n<8>(acc, op8, lds);
+    block_reduce_n<8, FIN>(acc, op8, lds);
     double dsum[3];
     int dbad[3];
-    const int n = run_defers(s, lds, dsum, dbad, 3);
+    const int n = run_defers<FIN>(s, lds, dsum, dbad, 3);
     if (threadIdx.x == 0) {
         acc[0] += dsum[0]; acc[2] = dbad[0];
         acc[1] += dsum[1]; acc[3] = dbad[1];
@@ -1014,18 +1029,18 @@ __device__ inline void gather_control(CgState& S, const double* acc, int* bad_co
     if (call_limit_s(S) || converged_s(S)) S.done = 1;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cg_gather_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
+__global__ void __launch_bounds__(FIN) k_cg_gather_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
     __shared__ CgState S;
-    __shared__ double lds[3 * (BLOCK / 64)];
+    __shared__ double lds[3 * (FIN / 64)];
     state_load(&S, st);
     if (S.done || (S.phase != PH_MOVE && S.phase != PH_GATHER)) return;
     double acc[3] = {0.0, 0.0, 0.0};
     const int op3[3] = {0, 0, 1};
-    for (int base = 0; base < nblk; base += 4 * BLOCK) {
+    for (int base = 0; base < nblk; base += 4 * FIN) {
         double p[4][3];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int bk = base + threadIdx.x + j * BLOCK;
+            const int bk = base + threadIdx.x + j * FIN;
             for (int q = 0; q < 3; q++) p[j][q] = bk < nblk ? B.partial[(long)PSTRIDE * bk + q] : 0.0;
         }
 #pragma unroll
@@ -1035,7 +1050,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather_fin(DevSys s, CgState* __re
             acc[2] = rmax(acc[2], p[j][2]);
         }
     }
-    block_reduce_n<3>(acc, op3, lds);
+    block_reduce_n<3, FIN>(acc, op3, lds);
     if (threadIdx.x == 0) gather_control(S, acc, B.bad_count);
     state_store(st, &S);
 }

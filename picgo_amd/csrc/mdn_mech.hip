@@ -975,11 +975,11 @@ static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
     const int T = s.blk[NFF], beads = bead_blocks(ctx);
     const int nls = NTRIAL * T + beads;
     hipLaunchKernelGGL(k_cg_ls<R>, dim3(nls), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_ls_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B, nls);
+    hipLaunchKernelGGL(k_cg_ls_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls);
     const int gbk = gather_blocks(ctx);
     hipLaunchKernelGGL(k_cg_move<R>, dim3(MOVE_GRID), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_gather<R>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, s, ctx->cg.p, B, gbk);
+    hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, gbk);
 }
 }  // extern "C++"
 
@@ -1122,7 +1122,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         else
             hipLaunchKernelGGL(k_cg_gather<double>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, ctx->sys,
                                (const CgState*)ctx->cg.p, B);
-        hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, ctx->sys, ctx->cg.p, B, gbk);
+        hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(FIN), 0, ctx->stream, ctx->sys, ctx->cg.p, B, gbk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
     }
