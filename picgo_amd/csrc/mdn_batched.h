@@ -4,8 +4,8 @@
 // only; then it pulls the next one. Every system keeps its own tables and vectors (the device
 // buffers of its context, described by a DevSys and CgBufs record).
 //
-// The work of one CG "slot" is the same as in the multi-kernel minimizer (k_cg_ls, k_cg_ls_fin,
-// k_cg_move, k_cg_gather, k_cg_gather_fin) and the scalar control is the same code
+// The work of one CG step is the same as in the multi-kernel minimizer (k_cg_ls, k_cg_gather,
+// k_cg_fin, with the move when it is needed) and the scalar control is the same code
 // (ls_control, gather_control), so a system follows the reference's CGMethod::minimize
 // (CGMethod.hpp:826-1743) exactly as the single-system path does; only the order of the
 // block-level energy and dot-product reductions differs.
@@ -109,9 +109,9 @@ __device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, 
     }
 }
 
-// The line-search slot: k_cg_ls (the trial energy at lambda with speculative forces, at lambda/2
-// too after a rejection, E(x) when needed, the bead work) and k_cg_ls_fin (reductions, Simpson
-// energies, ls_control).
+// The line search: k_cg_ls (the trial energy at lambda with speculative forces, at lambda/2
+// too after a rejection, E(x) when needed, the bead work) and the first half of k_cg_fin
+// (reductions, Simpson energies, ls_control).
 template <typename R, int NT>
 __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, double* red, int& sh_bad) {
     const bool first = S.ls_active == 0;
@@ -122,8 +122,9 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     const double* __restrict__ xc = B.x[par];
     const double* __restrict__ dc = B.d[dpar];
     R* slots = static_cast<R*>(B.slots);
-    const Pos p1{xc, dc, B.f, beta, lam0, pend ? 2 : 1};
-    const Pos p2{xc, dc, B.f, beta, lam0 * 0.5, pend ? 2 : 1};
+    const double* __restrict__ fcur = B.f[S.fpar];
+    const Pos p1{xc, dc, fcur, beta, lam0, pend ? 2 : 1};
+    const Pos p2{xc, dc, fcur, beta, lam0 * 0.5, pend ? 2 : 1};
     const Pos pz{xc, nullptr, nullptr, 0.0, 0.0, 0};
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
     int fbad = 0;
@@ -147,7 +148,7 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     double* __restrict__ xn = B.x[par ^ 1];
     double* __restrict__ dn = B.d[dpar ^ 1];
     for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) {
-        const double fv = B.f[i];
+        const double fv = fcur[i];
         const double de = pend ? fv + beta * dc[i] : dc[i];
         xn[i] = xc[i] + lam0 * de;   // speculative x_next
         if (first) {
@@ -172,13 +173,15 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
             S.status = 9;
             S.done = 1;
         } else {
-            ls_control(S, acc, &sh_bad);
+            CgState Sr = S;   // the scalar control on a register copy (LDS latency per field access)
+            ls_control(Sr, acc, &sh_bad);
+            S = Sr;
         }
     }
     __syncthreads();
 }
 
-// The move slot (k_cg_move): forces and x_next at the accepted step (or the recovery step).
+// The move (k_cg_ls in phase PH_MOVE): forces and x_next at the accepted step (or the recovery step).
 template <typename R, int NT>
 __device__ inline void blk_move(const DevSys& s, const CgBufs& B, const CgState& S, int& sh_bad) {
     const double lam = S.lambda;
@@ -196,17 +199,21 @@ __device__ inline void blk_move(const DevSys& s, const CgBufs& B, const CgState&
     __syncthreads();
 }
 
-// The gather slot (k_cg_gather, k_cg_gather_fin).
+// The gather (k_cg_gather and the second half of k_cg_fin) into the next force buffer.
 template <typename R, int NT>
 __device__ inline void blk_gather_step(const DevSys& s, const CgBufs& B, CgState& S, double* red, int& sh_bad) {
     // copycoordsifminimumE candidate (CGMethod.hpp:1564-1566)
     const bool cand = S.num_iter > S.force_call_limit / 2 && S.total_energy <= S.minimum_e;
     const int par = S.parity, bk = S.bkpar;
     double v[3] = {0.0, 0.0, 0.0};
-    blk_gather<R, NT>(s, static_cast<const R*>(B.slots), B.f, B.fp, cand, B.x[par ^ 1], B.xmin[bk ^ 1], v);
+    blk_gather<R, NT>(s, static_cast<const R*>(B.slots), B.f[S.fpar ^ 1], B.fp, cand, B.x[par ^ 1], B.xmin[bk ^ 1], v);
     const int op3[3] = {0, 0, 1};
     block_reduce_n<3, NT>(v, op3, red);
-    if (threadIdx.x == 0) gather_control(S, v, &sh_bad);
+    if (threadIdx.x == 0) {
+        CgState Sr = S;
+        gather_control(Sr, v, &sh_bad);
+        S = Sr;
+    }
     __syncthreads();
 }
 
@@ -243,7 +250,7 @@ __device__ inline void blk_minimize(const BatchItem& it, const BatchParams& P, C
     }
     __syncthreads();
     double v[3] = {0.0, 0.0, 0.0};
-    blk_gather<R, NT>(s, static_cast<const R*>(B.slots), B.f, nullptr, false, nullptr, nullptr, v);
+    blk_gather<R, NT>(s, static_cast<const R*>(B.slots), B.f[0], nullptr, false, nullptr, nullptr, v);   // fpar = 0
     const int op3[3] = {0, 0, 1};
     block_reduce_n<3, NT>(v, op3, red);
     if (threadIdx.x == 0) {

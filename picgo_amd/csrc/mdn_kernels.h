@@ -46,7 +46,6 @@ constexpr int MAXELEM = 8;       // boundary elements held in the kernel argumen
 constexpr int NTRIAL = 2;        // line-search step lengths evaluated per launch (lambda, lambda/2)
 constexpr int PSTRIDE = 4;       // partial slots per block
 constexpr int GB = 64;           // beads per gather block
-constexpr int MOVE_GRID = 256;   // grid of the (rarely active) move kernel
 constexpr int DEFER_CAP = 256;   // excluded-volume energies deferred to the Simpson pass per launch
 
 // An excluded-volume pair whose closed-form energy is non-finite (or < -1): the reference then
@@ -107,7 +106,7 @@ struct CgState {
     int method, try_recover, force_call_limit, energy_call_limit;
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
     // control
-    int phase, parity, dpar, bkpar, done, status;
+    int phase, parity, dpar, bkpar, fpar, done, status;
     int num_iter, num_energy_call, num_force_call, num_safe_mode, num_recovery;
     unsigned error_flags;
     int safe_mode, recovery_move;
@@ -449,13 +448,9 @@ __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, con
 // Returns the number of deferred pairs (DEFER_CAP exceeded when larger). The count is read and
 // reset with atomics: in the batched minimizer the pairs were pushed by other waves of the same
 // workgroup within the kernel.
+// run_defers_n: the same with the count n already read (by every thread).
 template <int NTH = BLOCK>
-__device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int* badc, int ncode) {
-    __shared__ int nsh;
-    __syncthreads();
-    if (threadIdx.x == 0) nsh = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int n = nsh;
+__device__ inline int run_defers_n(const DevSys& s, int n, double* lds, double* sum, int* badc, int ncode) {
     const int m = min(n, DEFER_CAP);
     const int op1[1] = {0};
     for (int k = 0; k < m; k++) {
@@ -486,6 +481,14 @@ __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int*
         __hip_atomic_store(s.defer_n, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return n;
+}
+template <int NTH = BLOCK>
+__device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int* badc, int ncode) {
+    __shared__ int nsh;
+    __syncthreads();
+    if (threadIdx.x == 0) nsh = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return run_defers_n<NTH>(s, nsh, lds, sum, badc, ncode);
 }
 
 // Per-FF energies at x, block partials. Grid: interaction blocks only.
@@ -645,23 +648,27 @@ __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, in
 }
 
 // =======================================================================================
-// CG kernels. One "slot" of the device-driven minimizer is
-//     k_cg_ls -> k_cg_move -> k_cg_gather
-// Each kernel reads the device state first and returns at once when it has nothing to do.
-// A typical CG iteration is one k_cg_ls (the trial energy at lambda0 with its forces,
-// speculatively; after a rejection the next two halvings per launch; the pending direction
-// update applied on the fly) and one k_cg_gather;
-// k_cg_move only runs when the accepted step is not the first trial, or for the recovery move.
+// CG kernels. One "slot" of the device-driven minimizer is three kernels:
+//     k_cg_ls -> k_cg_gather -> k_cg_fin
+// Each reads the device state first and returns at once when it has nothing to do.
+// A typical CG iteration is one slot: k_cg_ls evaluates the trial energy at lambda0 with its
+// forces and x + lambda0 d, speculatively (the pending direction update applied on the fly);
+// k_cg_gather sums those forces into the next force buffer, speculatively; k_cg_fin runs the
+// line-search decision and, when lambda0 is accepted, the post-step control at once. After a
+// rejection a launch evaluates the next two halvings; when the accepted step is not a
+// speculative one (or for the recovery move) the next slot's k_cg_ls is the move (forces and
+// x_next at the accepted step), its gather real, and its k_cg_fin the post-step control.
 // =======================================================================================
 
 struct CgBufs {
     double* x[2];       // current / next coordinates (parity)
     double* d[2];       // current / next search direction (dpar)
     double* xmin[2];    // committed / candidate minimum-energy backup (bkpar)
-    double* f;          // force
+    double* f[2];       // current / next force (fpar)
     double* fp;         // previous force
     void* slots;        // 3 values of the evaluation's arithmetic type per slot
-    double* partial;    // [blocks * PSTRIDE]
+    double* partial;    // line-search partials [blocks * PSTRIDE]
+    double* gpartial;   // gather partials [gather blocks * PSTRIDE]
     int* bad_count;     // excluded-volume pairs needing the numerical fallback in a CG move
 };
 
@@ -675,12 +682,40 @@ __device__ inline bool converged(const CgState* st) {
 
 // ---- line search: trial energies (and speculative forces at the first trial) ----------
 // Grid: NTRIAL copies of the interaction blocks, then the bead blocks. Writes block partials;
-// k_cg_ls_fin decides.
-// 3 waves per SIMD (at most 168 VGPRs, a few spilled) measured faster than the unconstrained 2.
+// k_cg_fin decides. In phase PH_MOVE the kernel is the move instead: forces at x + lambda d
+// (the accepted step when it was not a speculative one, or the recovery step) from the first
+// copy of the interaction blocks, x_next from the bead blocks (moveAlongSearchDir,
+// CGMethod.cpp:350-354; the recovery move, MController.h:318-321).
+// 3 waves per SIMD (at most 168 VGPRs) measured faster than the unconstrained 2.
 template <typename R>
 __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double lds[2 * (BLOCK / 64)];
-    if (st->done || st->phase != PH_LS) return;
+    if (st->done) return;
+    if (st->phase == PH_MOVE) {
+        const int T = s.blk[NFF];
+        const double lam = st->lambda;
+        const double* __restrict__ x = B.x[st->parity];
+        const double* __restrict__ d = B.d[st->dpar];
+        if ((int)blockIdx.x < T) {
+            const int ff = ff_of_block(s, blockIdx.x);
+            const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+            if (t < s.n[ff]) {
+                const Pos p{x, d, nullptr, 0.0, lam, 1};
+                int fbad = 0;
+                (void)term_ef<R>(s, ff, t, p, static_cast<R*>(B.slots), -1, &fbad);
+                if (fbad) atomicAdd(B.bad_count, 1);
+            }
+        } else if ((int)blockIdx.x >= NTRIAL * T) {
+            const int b = (blockIdx.x - NTRIAL * T) * BLOCK + threadIdx.x;
+            if (b < s.n_beads) {
+                double* xn = B.x[st->parity ^ 1];
+                for (int q = 0; q < 3; q++) xn[3 * b + q] = x[3 * b + q] + lam * d[3 * b + q];
+            }
+        }
+        return;
+    }
+    if (st->phase != PH_LS) return;
+    const double* __restrict__ fcur = B.f[st->fpar];
     const bool first = st->ls_active == 0;
     const bool pend = first && st->dir_pending;
     const int T = s.blk[NFF];
@@ -708,7 +743,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
         const int t = (blk - s.blk[ff]) * BLOCK + threadIdx.x;
         int fbad = 0;
         if (t < s.n[ff]) {
-            Pos p{xc, dc, B.f, beta, trial == 0 ? lam0 : lam0 * 0.5, pend ? 2 : 1};
+            Pos p{xc, dc, fcur, beta, trial == 0 ? lam0 : lam0 * 0.5, pend ? 2 : 1};
             if (trial < ntrial)
                 v[0] = term_ef<R>(s, ff, t, p, trial == 0 ? static_cast<R*>(B.slots) : nullptr, trial, &fbad);
             // E(x) rides with the energy-only trial (lower register pressure than the force trial)
@@ -725,7 +760,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
             double* xn = B.x[par ^ 1];
             double* dn = B.d[dpar ^ 1];
             for (int q = 0; q < 3; q++) {
-                const double fv = B.f[3 * b + q];
+                const double fv = fcur[3 * b + q];
                 const double de = pend ? fv + beta * dc[3 * b + q] : dc[3 * b + q];
                 xn[3 * b + q] = xc[3 * b + q] + lam0 * de;   // speculative x_next (first trial)
                 if (first) {
@@ -860,99 +895,21 @@ __device__ inline void ls_control(CgState& S, const double* acc, int* bad_count)
     S.done = 1;
 }
 
-// One block: reduce the line-search partials in a fixed order (8 independent loads in flight per
-// thread), add the deferred Simpson energies, and run ls_control.
-__global__ void __launch_bounds__(FIN) k_cg_ls_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
-    __shared__ CgState S;
-    __shared__ double lds[8 * (FIN / 64)];
-    state_load(&S, st);
-    if (S.done || S.phase != PH_LS) return;
-    const int T = s.blk[NFF];
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
-    const int op8[8] = {0, 0, 1, 1, 0, 1, 0, 2};   // E1, E2, bad1, bad2, E0, bad0, d.F, step
-    for (int base = 0; base < nblk; base += 8 * FIN) {
-        double p0[8], p1[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int bk = base + threadIdx.x + j * FIN;
-            p0[j] = bk < nblk ? B.partial[(long)PSTRIDE * bk] : 0.0;
-            p1[j] = bk < nblk ? B.partial[(long)PSTRIDE * bk + 1] : 0.0;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int bk = base + threadIdx.x + j * FIN;
-            if (bk >= nblk) continue;
-            if (bk < NTRIAL * T) {
-                const int trial = bk / T;
-                acc[trial] += p0[j];
-                if (trial == 1) acc[4] += p1[j];
-            } else {
-                acc[6] += p0[j];
-                acc[7] = rmin(acc[7], p1[j]);
-            }
-        }
-    }
-    block_reduce_n<8, FIN>(acc, op8, lds);
-    double dsum[3];
-    int dbad[3];
-    const int n = run_defers<FIN>(s, lds, dsum, dbad, 3);
-    if (threadIdx.x == 0) {
-        acc[0] += dsum[0]; acc[2] = dbad[0];
-        acc[1] += dsum[1]; acc[3] = dbad[1];
-        acc[4] += dsum[2]; acc[5] = dbad[2];
-        if (n > DEFER_CAP) {
-            S.status = 9;
-            S.done = 1;
-        } else {
-            ls_control(S, acc, B.bad_count);
-        }
-    }
-    state_store(st, &S);
-}
-
-// ---- forces and x_next at the accepted (non-first) step or the recovery step ----------
-// Small grid (MOVE_GRID blocks) striding over all interactions and beads: the kernel is a no-op
-// in most slots, so its empty launch must be cheap.
-template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_cg_move(DevSys s, const CgState* __restrict__ st, CgBufs B) {
-    if (st->done || st->phase != PH_MOVE) return;
-    const double lam = st->lambda;
-    const int par = st->parity;
-    const double* __restrict__ x = B.x[par];
-    const double* __restrict__ d = B.d[st->dpar];
-    const Pos p{x, d, nullptr, 0.0, lam, 1};
-    const long nterm = (long)s.blk[NFF] * BLOCK, ntot = nterm + s.n_beads;
-    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < ntot; i += (long)gridDim.x * BLOCK) {
-        if (i < nterm) {
-            const int blk = (int)(i / BLOCK);
-            const int ff = ff_of_block(s, blk);
-            const int t = (blk - s.blk[ff]) * BLOCK + (int)(i % BLOCK);
-            if (t < s.n[ff]) {
-                int fbad = 0;
-                (void)term_ef<R>(s, ff, t, p, static_cast<R*>(B.slots), -1, &fbad);
-                if (fbad) atomicAdd(B.bad_count, 1);
-            }
-        } else {
-            const int b = (int)(i - nterm);
-            double* xn = B.x[par ^ 1];
-            // moveAlongSearchDir (CGMethod.cpp:350-354) / the recovery move (MController.h:318-321)
-            for (int q = 0; q < 3; q++) xn[3 * b + q] = x[3 * b + q] + lam * d[3 * b + q];
-        }
-    }
-}
-
-// ---- gather F; F.F, F.F_prev, max|F| partials -----------------------------------------
-// Grid: bead blocks only.
+// ---- gather F into the next force buffer; F.F, F.F_prev, max|F| partials ---------------
+// After a line-search launch the gather is speculative (k_cg_fin uses it when lambda0 is
+// accepted); after a move it is the step's. Grid: bead blocks only.
 template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     if (st->done) return;
-    if (st->phase != PH_MOVE && st->phase != PH_GATHER) return;
+    if (st->phase != PH_LS && st->phase != PH_MOVE) return;
     const int b = blockIdx.x * GB + threadIdx.x;
-    // copycoordsifminimumE candidate (CGMethod.hpp:1564-1566): the energy and iteration parts of
-    // the condition are known now, the force part after the reduction
-    const bool cand = st->num_iter > st->force_call_limit / 2 && st->total_energy <= st->minimum_e;
+    // copycoordsifminimumE candidate (CGMethod.hpp:1564-1566): x_next goes to the candidate
+    // buffer whenever the iteration part of the condition can hold after this step; the energy
+    // and force parts decide in k_cg_fin whether it is committed
+    const bool cand = st->num_iter + 1 > st->force_call_limit / 2;
     const int par = st->parity, bk = st->bkpar;
+    double* __restrict__ fnext = B.f[st->fpar ^ 1];
     D3 acc = gather_block<R>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
     if (threadIdx.x >= GB) return;   // one wave finishes
     double v[3] = {0.0, 0.0, 0.0};
@@ -960,7 +917,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
         const double fv[3] = {acc.x, acc.y, acc.z};
         const double* xn = B.x[par ^ 1];
         for (int q = 0; q < 3; q++) {
-            B.f[3 * b + q] = fv[q];
+            fnext[3 * b + q] = fv[q];
             v[0] += fv[q] * fv[q];
             v[1] += fv[q] * B.fp[3 * b + q];
             v[2] = rmax(v[2], fabs(fv[q]));
@@ -971,7 +928,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     v[1] = wave_sum(v[1]);
     v[2] = wave_max(v[2]);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 3; q++) B.partial[(long)PSTRIDE * blockIdx.x + q] = v[q];
+        for (int q = 0; q < 3; q++) B.gpartial[(long)PSTRIDE * blockIdx.x + q] = v[q];
 }
 
 // Scalar logic after a force evaluation: maxF, energyRelChange, beta, the minimum-energy
@@ -988,6 +945,7 @@ __device__ inline void gather_control(CgState& S, const double* acc, int* bad_co
     const bool cand = S.num_iter > S.force_call_limit / 2 && S.total_energy <= S.minimum_e;
     S.n_gather_launch += 1;
     S.parity ^= 1;
+    S.fpar ^= 1;
     S.num_force_call += 1;
     S.grad_inf_norm = MX;
     S.fdotf = FF;
@@ -1029,30 +987,89 @@ __device__ inline void gather_control(CgState& S, const double* acc, int* bad_co
     if (call_limit_s(S) || converged_s(S)) S.done = 1;
 }
 
-__global__ void __launch_bounds__(FIN) k_cg_gather_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nblk) {
-    __shared__ CgState S;
-    __shared__ double lds[3 * (FIN / 64)];
-    state_load(&S, st);
-    if (S.done || (S.phase != PH_MOVE && S.phase != PH_GATHER)) return;
-    double acc[3] = {0.0, 0.0, 0.0};
-    const int op3[3] = {0, 0, 1};
-    for (int base = 0; base < nblk; base += 4 * FIN) {
+// One block after every slot: the line-search decision (phase PH_LS: the k_cg_ls partials in a
+// fixed order, the deferred Simpson energies, ls_control) and, when the step is accepted at the
+// speculative trial or a move just ran (phase PH_MOVE), the post-step control on the gather
+// partials (gather_control). The kernel is a chain of latencies, so every independent global
+// read (the state, the deferred-pair count, both sets of partials) is issued at the start, the
+// eleven sums and extrema are reduced in one pass, and thread 0 runs the scalar control on a
+// register copy of the state and writes it back.
+__global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nls, int ngb) {
+    constexpr int NW = sizeof(CgState) / sizeof(int);
+    static_assert(NW <= FIN, "the state is staged one word per thread");
+    __shared__ CgState Sl;
+    __shared__ double lds[11 * (FIN / 64)];
+    __shared__ int sh_ndef;
+    if (threadIdx.x < NW) reinterpret_cast<int*>(&Sl)[threadIdx.x] = reinterpret_cast<const int*>(st)[threadIdx.x];
+    if (threadIdx.x == 0) sh_ndef = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int T = s.blk[NFF];
+    // E(lambda), E(lambda/2), -, -, E(x), -, d.F, recovery step | F.F, F.F_prev, max|F|
+    double acc[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY, 0.0, 0.0, 0.0};
+    for (int base = 0; base < nls; base += 8 * FIN) {
+        double p0[8], p1[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bk = base + threadIdx.x + j * FIN;
+            p0[j] = bk < nls ? B.partial[(long)PSTRIDE * bk] : 0.0;
+            p1[j] = bk < nls ? B.partial[(long)PSTRIDE * bk + 1] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int bk = base + threadIdx.x + j * FIN;
+            if (bk >= nls) continue;
+            if (bk < NTRIAL * T) {
+                const int trial = bk / T;
+                acc[trial] += p0[j];
+                if (trial == 1) acc[4] += p1[j];
+            } else {
+                acc[6] += p0[j];
+                acc[7] = rmin(acc[7], p1[j]);
+            }
+        }
+    }
+    for (int base = 0; base < ngb; base += 4 * FIN) {
         double p[4][3];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int bk = base + threadIdx.x + j * FIN;
-            for (int q = 0; q < 3; q++) p[j][q] = bk < nblk ? B.partial[(long)PSTRIDE * bk + q] : 0.0;
+            for (int q = 0; q < 3; q++) p[j][q] = bk < ngb ? B.gpartial[(long)PSTRIDE * bk + q] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            acc[0] += p[j][0];
-            acc[1] += p[j][1];
-            acc[2] = rmax(acc[2], p[j][2]);
+            acc[8] += p[j][0];
+            acc[9] += p[j][1];
+            acc[10] = rmax(acc[10], p[j][2]);
         }
     }
-    block_reduce_n<3, FIN>(acc, op3, lds);
-    if (threadIdx.x == 0) gather_control(S, acc, B.bad_count);
-    state_store(st, &S);
+    __syncthreads();
+    const int phase0 = Sl.phase;
+    if (Sl.done || (phase0 != PH_LS && phase0 != PH_MOVE)) return;
+    const int op[11] = {0, 0, 1, 1, 0, 1, 0, 2, 0, 0, 1};
+    block_reduce_n<11, FIN>(acc, op, lds);
+    double dsum[3] = {0.0, 0.0, 0.0};
+    int dbad[3] = {0, 0, 0};
+    const int ndef = sh_ndef;
+    if (phase0 == PH_LS && ndef > 0) (void)run_defers_n<FIN>(s, ndef, lds, dsum, dbad, 3);
+    if (threadIdx.x == 0) {
+        CgState S = Sl;
+        bool gather = phase0 == PH_MOVE;
+        if (phase0 == PH_LS) {
+            acc[0] += dsum[0]; acc[2] = dbad[0];
+            acc[1] += dsum[1]; acc[3] = dbad[1];
+            acc[4] += dsum[2]; acc[5] = dbad[2];
+            if (ndef > DEFER_CAP) {
+                S.status = 9;
+                S.done = 1;
+            } else {
+                ls_control(S, acc, B.bad_count);
+                gather = !S.done && S.phase == PH_GATHER;
+            }
+        } else {
+            S.n_move_launch += 1;
+        }
+        if (gather) gather_control(S, acc + 8, B.bad_count);
+        *st = S;
+    }
 }
 
 // =======================================================================================

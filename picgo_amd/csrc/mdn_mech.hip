@@ -93,7 +93,7 @@ struct mdn_ctx {
     std::vector<int> bnd_nneighbors;
 
     // state vectors
-    DBuf<double> x[2], f, fp, d[2], xmin[2], scratch_f, ext;
+    DBuf<double> x[2], f, f2, fp, d[2], xmin[2], scratch_f, ext;   // f, f2: the CG force pair
     DBuf<double> slots;   // 3 values per slot; the mixed path uses the first half as float
     DBuf<double> partial;
     DBuf<int> bad_count, bad_list;
@@ -210,8 +210,9 @@ int ensure_eval_buffers(mdn_ctx* ctx) {
     const DevSys& s = ctx->sys;
     // padded by one gather chunk: the gather's chunk loads are unconditional
     HIPCHK(ctx->slots.ensure(3 * (size_t)std::max(1L, s.slot0[NFF]) + GCHUNK_PAD));
-    const long nls = NTRIAL * (long)s.blk[NFF] + (s.blk[NFF + 1] - s.blk[NFF]);
-    HIPCHK(ctx->partial.ensure((size_t)(std::max(nls, (long)gather_blocks(ctx)) + 1) * PSTRIDE));
+    // line-search partials, then the CG gather's
+    const long nls = NTRIAL * (long)s.blk[NFF] + bead_blocks(ctx);
+    HIPCHK(ctx->partial.ensure((size_t)(nls + gather_blocks(ctx) + 1) * PSTRIDE));
     HIPCHK(ctx->pos.ensure((size_t)std::max(1L, s.slot0[NFF])));
     HIPCHK(ctx->sorted_id.ensure((size_t)std::max(1L, s.slot0[NFF])));
     HIPCHK(ctx->bad_list.ensure((size_t)std::max(1, s.n[FF_EXVOL])));
@@ -407,7 +408,7 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                           &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
                           &ctx->mot_p2, &ctx->mot_sf, &ctx->ev_ks, &ctx->bnd_krep, &ctx->bnd_slen,
                           &ctx->d_cyl_kexvol, &ctx->d_cyl_eql, &ctx->d_krep_e, &ctx->d_slen_e, &ctx->mid,
-                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
+                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
                           &ctx->xmin[1], &ctx->scratch_f,
                           &ctx->ext, &ctx->partial, &ctx->eout, &ctx->fallback_partial};
     for (auto* b : db) b->release();
@@ -595,8 +596,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     }
     // state vectors
     size_t nv = 3 * (size_t)ctx->n_beads;
-    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0], &ctx->xmin[1],
-                    &ctx->scratch_f, &ctx->ext})
+    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
+                    &ctx->xmin[1], &ctx->scratch_f, &ctx->ext})
         HIPCHK(b->ensure(nv));
     ctx->cur = 0;
     HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
@@ -958,13 +959,17 @@ int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n
 // ----------------------------------------------------------------------------------------
 // CGMethod::minimize on the device
 // ----------------------------------------------------------------------------------------
+// Line-search blocks: NTRIAL copies of the interaction blocks, then the bead blocks.
+static int ls_blocks(mdn_ctx* ctx) { return NTRIAL * ctx->sys.blk[NFF] + bead_blocks(ctx); }
+
 static CgBufs cg_bufs(mdn_ctx* ctx) {
     CgBufs B;
     B.x[0] = ctx->x[0].p; B.x[1] = ctx->x[1].p;
     B.d[0] = ctx->d[0].p; B.d[1] = ctx->d[1].p;
     B.xmin[0] = ctx->xmin[0].p; B.xmin[1] = ctx->xmin[1].p;
-    B.f = ctx->f.p; B.fp = ctx->fp.p;
+    B.f[0] = ctx->f.p; B.f[1] = ctx->f2.p; B.fp = ctx->fp.p;
     B.slots = (void*)ctx->slots.p; B.partial = ctx->partial.p; B.bad_count = ctx->bad_count.p;
+    B.gpartial = ctx->partial.p + (long)PSTRIDE * ls_blocks(ctx);   // after the line-search partials
     return B;
 }
 
@@ -972,14 +977,10 @@ extern "C++" {
 template <typename R>
 static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
     const DevSys& s = ctx->sys;
-    const int T = s.blk[NFF], beads = bead_blocks(ctx);
-    const int nls = NTRIAL * T + beads;
+    const int nls = ls_blocks(ctx), gbk = gather_blocks(ctx);
     hipLaunchKernelGGL(k_cg_ls<R>, dim3(nls), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_ls_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls);
-    const int gbk = gather_blocks(ctx);
-    hipLaunchKernelGGL(k_cg_move<R>, dim3(MOVE_GRID), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_gather<R>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, gbk);
+    hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls, gbk);
 }
 }  // extern "C++"
 
@@ -1122,7 +1123,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         else
             hipLaunchKernelGGL(k_cg_gather<double>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, ctx->sys,
                                (const CgState*)ctx->cg.p, B);
-        hipLaunchKernelGGL(k_cg_gather_fin, dim3(1), dim3(FIN), 0, ctx->stream, ctx->sys, ctx->cg.p, B, gbk);
+        hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, ctx->sys, ctx->cg.p, B, ls_blocks(ctx), gbk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
     }
@@ -1160,7 +1161,9 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         res->num_energy_call += 1;
     }
     HIPCHK(hipMemcpyAsync(coord, xres, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
-    if (force_out) HIPCHK(hipMemcpyAsync(force_out, ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    if (force_out)
+        HIPCHK(hipMemcpyAsync(force_out, (S.fpar ? ctx->f2 : ctx->f).p, sizeof(double) * nv, hipMemcpyDeviceToHost,
+                              ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (S.error_flags & MDN_MIN_ERR_LINESEARCH) return fail(ctx, MDN_ERR_LINESEARCH, "line search failed (recovery disabled)");
     return MDN_OK;
@@ -1328,7 +1331,8 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
         fill_result(S, rep.data(), rep.data() + 9, ok, &results[i]);
         const int nv = 3 * c->n_beads;
         HIPCHK_CTX(L, hipMemcpy(coords[i], c->x[c->cur].p, sizeof(double) * nv, hipMemcpyDeviceToHost));
-        if (forces_out && forces_out[i]) HIPCHK_CTX(L, hipMemcpy(forces_out[i], c->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+        if (forces_out && forces_out[i])
+            HIPCHK_CTX(L, hipMemcpy(forces_out[i], (S.fpar ? c->f2 : c->f).p, sizeof(double) * nv, hipMemcpyDeviceToHost));
         int code = MDN_OK;
         if (S.status != 0) {
             code = (S.status == ST_NEED_FORCEN || S.status == 9) ? MDN_ERR_UNSUPPORTED : S.status;
