@@ -109,9 +109,9 @@ __device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, 
     }
 }
 
-// The line search: k_cg_ls (the trial energy at lambda with speculative forces, at lambda/2
-// too after a rejection, E(x) when needed, the bead work) and the first half of k_cg_fin
-// (reductions, Simpson energies, ls_control).
+// The line search: k_cg_pos and k_cg_ls (the bead work; the trial energy at lambda with
+// speculative forces, at lambda/2 too after a rejection, E(x) when needed) and the first half of
+// k_cg_fin (reductions, Simpson energies, ls_control).
 template <typename R, int NT>
 __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, double* red, int& sh_bad) {
     const bool first = S.ls_active == 0;
@@ -174,14 +174,15 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
             S.done = 1;
         } else {
             CgState Sr = S;   // the scalar control on a register copy (LDS latency per field access)
-            ls_control(Sr, acc, &sh_bad);
+            ls_control(Sr, acc, sh_bad);
             S = Sr;
         }
     }
     __syncthreads();
 }
 
-// The move (k_cg_ls in phase PH_MOVE): forces and x_next at the accepted step (or the recovery step).
+// The move (k_cg_pos and k_cg_ls in phase PH_MOVE): forces and x_next at the accepted step (or the
+// recovery step).
 template <typename R, int NT>
 __device__ inline void blk_move(const DevSys& s, const CgBufs& B, const CgState& S, int& sh_bad) {
     const double lam = S.lambda;
@@ -211,7 +212,7 @@ __device__ inline void blk_gather_step(const DevSys& s, const CgBufs& B, CgState
     block_reduce_n<3, NT>(v, op3, red);
     if (threadIdx.x == 0) {
         CgState Sr = S;
-        gather_control(Sr, v, &sh_bad);
+        gather_control(Sr, v, sh_bad);
         S = Sr;
     }
     __syncthreads();

@@ -648,16 +648,19 @@ __global__ void k_exvol_forceN_finish(DevSys s, const double* __restrict__ x, in
 }
 
 // =======================================================================================
-// CG kernels. One "slot" of the device-driven minimizer is three kernels:
-//     k_cg_ls -> k_cg_gather -> k_cg_fin
+// CG kernels. One "slot" of the device-driven minimizer is four kernels:
+//     k_cg_pos -> k_cg_ls -> k_cg_gather -> k_cg_fin
 // Each reads the device state first and returns at once when it has nothing to do.
-// A typical CG iteration is one slot: k_cg_ls evaluates the trial energy at lambda0 with its
-// forces and x + lambda0 d, speculatively (the pending direction update applied on the fly);
-// k_cg_gather sums those forces into the next force buffer, speculatively; k_cg_fin runs the
-// line-search decision and, when lambda0 is accepted, the post-step control at once. After a
-// rejection a launch evaluates the next two halvings; when the accepted step is not a
-// speculative one (or for the recovery move) the next slot's k_cg_ls is the move (forces and
-// x_next at the accepted step), its gather real, and its k_cg_fin the post-step control.
+// A typical CG iteration is one slot: k_cg_pos writes the trial coordinates x + lambda0 d (the
+// pending direction update d = F + beta d applied on the fly), which are also the speculative
+// next coordinates; k_cg_ls evaluates the trial energy there with its forces; k_cg_gather sums
+// those forces into the next force buffer, speculatively; k_cg_fin runs the line-search
+// decision and, when lambda0 is accepted, the post-step control at once. After a rejection a
+// launch evaluates the next two halvings; when the accepted step is not a speculative one (or
+// for the recovery move) the next slot is the move (k_cg_pos: x_next at the accepted step,
+// k_cg_ls: its forces), its gather real, and its k_cg_fin the post-step control.
+// Precomputing the trial coordinates once per bead lets every interaction read one coordinate
+// vector instead of three (x, d, F) per bead it touches.
 // =======================================================================================
 
 struct CgBufs {
@@ -666,6 +669,7 @@ struct CgBufs {
     double* xmin[2];    // committed / candidate minimum-energy backup (bkpar)
     double* f[2];       // current / next force (fpar)
     double* fp;         // previous force
+    double* xh;         // x + (lambda / 2) d, the second trial of a launch after a rejection
     void* slots;        // 3 values of the evaluation's arithmetic type per slot
     double* partial;    // line-search partials [blocks * PSTRIDE]
     double* gpartial;   // gather partials [gather blocks * PSTRIDE]
@@ -680,105 +684,107 @@ __device__ inline bool converged(const CgState* st) {
     return st->grad_inf_norm < st->grad_tol || st->energy_rel_change < st->energy_rel_tol;
 }
 
-// ---- line search: trial energies (and speculative forces at the first trial) ----------
-// Grid: NTRIAL copies of the interaction blocks, then the bead blocks. Writes block partials;
-// k_cg_fin decides. In phase PH_MOVE the kernel is the move instead: forces at x + lambda d
-// (the accepted step when it was not a speculative one, or the recovery step) from the first
-// copy of the interaction blocks, x_next from the bead blocks (moveAlongSearchDir,
-// CGMethod.cpp:350-354; the recovery move, MController.h:318-321).
-// 3 waves per SIMD (at most 168 VGPRs) measured faster than the unconstrained 2.
-template <typename R>
-__global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
+// ---- line search, bead part: trial coordinates, d.F, the recovery step ------------------
+// Grid: bead blocks. Phase PH_LS: x_next = x + lambda0 d_eff (trial 1, and the speculative next
+// coordinates), x + (lambda0 / 2) d after a rejection (trial 2), and on the first launch of a
+// search the d.F and max-step partials, the materialized direction and F_prev. Phase PH_MOVE:
+// x_next = x + lambda d (moveAlongSearchDir, CGMethod.cpp:350-354; the recovery move,
+// MController.h:318-321). The partials go after the interaction blocks' (index T + block).
+__global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double lds[2 * (BLOCK / 64)];
     if (st->done) return;
-    if (st->phase == PH_MOVE) {
-        const int T = s.blk[NFF];
-        const double lam = st->lambda;
-        const double* __restrict__ x = B.x[st->parity];
-        const double* __restrict__ d = B.d[st->dpar];
-        if ((int)blockIdx.x < T) {
-            const int ff = ff_of_block(s, blockIdx.x);
-            const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
-            if (t < s.n[ff]) {
-                const Pos p{x, d, nullptr, 0.0, lam, 1};
-                int fbad = 0;
-                (void)term_ef<R>(s, ff, t, p, static_cast<R*>(B.slots), -1, &fbad);
-                if (fbad) atomicAdd(B.bad_count, 1);
-            }
-        } else if ((int)blockIdx.x >= NTRIAL * T) {
-            const int b = (blockIdx.x - NTRIAL * T) * BLOCK + threadIdx.x;
-            if (b < s.n_beads) {
-                double* xn = B.x[st->parity ^ 1];
-                for (int q = 0; q < 3; q++) xn[3 * b + q] = x[3 * b + q] + lam * d[3 * b + q];
-            }
-        }
-        return;
-    }
-    if (st->phase != PH_LS) return;
-    const double* __restrict__ fcur = B.f[st->fpar];
-    const bool first = st->ls_active == 0;
-    const bool pend = first && st->dir_pending;
-    const int T = s.blk[NFF];
-    const double lam0 = st->ls_lambda;
-    const double beta = st->dir_beta;
-    const bool need_e0 = first && !st->ecur_valid;
-    const int ntrial = first ? 1 : NTRIAL;   // see ls_control
+    const int phase = st->phase;
+    if (phase != PH_LS && phase != PH_MOVE) return;
+    const int b = blockIdx.x * BLOCK + threadIdx.x;
     const int par = st->parity, dpar = st->dpar;
     const double* __restrict__ xc = B.x[par];
     const double* __restrict__ dc = B.d[dpar];
-    const bool term_block = (int)blockIdx.x < NTRIAL * T;
-    // block partials: interaction blocks (E(trial), E(x)); bead blocks (d.F, recovery step)
-    double v[2] = {0.0, term_block ? 0.0 : INFINITY};
-    if (term_block) {
-        const int trial = blockIdx.x / T;
-        if (trial >= ntrial && !(trial == 1 && need_e0)) {   // nothing to evaluate: zero partials
-            if (threadIdx.x == 0) {
-                B.partial[(long)PSTRIDE * blockIdx.x] = 0.0;
-                B.partial[(long)PSTRIDE * blockIdx.x + 1] = 0.0;
-            }
-            return;
-        }
-        const int blk = blockIdx.x - trial * T;
-        const int ff = ff_of_block(s, blk);
-        const int t = (blk - s.blk[ff]) * BLOCK + threadIdx.x;
-        int fbad = 0;
-        if (t < s.n[ff]) {
-            Pos p{xc, dc, fcur, beta, trial == 0 ? lam0 : lam0 * 0.5, pend ? 2 : 1};
-            if (trial < ntrial)
-                v[0] = term_ef<R>(s, ff, t, p, trial == 0 ? static_cast<R*>(B.slots) : nullptr, trial, &fbad);
-            // E(x) rides with the energy-only trial (lower register pressure than the force trial)
-            if (trial == 1 && need_e0) {
-                Pos p0{xc, nullptr, nullptr, 0.0, 0.0, 0};
-                v[1] = term_ef<R>(s, ff, t, p0, (R*)nullptr, 2, &fbad);
-            }
-            if (fbad) atomicAdd(B.bad_count, 1);
-        }
-    } else {
-        const int b = (blockIdx.x - NTRIAL * T) * BLOCK + threadIdx.x;
-        if (b < s.n_beads) {
-            const double maxd = st->max_dist;
-            double* xn = B.x[par ^ 1];
-            double* dn = B.d[dpar ^ 1];
-            for (int q = 0; q < 3; q++) {
-                const double fv = fcur[3 * b + q];
-                const double de = pend ? fv + beta * dc[3 * b + q] : dc[3 * b + q];
-                xn[3 * b + q] = xc[3 * b + q] + lam0 * de;   // speculative x_next (first trial)
-                if (first) {
-                    v[0] += de * fv;                          // d.F
-                    v[1] = rmin(v[1], maxd / fabs(de));       // MController.h:293-305
-                    if (pend) {
-                        dn[3 * b + q] = de;                   // shiftSearchDir materialized
-                        B.fp[3 * b + q] = fv;                 // forcePrev = force
-                    }
+    double* __restrict__ xn = B.x[par ^ 1];
+    if (phase == PH_MOVE) {
+        const double lam = st->lambda;
+        if (b < s.n_beads)
+            for (int q = 0; q < 3; q++) xn[3 * b + q] = xc[3 * b + q] + lam * dc[3 * b + q];
+        return;
+    }
+    const bool first = st->ls_active == 0;
+    const bool pend = first && st->dir_pending;
+    const double lam0 = st->ls_lambda, beta = st->dir_beta;
+    const double* __restrict__ fcur = B.f[st->fpar];
+    double v[2] = {0.0, INFINITY};
+    if (b < s.n_beads) {
+        const double maxd = st->max_dist;
+        double* dn = B.d[dpar ^ 1];
+        for (int q = 0; q < 3; q++) {
+            const double fv = fcur[3 * b + q];
+            const double de = pend ? fv + beta * dc[3 * b + q] : dc[3 * b + q];
+            xn[3 * b + q] = xc[3 * b + q] + lam0 * de;   // trial 1 = speculative x_next
+            if (!first) B.xh[3 * b + q] = xc[3 * b + q] + (lam0 * 0.5) * de;
+            if (first) {
+                v[0] += de * fv;                          // d.F
+                v[1] = rmin(v[1], maxd / fabs(de));       // MController.h:293-305
+                if (pend) {
+                    dn[3 * b + q] = de;                   // shiftSearchDir materialized
+                    B.fp[3 * b + q] = fv;                 // forcePrev = force
                 }
             }
         }
     }
-    const int op2t[2] = {0, 0}, op2b[2] = {0, 2};
-    block_reduce_n<2>(v, term_block ? op2t : op2b, lds);
+    if (!first) return;   // the partials are read on the first launch of a search only
+    const int op2[2] = {0, 2};
+    block_reduce_n<2>(v, op2, lds);
     if (threadIdx.x == 0) {
-        B.partial[(long)PSTRIDE * blockIdx.x] = v[0];
-        B.partial[(long)PSTRIDE * blockIdx.x + 1] = v[1];
+        const long k = (long)PSTRIDE * (s.blk[NFF] + blockIdx.x);
+        B.partial[k] = v[0];
+        B.partial[k + 1] = v[1];
+    }
+}
+
+// ---- line search, interaction part: trial energies, speculative forces -----------------
+// Grid: the interaction blocks. Phase PH_LS: E(lambda) with its forces at x_next (into the
+// slots), E(lambda / 2) at x_half after a rejection, E(x) when the current energy is not known;
+// block partials E(lambda), E(lambda / 2), E(x). Phase PH_MOVE: the forces at x_next.
+// 3 waves per SIMD (at most 168 VGPRs) measured faster than the unconstrained 2.
+template <typename R>
+__global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
+    __shared__ double lds[3 * (BLOCK / 64)];
+    if (st->done || (int)blockIdx.x >= s.blk[NFF]) return;   // (a 1-block grid when there are no interactions)
+    const int phase = st->phase;
+    if (phase != PH_LS && phase != PH_MOVE) return;
+    const int ff = ff_of_block(s, blockIdx.x);
+    const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+    const double* __restrict__ xn = B.x[st->parity ^ 1];
+    const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
+    if (phase == PH_MOVE) {
+        if (t < s.n[ff]) {
+            int fbad = 0;
+            (void)term_ef<R>(s, ff, t, p1, static_cast<R*>(B.slots), -1, &fbad);
+            if (fbad) atomicAdd(B.bad_count, 1);
+        }
+        return;
+    }
+    const bool first = st->ls_active == 0;
+    const bool need_e0 = first && !st->ecur_valid;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (t < s.n[ff]) {
+        int fbad = 0;
+        v[0] = term_ef<R>(s, ff, t, p1, static_cast<R*>(B.slots), 0, &fbad);
+        if (fbad) atomicAdd(B.bad_count, 1);
+        if (!first) {   // the first launch of a search evaluates lambda0 alone (see ls_control)
+            const Pos p2{B.xh, nullptr, nullptr, 0.0, 0.0, 0};
+            v[1] = term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fbad);
+        }
+        if (need_e0) {
+            const Pos p0{B.x[st->parity], nullptr, nullptr, 0.0, 0.0, 0};
+            v[2] = term_ef<R>(s, ff, t, p0, (R*)nullptr, 2, &fbad);
+        }
+    }
+    const int op3[3] = {0, 0, 0};
+    block_reduce_n<3>(v, op3, lds);
+    if (threadIdx.x == 0) {
+        const long k = (long)PSTRIDE * blockIdx.x;
+        B.partial[k] = v[0];
+        B.partial[k + 1] = v[1];
+        B.partial[k + 2] = v[2];
     }
 }
 
@@ -793,7 +799,7 @@ __device__ inline bool converged_s(const CgState& S) {
 // Scalar logic after the trial energies: backtrackingLineSearch (CGMethod.hpp:238-373) over the
 // evaluated step lengths, and the failure handling of CGMethod::minimize (:1337-1367).
 // acc = {E(lambda), E(lambda/2), bad, bad2, E(x), bad0, d.F, recovery step}
-__device__ inline void ls_control(CgState& S, const double* acc, int* bad_count) {
+__device__ inline void ls_control(CgState& S, const double* acc, int& bad_count) {
     const bool first = S.ls_active == 0;
     const double E1 = acc[2] > 0.0 ? INFINITY : acc[0];   // computeEnergy: -1 sentinel -> +inf
     const double E2 = acc[3] > 0.0 ? INFINITY : acc[1];
@@ -806,7 +812,7 @@ __device__ inline void ls_control(CgState& S, const double* acc, int* bad_count)
                 // redo this line-search launch with d = F
                 S.dir_beta = 0.0;
                 S.dir_reset = 1;
-                *bad_count = 0;
+                bad_count = 0;
                 return;
             }
             S.dpar ^= 1;
@@ -852,8 +858,8 @@ __device__ inline void ls_control(CgState& S, const double* acc, int* bad_count)
             energyLambda = S.ls_cur;
         }
     }
-    const bool spec = *bad_count == 0;
-    *bad_count = 0;
+    const bool spec = bad_count == 0;
+    bad_count = 0;
     if (!conv) {                 // keep halving in the next launch
         S.ls_lambda = lambda;
         return;
@@ -933,8 +939,8 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
 
 // Scalar logic after a force evaluation: maxF, energyRelChange, beta, the minimum-energy
 // backup and the loop condition (CGMethod.hpp:1406-1576).
-__device__ inline void gather_control(CgState& S, const double* acc, int* bad_count) {
-    if (*bad_count != 0) {
+__device__ inline void gather_control(CgState& S, const double* acc, int bad_count) {
+    if (bad_count != 0) {
         // an excluded-volume pair needs the numerical force fallback (forceN): pause before any
         // state change; the host completes this step's forces and re-runs the gather
         S.status = ST_NEED_FORCEN;
@@ -987,40 +993,44 @@ __device__ inline void gather_control(CgState& S, const double* acc, int* bad_co
     if (call_limit_s(S) || converged_s(S)) S.done = 1;
 }
 
-// One block after every slot: the line-search decision (phase PH_LS: the k_cg_ls partials in a
-// fixed order, the deferred Simpson energies, ls_control) and, when the step is accepted at the
+// One block after every slot: the line-search decision (phase PH_LS: the k_cg_ls and k_cg_pos
+// partials in a fixed order, the deferred Simpson energies, ls_control) and, when the step is accepted at the
 // speculative trial or a move just ran (phase PH_MOVE), the post-step control on the gather
 // partials (gather_control). The kernel is a chain of latencies, so every independent global
-// read (the state, the deferred-pair count, both sets of partials) is issued at the start, the
-// eleven sums and extrema are reduced in one pass, and thread 0 runs the scalar control on a
-// register copy of the state and writes it back.
+// read (the state, the deferred-pair and bad-pair counts, both sets of partials) is issued at
+// the start, the eleven sums and extrema are reduced in one pass, and thread 0 runs the scalar
+// control on a register copy of the state and writes it back.
 __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nls, int ngb) {
     constexpr int NW = sizeof(CgState) / sizeof(int);
     static_assert(NW <= FIN, "the state is staged one word per thread");
     __shared__ CgState Sl;
     __shared__ double lds[11 * (FIN / 64)];
-    __shared__ int sh_ndef;
+    __shared__ int sh_ndef, sh_bad;
     if (threadIdx.x < NW) reinterpret_cast<int*>(&Sl)[threadIdx.x] = reinterpret_cast<const int*>(st)[threadIdx.x];
-    if (threadIdx.x == 0) sh_ndef = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) {
+        sh_ndef = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sh_bad = *B.bad_count;
+    }
     const int T = s.blk[NFF];
     // E(lambda), E(lambda/2), -, -, E(x), -, d.F, recovery step | F.F, F.F_prev, max|F|
     double acc[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY, 0.0, 0.0, 0.0};
     for (int base = 0; base < nls; base += 8 * FIN) {
-        double p0[8], p1[8];
+        double p0[8], p1[8], p2[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int bk = base + threadIdx.x + j * FIN;
             p0[j] = bk < nls ? B.partial[(long)PSTRIDE * bk] : 0.0;
             p1[j] = bk < nls ? B.partial[(long)PSTRIDE * bk + 1] : 0.0;
+            p2[j] = bk < T ? B.partial[(long)PSTRIDE * bk + 2] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int bk = base + threadIdx.x + j * FIN;
             if (bk >= nls) continue;
-            if (bk < NTRIAL * T) {
-                const int trial = bk / T;
-                acc[trial] += p0[j];
-                if (trial == 1) acc[4] += p1[j];
+            if (bk < T) {
+                acc[0] += p0[j];
+                acc[1] += p1[j];
+                acc[4] += p2[j];
             } else {
                 acc[6] += p0[j];
                 acc[7] = rmin(acc[7], p1[j]);
@@ -1052,6 +1062,7 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
     if (phase0 == PH_LS && ndef > 0) (void)run_defers_n<FIN>(s, ndef, lds, dsum, dbad, 3);
     if (threadIdx.x == 0) {
         CgState S = Sl;
+        int bad = sh_bad;
         bool gather = phase0 == PH_MOVE;
         if (phase0 == PH_LS) {
             acc[0] += dsum[0]; acc[2] = dbad[0];
@@ -1061,13 +1072,14 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
                 S.status = 9;
                 S.done = 1;
             } else {
-                ls_control(S, acc, B.bad_count);
+                ls_control(S, acc, bad);
                 gather = !S.done && S.phase == PH_GATHER;
             }
         } else {
             S.n_move_launch += 1;
         }
-        if (gather) gather_control(S, acc + 8, B.bad_count);
+        if (gather) gather_control(S, acc + 8, bad);
+        if (bad != sh_bad) *B.bad_count = bad;
         *st = S;
     }
 }

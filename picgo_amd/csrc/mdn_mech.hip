@@ -93,7 +93,7 @@ struct mdn_ctx {
     std::vector<int> bnd_nneighbors;
 
     // state vectors
-    DBuf<double> x[2], f, f2, fp, d[2], xmin[2], scratch_f, ext;   // f, f2: the CG force pair
+    DBuf<double> x[2], f, f2, fp, xh, d[2], xmin[2], scratch_f, ext;   // f, f2: the CG force pair
     DBuf<double> slots;   // 3 values per slot; the mixed path uses the first half as float
     DBuf<double> partial;
     DBuf<int> bad_count, bad_list;
@@ -210,8 +210,8 @@ int ensure_eval_buffers(mdn_ctx* ctx) {
     const DevSys& s = ctx->sys;
     // padded by one gather chunk: the gather's chunk loads are unconditional
     HIPCHK(ctx->slots.ensure(3 * (size_t)std::max(1L, s.slot0[NFF]) + GCHUNK_PAD));
-    // line-search partials, then the CG gather's
-    const long nls = NTRIAL * (long)s.blk[NFF] + bead_blocks(ctx);
+    // line-search partials (interaction blocks, then bead blocks), then the CG gather's
+    const long nls = (long)s.blk[NFF] + bead_blocks(ctx);
     HIPCHK(ctx->partial.ensure((size_t)(nls + gather_blocks(ctx) + 1) * PSTRIDE));
     HIPCHK(ctx->pos.ensure((size_t)std::max(1L, s.slot0[NFF])));
     HIPCHK(ctx->sorted_id.ensure((size_t)std::max(1L, s.slot0[NFF])));
@@ -408,7 +408,7 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                           &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
                           &ctx->mot_p2, &ctx->mot_sf, &ctx->ev_ks, &ctx->bnd_krep, &ctx->bnd_slen,
                           &ctx->d_cyl_kexvol, &ctx->d_cyl_eql, &ctx->d_krep_e, &ctx->d_slen_e, &ctx->mid,
-                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
+                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->xh, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
                           &ctx->xmin[1], &ctx->scratch_f,
                           &ctx->ext, &ctx->partial, &ctx->eout, &ctx->fallback_partial};
     for (auto* b : db) b->release();
@@ -596,8 +596,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     }
     // state vectors
     size_t nv = 3 * (size_t)ctx->n_beads;
-    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
-                    &ctx->xmin[1], &ctx->scratch_f, &ctx->ext})
+    for (auto* b : {&ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->xh, &ctx->d[0], &ctx->d[1],
+                    &ctx->xmin[0], &ctx->xmin[1], &ctx->scratch_f, &ctx->ext})
         HIPCHK(b->ensure(nv));
     ctx->cur = 0;
     HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
@@ -959,15 +959,15 @@ int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n
 // ----------------------------------------------------------------------------------------
 // CGMethod::minimize on the device
 // ----------------------------------------------------------------------------------------
-// Line-search blocks: NTRIAL copies of the interaction blocks, then the bead blocks.
-static int ls_blocks(mdn_ctx* ctx) { return NTRIAL * ctx->sys.blk[NFF] + bead_blocks(ctx); }
+// Line-search partial blocks: the interaction blocks (k_cg_ls), then the bead blocks (k_cg_pos).
+static int ls_blocks(mdn_ctx* ctx) { return ctx->sys.blk[NFF] + bead_blocks(ctx); }
 
 static CgBufs cg_bufs(mdn_ctx* ctx) {
     CgBufs B;
     B.x[0] = ctx->x[0].p; B.x[1] = ctx->x[1].p;
     B.d[0] = ctx->d[0].p; B.d[1] = ctx->d[1].p;
     B.xmin[0] = ctx->xmin[0].p; B.xmin[1] = ctx->xmin[1].p;
-    B.f[0] = ctx->f.p; B.f[1] = ctx->f2.p; B.fp = ctx->fp.p;
+    B.f[0] = ctx->f.p; B.f[1] = ctx->f2.p; B.fp = ctx->fp.p; B.xh = ctx->xh.p;
     B.slots = (void*)ctx->slots.p; B.partial = ctx->partial.p; B.bad_count = ctx->bad_count.p;
     B.gpartial = ctx->partial.p + (long)PSTRIDE * ls_blocks(ctx);   // after the line-search partials
     return B;
@@ -978,7 +978,9 @@ template <typename R>
 static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
     const DevSys& s = ctx->sys;
     const int nls = ls_blocks(ctx), gbk = gather_blocks(ctx);
-    hipLaunchKernelGGL(k_cg_ls<R>, dim3(nls), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_pos, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    hipLaunchKernelGGL(k_cg_ls<R>, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s,
+                       (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_gather<R>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls, gbk);
 }
