@@ -671,8 +671,8 @@ struct CgBufs {
     double* fp;         // previous force
     double* xh;         // x + (lambda / 2) d, the second trial of a launch after a rejection
     void* slots;        // 3 values of the evaluation's arithmetic type per slot
-    double* partial;    // line-search partials [blocks * PSTRIDE]
-    double* gpartial;   // gather partials [gather blocks * PSTRIDE]
+    double* partial;    // line-search partials, value-major: [value][interaction blocks + bead blocks]
+    double* gpartial;   // gather partials, value-major: [value][gather blocks]
     int* bad_count;     // excluded-volume pairs needing the numerical fallback in a CG move
 };
 
@@ -695,15 +695,19 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
     if (st->done) return;
     const int phase = st->phase;
     if (phase != PH_LS && phase != PH_MOVE) return;
-    const int b = blockIdx.x * BLOCK + threadIdx.x;
+    // block b covers the 3 * BLOCK coordinates of beads [BLOCK b, BLOCK (b + 1)), coalesced
+    const long e0 = 3L * BLOCK * blockIdx.x + threadIdx.x, ne = 3L * s.n_beads;
     const int par = st->parity, dpar = st->dpar;
     const double* __restrict__ xc = B.x[par];
     const double* __restrict__ dc = B.d[dpar];
     double* __restrict__ xn = B.x[par ^ 1];
     if (phase == PH_MOVE) {
         const double lam = st->lambda;
-        if (b < s.n_beads)
-            for (int q = 0; q < 3; q++) xn[3 * b + q] = xc[3 * b + q] + lam * dc[3 * b + q];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const long e = e0 + (long)k * BLOCK;
+            if (e < ne) xn[e] = xc[e] + lam * dc[e];
+        }
         return;
     }
     const bool first = st->ls_active == 0;
@@ -711,21 +715,22 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
     const double lam0 = st->ls_lambda, beta = st->dir_beta;
     const double* __restrict__ fcur = B.f[st->fpar];
     double v[2] = {0.0, INFINITY};
-    if (b < s.n_beads) {
-        const double maxd = st->max_dist;
-        double* dn = B.d[dpar ^ 1];
-        for (int q = 0; q < 3; q++) {
-            const double fv = fcur[3 * b + q];
-            const double de = pend ? fv + beta * dc[3 * b + q] : dc[3 * b + q];
-            xn[3 * b + q] = xc[3 * b + q] + lam0 * de;   // trial 1 = speculative x_next
-            if (!first) B.xh[3 * b + q] = xc[3 * b + q] + (lam0 * 0.5) * de;
-            if (first) {
-                v[0] += de * fv;                          // d.F
-                v[1] = rmin(v[1], maxd / fabs(de));       // MController.h:293-305
-                if (pend) {
-                    dn[3 * b + q] = de;                   // shiftSearchDir materialized
-                    B.fp[3 * b + q] = fv;                 // forcePrev = force
-                }
+    const double maxd = st->max_dist;
+    double* __restrict__ dn = B.d[dpar ^ 1];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const long e = e0 + (long)k * BLOCK;
+        if (e >= ne) continue;
+        const double fv = fcur[e];
+        const double de = pend ? fv + beta * dc[e] : dc[e];
+        xn[e] = xc[e] + lam0 * de;   // trial 1 = speculative x_next
+        if (!first) B.xh[e] = xc[e] + (lam0 * 0.5) * de;
+        if (first) {
+            v[0] += de * fv;                          // d.F
+            v[1] = rmin(v[1], maxd / fabs(de));       // MController.h:293-305
+            if (pend) {
+                dn[e] = de;                           // shiftSearchDir materialized
+                B.fp[e] = fv;                         // forcePrev = force
             }
         }
     }
@@ -733,9 +738,9 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
     const int op2[2] = {0, 2};
     block_reduce_n<2>(v, op2, lds);
     if (threadIdx.x == 0) {
-        const long k = (long)PSTRIDE * (s.blk[NFF] + blockIdx.x);
+        const long np = s.blk[NFF + 1], k = s.blk[NFF] + blockIdx.x;   // value-major
         B.partial[k] = v[0];
-        B.partial[k + 1] = v[1];
+        B.partial[np + k] = v[1];
     }
 }
 
@@ -781,10 +786,10 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     const int op3[3] = {0, 0, 0};
     block_reduce_n<3>(v, op3, lds);
     if (threadIdx.x == 0) {
-        const long k = (long)PSTRIDE * blockIdx.x;
+        const long np = s.blk[NFF + 1], k = blockIdx.x;   // value-major
         B.partial[k] = v[0];
-        B.partial[k + 1] = v[1];
-        B.partial[k + 2] = v[2];
+        B.partial[np + k] = v[1];
+        B.partial[2 * np + k] = v[2];
     }
 }
 
@@ -934,7 +939,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     v[1] = wave_sum(v[1]);
     v[2] = wave_max(v[2]);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 3; q++) B.gpartial[(long)PSTRIDE * blockIdx.x + q] = v[q];
+        for (int q = 0; q < 3; q++) B.gpartial[(long)q * gridDim.x + blockIdx.x] = v[q];   // value-major
 }
 
 // Scalar logic after a force evaluation: maxF, energyRelChange, beta, the minimum-energy
@@ -1019,9 +1024,9 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int bk = base + threadIdx.x + j * FIN;
-            p0[j] = bk < nls ? B.partial[(long)PSTRIDE * bk] : 0.0;
-            p1[j] = bk < nls ? B.partial[(long)PSTRIDE * bk + 1] : 0.0;
-            p2[j] = bk < T ? B.partial[(long)PSTRIDE * bk + 2] : 0.0;
+            p0[j] = bk < nls ? B.partial[bk] : 0.0;
+            p1[j] = bk < nls ? B.partial[(long)nls + bk] : 0.0;
+            p2[j] = bk < T ? B.partial[2L * nls + bk] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
@@ -1042,7 +1047,7 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int bk = base + threadIdx.x + j * FIN;
-            for (int q = 0; q < 3; q++) p[j][q] = bk < ngb ? B.gpartial[(long)PSTRIDE * bk + q] : 0.0;
+            for (int q = 0; q < 3; q++) p[j][q] = bk < ngb ? B.gpartial[(long)q * ngb + bk] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 4; j++) {
