@@ -132,19 +132,38 @@ __device__ inline int ff_of_block(const DevSys& s, int b) {
     return ff;   // NFF means a bead block
 }
 
-// ---- deterministic block reduction helpers (64-wide waves) ----------------------------
-__device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
+// ---- deterministic reduction helpers (64-wide waves) ----------------------------------
+// A wave reduction in six DPP steps on the VALU (no LDS traffic): row shifts by 1, 2, 4 and 8
+// leave each 16-lane row's total in its lane 15, row_bcast:15 adds row 0's total into row 1 and
+// row 2's into row 3, row_bcast:31 adds lane 31 into rows 2 and 3; lane 63 then holds the
+// reduction of all 64 lanes, read back into every lane. Lanes without a source keep the
+// identity. The whole wave must be active.
+template <int CTRL, int ROWS>
+__device__ inline double dpp_or(double v, double ident) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(ident), __double2loint(v), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(ident), __double2hiint(v), CTRL, ROWS, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
-__device__ inline double wave_max(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = rmax(v, __shfl_down(v, o, 64));
-    return v;
+__device__ inline double lane63(double v) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
 }
-__device__ inline double wave_min(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = rmin(v, __shfl_down(v, o, 64));
-    return v;
+template <int OP>   // 0 sum, 1 max, 2 min
+__device__ inline double comb(double a, double b) { return OP == 0 ? a + b : OP == 1 ? rmax(a, b) : rmin(a, b); }
+template <int OP>
+__device__ inline double wave_reduce(double v) {
+    const double id = OP == 0 ? 0.0 : OP == 1 ? -INFINITY : INFINITY;
+    v = comb<OP>(v, dpp_or<0x111, 0xF>(v, id));   // row_shr:1
+    v = comb<OP>(v, dpp_or<0x112, 0xF>(v, id));   // row_shr:2
+    v = comb<OP>(v, dpp_or<0x114, 0xF>(v, id));   // row_shr:4
+    v = comb<OP>(v, dpp_or<0x118, 0xF>(v, id));   // row_shr:8
+    v = comb<OP>(v, dpp_or<0x142, 0xA>(v, id));   // row_bcast:15 -> rows 1, 3
+    v = comb<OP>(v, dpp_or<0x143, 0xC>(v, id));   // row_bcast:31 -> rows 2, 3
+    return lane63(v);
 }
+__device__ inline double wave_sum(double v) { return wave_reduce<0>(v); }
+__device__ inline double wave_max(double v) { return wave_reduce<1>(v); }
+__device__ inline double wave_min(double v) { return wave_reduce<2>(v); }
 // op: 0 sum, 1 max, 2 min, for NV values at once over a block of NTH threads. Results valid in
 // thread 0.
 template <int NV, int NTH = BLOCK>
