@@ -510,6 +510,39 @@ __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int*
     return run_defers_n<NTH>(s, nsh, lds, sum, badc, ncode);
 }
 
+// v.v and max|v| of a vector (the minimizer's start: F.F and maxF): grid-stride block partials,
+// value-major, then one block in a fixed order into out[0..1].
+__global__ void __launch_bounds__(BLOCK) k_vec_stats(const double* __restrict__ v, long n, double* __restrict__ partial) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    double a[2] = {0.0, 0.0};
+    for (long i = (long)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (long)gridDim.x * BLOCK) {
+        const double x = v[i];
+        a[0] += x * x;
+        a[1] = rmax(a[1], fabs(x));
+    }
+    const int op[2] = {0, 1};
+    block_reduce_n<2>(a, op, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = a[0];
+        partial[gridDim.x + blockIdx.x] = a[1];
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_vec_stats_fin(const double* __restrict__ partial, int nb,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    double a[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += BLOCK) {
+        a[0] += partial[b];
+        a[1] = rmax(a[1], partial[nb + b]);
+    }
+    const int op[2] = {0, 1};
+    block_reduce_n<2>(a, op, lds);
+    if (threadIdx.x == 0) {
+        out[0] = a[0];
+        out[1] = a[1];
+    }
+}
+
 // Per-FF energies at x, block partials. Grid: interaction blocks only.
 template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_energy_report(DevSys s, const double* __restrict__ x,

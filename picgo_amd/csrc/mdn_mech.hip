@@ -99,6 +99,7 @@ struct mdn_ctx {
     DBuf<int> bad_count, bad_list;
     DBuf<unsigned> ticket;
     DBuf<double> eout;
+    DBuf<double> vstat;   // k_vec_stats partials (2 * VSTAT_BLOCKS)
     DBuf<CgState> cg;
     DBuf<double> fallback_partial;
     DBuf<EnDefer> defer;
@@ -168,6 +169,7 @@ hipError_t upload(mdn_ctx* ctx, DBuf<T>& b, const std::vector<T>& v) {
 }
 
 int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+const int VSTAT_BLOCKS = 1024;   // grid of k_vec_stats
 
 // Block and slot layout of the term / bead grids for the current interaction counts.
 void layout(mdn_ctx* ctx) {
@@ -376,6 +378,7 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
         return MDN_ERR_HIP;
     }
     if (ctx->ticket.ensure(8) != hipSuccess || ctx->eout.ensure(32) != hipSuccess ||
+        ctx->vstat.ensure(2 * VSTAT_BLOCKS) != hipSuccess ||
         ctx->bad_count.ensure(1) != hipSuccess || ctx->cg.ensure(1) != hipSuccess ||
         ctx->nl_err.ensure(1) != hipSuccess || ctx->defer.ensure(DEFER_CAP) != hipSuccess ||
         ctx->defer_n.ensure(1) != hipSuccess) {
@@ -408,9 +411,9 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                           &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
                           &ctx->mot_p2, &ctx->mot_sf, &ctx->ev_ks, &ctx->bnd_krep, &ctx->bnd_slen,
                           &ctx->d_cyl_kexvol, &ctx->d_cyl_eql, &ctx->d_krep_e, &ctx->d_slen_e, &ctx->mid,
-                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->xh, &ctx->d[0], &ctx->d[1], &ctx->xmin[0],
-                          &ctx->xmin[1], &ctx->scratch_f,
-                          &ctx->ext, &ctx->partial, &ctx->eout, &ctx->fallback_partial};
+                          &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->xh, &ctx->d[0], &ctx->d[1],
+                          &ctx->xmin[0], &ctx->xmin[1], &ctx->scratch_f, &ctx->ext, &ctx->partial, &ctx->eout,
+                          &ctx->vstat, &ctx->fallback_partial};
     for (auto* b : db) b->release();
     ctx->d_cyl_minus.release();
     ctx->pmask.release();
@@ -1033,14 +1036,15 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     }
     if ((st = eval_slots(ctx, ctx->x[0].p, prec))) return st;
     if ((st = gather(ctx, 3, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p, prec))) return st;
-    std::vector<double> hf(nv);
-    HIPCHK(hipMemcpyAsync(hf.data(), ctx->f.p, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    // F.F and maxF on the device (fixed-order reduction), two doubles back
+    const int nsb = std::min(VSTAT_BLOCKS, std::max(1, ceil_div(nv, BLOCK)));
+    hipLaunchKernelGGL(k_vec_stats, dim3(nsb), dim3(BLOCK), 0, ctx->stream, (const double*)ctx->f.p, (long)nv,
+                       ctx->vstat.p);
+    hipLaunchKernelGGL(k_vec_stats_fin, dim3(1), dim3(BLOCK), 0, ctx->stream, (const double*)ctx->vstat.p, nsb,
+                       ctx->eout.p + 16);
+    HIPCHK(hipMemcpyAsync(ctx->h_eout + 16, ctx->eout.p + 16, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    double mx = 0.0, ff = 0.0;
-    for (int i = 0; i < nv; i++) {
-        mx = rmax(mx, std::fabs(hf[i]));
-        ff += hf[i] * hf[i];
-    }
+    const double ff = ctx->h_eout[16], mx = ctx->h_eout[17];
     CgState& S = *ctx->h_cg;
     memset(&S, 0, sizeof(S));
     S.method = p->method;
@@ -1069,7 +1073,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     res->energy_call_limit = S.energy_call_limit;
     res->grad_tol = S.grad_tol;
     res->energy_rel_tol = S.energy_rel_tol;
-    if (!std::isfinite(mx)) return fail(ctx, MDN_ERR_NONFINITE, "maxF is not finite at the start");
+    if (!std::isfinite(mx) || !std::isfinite(ff)) return fail(ctx, MDN_ERR_NONFINITE, "maxF is not finite at the start");
     const bool limit0 = (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
     S.done = (S.grad_inf_norm < S.grad_tol || S.energy_rel_change < S.energy_rel_tol || limit0) ? 1 : 0;
     HIPCHK(hipMemsetAsync(ctx->d[0].p, 0, sizeof(double) * nv, ctx->stream));   // searchDir.assign(numDof, 0)

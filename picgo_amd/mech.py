@@ -189,14 +189,14 @@ class ForceFieldManager:
         self._chk(self._L.mdn_get_coords(self._h, x.ctypes.data_as(dp)))
         return x
 
-    def minimize(self, params, coord):
+    def minimize(self, params, coord, want_forces=True):
         """CGMethod::minimize from ``coord`` with the neighbor lists frozen.
-        Returns (coord_out, force_out, MinResult)."""
+        Returns (coord_out, force_out or None, MinResult)."""
         x = _f64(coord).copy()
-        f = np.zeros_like(x)
+        f = np.zeros_like(x) if want_forces else None
         res = abi.MinResult()
         self._chk(self._L.mdn_minimize(self._h, C.byref(params), x.ctypes.data_as(dp),
-                                       f.ctypes.data_as(dp), C.byref(res)))
+                                       f.ctypes.data_as(dp) if want_forces else None, C.byref(res)))
         return x, f, res
 
     def last_min_stats(self):
