@@ -130,6 +130,33 @@ def run_stress(args):
     print(json.dumps(line), flush=True)
 
 
+def run_minimize(args):
+    """One full minimization (reference criterion) of the config-1 or config-4 network from the
+    seeded start, after a 50-energy-call warm-up: the unit the CG kernel profiles are taken on."""
+    net = synth.Network.config(args.net)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    ffm.reset_neighbor_lists(net.coord)
+    prec = abi.MDN_PREC_MIXED if args.prec == "mixed" else abi.MDN_PREC_FP64
+    ffm.minimize(abi.default_cg_params(precision=prec, energy_call_limit=50), net.coord)
+    t0 = time.perf_counter()
+    x, f, res = ffm.minimize(abi.default_cg_params(precision=prec), net.coord, want_forces=False)
+    dt = time.perf_counter() - t0
+    st = ffm.last_min_stats()
+    line = {
+        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
+        "value": 1.0 / dt, "unit": "minimizations/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == abi.MDN_PREC_FP64 else "f32 term math + f64 excluded volume / accumulation",
+        "data": f"synthetic (seeded generator, config {args.net})",
+        "config": {"workload": f"one minimization of the config-{args.net} network", "n_beads": net.n_beads},
+        "cg_iterations": res.num_iter, "us_per_cg_iteration": 1e6 * dt / max(1, res.num_iter),
+        "line_search_launches": st["ls_launches"],
+        "criterion": "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit"),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_batched(args):
     """The config-3 batch in one persistent kernel (mixed and FP64), against the same systems
     minimized one after another by the single-system minimizer (FP64)."""
@@ -183,10 +210,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-min", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["config1", "stress", "batched"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "stress", "batched", "minimize"], default="config1",
                     help="config1: the metric's single network (default); stress: 20,000 x 20 filaments in a "
                          "10 um sphere, successive minimizations with 5 nm noise and a neighbor-list rebuild "
-                         "each; batched: 256 uneven networks in one persistent kernel")
+                         "each; batched: 256 uneven networks in one persistent kernel; minimize: one "
+                         "minimization of the --net network in --prec precision (for profiling)")
+    ap.add_argument("--net", type=int, choices=[1, 4], default=1)
+    ap.add_argument("--prec", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--stress-count", type=int, default=100)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--calib-copy", action="store_true",
@@ -209,6 +239,8 @@ def main():
         return run_stress(args)
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "minimize":
+        return run_minimize(args)
 
     net = synth.Network.config(1)
     ffm = mech.ForceFieldManager(0)
