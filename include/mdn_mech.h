@@ -212,6 +212,18 @@ int mdn_ff_forces(mdn_ctx* ctx, int32_t ff, const double* coord, double* force, 
  * (LinkerStretchingHarmonic.cpp:291, ForceField::assignforcemags). */
 int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor_out);
 
+/* Boundary load forces on the filament ends after a minimization, for the chemistry's rate
+ * changers (BoundaryCylinderRepulsion::computeLoadForces, BoundaryCylinderRepulsion.cpp:122-187,
+ * reached through ForceFieldManager::computeLoadForces, ForceFieldManager.cpp:292-310, and
+ * MController's afterMinimization_ -> computeAuxParams, MController.h:238-266), from the boundary
+ * neighbor list of the last mdn_nl_build and the coordinates `coord` (NULL: the current device
+ * coordinates, i.e. the last minimization's result). For filament f, load_plus[f * stride + i] is
+ * Bead::loadForcesP[i] of its plus-end bead and load_minus[f * stride + i] Bead::loadForcesM[i] of
+ * its minus-end bead, i < num_monomers[type] (cylinderNumMon; monomer_size[type] is monomerSize);
+ * entries up to `stride` are zero. Filament types must be below n_types (<= 8). FP64. */
+int mdn_boundary_load_forces(mdn_ctx* ctx, const double* coord, int32_t n_types, const double* monomer_size,
+                             const int32_t* num_monomers, int32_t stride, double* load_plus, double* load_minus);
+
 /* CGMethod::minimize (CGMethod.hpp:826-1759) driven by MController::run
  * (MController.h:108-211): minimizes from `coord` (host, in/out), writes the final forces to
  * force_out (3*n_beads, may be NULL) and the statistics to res. The neighbor list must have

@@ -121,6 +121,7 @@ typedef struct or_system {
     /* boundary (from the boundary NL) */
     int n_elem; mdn_boundary_element* elems; double bnd_cutoff;
     int n_bint; int* bnd_beadSet; double *bnd_krep, *bnd_slen; int* bnd_nneighbors;
+    char* bnd_member;   /* [n_elem * n_cyl]: cylinder c in element e's neighbor list */
     float nl_rmax, nl_rmin; double grid[3];
     int use_cells;
 } or_system;
@@ -166,6 +167,16 @@ int or_cg_minimize(const mdn_cg_params* p, double* coord, int ndof, int nvar,
  * (MController.h:273-339) as the recovery function. The neighbor lists must be built. */
 int or_system_minimize(or_system* s, const mdn_cg_params* p, double* coord, double* force_out,
                        mdn_min_result* res);
+
+/* Boundary load forces on the filament ends after a minimization
+ * (BoundaryCylinderRepulsion::computeLoadForces, BoundaryCylinderRepulsion.cpp:122-187, with the
+ * reset of ForceFieldManager::computeLoadForces, ForceFieldManager.cpp:292-310), from the
+ * boundary neighbor list of the last or_system_nl. out_plus[f * stride + i] = loadForcesP[i] of
+ * filament f's plus-end bead, out_minus likewise for its minus-end bead; i < num_mon[type]
+ * (cylinderNumMon), zeros up to stride. Returns MDN_ERR_ARG on an unknown type or a stride
+ * below num_mon. */
+int or_system_load_forces(const or_system* s, const double* coord, int n_types, const double* mon_size,
+                          const int* num_mon, int stride, double* out_plus, double* out_minus);
 
 /* ---- handle API for language bindings ------------------------------------------------ */
 or_system* or_system_create(const mdn_system_desc* d, int* status);

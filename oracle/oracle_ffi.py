@@ -88,6 +88,8 @@ def lib(fast=False):
         L.or_system_nbend.argtypes = [C.c_void_p]
         L.or_system_stretchforce.restype = dp
         L.or_system_stretchforce.argtypes = [C.c_void_p, C.c_int]
+        L.or_system_load_forces.restype = C.c_int
+        L.or_system_load_forces.argtypes = [C.c_void_p, dp, C.c_int, dp, ip, C.c_int, dp, dp]
         L.or_ffm_energy.restype = C.c_double
         L.or_ffm_energy.argtypes = [C.c_void_p, dp, C.POINTER(abi.EnergyReport)]
         L.or_ffm_forces.argtypes = [C.c_void_p, dp, dp]
@@ -264,6 +266,7 @@ class OracleSystem:
             raise ValueError(f"or_system_create: {abi.status_name(st.value)}")
         self._L.or_system_set_cells(self._h, 1 if use_cells else 0)
         self.n_beads = desc.contents.n_beads if isinstance(desc, C._Pointer) else desc.n_beads
+        self.n_filaments = desc.contents.n_filaments if isinstance(desc, C._Pointer) else desc.n_filaments
 
     def build_nl(self, coord):
         c = _arr(coord, np.float64)
@@ -316,6 +319,20 @@ class OracleSystem:
     def stretch_forces_of(self, motor=False):
         n = self._L.or_system_ncyl(self._h)  # placeholder size guard
         return self._L.or_system_stretchforce(self._h, 1 if motor else 0), n
+
+    def load_forces(self, coord, mon_size, num_mon, stride):
+        """Boundary load forces on the filament ends: (plus, minus), each [n_filaments, stride]
+        (or_system_load_forces)."""
+        ms = _arr(mon_size, np.float64)
+        nm = _arr(num_mon, np.int32)
+        c = _arr(coord, np.float64)
+        nf = self.n_filaments
+        outp, outm = np.zeros(nf * stride), np.zeros(nf * stride)
+        st = self._L.or_system_load_forces(self._h, _p(c), len(ms), _p(ms), nm.ctypes.data_as(ip), stride,
+                                           _p(outp), _p(outm))
+        if st != 0:
+            raise ValueError(f"or_system_load_forces: {abi.status_name(st)}")
+        return outp.reshape(nf, stride), outm.reshape(nf, stride)
 
     def minimize(self, params, coord):
         x = _arr(coord, np.float64).copy()

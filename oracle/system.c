@@ -168,11 +168,14 @@ int or_system_nl(or_system* s, const double* coord) {
                 nint += s->cyl_minus_end[c] ? 2 : 1;
     s->n_bint = nint;
     OR_ALLOC(s->bnd_beadSet, nint); OR_ALLOC(s->bnd_krep, nint); OR_ALLOC(s->bnd_slen, nint);
+    free(s->bnd_member);
+    OR_ALLOC(s->bnd_member, (long)s->n_elem * s->n_cyl);
     int cum = 0;
     for (int e = 0; e < s->n_elem; e++) {
         int idx = 0;
         for (int c = 0; c < s->n_cyl; c++) {
             if (!(or_boundary_distance(&s->elems[e], &mid[3 * c]) < s->bnd_cutoff)) continue;
+            s->bnd_member[(long)e * s->n_cyl + c] = 1;
             if (s->cyl_minus_end[c]) {
                 s->bnd_beadSet[cum + idx] = 3 * s->cyl_b1[c];
                 s->bnd_krep[cum + idx] = s->elems[e].krep;
@@ -191,6 +194,58 @@ int or_system_nl(or_system* s, const double* coord) {
     return MDN_OK;
 }
 
+/* boundaryCylinderRepulsionLoadForce (BoundaryCylinderRepulsion.cpp:125-162) for bead bd with
+ * its cylinder neighbour bo, accumulated into out[0 .. nmon) */
+static void load_force_end(const mdn_boundary_element* be, const double* xd, const double* xo, double msize,
+                           int nmon, double* out) {
+    /* dir = normalizedVector(bd - bo): magnitude2 summed from zero, then each component / norm */
+    double dir[3] = {xd[0] - xo[0], xd[1] - xo[1], xd[2] - xo[2]};
+    const double norm = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    for (int k = 0; k < 3; k++) dir[k] /= norm;
+    for (int i = 0; i < nmon; i++) {
+        const double step = i * msize;
+        const double p[3] = {xd[0] + step * dir[0], xd[1] + step * dir[1], xd[2] + step * dir[2]};
+        double n[3];
+        or_boundary_normal(be, p, n);
+        const double dotnd = n[0] * dir[0] + n[1] * dir[1] + n[2] * dir[2];
+        const double proj = (-dotnd < 0.0) ? 0.0 : -dotnd;   /* std::max(-dot, 0.0) */
+        /* BoundaryCylinderRepulsionExp::loadForces (BoundaryCylinderRepulsionExp.cpp:113-118) */
+        const double R = -or_boundary_distance(be, p) / be->slen;
+        const double lf = be->krep * exp(R) / be->slen;
+        out[i] += proj * lf;
+    }
+}
+
+int or_system_load_forces(const or_system* s, const double* coord, int n_types, const double* mon_size,
+                          const int* num_mon, int stride, double* out_plus, double* out_minus) {
+    for (int t = 0; t < n_types; t++)
+        if (num_mon[t] < 0 || num_mon[t] > stride) return MDN_ERR_ARG;
+    memset(out_plus, 0, sizeof(double) * (size_t)s->n_fil * stride);
+    memset(out_minus, 0, sizeof(double) * (size_t)s->n_fil * stride);
+    if (!s->bnd_member) return MDN_OK;
+    /* the last cylinder of each filament is its plus end (Cylinder::isPlusEnd) */
+    int* last = calloc((size_t)(s->n_fil > 0 ? s->n_fil : 1), sizeof(int));
+    for (int c = 0; c < s->n_cyl; c++) last[s->cyl_fil[c]] = c;
+    int st = MDN_OK;
+    for (int e = 0; e < s->n_elem && st == MDN_OK; e++) {
+        for (int c = 0; c < s->n_cyl; c++) {
+            if (!s->bnd_member[(long)e * s->n_cyl + c]) continue;
+            const int f = s->cyl_fil[c], t = s->cyl_type[c];
+            if (t < 0 || t >= n_types) {
+                st = MDN_ERR_ARG;
+                break;
+            }
+            const double* x1 = &coord[3 * s->cyl_b1[c]];
+            const double* x2 = &coord[3 * s->cyl_b2[c]];
+            if (last[f] == c) load_force_end(&s->elems[e], x2, x1, mon_size[t], num_mon[t], &out_plus[(long)f * stride]);
+            if (s->cyl_minus_end[c])
+                load_force_end(&s->elems[e], x1, x2, mon_size[t], num_mon[t], &out_minus[(long)f * stride]);
+        }
+    }
+    free(last);
+    return st;
+}
+
 void or_system_free(or_system* s) {
     void* ptrs[] = {s->cyl_b1, s->cyl_b2, s->cyl_fil, s->cyl_pos, s->cyl_type, s->cyl_minus_end,
                     s->cyl_kexvol, s->cyl_eql, s->str_beadSet, s->str_k, s->str_eql,
@@ -198,7 +253,7 @@ void or_system_free(or_system* s) {
                     s->lnk_eql, s->lnk_p1, s->lnk_p2, s->lnk_sf, s->mot_beadSet, s->mot_k,
                     s->mot_eql, s->mot_p1, s->mot_p2, s->mot_sf, s->pairs, s->ev_beadSet,
                     s->ev_krep, s->ev_eqlen, s->elems, s->bnd_beadSet, s->bnd_krep,
-                    s->bnd_slen, s->bnd_nneighbors};
+                    s->bnd_slen, s->bnd_nneighbors, s->bnd_member};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     memset(s, 0, sizeof(*s));
 }

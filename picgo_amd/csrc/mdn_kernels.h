@@ -1416,6 +1416,52 @@ __global__ void k_bnd_emit(NLDev n, DevSys s, const int* __restrict__ cnt, const
     bnd_b[o] = n.cyl_b2[c]; bnd_e[o] = e; krep[o] = krep_e[e]; slen[o] = slen_e[e];
 }
 
+// ---- boundary load forces on the filament ends ---------------------------------------
+// BoundaryCylinderRepulsion::computeLoadForces (BoundaryCylinderRepulsion.cpp:122-187, the
+// helper :125-162) after the reset of ForceFieldManager::computeLoadForces
+// (ForceFieldManager.cpp:292-310): for the plus-end cylinder of every filament (bead bd = its
+// second bead, bo = its first) and the minus-end cylinder (bd = its first bead, bo = its
+// second), loadForces[i] = sum over the boundary elements whose neighbor list holds the cylinder,
+// in element order, of max(-n(p_i).dir, 0) * kRep exp(-d(p_i) / lambda) / lambda
+// (BoundaryCylinderRepulsionExp::loadForces, BoundaryCylinderRepulsionExp.cpp:113-118) at
+// p_i = bd + (i * monomerSize) * dir, dir = normalizedVector(bd - bo), i < cylinderNumMon of the
+// filament's type. FP64. One thread per (end, filament, monomer); out[end][filament][stride].
+constexpr int MAX_FIL_TYPES = 8;
+struct LoadParams {
+    int n_types, stride;
+    double mon_size[MAX_FIL_TYPES];
+    int num_mon[MAX_FIL_TYPES];
+};
+__global__ void k_load_forces(NLDev n, DevSys s, const double* __restrict__ x, const int* __restrict__ fil_ends,
+                              const int* __restrict__ bnd_cnt, const double* __restrict__ krep_e,
+                              const double* __restrict__ slen_e, int n_fil, LoadParams P, double* __restrict__ out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2L * n_fil * P.stride) return;
+    const int end = (int)(i / ((long)n_fil * P.stride));          // 0: plus, 1: minus
+    const int f = (int)((i / P.stride) % n_fil), k = (int)(i % P.stride);
+    const int c = fil_ends[2 * f + (end == 0 ? 1 : 0)];             // last / first cylinder
+    const int t = n.cyl_type[c];
+    double acc = 0.0;
+    if (t >= 0 && t < P.n_types && k < P.num_mon[t]) {
+        const D3 xd = ld3(x, end == 0 ? n.cyl_b2[c] : n.cyl_b1[c]);
+        const D3 xo = ld3(x, end == 0 ? n.cyl_b1[c] : n.cyl_b2[c]);
+        double dir[3] = {xd.x - xo.x, xd.y - xo.y, xd.z - xo.z};
+        const double norm = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        for (int q = 0; q < 3; q++) dir[q] /= norm;
+        const double step = k * P.mon_size[t];
+        const D3 p{xd.x + step * dir[0], xd.y + step * dir[1], xd.z + step * dir[2]};
+        for (int e = 0; e < s.n_elem; e++) {
+            if (bnd_cnt[(long)e * n.n_cyl + c] == 0) continue;   // not in element e's neighbor list
+            const V3<double> nn = boundary_normal<double>(s.elem[e], p);
+            const double dotnd = nn.x * dir[0] + nn.y * dir[1] + nn.z * dir[2];
+            const double proj = rmax(-dotnd, 0.0);
+            const double R = -boundary_distance(s.elem[e], p) / slen_e[e];
+            acc += proj * (krep_e[e] * exp(R) / slen_e[e]);
+        }
+    }
+    out[i] = acc;
+}
+
 // dynamic incidence lists: per bead, the excluded-volume and boundary slot ids, ascending
 __global__ void k_inc_count(DevSys s, int* __restrict__ cnt) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -82,6 +82,8 @@ struct mdn_ctx {
     // cylinder tables for the NL
     DBuf<int> d_cyl_b1, d_cyl_b2, d_cyl_fil, d_cyl_pos, d_cyl_type;
     DBuf<char> d_cyl_minus;
+    DBuf<int> d_fil_ends;      // [2 * n_fil]: first and last cylinder of each filament
+    DBuf<double> load_out;     // boundary load forces [2][n_fil][stride]
     DBuf<double> d_cyl_kexvol, d_cyl_eql, d_krep_e, d_slen_e;
     // NL workspace
     DBuf<double> mid;
@@ -416,6 +418,8 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                           &ctx->vstat, &ctx->fallback_partial};
     for (auto* b : db) b->release();
     ctx->d_cyl_minus.release();
+    ctx->d_fil_ends.release();
+    ctx->load_out.release();
     ctx->pmask.release();
     ctx->d_cyl_bend.release();
     ctx->ev_U.release();
@@ -586,6 +590,16 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     HIPCHK(upload(ctx, ctx->d_cyl_fil, ctx->cyl_fil)); HIPCHK(upload(ctx, ctx->d_cyl_pos, ctx->cyl_pos));
     HIPCHK(upload(ctx, ctx->d_cyl_type, ctx->cyl_type)); HIPCHK(upload(ctx, ctx->d_cyl_minus, ctx->cyl_minus));
     HIPCHK(upload(ctx, ctx->d_cyl_kexvol, ctx->cyl_kexvol)); HIPCHK(upload(ctx, ctx->d_cyl_eql, ctx->cyl_eql));
+    {
+        std::vector<int> ends(2 * (size_t)std::max(1, ctx->n_fil), 0);
+        for (int f = 0, c0 = 0; f < ctx->n_fil; f++) {
+            const int nb = d->fil_bead_offset[f + 1] - d->fil_bead_offset[f];
+            ends[2 * f] = c0;               // minus-end cylinder
+            ends[2 * f + 1] = c0 + nb - 2;  // plus-end cylinder
+            c0 += nb - 1;
+        }
+        HIPCHK(upload(ctx, ctx->d_fil_ends, ends));
+    }
     {
         std::vector<double> ke(ctx->elems.size()), se(ctx->elems.size());
         for (size_t e = 0; e < ctx->elems.size(); e++) { ke[e] = ctx->elems[e].krep; se[e] = ctx->elems[e].slen; }
@@ -764,6 +778,50 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nl_built = true;
     if (n_pairs_out) *n_pairs_out = np;
+    return MDN_OK;
+}
+
+int mdn_boundary_load_forces(mdn_ctx* ctx, const double* coord, int32_t n_types, const double* monomer_size,
+                             const int32_t* num_monomers, int32_t stride, double* load_plus, double* load_minus) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!monomer_size || !num_monomers || !load_plus || !load_minus) return fail(ctx, MDN_ERR_ARG, "null argument");
+    if (n_types < 1 || n_types > MAX_FIL_TYPES)
+        return fail(ctx, MDN_ERR_ARG, "n_types must be in [1, %d]", MAX_FIL_TYPES);
+    LoadParams P;
+    memset(&P, 0, sizeof(P));
+    P.n_types = n_types;
+    P.stride = stride;
+    for (int t = 0; t < n_types; t++) {
+        if (num_monomers[t] < 0 || num_monomers[t] > stride)
+            return fail(ctx, MDN_ERR_ARG, "num_monomers[%d] = %d outside [0, stride = %d]", t, num_monomers[t], stride);
+        P.mon_size[t] = monomer_size[t];
+        P.num_mon[t] = num_monomers[t];
+    }
+    for (int c = 0; c < ctx->n_cyl; c++)
+        if (ctx->cyl_type[c] < 0 || ctx->cyl_type[c] >= n_types)
+            return fail(ctx, MDN_ERR_ARG, "filament %d has type %d, not below n_types = %d", ctx->cyl_fil[c],
+                        ctx->cyl_type[c], n_types);
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * 3 * ctx->n_beads, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    const long n_out = 2L * ctx->n_fil * stride;
+    if (n_out > 0) {
+        HIPCHK(ctx->load_out.ensure(n_out));
+        hipLaunchKernelGGL(k_load_forces, dim3(ceil_div(n_out, BLOCK)), dim3(BLOCK), 0, ctx->stream, nl_view(ctx),
+                           ctx->sys, x, (const int*)ctx->d_fil_ends.p, (const int*)ctx->bnd_cnt.p,
+                           (const double*)ctx->d_krep_e.p, (const double*)ctx->d_slen_e.p, ctx->n_fil, P,
+                           ctx->load_out.p);
+        HIPCHK(hipGetLastError());
+        const size_t half = sizeof(double) * (size_t)ctx->n_fil * stride;
+        HIPCHK(hipMemcpyAsync(load_plus, ctx->load_out.p, half, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(load_minus, ctx->load_out.p + (size_t)ctx->n_fil * stride, half, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
 }
 

@@ -30,7 +30,7 @@ EXPORTED = [
     "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
-    "mdn_param_grad", "mdn_hvp", "mdn_set_params",
+    "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces",
 ]
 
 
@@ -66,6 +66,7 @@ def lib():
         L.mdn_forces.argtypes = [C.c_void_p, dp, dp, C.c_int32]
         L.mdn_ff_forces.argtypes = [C.c_void_p, C.c_int32, dp, dp, C.c_int32]
         L.mdn_crosslink_stretch_forces.argtypes = [C.c_void_p, dp, dp]
+        L.mdn_boundary_load_forces.argtypes = [C.c_void_p, dp, C.c_int32, dp, i32p, C.c_int32, dp, dp]
         L.mdn_minimize.argtypes = [C.c_void_p, C.POINTER(abi.CgParams), dp, dp, C.POINTER(abi.MinResult)]
         L.mdn_bench_forces.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, dp]
         L.mdn_bench_energy.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
@@ -109,6 +110,7 @@ class ForceFieldManager:
         c = _f64(coord)
         self._chk(self._L.mdn_system_upload(self._h, ptr, c.ctypes.data_as(dp)))
         self.n_beads = ptr.contents.n_beads
+        self.n_filaments = ptr.contents.n_filaments
         return self
 
     upload = vectorize_all_force_fields
@@ -198,6 +200,21 @@ class ForceFieldManager:
         self._chk(self._L.mdn_minimize(self._h, C.byref(params), x.ctypes.data_as(dp),
                                        f.ctypes.data_as(dp) if want_forces else None, C.byref(res)))
         return x, f, res
+
+    def boundary_load_forces(self, monomer_size, num_monomers, coord=None, stride=None):
+        """Boundary load forces on the filament ends (mdn_boundary_load_forces): (plus, minus),
+        each [n_filaments, stride] (stride defaults to max(num_monomers)). ``monomer_size`` and
+        ``num_monomers`` are per filament type."""
+        ms = np.ascontiguousarray(np.atleast_1d(monomer_size), dtype=np.float64)
+        nm = np.ascontiguousarray(np.atleast_1d(num_monomers), dtype=np.int32)
+        stride = int(nm.max()) if stride is None else int(stride)
+        nf = self.n_filaments
+        lp, lm = np.zeros(nf * stride), np.zeros(nf * stride)
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_boundary_load_forces(self._h, cp, len(ms), ms.ctypes.data_as(dp),
+                                                   nm.ctypes.data_as(i32p), stride, lp.ctypes.data_as(dp),
+                                                   lm.ctypes.data_as(dp)))
+        return lp.reshape(nf, stride), lm.reshape(nf, stride)
 
     def last_min_stats(self):
         out = (C.c_int64 * 4)()

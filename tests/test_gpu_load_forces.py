@@ -1,0 +1,74 @@
+"""GPU: boundary load forces on the filament ends (mdn_boundary_load_forces) against the oracle
+(pinned on closed forms in tests/test_load_forces.py): the hand geometries, the config-0 network
+in its cube after a short minimization, and a config-0-sized network with one- to three-cylinder
+filaments in a sphere. The device's exp differs from the host's by at most an ulp, so the bar is
+1e-13 relative to the largest load."""
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi, mech, synth
+
+from .test_load_forces import MON, NMON, hand_systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(g, o, coord, types=1, stride=NMON):
+    ms, nm = [MON] * types, [NMON] * types
+    gp, gm = g.boundary_load_forces(ms, nm, coord=coord, stride=stride)
+    op, om = o.load_forces(coord, ms, nm, stride)
+    scale = max(float(np.abs(op).max()), float(np.abs(om).max()), 1e-300)
+    print(f"max load {scale:.4e}; max |dev - oracle| {max(np.abs(gp - op).max(), np.abs(gm - om).max()):.3e}; "
+          f"ends loaded {int((op[:, 0] > 0).sum())} plus, {int((om[:, 0] > 0).sum())} minus", flush=True)
+    np.testing.assert_allclose(gp, op, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_allclose(gm, om, rtol=0, atol=1e-13 * scale)
+    return op, om
+
+
+def test_load_forces_hand_geometries(gpu):
+    cube, ep, em, ball, eb = hand_systems()
+    for hs in (cube, ball):
+        g = gpu()
+        g.upload(hs.desc, hs.coord)
+        g.reset_neighbor_lists(hs.coord)
+        o = of.OracleSystem(hs.desc)
+        o.build_nl(hs.coord)
+        _compare(g, o, hs.coord)
+
+
+def test_load_forces_config0_after_minimization(gpu):
+    net = synth.Network.config(0)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    x, _, _ = g.minimize(abi.default_cg_params(energy_call_limit=40), net.coord)
+    o = of.OracleSystem(net.desc)
+    o.build_nl(net.coord)
+    op, om = _compare(g, o, x)
+    # the device's current coordinates are the minimization's result: coord=None gives the same
+    gp, gm = g.boundary_load_forces([MON], [NMON])
+    np.testing.assert_array_equal(gp, g.boundary_load_forces([MON], [NMON], coord=x)[0])
+    assert (op > 0).any()
+
+
+def test_load_forces_sphere_short_filaments(gpu):
+    net = synth.Network.config(0, boundary=1, radius=900.0, n_cyl_min=1, n_cyl_max=3, wall_margin=5.0)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    o = of.OracleSystem(net.desc)
+    o.build_nl(net.coord)
+    op, om = _compare(g, o, net.coord, stride=NMON + 3)
+    assert (op[:, NMON:] == 0).all()
+
+
+def test_load_forces_rejects_bad_arguments(gpu):
+    net = synth.Network.config(0)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    with pytest.raises(mech.MdnError):
+        g.boundary_load_forces([MON], [NMON], stride=NMON - 1)   # stride below cylinderNumMon
+    with pytest.raises(mech.MdnError):
+        g.boundary_load_forces([MON] * 9, [NMON] * 9)            # more than 8 types
