@@ -3,7 +3,8 @@
 //
 //   mdn::ForceFieldManager  ~ medyan::ForceFieldManager (ForceFieldManager.h:64-123):
 //       vectorizeAllForceFields, computeEnergy (+inf sentinel), computeIndividualEnergies,
-//       computeForces (zero-fill + accumulate), plus resetNeighborLists.
+//       computeForces (zero-fill + accumulate), assignallforcemags, computeLoadForces, plus
+//       resetNeighborLists.
 //   mdn::CGMethod           ~ medyan::CGMethod::minimize (CGMethod.hpp:1746-1759) with the
 //       ConjugateGradientParams fields (CGMethod.hpp:44-63).
 //
@@ -82,6 +83,30 @@ public:
     // ForceField::computeForces of one force field (accumulates)
     void computeForces(int ff, const double* coord, double* force, int numVar) {
         check(mdn_ff_forces(ctx_, ff, coord, force, numVar));
+    }
+    // ForceFieldManager::assignallforcemags (ForceFieldManager.cpp:347-351): the linkers' and
+    // motors' stretchForce after the last force evaluation, in vectorization order
+    void assignallforcemags(std::vector<double>& linkerStretchForce, std::vector<double>& motorStretchForce) {
+        int64_t counts[2 + MDN_NUM_FF];
+        check(mdn_system_counts(ctx_, counts));
+        linkerStretchForce.assign((size_t)counts[2 + MDN_FF_LINKER], 0.0);
+        motorStretchForce.assign((size_t)counts[2 + MDN_FF_MOTOR], 0.0);
+        check(mdn_crosslink_stretch_forces(ctx_, linkerStretchForce.data(), motorStretchForce.data()));
+    }
+    // ForceFieldManager::computeLoadForces (ForceFieldManager.cpp:292-310) at the current
+    // coordinates: per filament f, loadForcesP[f * stride + i] of its plus-end bead and
+    // loadForcesM[f * stride + i] of its minus-end bead, stride = max(cylinderNumMon);
+    // monomerSize and cylinderNumMon are per filament type (SysParams::Geometry())
+    int computeLoadForces(int nFilaments, const std::vector<double>& monomerSize, const std::vector<int>& cylinderNumMon,
+                          std::vector<double>& loadForcesP, std::vector<double>& loadForcesM) {
+        int stride = 0;
+        for (int n : cylinderNumMon) stride = n > stride ? n : stride;
+        loadForcesP.assign((size_t)nFilaments * stride, 0.0);
+        loadForcesM.assign((size_t)nFilaments * stride, 0.0);
+        std::vector<int32_t> nm(cylinderNumMon.begin(), cylinderNumMon.end());
+        check(mdn_boundary_load_forces(ctx_, nullptr, (int32_t)monomerSize.size(), monomerSize.data(), nm.data(), stride,
+                                       loadForcesP.data(), loadForcesM.data()));
+        return stride;
     }
     mdn_ctx* ctx() const { return ctx_; }
     int numVar() const { return nvar_; }

@@ -1,6 +1,7 @@
 // C++ caller of the drop-in adapters (include/mdn_mech.hpp): the MController-shaped sequence
-// serialize/vectorize -> reset neighbor lists -> minimize, on the seeded 50-filament network.
-// Prints "pairs energy_before energy_after iterations max_force converged".
+// serialize/vectorize -> reset neighbor lists -> minimize -> stretch forces and boundary load
+// forces, on the seeded 50-filament network. Prints "pairs energy_before energy_after iterations
+// max_force converged n_linker_stretch_forces n_loaded_filament_ends".
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -28,8 +29,14 @@ int main() {
         auto res = cg.minimize(ffm, p, coord, &force);
         double mx = 0;
         for (double v : force) mx = std::fmax(mx, std::fabs(v));
-        std::printf("%ld %.10e %.10e %d %.6e %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
-                    (int)res.converged());
+        // the post-minimization consumers (MController::afterMinimization_)
+        std::vector<double> lsf, msf, lp, lm;
+        ffm.assignallforcemags(lsf, msf);
+        const int stride = ffm.computeLoadForces(d->n_filaments, {2.7}, {40}, lp, lm);
+        int loaded = 0;
+        for (int fi = 0; fi < d->n_filaments; fi++) loaded += (lp[(size_t)fi * stride] > 0) + (lm[(size_t)fi * stride] > 0);
+        std::printf("%ld %.10e %.10e %d %.6e %d %zu %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
+                    (int)res.converged(), lsf.size(), loaded);
     } catch (const mdn::Error& e) {
         std::printf("error %d %s\n", e.code, e.what());
         mdn_synth_destroy(net);
