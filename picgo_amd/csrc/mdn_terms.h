@@ -504,14 +504,13 @@ MDN_HD inline double exvol_energy_forces(D3 c1, D3 c2, D3 c3, D3 c4, double kRep
     ExvTerms<R> t;
     exvol_terms<R>(c1, c2, c3, c4, t);
     R U = exvol_energy_of<R>(t, (R)kRepScaled);
+    // a coplanar pair's invariants are replaced by the shifted geometry's, so a wave holding
+    // both kinds of pairs runs the force assembly once (two calls would run it under both masks)
     if (are_in_plane(c1, c2, c3, c4)) {
         D3 c2s = move_out_of_plane(c1, c2, c3, 2, 0.01);
-        ExvTerms<R> ts;
-        exvol_terms<R>(c1, c2s, c3, c4, ts);
-        *fok = exvol_forces_from<R>(ts, (R)kRepScaled, out);
-    } else {
-        *fok = exvol_forces_from<R>(t, (R)kRepScaled, out);
+        exvol_terms<R>(c1, c2s, c3, c4, t);
     }
+    *fok = exvol_forces_from<R>(t, (R)kRepScaled, out);
     if (energy_bad<R>(U)) {
         *deferred = 1;
         return 0.0;
