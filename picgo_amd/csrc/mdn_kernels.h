@@ -311,13 +311,21 @@ __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R
         D3 c1 = pos_of(p, s.ev_b[4 * tp]), c2 = pos_of(p, s.ev_b[4 * tp + 1]), c3 = pos_of(p, s.ev_b[4 * tp + 2]),
            c4 = pos_of(p, s.ev_b[4 * tp + 3]);
         const long id = s.slot0[FF_EXVOL] + 4L * tp;
-        // FP64, the reference's expression order
+        // FP64 in both paths: the reference's expression order (FP64 path), the reduced-operation
+        // form of the same closed form (mixed path)
+        constexpr bool fast = sizeof(R) == 4;
         int deferred = 0;
         double U = 0.0;
         if (slots) {
             bool ok = true;
-            if (code < 0) ok = exvol_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], out);
-            else U = exvol_energy_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
+            if (fast) {
+                if (code < 0) ok = exvol_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], out);
+                else U = exvol_energy_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
+            } else if (code < 0) {
+                ok = exvol_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], out);
+            } else {
+                U = exvol_energy_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
+            }
             if (!ok) {
                 out[0] = out[1] = out[2] = out[3] = D3{0.0, 0.0, 0.0};
                 *fbad = 1;
@@ -325,7 +333,8 @@ __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R
             }
             for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
         } else {
-            U = exvol_energy<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred);
+            U = fast ? exvol_energy_fast(c1, c2, c3, c4, s.ev_ks[tp], &deferred)
+                     : exvol_energy<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred);
         }
         if (deferred && code >= 0) push_defer(s, code, tp, c1, c2, c3, c4);
         return U;

@@ -273,7 +273,7 @@ template <typename R> struct ExvTerms {
 
 // The pair invariants (CylinderExclVolRepulsion.cpp:353-397) from five FP64 difference vectors:
 // scalarProduct(p1,p2,p3,p4) = (p2 - p1).(p4 - p3), c1 - c2 = -(c2 - c1) exactly, etc.
-template <typename R, typename P> MDN_HD inline void exvol_terms(P c1, P c2, P c3, P c4, ExvTerms<R>& t) {
+template <typename R, typename P, bool FAST = false> MDN_HD inline void exvol_terms(P c1, P c2, P c3, P c4, ExvTerms<R>& t) {
     const V3<R> u1 = dif<R>(c2, c1), u2 = dif<R>(c4, c3), u3 = dif<R>(c1, c3);
     const V3<R> w14 = dif<R>(c4, c1), w32 = dif<R>(c2, c3);
     t.u1 = u1; t.u2 = u2; t.u3 = u3;
@@ -307,10 +307,17 @@ template <typename R, typename P> MDN_HD inline void exvol_terms(P c1, P c2, P c
     R JJ = (cp0 * -u3.x + cp1 * -u3.y + cp2 * -u3.z);
     t.JJ = -JJ * JJ;
     const R AA = t.AA, BB = t.BB, EE = t.EE, FF = t.FF;
-    t.ATG1 = atan((a + e) / AA) - atan(e / AA);
-    t.ATG2 = atan((a + e - d) / EE) - atan((e - d) / EE);
-    t.ATG3 = atan((F) / BB) - atan((F - b) / BB);
-    t.ATG4 = atan((d + F) / FF) - atan((d + F - b) / FF);
+    if constexpr (FAST) {   // the mixed path's form (below)
+        t.ATG1 = atan2(a * AA, AA * AA + (a + e) * e);
+        t.ATG2 = atan2(a * EE, EE * EE + (a + e - d) * (e - d));
+        t.ATG3 = atan2(b * BB, BB * BB + F * (F - b));
+        t.ATG4 = atan2(b * FF, FF * FF + (d + F) * (d + F - b));
+    } else {
+        t.ATG1 = atan((a + e) / AA) - atan(e / AA);
+        t.ATG2 = atan((a + e - d) / EE) - atan((e - d) / EE);
+        t.ATG3 = atan((F) / BB) - atan((F - b) / BB);
+        t.ATG4 = atan((d + F) / FF) - atan((d + F - b) / FF);
+    }
 }
 
 template <typename R> MDN_HD inline bool energy_bad(R U) {
@@ -516,6 +523,144 @@ MDN_HD inline double exvol_energy_forces(D3 c1, D3 c2, D3 c3, D3 c4, double kRep
         return 0.0;
     }
     return (double)U;
+}
+
+// ---- the mixed path's excluded volume: the same closed form in fewer FP64 operations -------
+// The reference form (above) spends most of its issue slots on eight atan calls and about fifty
+// divisions. With AA > 0 both angles of atan(p / AA) - atan(q / AA) lie in (-pi/2, pi/2), so
+//     atan(p / AA) - atan(q / AA) = atan2((p - q) AA, AA^2 + p q)
+// (tangent of a difference; the cosine of the difference has the sign of AA^2 + p q): four
+// atan2 instead of eight atan, with p - q (= a or b) entering exactly instead of through the
+// cancellation of two angles. In the force blocks the divisions by AA, BB, EE, FF and their
+// powers become products with four reciprocals, the eight cos^2 = 1 / (1 + tan^2) factors share
+// one division per pair of angles, and the expressions contract to FMAs. In exact arithmetic
+// every quantity equals the reference's; the rounding differs (the FP64 path keeps the
+// reference's operand order). The invariants a ... JJ are the reference's, bit for bit, so the
+// degenerate geometries (AA, BB, EE, FF or JJ exactly zero) give non-finite values exactly where
+// the reference's do and the Simpson fallbacks still trigger there.
+MDN_HD inline double exvol_energy_of_fast(const ExvTerms<double>& t, double kRepScaled) {
+    return exvol_energy_of<double>(t, kRepScaled);   // the atan2 differences are in t
+}
+
+// the force blocks of CylinderExclVolRepulsion.cpp:668-858 with the reciprocals; a block that
+// the reference only uses divided by AA (blockA_A and its E, B, F kin) carries that factor
+template <typename O> MDN_HD inline bool exvol_forces_from_fast(const ExvTerms<double>& t, double kRepScaled, O* out) {
+#pragma clang fp contract(fast)
+    const double a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
+    const double CC = t.CC, DD = t.DD, GG = t.GG, HH = t.HH;
+    const double iA = 1.0 / t.AA, iB = 1.0 / t.BB, iE = 1.0 / t.EE, iF = 1.0 / t.FF, iJJ = 1.0 / t.JJ;
+    const double iA2 = iA * iA, iB2 = iB * iB, iE2 = iE * iE, iF2 = iF * iF;
+    const double ATG1 = t.ATG1, ATG2 = t.ATG2, ATG3 = t.ATG3, ATG4 = t.ATG4;
+    const double sumBlock = CC * iA * ATG1 + GG * iE * ATG2 + DD * iB * ATG3 + HH * iF * ATG4;
+    const double enFactor = kRepScaled * iJJ * 0.5;
+
+    // cos^2 of the eight angles, 1 / (1 + tan^2), two per division
+    const double tA1 = (a + e) * iA, tA2 = e * iA, tE1 = (a + e - d) * iE, tE2 = (e - d) * iE;
+    const double tB1 = F * iB, tB2 = (F - b) * iB, tF1 = (d + F) * iF, tF2 = (d + F - b) * iF;
+    const double pA1 = 1 + tA1 * tA1, pA2 = 1 + tA2 * tA2, pE1 = 1 + tE1 * tE1, pE2 = 1 + tE2 * tE2;
+    const double pB1 = 1 + tB1 * tB1, pB2 = 1 + tB2 * tB2, pF1 = 1 + tF1 * tF1, pF2 = 1 + tF2 * tF2;
+    const double rA = 1.0 / (pA1 * pA2), rE = 1.0 / (pE1 * pE2), rB = 1.0 / (pB1 * pB2), rF = 1.0 / (pF1 * pF2);
+    const double A1 = pA2 * rA, A2 = pA1 * rA, E1 = pE2 * rE, E2 = pE1 * rE;
+    const double B1 = pB2 * rB, B2 = pB1 * rB, F1 = pF2 * rF, F2 = pF1 * rF;
+
+    const double bA_C = ATG1 * iA;
+    const double bA_An = (-(ATG1 * CC) + (-A1 * (a + e) + A2 * e) * CC * iA) * iA2 * iA;   // blockA_A / AA
+    const double bA_e = (A1 - A2) * CC * iA2;
+    const double bA_a = A1 * CC * iA2;
+    const double bE_G = ATG2 * iE;
+    const double bE_En = (-(ATG2 * GG) + (-E1 * (a + e - d) + E2 * (e - d)) * GG * iE) * iE2 * iE;
+    const double bE_emd = (E1 - E2) * GG * iE2;
+    const double bE_a = E1 * GG * iE2;
+    const double bB_D = ATG3 * iB;
+    const double bB_Bn = (-(ATG3 * DD) + (-B1 * F + B2 * (F - b)) * DD * iB) * iB2 * iB;
+    const double bB_fmb = (B1 - B2) * DD * iB2;
+    const double bB_b = B1 * DD * iB2;
+    const double bF_H = ATG4 * iF;
+    const double bF_Fn = (-(ATG4 * HH) + (-F1 * (d + F) + F2 * (d + F - b)) * HH * iF) * iF2 * iF;
+    const double bF_dfb = (F1 - F2) * HH * iF2;
+    const double bF_b = F1 * HH * iF2;
+
+    const double block_11 = bA_C * (-2 * F) + bA_An * c + bA_a * 2 + bE_G * (2 * (F - b)) +
+                            bE_En * (b + c - 2 * F) + bE_a * 2 + bF_H * (-2 * b) + bF_Fn * b;
+    const double block_12 = bA_C * e + bE_G * (2 * d - e) + bE_En * (e - d) - bE_emd + bB_D * (-F) +
+                            bF_H * (2 * d + F) - bF_Fn * (d + F) + bF_dfb;
+    const double block_13 = bA_C * d - bA_An * e + bA_e - bE_G * d - bE_En * (e - d) + bE_emd + bB_D * b -
+                            bF_H * b + bF_Fn * b;
+    const double block_22 = bE_G * (-2 * a) + bE_En * a + bB_D * (2 * e) + bB_Bn * c - 2 * bB_fmb + 2 * bB_b +
+                            bF_H * (-2 * (a + e)) + bF_Fn * (a + c + 2 * e) - 2 * bF_dfb + 2 * bF_b;
+    const double block_23 = -a * bA_C + a * bE_G - a * bE_En - d * bB_D - F * bB_Bn + bB_fmb + d * bF_H -
+                            (d + F) * bF_Fn + bF_dfb;
+    const double block_33 = a * bA_An + a * bE_En + b * bB_Bn + b * bF_Fn;
+    const double block_21 = block_12, block_31 = block_13, block_32 = block_23;
+
+    const double JJ_11 = 2 * (-b * c + F * F), JJ_12 = 2 * (c * d - e * F), JJ_13 = 2 * (b * e - d * F);
+    const double JJ_21 = JJ_12, JJ_22 = 2 * (-a * c + e * e), JJ_23 = 2 * (-d * e + a * F);
+    const double JJ_31 = JJ_13, JJ_32 = JJ_23, JJ_33 = 2 * (d * d - a * b);
+    const double sj = -iJJ * sumBlock;
+
+    double dv[4][3];
+    dv[0][0] = enFactor * (sj * (JJ_31 - JJ_11) + (block_31 - block_11));
+    dv[0][1] = enFactor * (sj * (JJ_32 - JJ_12) + (block_32 - block_12));
+    dv[0][2] = enFactor * (sj * (JJ_33 - JJ_13) + (block_33 - block_13));
+    dv[1][0] = enFactor * (sj * JJ_11 + block_11);
+    dv[1][1] = enFactor * (sj * JJ_12 + block_12);
+    dv[1][2] = enFactor * (sj * JJ_13 + block_13);
+    dv[2][0] = enFactor * (sj * (-JJ_31 - JJ_21) + (-block_31 - block_21));
+    dv[2][1] = enFactor * (sj * (-JJ_32 - JJ_22) + (-block_32 - block_22));
+    dv[2][2] = enFactor * (sj * (-JJ_33 - JJ_23) + (-block_33 - block_23));
+    dv[3][0] = enFactor * (sj * JJ_21 + block_21);
+    dv[3][1] = enFactor * (sj * JJ_22 + block_22);
+    dv[3][2] = enFactor * (sj * JJ_23 + block_23);
+
+    const double u1[3] = {t.u1.x, t.u1.y, t.u1.z};
+    const double u2[3] = {t.u2.x, t.u2.y, t.u2.z};
+    const double u3[3] = {t.u3.x, t.u3.y, t.u3.z};
+    bool ok = true;
+    for (int bi = 0; bi < 4; bi++) {
+        double f[3];
+        for (int k = 0; k < 3; k++) {
+            f[k] = -(dv[bi][0] * u1[k] + dv[bi][1] * u2[k] + dv[bi][2] * u3[k]);
+            ok = ok && !bad_component<double>(f[k]);
+        }
+        put3(out[bi], f[0], f[1], f[2]);
+    }
+    return ok;
+}
+
+// forces only (the coplanar shift applies), energy only, energy and forces: as the reference
+// forms above, on the atan2 terms
+MDN_HD inline bool exvol_forces_fast(D3 c1, D3 c2in, D3 c3, D3 c4, double kRepScaled, D3* out) {
+    D3 c2 = c2in;
+    if (are_in_plane(c1, c2, c3, c4)) c2 = move_out_of_plane(c1, c2, c3, 2, 0.01);
+    ExvTerms<double> t;
+    exvol_terms<double, D3, true>(c1, c2, c3, c4, t);
+    return exvol_forces_from_fast(t, kRepScaled, out);
+}
+MDN_HD inline double exvol_energy_fast(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, int* deferred) {
+    ExvTerms<double> t;
+    exvol_terms<double, D3, true>(c1, c2, c3, c4, t);
+    const double U = exvol_energy_of_fast(t, kRepScaled);
+    if (energy_bad<double>(U)) {
+        *deferred = 1;
+        return 0.0;
+    }
+    return U;
+}
+MDN_HD inline double exvol_energy_forces_fast(D3 c1, D3 c2, D3 c3, D3 c4, double kRepScaled, int* deferred,
+                                              bool* fok, D3* out) {
+    ExvTerms<double> t;
+    exvol_terms<double, D3, true>(c1, c2, c3, c4, t);
+    const double U = exvol_energy_of_fast(t, kRepScaled);
+    if (are_in_plane(c1, c2, c3, c4)) {
+        const D3 c2s = move_out_of_plane(c1, c2, c3, 2, 0.01);
+        exvol_terms<double, D3, true>(c1, c2s, c3, c4, t);
+    }
+    *fok = exvol_forces_from_fast(t, kRepScaled, out);
+    if (energy_bad<double>(U)) {
+        *deferred = 1;
+        return 0.0;
+    }
+    return U;
 }
 
 // ---------------- BoundaryCylinderRepulsionExp (BoundaryCylinderRepulsionExp.cpp:27-110) ---

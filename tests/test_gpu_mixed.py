@@ -1,8 +1,10 @@
 """GPU: the mixed-precision path (MDN_PREC_MIXED) against the FP64 path on the same inputs.
 
 The mixed path computes the stretching, bending, crosslink and boundary terms in FP32 from bond
-vectors formed in FP64, keeps excluded volume in FP64 and accumulates in FP64. Bars (the FP64
-path itself is pinned to the oracle by test_gpu_parity.py):
+vectors formed in FP64, keeps excluded volume in FP64 (the same closed form with four atan2 and
+reciprocal force blocks instead of the reference's eight atan and fifty divisions; its accuracy
+against extended precision is checked on the host by test_exvol_fast.py) and accumulates in FP64.
+Bars (the FP64 path itself is pinned to the oracle by test_gpu_parity.py):
   unrelaxed networks (forces up to 1e14 pN from overlapping cylinders): normwise relative force
     error max|dF| / max|F| <= 1e-6 (the target is 1e-5; measured 2e-8 .. 7e-8);
   total energy: relative 1e-6;
@@ -10,12 +12,15 @@ path itself is pinned to the oracle by test_gpu_parity.py):
     larger): absolute force error <= 1e-2 pN, i.e. 1 % of the 1 pN force tolerance (measured
     1.7e-3 pN; FP32 rounding of the large cancelling bonded terms);
   minimization: stops by the reference criterion with a state whose FP64 forces agree with the
-    mixed path's reported max|F| to that absolute bar, and lowers the FP64 energy.
+    mixed path's reported max|F| and forces to that absolute bar (outside the beads of
+    excluded-volume pairs with normalized triple product below 1e-3, where both FP64 forms of the
+    closed form are inaccurate), and lowers the FP64 energy.
 """
 import numpy as np
 import pytest
 
 from picgo_amd import abi, synth
+from tests.exvol_conditioning import well_conditioned
 
 pytestmark = pytest.mark.gpu
 
@@ -42,8 +47,11 @@ def test_mixed_force_error_unrelaxed(gpu, cfg):
     e64 = g.compute_individual_energies(x)
     e32 = g.compute_individual_energies(x, precision=abi.MDN_PREC_MIXED)
     assert abs(e32.total - e64.total) <= 1e-6 * abs(e64.total)
-    # excluded volume is FP64 in both paths: identical energies
-    assert e32.ff[abi.MDN_FF_EXVOL] == e64.ff[abi.MDN_FF_EXVOL]
+    # excluded volume is FP64 in both paths (the mixed path's reduced-operation form of the same
+    # closed form): energies agree to rounding on these unrelaxed networks
+    ev32, ev64 = e32.ff[abi.MDN_FF_EXVOL], e64.ff[abi.MDN_FF_EXVOL]
+    print(f"config {cfg}: excluded-volume energy rel diff {abs(ev32 - ev64) / abs(ev64):.3e}", flush=True)
+    assert abs(ev32 - ev64) <= 1e-12 * abs(ev64)
 
 
 def test_mixed_force_error_relaxed(gpu):
@@ -69,8 +77,18 @@ def test_mixed_minimize(gpu, cfg):
     assert not res.call_limit_exceeded
     fx = g.compute_forces(x)
     assert abs(float(np.abs(fx).max()) - res.grad_inf_norm) <= 1e-2
-    # forces returned are the mixed path's forces at the result
-    assert float(np.abs(f - fx).max()) <= 1e-2
+    # forces returned are the mixed path's forces at the result: the FP64 path's within 1e-2 pN
+    # outside the beads of excluded-volume pairs with tau < 1e-3. On those pairs the closed form
+    # loses most of its digits in FP64 whatever the operation order: against extended precision
+    # the reference form and the mixed path's form err alike (test_exvol_fast.py; a relative
+    # error of order 1 on the worst shifted pairs), so the two only agree to that error there.
+    m = well_conditioned(g, net, x)
+    dF = np.abs(f - fx)
+    print(f"config {cfg}: {int((~m).sum())} coordinates on pairs with tau < 1e-3 (max |dF| "
+          f"{float(dF[~m].max()) if (~m).any() else 0.0:.3e} pN), max |dF| elsewhere "
+          f"{float(dF[m].max()):.3e} pN", flush=True)
+    assert np.all(np.isfinite(f))
+    assert float(dF[m].max()) <= 1e-2
     # reported energies are FP64 and consistent with an independent FP64 evaluation
     assert res.energies_after.total == pytest.approx(g.compute_energy(x), rel=1e-12)
     assert res.energies_after.total < res.energies_before.total
