@@ -84,6 +84,30 @@ def test_minimize_trajectory_prefix(gpu, ncalls):
     np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
 
 
+@pytest.mark.parametrize("method", [abi.MDN_CG_STEEPEST, abi.MDN_CG_FLETCHER_REEVES], ids=["steepest", "fletcher_reeves"])
+def test_minimize_trajectory_prefix_other_methods(gpu, method):
+    """The descent-direction variants of CGMethod (steepest descent: beta = 0; Fletcher-Reeves:
+    beta = F.F / F_prev.F_prev, CGMethod.hpp:1322-1332) follow the oracle's trajectory for 60
+    energy calls on config 0, like Polak-Ribiere above, and a full minimization converges by the
+    reference criterion at a state whose oracle forces confirm it."""
+    net, g, o, _ = _pair(gpu, 0)
+    p = abi.default_cg_params(method=method, energy_call_limit=60)
+    xg, fg, rg = g.minimize(p, net.coord)
+    st, xo, fo, ro = o.minimize(p, net.coord)
+    print(method, "gpu", rg.num_iter, rg.num_energy_call, rg.num_force_call, "oracle", ro.num_iter,
+          ro.num_energy_call, ro.num_force_call, "max |dx|", float(np.abs(xg - xo).max()), flush=True)
+    assert (rg.num_iter, rg.num_energy_call, rg.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+    np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
+    p = abi.default_cg_params(method=method)
+    xg, fg, rg = g.minimize(p, net.coord)
+    print(method, "full:", rg.as_dict(), flush=True)
+    assert rg.success() and rg.converged() and not rg.call_limit_exceeded
+    f_at_xg = o.forces(xg)
+    np.testing.assert_allclose(fg, f_at_xg, rtol=0, atol=1e-9 * max(1.0, float(np.abs(fg).max())))
+    if rg.converged_grad:
+        assert float(np.abs(f_at_xg).max()) < p.grad_tol
+
+
 def test_minimize_config0(gpu):
     """Full minimization: both paths converge by the reference criterion (max |F| < 1 pN or
     |dE/E| < 1e-9); the GPU's final state is checked by the oracle (its forces there satisfy the
