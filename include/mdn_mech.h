@@ -232,7 +232,9 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* params, double* coord, doubl
                  mdn_min_result* res);
 
 /* Launch statistics of the last mdn_minimize on this context: out[0] line-search launches
- * that did work, out[1] reserved, out[2] force gathers, out[3] CG iterations. */
+ * that did work, out[1] steps whose excluded-volume forces needed the numerical fallback
+ * (forceN; the device state machine paused and the host resumed it), out[2] force gathers,
+ * out[3] CG iterations. */
 int mdn_last_min_stats(mdn_ctx* ctx, int64_t* out);
 
 /* ---- device-resident variants (benchmarking, zero host traffic) ---------------------- */

@@ -162,6 +162,14 @@ def exvol_energyN(coord, beadSet, krep, eqlen, i, movebeads=0):
     return lib().or_exvol_energyN(_p(c), _p(b), _p(k), _p(e), i, movebeads)
 
 
+def exvol_forceN(coord, beadSet, krep, eqlen, i, movebeads=0):
+    """The 10^4 x 10^4 Simpson force rule of pair i, accumulated into a zero force vector."""
+    c, b, k, e = _arr(coord, np.float64), _arr(beadSet, np.int32), _arr(krep, np.float64), _arr(eqlen, np.float64)
+    f = np.zeros_like(c)
+    lib().or_exvol_forceN(_p(c), _p(f), _p(b), _p(k), _p(e), i, movebeads)
+    return f
+
+
 def are_in_plane(p1, p2, p3, p4):
     a = [_arr(x, np.float64) for x in (p1, p2, p3, p4)]
     return bool(lib().or_are_in_plane(*[_p(x) for x in a]))

@@ -1437,7 +1437,7 @@ int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms4) {
 int mdn_last_min_stats(mdn_ctx* ctx, int64_t* out) {
     if (!ctx || !out) return MDN_ERR_ARG;
     out[0] = ctx->last_cg.n_ls_launch;
-    out[1] = ctx->last_cg.n_move_launch;
+    out[1] = ctx->last_cg.reserved[1];   // steps resumed after the host-run forceN fallback
     out[2] = ctx->last_cg.n_gather_launch;
     out[3] = ctx->last_cg.num_iter;
     return MDN_OK;

@@ -219,7 +219,7 @@ class ForceFieldManager:
     def last_min_stats(self):
         out = (C.c_int64 * 4)()
         self._chk(self._L.mdn_last_min_stats(self._h, out))
-        return {"ls_launches": out[0], "gathers": out[2], "iterations": out[3]}
+        return {"ls_launches": out[0], "forceN_steps": out[1], "gathers": out[2], "iterations": out[3]}
 
     def param_grad(self, coord=None):
         """dE/dtheta for theta = (stretching k, rest length L0, bending k) per cylinder; returns

@@ -1,0 +1,102 @@
+"""GPU: the Simpson fallbacks of the excluded-volume closed form on degenerate pairs, against the
+oracle (CylinderExclVolRepulsion.cpp:884-1254).
+
+* Parallel coplanar cylinders (JJ = 0): the closed-form energy is 0/0, so the energy comes from the
+  400 x 400 Simpson rule (energyN), which the device evaluates as a deferred block-parallel pass;
+  the forces come from the closed form at the coplanar-shifted geometry.
+* Two cylinders on one line (AA = BB = EE = FF = JJ = 0): the energy comes from energyN and the
+  forces from the 10^4 x 10^4 Simpson rule (forceN), run on the device for the flagged pair.
+* A minimization of the collinear pair: every force is along the line, so the pair stays collinear.
+  Every line-search trial then defers its energy to energyN, and every step's forces need forceN:
+  the device state machine pauses, the host runs forceN on the device and the machine resumes. The
+  trajectory prefix must follow the oracle's.
+
+Bars: energies 1e-12 relative (measured 9e-14, 3e-13: the device sums the rules' samples in a
+tree, the oracle sequentially); forceN forces 1e-12 of max|F| on the FP64 path (measured 1.5e-13)
+and 1e-9 on the mixed path, whose slots hold the result in FP32 (measured 1e-10); the
+trajectory's counts identical and its coordinates within 1e-8 relative. The shifted closed form
+of the parallel pair is ill-conditioned (normalized triple product of the shifted geometry
+~1e-4), so its forces carry the FP64 rounding of that conditioning: 1e-8 of max|F| on the FP64
+path (measured 5e-10) and 1e-6 on the mixed path with its own operation order (measured 2e-8).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi
+from tests.hand_systems import HandSystem
+
+pytestmark = pytest.mark.gpu
+
+P = np.array
+
+
+def _parallel():
+    return HandSystem([[P([500., 500, 500]), P([608., 500, 500])], [P([500., 530, 500]), P([608., 530, 500])]], [])
+
+
+def _collinear():
+    # 60 nm cylinders on the line y = z = 500 with a 20 nm gap (midpoints 80 nm apart)
+    return HandSystem([[P([500., 500, 500]), P([560., 500, 500])], [P([580., 500, 500]), P([640., 500, 500])]], [])
+
+
+def _setup(gpu, hs):
+    g = gpu()
+    g.upload(hs.desc, hs.coord)
+    assert g.reset_neighbor_lists(hs.coord) == 1
+    o = of.OracleSystem(hs.desc)
+    o.build_nl(hs.coord)
+    np.testing.assert_array_equal(g.neighbor_pairs(), o.pairs())
+    return g, o
+
+
+@pytest.mark.parametrize("case", ["parallel", "collinear"])
+def test_degenerate_pair_energy_and_forces(gpu, case):
+    hs = _parallel() if case == "parallel" else _collinear()
+    g, o = _setup(gpu, hs)
+    x = hs.coord
+    eo, repo = o.energy(x)
+    fo = o.forces(x)
+    c = x
+    # the oracle's closed form flags the pair; its value is the Simpson rule's
+    ev = of.exvol_energy(c, [0, 3, 6, 9], [1e5], [108.0, 108.0])
+    assert ev == of.exvol_energyN(c, [0, 3, 6, 9], [1e5], [108.0, 108.0], 0)
+    for prec in (abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED):
+        rep = g.compute_individual_energies(x, precision=prec)
+        f = g.compute_forces(x, precision=prec)
+        de = abs(rep.ff[abi.MDN_FF_EXVOL] - repo.ff[abi.MDN_FF_EXVOL]) / repo.ff[abi.MDN_FF_EXVOL]
+        df = float(np.abs(f - fo).max() / np.abs(fo).max())
+        print(f"{case} precision {prec}: exvol energy {rep.ff[abi.MDN_FF_EXVOL]:.12e} (oracle "
+              f"{repo.ff[abi.MDN_FF_EXVOL]:.12e}, rel {de:.2e}), forces rel {df:.2e}", flush=True)
+        assert de <= 1e-12
+        assert rep.total == pytest.approx(eo, rel=1e-12)
+        if case == "collinear":   # forceN in FP64 on both paths; the mixed path's slots are FP32
+            assert df <= (1e-12 if prec == abi.MDN_PREC_FP64 else 1e-9)
+        else:
+            assert df <= (1e-8 if prec == abi.MDN_PREC_FP64 else 1e-6)
+
+
+def test_minimize_collinear_pair_through_forceN(gpu):
+    """Every step of this minimization pauses the device state machine for forceN; the prefix
+    of the trajectory must still be the oracle's, and a longer run must finish cleanly."""
+    hs = _collinear()
+    g, o = _setup(gpu, hs)
+    for ncalls in (4, 8):
+        p = abi.default_cg_params(energy_call_limit=ncalls)
+        xg, fg, rg = g.minimize(p, hs.coord)
+        st, xo, fo, ro = o.minimize(p, hs.coord)
+        stats = g.last_min_stats()
+        print(ncalls, "gpu", rg.num_iter, rg.num_energy_call, rg.num_force_call, "oracle", ro.num_iter,
+              ro.num_energy_call, ro.num_force_call, "max |dx|", float(np.abs(xg - xo).max()), "stats", stats,
+              flush=True)
+        assert st == 0 and rg.error_flags == 0
+        assert 0 < stats["forceN_steps"] <= rg.num_force_call   # steps resumed after forceN
+        assert (rg.num_iter, rg.num_energy_call, rg.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+        np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
+        # still on the line: forces and motion are along x only
+        X = xg.reshape(-1, 3)
+        assert np.all(X[:, 1] == 500.0) and np.all(X[:, 2] == 500.0)
+    p = abi.default_cg_params(energy_call_limit=60)
+    xg, fg, rg = g.minimize(p, hs.coord)
+    print("60 calls:", rg.as_dict(), flush=True)
+    assert rg.error_flags == 0 and rg.energies_after.total < rg.energies_before.total

@@ -127,6 +127,37 @@ def test_exvol_parallel_segments_use_simpson():
     assert e == pytest.approx(gauss_legendre_exvol(c, 1.0, 100.0, 100.0), rel=2e-2)
 
 
+def test_simpson_rules_vs_closed_form():
+    """energyN (400 x 400) and forceN (10^4 x 10^4), CylinderExclVolRepulsion.cpp:884-1254,
+    approximate the same integral as the closed form, so on regular (non-coplanar, separated)
+    pairs they must agree with it to the rules' accuracy. The reference's loops advance s and t
+    before use (samples at (i + 1) / N), which makes both rules first order in 1/N: the energy is
+    off by 1e-3 .. 2e-2 (1/N = 2.5e-3) and the forces by 1e-4 .. 2e-3 (1/N = 1e-4), far above the
+    1e-8 of a correctly placed Simpson rule. Both properties are checked: a rule with another
+    sample placement, a dropped edge term or a wrong weight fails one of them. The Simpson forces
+    sum to zero over the four beads (Newton's third law)."""
+    rng = np.random.default_rng(5)
+    done = 0
+    while done < 3:
+        c1 = rng.normal(0, 30, 3)
+        c2 = c1 + rng.normal(0, 1, 3) * 50
+        off = rng.normal(0, 1, 3)
+        c3 = 0.5 * (c1 + c2) + 50 * off / np.linalg.norm(off) + rng.normal(0, 10, 3)
+        c4 = c3 + rng.normal(0, 1, 3) * 50
+        if of.are_in_plane(c1, c2, c3, c4):
+            continue
+        c = np.concatenate([c1, c2, c3, c4])
+        args = (c, [0, 3, 6, 9], [1e5], [108.0, 97.0])
+        e, en = of.exvol_energy(*args), of.exvol_energyN(*args, 0)
+        f, fn = of.exvol_forces(*args), of.exvol_forceN(*args, 0)
+        e_rel = abs(en - e) / e
+        f_rel = float(np.abs(fn - f).max() / np.abs(f).max())
+        assert 1e-5 <= e_rel <= 2e-2, e_rel
+        assert 1e-6 <= f_rel <= 2e-3, f_rel
+        np.testing.assert_allclose(fn.reshape(4, 3).sum(0), 0, atol=1e-12 * float(np.abs(fn).max()))
+        done += 1
+
+
 def test_are_in_plane_threshold():
     p1, p2, p3 = np.zeros(3), np.array([1.0, 0, 0]), np.array([0, 1.0, 0])
     assert of.are_in_plane(p1, p2, p3, np.array([1, 1, 0.09]))          # |n.v| = 0.0634 < 0.1
