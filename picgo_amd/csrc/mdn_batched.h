@@ -277,7 +277,9 @@ __device__ inline void blk_minimize(const BatchItem& it, const BatchParams& P, C
         S.phase = PH_LS;
         S.ls_lambda = rmin(S.lambda_max, S.max_dist / S.grad_inf_norm);
         if (sh_bad) {
-            S.status = 9;   // MDN_ERR_UNSUPPORTED: an excluded-volume pair needs the numerical force fallback
+            // an excluded-volume pair needs the numerical force fallback, which this kernel does
+            // not run (reported as MDN_ERR_UNSUPPORTED, like a pause for it later in the run)
+            S.status = ST_NEED_FORCEN;
             S.done = 1;
         } else if (!isfinite(v[2])) {
             S.status = 4;   // MDN_ERR_NONFINITE: maxF not finite (CGMethod.hpp:202-212)

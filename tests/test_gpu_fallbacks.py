@@ -23,7 +23,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_ffi as of
-from picgo_amd import abi
+from picgo_amd import abi, mech, synth
 from tests.hand_systems import HandSystem
 
 pytestmark = pytest.mark.gpu
@@ -100,3 +100,36 @@ def test_minimize_collinear_pair_through_forceN(gpu):
     xg, fg, rg = g.minimize(p, hs.coord)
     print("60 calls:", rg.as_dict(), flush=True)
     assert rg.error_flags == 0 and rg.energies_after.total < rg.energies_before.total
+
+
+def test_parallel_pair_start_energy_in_minimizers(gpu):
+    """The single-system and the batched minimizer evaluate the start energy of the parallel pair
+    with energyN (the batched workgroup runs the deferred pass itself). Only the start is
+    compared: once a step lifts the pair out of its plane, the closed form's normalized triple
+    product is ~1e-10 and its rounding (device vs host atan) decides the line search, so the two
+    trajectories part at once."""
+    hs = _parallel()
+    g, o = _setup(gpu, hs)
+    eo, _ = o.energy(hs.coord)
+    p = abi.default_cg_params(energy_call_limit=2)
+    xg, fg, rg = g.minimize(p, hs.coord)
+    xs, fs, rs = mech.minimize_batched([g], p, [hs.coord])
+    for name, r, x in (("single", rg, xg), ("batched", rs[0], xs[0])):
+        print(name, "start energy", r.energies_before.total, "oracle", eo, flush=True)
+        assert r.energies_before.total == pytest.approx(eo, rel=1e-12)
+        assert r.energies_before.ff[abi.MDN_FF_EXVOL] > 0 and np.all(np.isfinite(x))
+
+
+def test_batched_refuses_forceN(gpu):
+    """The batched minimizer does not run forceN: a system that needs it is reported as
+    MDN_ERR_UNSUPPORTED, naming the fallback, while the call still returns."""
+    hs = _collinear()
+    g, _ = _setup(gpu, hs)
+    net = synth.Network.config(3, index=0)
+    g2 = gpu()
+    g2.upload(net.desc, net.coord)
+    g2.reset_neighbor_lists(net.coord)
+    with pytest.raises(mech.MdnError) as ei:
+        mech.minimize_batched([g2, g], abi.default_cg_params(), [net.coord, hs.coord])
+    print(ei.value, flush=True)
+    assert ei.value.code == 9 and "numerical force fallback" in str(ei.value)   # MDN_ERR_UNSUPPORTED
