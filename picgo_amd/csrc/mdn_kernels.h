@@ -161,6 +161,15 @@ __device__ inline double wave_reduce(double v) {
     v = comb<OP>(v, dpp_or<0x143, 0xC>(v, id));   // row_bcast:31 -> rows 2, 3
     return lane63(v);
 }
+// The first three steps of wave_reduce: lane 8k + 7 then holds the reduction of lanes 8k .. 8k + 7.
+template <int OP>
+__device__ inline double wave_fold8(double v) {
+    const double id = OP == 0 ? 0.0 : OP == 1 ? -INFINITY : INFINITY;
+    v = comb<OP>(v, dpp_or<0x111, 0xF>(v, id));   // row_shr:1
+    v = comb<OP>(v, dpp_or<0x112, 0xF>(v, id));   // row_shr:2
+    v = comb<OP>(v, dpp_or<0x114, 0xF>(v, id));   // row_shr:4
+    return v;
+}
 __device__ inline double wave_sum(double v) { return wave_reduce<0>(v); }
 __device__ inline double wave_max(double v) { return wave_reduce<1>(v); }
 __device__ inline double wave_min(double v) { return wave_reduce<2>(v); }
@@ -188,6 +197,34 @@ __device__ inline void block_reduce_n(double* v, const int* op, double* lds /* >
             }
             v[i] = r;
         }
+}
+
+// The same for a large block whose waves would otherwise each run NV full wave reductions on
+// shared SIMDs: every wave folds groups of 8 lanes (three DPP row shifts), the NTH / 8 group
+// results go through LDS, and wave 0 alone combines them and runs the full wave reductions.
+// Fixed order, so deterministic. Results valid in every lane of wave 0.
+template <int NV, int NTH>
+__device__ inline void block_reduce_fold(double* v, const int* op, double* lds /* >= NV * NTH / 8 */) {
+    constexpr int NP = NTH / 8;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+        v[i] = op[i] == 0 ? wave_fold8<0>(v[i]) : op[i] == 1 ? wave_fold8<1>(v[i]) : wave_fold8<2>(v[i]);
+    if ((lane & 7) == 7)
+#pragma unroll
+        for (int i = 0; i < NV; i++) lds[i * NP + w * 8 + (lane >> 3)] = v[i];
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        const double id = op[i] == 0 ? 0.0 : op[i] == 1 ? -INFINITY : INFINITY;
+        double r = lane < NP ? lds[i * NP + lane] : id;
+        for (int q = lane + 64; q < NP; q += 64) {
+            const double u = lds[i * NP + q];
+            r = op[i] == 0 ? r + u : op[i] == 1 ? rmax(r, u) : rmin(r, u);
+        }
+        v[i] = op[i] == 0 ? wave_sum(r) : op[i] == 1 ? wave_max(r) : wave_min(r);
+    }
 }
 
 // Stage the whole CG state through LDS: one parallel round trip instead of a chain of
@@ -1070,7 +1107,7 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
     constexpr int NW = sizeof(CgState) / sizeof(int);
     static_assert(NW <= FIN, "the state is staged one word per thread");
     __shared__ CgState Sl;
-    __shared__ double lds[11 * (FIN / 64)];
+    __shared__ double lds[8 * (FIN / 8)];
     __shared__ int sh_ndef, sh_bad;
     if (threadIdx.x < NW) reinterpret_cast<int*>(&Sl)[threadIdx.x] = reinterpret_cast<const int*>(st)[threadIdx.x];
     if (threadIdx.x == 0) {
@@ -1120,8 +1157,12 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
     __syncthreads();
     const int phase0 = Sl.phase;
     if (Sl.done || (phase0 != PH_LS && phase0 != PH_MOVE)) return;
-    const int op[11] = {0, 0, 1, 1, 0, 1, 0, 2, 0, 0, 1};
-    block_reduce_n<11, FIN>(acc, op, lds);
+    // the eight reduced values (the three flags of acc are set from the deferred pass below)
+    double r8[8] = {acc[0], acc[1], acc[4], acc[6], acc[7], acc[8], acc[9], acc[10]};
+    const int op[8] = {0, 0, 0, 0, 2, 0, 0, 1};
+    block_reduce_fold<8, FIN>(r8, op, lds);
+    acc[0] = r8[0]; acc[1] = r8[1]; acc[4] = r8[2]; acc[6] = r8[3]; acc[7] = r8[4];
+    acc[8] = r8[5]; acc[9] = r8[6]; acc[10] = r8[7];
     double dsum[3] = {0.0, 0.0, 0.0};
     int dbad[3] = {0, 0, 0};
     const int ndef = sh_ndef;

@@ -9,8 +9,9 @@ Bars (the FP64 path itself is pinned to the oracle by test_gpu_parity.py):
     error max|dF| / max|F| <= 1e-6 (the target is 1e-5; measured 2e-8 .. 7e-8);
   total energy: relative 1e-6;
   relaxed network (config 1 after an FP64 minimization, max|F| ~ 50 pN, per-term forces much
-    larger): absolute force error <= 1e-2 pN, i.e. 1 % of the 1 pN force tolerance (measured
-    1.7e-3 pN; FP32 rounding of the large cancelling bonded terms);
+    larger): absolute force error <= 1e-2 pN, i.e. 1 % of the 1 pN force tolerance, outside the
+    beads of excluded-volume pairs with normalized triple product below 1e-3 (measured 1e-4 pN;
+    FP32 rounding of the large cancelling bonded terms);
   minimization: stops by the reference criterion with a state whose FP64 forces agree with the
     mixed path's reported max|F| and forces to that absolute bar (outside the beads of
     excluded-volume pairs with normalized triple product below 1e-3, where both FP64 forms of the
@@ -60,9 +61,14 @@ def test_mixed_force_error_relaxed(gpu):
     assert res.converged_grad or res.converged_energy
     f64 = g.compute_forces(x)
     f32 = g.compute_forces(x, precision=abi.MDN_PREC_MIXED)
-    err = float(np.abs(f32 - f64).max())
-    print(f"relaxed config 1: max|F| {np.abs(f64).max():.3f} pN, mixed abs error {err:.3e} pN", flush=True)
-    assert err <= 1e-2
+    # outside the beads of excluded-volume pairs with tau < 1e-3, where the two FP64 forms of the
+    # closed form carry rounding errors of their own (test_exvol_fast.py)
+    m = well_conditioned(g, net, x)
+    dF = np.abs(f32 - f64)
+    err = float(dF[m].max())
+    print(f"relaxed config 1: max|F| {np.abs(f64).max():.3f} pN, mixed abs error {err:.3e} pN outside "
+          f"{int((~m).sum())} coordinates on pairs with tau < 1e-3 (max there {float(dF.max()):.3e})", flush=True)
+    assert err <= 1e-2 and np.all(np.isfinite(f32))
 
 
 @pytest.mark.parametrize("cfg", [0, 1])
