@@ -227,6 +227,12 @@ __device__ inline void block_reduce_fold(double* v, const int* op, double* lds /
     }
 }
 
+// Pins values loaded at a kernel's entry before its first branch: without a use there, the
+// compiler sinks each load into the branch that needs it, which turns one round trip to the
+// state into a chain of dependent ones. An empty asm statement, no instruction.
+__device__ inline void keep_loaded(int a) { asm volatile("" ::"s"(a)); }
+__device__ inline void keep_loaded(double a) { asm volatile("" ::"s"(a)); }
+
 // Stage the whole CG state through LDS: one parallel round trip instead of a chain of
 // dependent global accesses by the thread that runs the scalar logic.
 template <typename S>
@@ -790,17 +796,23 @@ __device__ inline bool converged(const CgState* st) {
 // MController.h:318-321). The partials go after the interaction blocks' (index T + block).
 __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double lds[2 * (BLOCK / 64)];
-    if (st->done) return;
-    const int phase = st->phase;
+    // every state field this kernel reads, loaded before the first branch: one round trip to the
+    // state instead of a chain of dependent ones
+    const int done = st->done, phase = st->phase, par = st->parity, dpar = st->dpar, fpar = st->fpar;
+    const int ls_active = st->ls_active, dir_pending = st->dir_pending;
+    const double lambda = st->lambda, lam0 = st->ls_lambda, beta = st->dir_beta, maxd = st->max_dist;
+    keep_loaded(done); keep_loaded(phase); keep_loaded(par); keep_loaded(dpar); keep_loaded(fpar);
+    keep_loaded(ls_active); keep_loaded(dir_pending);
+    keep_loaded(lambda); keep_loaded(lam0); keep_loaded(beta); keep_loaded(maxd);
+    if (done) return;
     if (phase != PH_LS && phase != PH_MOVE) return;
     // block b covers the 3 * BLOCK coordinates of beads [BLOCK b, BLOCK (b + 1)), coalesced
     const long e0 = 3L * BLOCK * blockIdx.x + threadIdx.x, ne = 3L * s.n_beads;
-    const int par = st->parity, dpar = st->dpar;
     const double* __restrict__ xc = B.x[par];
     const double* __restrict__ dc = B.d[dpar];
     double* __restrict__ xn = B.x[par ^ 1];
     if (phase == PH_MOVE) {
-        const double lam = st->lambda;
+        const double lam = lambda;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const long e = e0 + (long)k * BLOCK;
@@ -808,12 +820,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
         }
         return;
     }
-    const bool first = st->ls_active == 0;
-    const bool pend = first && st->dir_pending;
-    const double lam0 = st->ls_lambda, beta = st->dir_beta;
-    const double* __restrict__ fcur = B.f[st->fpar];
+    const bool first = ls_active == 0;
+    const bool pend = first && dir_pending;
+    const double* __restrict__ fcur = B.f[fpar];
     double v[2] = {0.0, INFINITY};
-    const double maxd = st->max_dist;
     double* __restrict__ dn = B.d[dpar ^ 1];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -850,12 +860,15 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
 template <typename R>
 __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double lds[3 * (BLOCK / 64)];
-    if (st->done || (int)blockIdx.x >= s.blk[NFF]) return;   // (a 1-block grid when there are no interactions)
-    const int phase = st->phase;
+    // the state fields, loaded before the first branch (one round trip)
+    const int done = st->done, phase = st->phase, par = st->parity, ls_active = st->ls_active;
+    const int ecur_valid = st->ecur_valid;
+    keep_loaded(done); keep_loaded(phase); keep_loaded(par); keep_loaded(ls_active); keep_loaded(ecur_valid);
+    if (done || (int)blockIdx.x >= s.blk[NFF]) return;   // (a 1-block grid when there are no interactions)
     if (phase != PH_LS && phase != PH_MOVE) return;
     const int ff = ff_of_block(s, blockIdx.x);
     const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
-    const double* __restrict__ xn = B.x[st->parity ^ 1];
+    const double* __restrict__ xn = B.x[par ^ 1];
     const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
     if (phase == PH_MOVE) {
         if (t < s.n[ff]) {
@@ -865,8 +878,8 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
         }
         return;
     }
-    const bool first = st->ls_active == 0;
-    const bool need_e0 = first && !st->ecur_valid;
+    const bool first = ls_active == 0;
+    const bool need_e0 = first && !ecur_valid;
     double v[3] = {0.0, 0.0, 0.0};
     if (t < s.n[ff]) {
         int fbad = 0;
@@ -877,7 +890,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
             v[1] = term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fbad);
         }
         if (need_e0) {
-            const Pos p0{B.x[st->parity], nullptr, nullptr, 0.0, 0.0, 0};
+            const Pos p0{B.x[par], nullptr, nullptr, 0.0, 0.0, 0};
             v[2] = term_ef<R>(s, ff, t, p0, (R*)nullptr, 2, &fbad);
         }
     }
@@ -1010,15 +1023,19 @@ __device__ inline void ls_control(CgState& S, const double* acc, int& bad_count)
 template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
-    if (st->done) return;
-    if (st->phase != PH_LS && st->phase != PH_MOVE) return;
+    // the state fields, loaded before the first branch (one round trip)
+    const int done = st->done, phase = st->phase, par = st->parity, bk = st->bkpar, fpar = st->fpar;
+    const int num_iter = st->num_iter, force_call_limit = st->force_call_limit;
+    keep_loaded(done); keep_loaded(phase); keep_loaded(par); keep_loaded(bk); keep_loaded(fpar);
+    keep_loaded(num_iter); keep_loaded(force_call_limit);
+    if (done) return;
+    if (phase != PH_LS && phase != PH_MOVE) return;
     const int b = blockIdx.x * GB + threadIdx.x;
     // copycoordsifminimumE candidate (CGMethod.hpp:1564-1566): x_next goes to the candidate
     // buffer whenever the iteration part of the condition can hold after this step; the energy
     // and force parts decide in k_cg_fin whether it is committed
-    const bool cand = st->num_iter + 1 > st->force_call_limit / 2;
-    const int par = st->parity, bk = st->bkpar;
-    double* __restrict__ fnext = B.f[st->fpar ^ 1];
+    const bool cand = num_iter + 1 > force_call_limit / 2;
+    double* __restrict__ fnext = B.f[fpar ^ 1];
     D3 acc = gather_block<R>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
     if (threadIdx.x >= GB) return;   // one wave finishes
     double v[3] = {0.0, 0.0, 0.0};
