@@ -63,7 +63,12 @@ enum { FF_STRETCH = 0, FF_BEND, FF_LINKER, FF_MOTOR, FF_EXVOL, FF_BOUNDARY };
 struct DevSys {
     int n_beads;
     int n[NFF];                 // interactions per force field
-    int blk[NFF + 2];           // first block of each FF range, beads, end
+    // interaction blocks in dispatch order: range k = [blk[k], blk[k + 1]) holds force field
+    // dff[k]; blk[NFF] = all interaction blocks, blk[NFF + 1] = + bead blocks. fblk[ff] = first
+    // block of force field ff.
+    int blk[NFF + 2];
+    int dff[NFF];
+    int fblk[NFF];
     long slot0[NFF + 1];        // first slot id of each force field, total
     // stretching
     const int* str_b;  const double* str_k; const double* str_eql;
@@ -126,10 +131,10 @@ struct CgState {
 };
 
 __device__ inline int ff_of_block(const DevSys& s, int b) {
-    int ff = 0;
+    int k = 0;
 #pragma unroll
-    for (int k = 1; k <= NFF; k++) ff += (b >= s.blk[k]) ? 1 : 0;
-    return ff;   // NFF means a bead block
+    for (int q = 1; q <= NFF; q++) k += (b >= s.blk[q]) ? 1 : 0;
+    return k < NFF ? s.dff[k] : NFF;   // NFF means a bead block
 }
 
 // ---- deterministic reduction helpers (64-wide waves) ----------------------------------
@@ -488,7 +493,7 @@ __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* 
                                                         R* __restrict__ slots, int* bad_count, int* bad_list) {
     const int ff = ff_of_block(s, blockIdx.x);
     if (ff >= NFF) return;
-    const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+    const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     if (t >= s.n[ff]) return;
     Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
     int fbad = 0, bad_t = -1;
@@ -603,7 +608,7 @@ __global__ void __launch_bounds__(BLOCK) k_energy_report(DevSys s, const double*
     const int ff = ff_of_block(s, blockIdx.x);
     double v = 0.0;
     if (ff < NFF) {
-        const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+        const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
         int fbad = 0;
         Pos p{x, nullptr, nullptr, 0.0, 0.0, 0};
         if (t < s.n[ff]) v = term_ef<R>(s, ff, t, p, (R*)nullptr, 0, &fbad);
@@ -625,7 +630,8 @@ __global__ void __launch_bounds__(BLOCK) k_energy_report_fin(DevSys s, const dou
     for (int f2 = 0; f2 < NFF; f2++) {
         acc[f2] = 0.0;
         opn[f2] = 0;
-        for (int b = s.blk[f2] + threadIdx.x; b < s.blk[f2 + 1]; b += BLOCK) acc[f2] += partial[PSTRIDE * b];
+        const int b1 = s.fblk[f2] + (s.n[f2] + BLOCK - 1) / BLOCK;
+        for (int b = s.fblk[f2] + threadIdx.x; b < b1; b += BLOCK) acc[f2] += partial[PSTRIDE * b];
     }
     block_reduce_n<NFF>(acc, opn, lds);
     double dsum[1];
@@ -867,7 +873,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     if (done || (int)blockIdx.x >= s.blk[NFF]) return;   // (a 1-block grid when there are no interactions)
     if (phase != PH_LS && phase != PH_MOVE) return;
     const int ff = ff_of_block(s, blockIdx.x);
-    const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+    const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     const double* __restrict__ xn = B.x[par ^ 1];
     const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
     if (phase == PH_MOVE) {
@@ -1272,7 +1278,7 @@ __global__ void __launch_bounds__(BLOCK) k_tangent_terms(DevSys s, const double*
                                                          int* bad_count) {
     const int ff = ff_of_block(s, blockIdx.x);
     if (ff >= NFF) return;
-    const int t = (blockIdx.x - s.blk[ff]) * BLOCK + threadIdx.x;
+    const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     if (t >= s.n[ff]) return;
     int fbad = 0;
     term_tangent(s, ff, t, x, v, slots, &fbad);

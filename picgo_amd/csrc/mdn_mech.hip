@@ -177,12 +177,18 @@ const int VSTAT_BLOCKS = 1024;   // grid of k_vec_stats
 void layout(mdn_ctx* ctx) {
     DevSys& s = ctx->sys;
     s.n_beads = ctx->n_beads;
+    // the longest-running blocks are dispatched first (excluded volume, crosslinks, boundary,
+    // then the bonded terms): on grids larger than the chip holds at once, the cheap bonded
+    // blocks then fill in around them instead of delaying them to the kernel's tail
+    static const int order[NFF] = {FF_EXVOL, FF_LINKER, FF_MOTOR, FF_BOUNDARY, FF_BEND, FF_STRETCH};
     s.blk[0] = 0;
-    s.slot0[0] = 0;
-    for (int ff = 0; ff < NFF; ff++) {
-        s.blk[ff + 1] = s.blk[ff] + ceil_div(s.n[ff], BLOCK);
-        s.slot0[ff + 1] = s.slot0[ff] + (long)SLOTS_PER_TERM[ff] * s.n[ff];
+    for (int k = 0; k < NFF; k++) {
+        s.dff[k] = order[k];
+        s.fblk[order[k]] = s.blk[k];
+        s.blk[k + 1] = s.blk[k] + ceil_div(s.n[order[k]], BLOCK);
     }
+    s.slot0[0] = 0;
+    for (int ff = 0; ff < NFF; ff++) s.slot0[ff + 1] = s.slot0[ff] + (long)SLOTS_PER_TERM[ff] * s.n[ff];
     s.blk[NFF + 1] = s.blk[NFF] + ceil_div(ctx->n_beads, BLOCK);
 }
 
