@@ -178,6 +178,17 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* sys, const double* co
  * for MDN_FF_EXVOL and MDN_FF_BOUNDARY). out must hold 2 + MDN_NUM_FF int64 values. */
 int mdn_system_counts(mdn_ctx* ctx, int64_t* out);
 
+/* Incremental re-vectorization of the crosslink force fields alone: replaces the linker and
+ * motor-ghost tables of the uploaded system (same filaments, beads and cylinders) and keeps the
+ * filament tables, the neighbor lists and the excluded-volume / boundary tables of the last
+ * mdn_nl_build. Replaces the LinkerStretching / MotorGhostStretching part of
+ * ForceFieldManager::vectorizeAllForceFields (ForceFieldManager.cpp:35-50 ->
+ * LinkerStretching.cpp:30-57, MotorGhostStretching.cpp:30-57), which the reference reruns for
+ * every force field before each minimization although between chemistry steps only the bound
+ * linkers and motors change. Indices are validated before any state changes (MDN_ERR_ARG). */
+int mdn_crosslinks_update(mdn_ctx* ctx, const mdn_crosslink* linkers, int32_t n_linkers,
+                          const mdn_crosslink* motors, int32_t n_motors);
+
 /* Rebuild the excluded-volume neighbor list (HybridCylinderCylinderNL::reset,
  * HybridNeighborListImpl.cpp:527-556) and the boundary neighbor list
  * (BoundaryCylinderNL::reset, NeighborListImpl.cpp:801-809) from `coord` (host, or NULL for

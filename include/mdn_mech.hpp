@@ -52,6 +52,12 @@ public:
         check(mdn_system_upload(ctx_, &sys, coord.data()));
         nvar_ = 3 * sys.n_beads;
     }
+    // LinkerStretching / MotorGhostStretching::vectorize alone, between minimizations whose
+    // chemistry only bound or unbound crosslinks (filaments, neighbor lists kept)
+    void vectorizeCrosslinks(const std::vector<mdn_crosslink>& linkers, const std::vector<mdn_crosslink>& motors) {
+        check(mdn_crosslinks_update(ctx_, linkers.data(), (int32_t)linkers.size(), motors.data(),
+                                    (int32_t)motors.size()));
+    }
     // SubSystem::resetNeighborLists (hybrid cylinder NL + boundary NL) and the excluded-volume /
     // boundary vectorization; returns the number of excluded-volume pairs
     long resetNeighborLists(const std::vector<double>& coord) {

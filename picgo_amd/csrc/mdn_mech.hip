@@ -449,12 +449,66 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     return MDN_OK;
 }
 
+int relayout(mdn_ctx* ctx);
+
+// LinkerStretching / MotorGhostStretching::vectorize (LinkerStretching.cpp:30-57,
+// MotorGhostStretching.cpp:30-57): the crosslink tables (beads of both cylinders, k, rest length,
+// binding positions) and the static slot incidence, per bead the crosslink slots in ascending
+// slot order. The counts are set and the block / slot layout recomputed; the merged bead-sorted
+// layout is then stale until relayout (or the next neighbor-list build). Indices are validated
+// by the callers.
+int set_crosslinks(mdn_ctx* ctx, const mdn_crosslink* linkers, int n_linkers, const mdn_crosslink* motors,
+                   int n_motors) {
+    DevSys& s = ctx->sys;
+    std::vector<int> xb[2];
+    std::vector<double> xk[2], xe[2], xp1[2], xp2[2];
+    for (int w = 0; w < 2; w++) {
+        const int n = w ? n_motors : n_linkers;
+        const mdn_crosslink* xl = w ? motors : linkers;
+        xb[w].resize(4 * (size_t)n);
+        xk[w].resize(n); xe[w].resize(n); xp1[w].resize(n); xp2[w].resize(n);
+        for (int j = 0; j < n; j++) {
+            const int c1 = xl[j].cyl1, c2 = xl[j].cyl2;
+            xb[w][4 * j] = ctx->cyl_b1[c1]; xb[w][4 * j + 1] = ctx->cyl_b2[c1];
+            xb[w][4 * j + 2] = ctx->cyl_b1[c2]; xb[w][4 * j + 3] = ctx->cyl_b2[c2];
+            xk[w][j] = xl[j].kstr; xe[w][j] = xl[j].eql; xp1[w][j] = xl[j].pos1; xp2[w][j] = xl[j].pos2;
+        }
+    }
+    s.n[FF_LINKER] = n_linkers;
+    s.n[FF_MOTOR] = n_motors;
+    layout(ctx);
+    if (s.slot0[NFF] > 0x7fffffffL) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many interactions");
+    // static incidence CSR: per bead, slots of linkers / motors ascending
+    std::vector<int> cnt(ctx->n_beads + 1, 0);
+    auto each_static = [&](auto&& fn) {
+        for (int w = 0; w < 2; w++)
+            for (size_t t = 0; t < xk[w].size(); t++)
+                for (int q = 0; q < 4; q++) fn(xb[w][4 * t + q], s.slot0[2 + w] + 4L * t + q);
+    };
+    each_static([&](int b, long) { cnt[b + 1]++; });
+    for (int b = 0; b < ctx->n_beads; b++) cnt[b + 1] += cnt[b];
+    std::vector<int> inc(std::max(1, cnt[ctx->n_beads]));
+    {
+        std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+        each_static([&](int b, long sl) { inc[cur[b]++] = (int)sl; });   // visited in slot order
+    }
+    HIPCHK(upload(ctx, ctx->lnk_b, xb[0])); HIPCHK(upload(ctx, ctx->lnk_k, xk[0])); HIPCHK(upload(ctx, ctx->lnk_eql, xe[0]));
+    HIPCHK(upload(ctx, ctx->lnk_p1, xp1[0])); HIPCHK(upload(ctx, ctx->lnk_p2, xp2[0]));
+    HIPCHK(ctx->lnk_sf.ensure(std::max<size_t>(1, xk[0].size())));
+    HIPCHK(upload(ctx, ctx->mot_b, xb[1])); HIPCHK(upload(ctx, ctx->mot_k, xk[1])); HIPCHK(upload(ctx, ctx->mot_eql, xe[1]));
+    HIPCHK(upload(ctx, ctx->mot_p1, xp1[1])); HIPCHK(upload(ctx, ctx->mot_p2, xp2[1]));
+    HIPCHK(ctx->mot_sf.ensure(std::max<size_t>(1, xk[1].size())));
+    HIPCHK(upload(ctx, ctx->inc_s_start, cnt)); HIPCHK(upload(ctx, ctx->inc_s, inc));
+    return MDN_OK;
+}
+
 int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coord) {
     if (!ctx || !d || !coord) return fail(ctx, MDN_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(ctx->device));
     if (d->n_beads <= 0 || d->n_filaments < 0) return fail(ctx, MDN_ERR_ARG, "bad bead / filament counts");
     if (d->n_boundary > MAXELEM) return fail(ctx, MDN_ERR_UNSUPPORTED, "at most %d boundary elements", MAXELEM);
     ctx->uploaded = ctx->nl_built = false;
+    int st = MDN_OK;
     ctx->n_beads = d->n_beads;
     ctx->n_fil = d->n_filaments;
     int n_cyl = 0;
@@ -510,22 +564,13 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
             c += nb - 1;
         }
     }
-    // LinkerStretching / MotorGhostStretching::vectorize (LinkerStretching.cpp:30-57)
-    std::vector<int> xb[2];
-    std::vector<double> xk[2], xe[2], xp1[2], xp2[2];
     for (int w = 0; w < 2; w++) {
-        int n = w ? d->n_motors : d->n_linkers;
+        const int n = w ? d->n_motors : d->n_linkers;
         const mdn_crosslink* xl = w ? d->motors : d->linkers;
-        xb[w].resize(4 * (size_t)n);
-        xk[w].resize(n); xe[w].resize(n); xp1[w].resize(n); xp2[w].resize(n);
-        for (int j = 0; j < n; j++) {
-            int c1 = xl[j].cyl1, c2 = xl[j].cyl2;
-            if (c1 < 0 || c1 >= n_cyl || c2 < 0 || c2 >= n_cyl)
+        if (n < 0 || (n > 0 && !xl)) return fail(ctx, MDN_ERR_ARG, "bad %s table", w ? "motor" : "linker");
+        for (int j = 0; j < n; j++)
+            if (xl[j].cyl1 < 0 || xl[j].cyl1 >= n_cyl || xl[j].cyl2 < 0 || xl[j].cyl2 >= n_cyl)
                 return fail(ctx, MDN_ERR_ARG, "%s %d: cylinder index out of range", w ? "motor" : "linker", j);
-            xb[w][4 * j] = ctx->cyl_b1[c1]; xb[w][4 * j + 1] = ctx->cyl_b2[c1];
-            xb[w][4 * j + 2] = ctx->cyl_b1[c2]; xb[w][4 * j + 3] = ctx->cyl_b2[c2];
-            xk[w][j] = xl[j].kstr; xe[w][j] = xl[j].eql; xp1[w][j] = xl[j].pos1; xp2[w][j] = xl[j].pos2;
-        }
     }
     ctx->elems.assign(d->boundary, d->boundary + d->n_boundary);
     for (auto& e : ctx->elems)
@@ -541,8 +586,8 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     s = DevSys{};
     s.n[FF_STRETCH] = n_cyl;
     s.n[FF_BEND] = n_bend;
-    s.n[FF_LINKER] = d->n_linkers;
-    s.n[FF_MOTOR] = d->n_motors;
+    s.n[FF_LINKER] = 0;
+    s.n[FF_MOTOR] = 0;
     s.n[FF_EXVOL] = 0;
     s.n[FF_BOUNDARY] = 0;
     layout(ctx);
@@ -563,31 +608,9 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
             ok = put(bend_b[3 * t + 2], 2) && put(bend_b[3 * t + 1], 3) && put(bend_b[3 * t], 4) && ok;
         if (!ok) return fail(ctx, MDN_ERR_ARG, "a bead appears more than once in the filament bead lists");
     }
-    // static incidence CSR: per bead, slots of linkers / motors ascending
-    std::vector<int> cnt(ctx->n_beads + 1, 0);
-    auto each_static = [&](auto&& fn) {
-        for (int w = 0; w < 2; w++)
-            for (size_t t = 0; t < xk[w].size(); t++)
-                for (int q = 0; q < 4; q++) fn(xb[w][4 * t + q], s.slot0[2 + w] + 4L * t + q);
-    };
-    each_static([&](int b, long) { cnt[b + 1]++; });
-    for (int b = 0; b < ctx->n_beads; b++) cnt[b + 1] += cnt[b];
-    std::vector<int> inc(cnt[ctx->n_beads]);
-    {
-        std::vector<int> cur(cnt.begin(), cnt.end() - 1);
-        each_static([&](int b, long sl) { inc[cur[b]++] = (int)sl; });   // visited in slot order
-    }
-    if (s.slot0[NFF] > 0x7fffffffL) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many interactions");
-
     HIPCHK(upload(ctx, ctx->str_b, str_b)); HIPCHK(upload(ctx, ctx->str_k, str_k)); HIPCHK(upload(ctx, ctx->str_eql, str_eql));
     HIPCHK(upload(ctx, ctx->bend_b, bend_b)); HIPCHK(upload(ctx, ctx->bend_k, bend_k)); HIPCHK(upload(ctx, ctx->bend_eqt, bend_eqt));
-    HIPCHK(upload(ctx, ctx->lnk_b, xb[0])); HIPCHK(upload(ctx, ctx->lnk_k, xk[0])); HIPCHK(upload(ctx, ctx->lnk_eql, xe[0]));
-    HIPCHK(upload(ctx, ctx->lnk_p1, xp1[0])); HIPCHK(upload(ctx, ctx->lnk_p2, xp2[0]));
-    HIPCHK(ctx->lnk_sf.ensure(std::max<size_t>(1, xk[0].size())));
-    HIPCHK(upload(ctx, ctx->mot_b, xb[1])); HIPCHK(upload(ctx, ctx->mot_k, xk[1])); HIPCHK(upload(ctx, ctx->mot_eql, xe[1]));
-    HIPCHK(upload(ctx, ctx->mot_p1, xp1[1])); HIPCHK(upload(ctx, ctx->mot_p2, xp2[1]));
-    HIPCHK(ctx->mot_sf.ensure(std::max<size_t>(1, xk[1].size())));
-    HIPCHK(upload(ctx, ctx->inc_s_start, cnt)); HIPCHK(upload(ctx, ctx->inc_s, inc));
+    if ((st = set_crosslinks(ctx, d->linkers, d->n_linkers, d->motors, d->n_motors))) return st;
     HIPCHK(upload(ctx, ctx->pmask, pmask));
     HIPCHK(upload(ctx, ctx->d_cyl_bend, cyl_bend));
     HIPCHK(ctx->piece.ensure(NBOND * 3 * (size_t)ctx->n_beads));
@@ -625,11 +648,34 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
     ctx->cur = 0;
     HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
     refresh_ptrs(ctx);
-    int st = ensure_eval_buffers(ctx);
-    if (st) return st;
+    if ((st = ensure_eval_buffers(ctx))) return st;
     refresh_ptrs(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->uploaded = true;
+    return MDN_OK;
+}
+
+int mdn_crosslinks_update(mdn_ctx* ctx, const mdn_crosslink* linkers, int32_t n_linkers,
+                          const mdn_crosslink* motors, int32_t n_motors) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    HIPCHK(hipSetDevice(ctx->device));
+    for (int w = 0; w < 2; w++) {   // validate everything before any state changes
+        const int n = w ? n_motors : n_linkers;
+        const mdn_crosslink* xl = w ? motors : linkers;
+        if (n < 0 || (n > 0 && !xl)) return fail(ctx, MDN_ERR_ARG, "bad %s table", w ? "motor" : "linker");
+        for (int j = 0; j < n; j++)
+            if (xl[j].cyl1 < 0 || xl[j].cyl1 >= ctx->n_cyl || xl[j].cyl2 < 0 || xl[j].cyl2 >= ctx->n_cyl)
+                return fail(ctx, MDN_ERR_ARG, "%s %d: cylinder index out of range", w ? "motor" : "linker", j);
+    }
+    if ((st = set_crosslinks(ctx, linkers, n_linkers, motors, n_motors))) return st;
+    refresh_ptrs(ctx);
+    if (ctx->nl_built) {
+        if ((st = relayout(ctx))) return st;
+    } else {
+        ctx->sys_gen++;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
 }
 
@@ -657,6 +703,50 @@ int mdn_get_coords(mdn_ctx* ctx, double* coord) {
     HIPCHK(hipMemcpyAsync(coord, ctx->x[ctx->cur].p, sizeof(double) * 3 * ctx->n_beads, hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+// The block / slot layout for the current interaction counts, the dynamic incidence (excluded-
+// volume and boundary slots per bead, from the tables of the last neighbor-list build) and the
+// merged bead-sorted slot layout with the static crosslink incidence.
+int relayout(mdn_ctx* ctx) {
+    int st = MDN_OK;
+    const long np = ctx->sys.n[FF_EXVOL];
+    const int nbi = ctx->sys.n[FF_BOUNDARY];
+    layout(ctx);
+    if ((st = ensure_eval_buffers(ctx))) return st;
+    refresh_ptrs(ctx);
+    // dynamic incidence CSR
+    HIPCHK(ctx->inc_cnt.ensure(ctx->n_beads + 1));
+    HIPCHK(ctx->inc_cursor.ensure(ctx->n_beads + 1));
+    HIPCHK(ctx->inc_d_start.ensure(ctx->n_beads + 1));
+    const long ndyn = 4L * np + nbi;
+    HIPCHK(ctx->inc_d.ensure(std::max(1L, ndyn)));
+    HIPCHK(hipMemsetAsync(ctx->inc_cnt.p, 0, sizeof(int) * (ctx->n_beads + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->inc_cursor.p, 0, sizeof(int) * (ctx->n_beads + 1), ctx->stream));
+    refresh_ptrs(ctx);
+    if (ndyn > 0)
+        hipLaunchKernelGGL(k_inc_count, dim3(ceil_div(ndyn, BLOCK)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                           ctx->inc_cnt.p);
+    if ((st = exclusive_scan(ctx, ctx->inc_cnt.p, ctx->inc_d_start.p, ctx->n_beads + 1))) return st;
+    if (ndyn > 0) {
+        hipLaunchKernelGGL(k_inc_fill, dim3(ceil_div(ndyn, BLOCK)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
+                           ctx->inc_d_start.p, ctx->inc_cursor.p, ctx->inc_d.p);
+        hipLaunchKernelGGL(k_inc_sort, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
+                           ctx->inc_d_start.p, ctx->inc_d.p);
+    }
+    // bead-sorted slot layout (static then dynamic slots of each bead)
+    HIPCHK(ctx->bstart.ensure(ctx->n_beads + 1));
+    hipLaunchKernelGGL(k_inc_total, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
+                       ctx->inc_s_start.p, ctx->inc_d_start.p, ctx->inc_cnt.p);
+    HIPCHK(hipMemsetAsync(ctx->inc_cnt.p + ctx->n_beads, 0, sizeof(int), ctx->stream));
+    if ((st = exclusive_scan(ctx, ctx->inc_cnt.p, ctx->bstart.p, ctx->n_beads + 1))) return st;
+    hipLaunchKernelGGL(k_inc_merge, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
+                       ctx->inc_s_start.p, ctx->inc_s.p, ctx->inc_d_start.p, ctx->inc_d.p, ctx->bstart.p,
+                       ctx->sorted_id.p, ctx->pos.p);
+    HIPCHK(hipGetLastError());
+    refresh_ptrs(ctx);
+    ctx->sys_gen++;
     return MDN_OK;
 }
 
@@ -743,44 +833,10 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
                            ctx->bnd_e.p, ctx->bnd_krep.p, ctx->bnd_slen.p);
     ctx->n_bint = nbi;
     // new layout
-    DevSys& s = ctx->sys;
     if (np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many excluded-volume pairs");
-    s.n[FF_EXVOL] = (int)np;
-    s.n[FF_BOUNDARY] = nbi;
-    layout(ctx);
-    if ((st = ensure_eval_buffers(ctx))) return st;
-    refresh_ptrs(ctx);
-    // dynamic incidence CSR
-    HIPCHK(ctx->inc_cnt.ensure(ctx->n_beads + 1));
-    HIPCHK(ctx->inc_cursor.ensure(ctx->n_beads + 1));
-    HIPCHK(ctx->inc_d_start.ensure(ctx->n_beads + 1));
-    const long ndyn = 4L * np + nbi;
-    HIPCHK(ctx->inc_d.ensure(std::max(1L, ndyn)));
-    HIPCHK(hipMemsetAsync(ctx->inc_cnt.p, 0, sizeof(int) * (ctx->n_beads + 1), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->inc_cursor.p, 0, sizeof(int) * (ctx->n_beads + 1), ctx->stream));
-    refresh_ptrs(ctx);
-    if (ndyn > 0)
-        hipLaunchKernelGGL(k_inc_count, dim3(ceil_div(ndyn, BLOCK)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                           ctx->inc_cnt.p);
-    if ((st = exclusive_scan(ctx, ctx->inc_cnt.p, ctx->inc_d_start.p, ctx->n_beads + 1))) return st;
-    if (ndyn > 0) {
-        hipLaunchKernelGGL(k_inc_fill, dim3(ceil_div(ndyn, BLOCK)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                           ctx->inc_d_start.p, ctx->inc_cursor.p, ctx->inc_d.p);
-        hipLaunchKernelGGL(k_inc_sort, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
-                           ctx->inc_d_start.p, ctx->inc_d.p);
-    }
-    // bead-sorted slot layout (static then dynamic slots of each bead)
-    HIPCHK(ctx->bstart.ensure(ctx->n_beads + 1));
-    hipLaunchKernelGGL(k_inc_total, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
-                       ctx->inc_s_start.p, ctx->inc_d_start.p, ctx->inc_cnt.p);
-    HIPCHK(hipMemsetAsync(ctx->inc_cnt.p + ctx->n_beads, 0, sizeof(int), ctx->stream));
-    if ((st = exclusive_scan(ctx, ctx->inc_cnt.p, ctx->bstart.p, ctx->n_beads + 1))) return st;
-    hipLaunchKernelGGL(k_inc_merge, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->n_beads,
-                       ctx->inc_s_start.p, ctx->inc_s.p, ctx->inc_d_start.p, ctx->inc_d.p, ctx->bstart.p,
-                       ctx->sorted_id.p, ctx->pos.p);
-    HIPCHK(hipGetLastError());
-    refresh_ptrs(ctx);
-    ctx->sys_gen++;
+    ctx->sys.n[FF_EXVOL] = (int)np;
+    ctx->sys.n[FF_BOUNDARY] = nbi;
+    if ((st = relayout(ctx))) return st;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->nl_built = true;
     if (n_pairs_out) *n_pairs_out = np;

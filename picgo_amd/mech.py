@@ -30,7 +30,7 @@ EXPORTED = [
     "mdn_forces", "mdn_ff_forces", "mdn_crosslink_stretch_forces", "mdn_minimize",
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
-    "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces",
+    "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
 ]
 
 
@@ -67,6 +67,8 @@ def lib():
         L.mdn_ff_forces.argtypes = [C.c_void_p, C.c_int32, dp, dp, C.c_int32]
         L.mdn_crosslink_stretch_forces.argtypes = [C.c_void_p, dp, dp]
         L.mdn_boundary_load_forces.argtypes = [C.c_void_p, dp, C.c_int32, dp, i32p, C.c_int32, dp, dp]
+        xlp = C.POINTER(abi.Crosslink)
+        L.mdn_crosslinks_update.argtypes = [C.c_void_p, xlp, C.c_int32, xlp, C.c_int32]
         L.mdn_minimize.argtypes = [C.c_void_p, C.POINTER(abi.CgParams), dp, dp, C.POINTER(abi.MinResult)]
         L.mdn_bench_forces.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, dp]
         L.mdn_bench_energy.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
@@ -200,6 +202,18 @@ class ForceFieldManager:
         self._chk(self._L.mdn_minimize(self._h, C.byref(params), x.ctypes.data_as(dp),
                                        f.ctypes.data_as(dp) if want_forces else None, C.byref(res)))
         return x, f, res
+
+    def update_crosslinks(self, linkers, motors):
+        """LinkerStretching / MotorGhostStretching::vectorize alone (mdn_crosslinks_update): new
+        linker and motor-ghost tables (sequences of abi.Crosslink) for the uploaded system; the
+        filament tables and the neighbor lists are kept."""
+        arrs = []
+        for xl in (linkers, motors):
+            n = len(xl)
+            a = (abi.Crosslink * max(1, n))(*xl)
+            arrs.append((a, n))
+        (la, ln), (ma, mn) = arrs
+        self._chk(self._L.mdn_crosslinks_update(self._h, la, ln, ma, mn))
 
     def boundary_load_forces(self, monomer_size, num_monomers, coord=None, stride=None):
         """Boundary load forces on the filament ends (mdn_boundary_load_forces): (plus, minus),
