@@ -461,13 +461,30 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
             if (j * BLOCK < n) lds[threadIdx.x + j * BLOCK] = v[j];
         __syncthreads();
         const int q0 = max(my0, c0), q1 = min(my1, c1);
-        for (int q = q0; q < q1; q++) {
-            if (!all) {
-                long id = s.sorted_id[q];
-                if (id < lo || id >= hi) continue;
+        if (all) {
+            // four slots' LDS reads in flight at once; the sums stay in slot order
+            int q = q0;
+            for (; q + 4 <= q1; q += 4) {
+                const R* w = lds + 3 * (q - c0);
+                R u[12];
+#pragma unroll
+                for (int j = 0; j < 12; j++) u[j] = w[j];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    acc.x += (double)u[3 * j]; acc.y += (double)u[3 * j + 1]; acc.z += (double)u[3 * j + 2];
+                }
             }
-            const R* w = lds + 3 * (q - c0);
-            acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
+            for (; q < q1; q++) {
+                const R* w = lds + 3 * (q - c0);
+                acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
+            }
+        } else {
+            for (int q = q0; q < q1; q++) {
+                const long id = s.sorted_id[q];
+                if (id < lo || id >= hi) continue;
+                const R* w = lds + 3 * (q - c0);
+                acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
+            }
         }
         c0 = c1;
         if (c0 < r1) {
