@@ -474,9 +474,22 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
                     acc.x += (double)u[3 * j]; acc.y += (double)u[3 * j + 1]; acc.z += (double)u[3 * j + 2];
                 }
             }
-            for (; q < q1; q++) {
+            if (q < q1) {   // the last one to three slots, their reads in flight together
                 const R* w = lds + 3 * (q - c0);
-                acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
+                const int nr = q1 - q;
+                R u[9];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const bool on = j < nr;
+                    u[3 * j] = on ? w[3 * j] : (R)0;
+                    u[3 * j + 1] = on ? w[3 * j + 1] : (R)0;
+                    u[3 * j + 2] = on ? w[3 * j + 2] : (R)0;
+                }
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (j < nr) {
+                        acc.x += (double)u[3 * j]; acc.y += (double)u[3 * j + 1]; acc.z += (double)u[3 * j + 2];
+                    }
             }
         } else {
             for (int q = q0; q < q1; q++) {
