@@ -45,7 +45,9 @@ constexpr int NFF = 6;
 constexpr int MAXELEM = 8;       // boundary elements held in the kernel argument
 constexpr int NTRIAL = 2;        // line-search step lengths evaluated per launch (lambda, lambda/2)
 constexpr int PSTRIDE = 4;       // partial slots per block
-constexpr int GB = 64;           // beads per gather block
+// beads per gather block (GB / 64 waves sum, all 256 threads stage slots): 64 below
+// GB_LARGE_BEADS beads, 128 from there on (half the blocks to dispatch and to finalize)
+constexpr int GB_LARGE_BEADS = 100000;
 constexpr int DEFER_CAP = 256;   // excluded-volume energies deferred to the Simpson pass per launch
 
 // An excluded-volume pair whose closed-form energy is non-finite (or < -1): the reference then
@@ -414,7 +416,7 @@ constexpr int NBOND = 5;              // bonded pieces per bead
 // every bead, so the loads below are unconditional (the results are selected, not branched on).
 constexpr long GCHUNK_PAD = GCHUNK_BYTES / 4;   // values (>= 3 * gchunk<R>() for R = float, double)
 
-template <typename R>
+template <typename R, int GB>
 __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restrict__ slots, long lo, long hi,
                                   bool all, D3 acc, double* lds_raw /* GCHUNK_BYTES */) {
     constexpr int CH = gchunk<R>(), UN = gunroll<R>();
@@ -428,7 +430,7 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
     // bonded pieces in flight first
     const unsigned m = (mine && bmask) ? s.pmask[bc] & ((bmask & 1 ? 0x3u : 0u) | (bmask & 2 ? 0x1cu : 0u)) : 0u;
     R pv[NBOND][3] = {};
-    if (threadIdx.x < GB) {   // wave 0 (uniform branch)
+    if (threadIdx.x < GB) {   // the summing waves (uniform branch)
         const R* pc = static_cast<const R*>(s.piece);
 #pragma unroll
         for (int k = 0; k < NBOND; k++)
@@ -536,14 +538,14 @@ __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* 
 
 // force[b] = base[b] + the bonded pieces selected by bmask + b's slots with id in [lo, hi)
 // (base may be null: zero-fill). Grid: gather blocks (GB beads each).
-template <typename R>
+template <typename R, int GB>
 __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, const R* __restrict__ slots, long lo,
                                                         long hi, const double* __restrict__ base, double* __restrict__ f) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     const int b = blockIdx.x * GB + threadIdx.x;
     const bool mine = threadIdx.x < GB && b < s.n_beads;
     D3 acc = (base && mine) ? D3{base[3 * b], base[3 * b + 1], base[3 * b + 2]} : D3{0.0, 0.0, 0.0};
-    acc = gather_block<R>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L);
+    acc = gather_block<R, GB>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L);
     if (!mine) return;
     f[3 * b] = acc.x; f[3 * b + 1] = acc.y; f[3 * b + 2] = acc.z;
 }
@@ -1056,7 +1058,7 @@ __device__ inline void ls_control(CgState& S, const double* acc, int& bad_count)
 // ---- gather F into the next force buffer; F.F, F.F_prev, max|F| partials ---------------
 // After a line-search launch the gather is speculative (k_cg_fin uses it when lambda0 is
 // accepted); after a move it is the step's. Grid: bead blocks only.
-template <typename R>
+template <typename R, int GB>
 __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     // the state fields, loaded before the first branch (one round trip)
@@ -1072,25 +1074,38 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     // and force parts decide in k_cg_fin whether it is committed
     const bool cand = num_iter + 1 > force_call_limit / 2;
     double* __restrict__ fnext = B.f[fpar ^ 1];
-    D3 acc = gather_block<R>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
-    if (threadIdx.x >= GB) return;   // one wave finishes
-    double v[3] = {0.0, 0.0, 0.0};
-    if (b < s.n_beads) {
-        const double fv[3] = {acc.x, acc.y, acc.z};
-        const double* xn = B.x[par ^ 1];
-        for (int q = 0; q < 3; q++) {
-            fnext[3 * b + q] = fv[q];
-            v[0] += fv[q] * fv[q];
-            v[1] += fv[q] * B.fp[3 * b + q];
-            v[2] = rmax(v[2], fabs(fv[q]));
-            if (cand) B.xmin[bk ^ 1][3 * b + q] = xn[3 * b + q];
+    D3 acc = gather_block<R, GB>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
+    constexpr int NSW = GB / 64;   // summing waves
+    __shared__ double red[3 * NSW];
+    if (threadIdx.x < GB) {   // whole waves
+        double v[3] = {0.0, 0.0, 0.0};
+        if (b < s.n_beads) {
+            const double fv[3] = {acc.x, acc.y, acc.z};
+            const double* xn = B.x[par ^ 1];
+            for (int q = 0; q < 3; q++) {
+                fnext[3 * b + q] = fv[q];
+                v[0] += fv[q] * fv[q];
+                v[1] += fv[q] * B.fp[3 * b + q];
+                v[2] = rmax(v[2], fabs(fv[q]));
+                if (cand) B.xmin[bk ^ 1][3 * b + q] = xn[3 * b + q];
+            }
         }
+        v[0] = wave_sum(v[0]);
+        v[1] = wave_sum(v[1]);
+        v[2] = wave_max(v[2]);
+        if ((threadIdx.x & 63) == 0)
+            for (int q = 0; q < 3; q++) red[q * NSW + (threadIdx.x >> 6)] = v[q];
     }
-    v[0] = wave_sum(v[0]);
-    v[1] = wave_sum(v[1]);
-    v[2] = wave_max(v[2]);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 3; q++) B.gpartial[(long)q * gridDim.x + blockIdx.x] = v[q];   // value-major
+    __syncthreads();
+    if (threadIdx.x == 0) {   // the summing waves' results in wave order
+        double r[3] = {red[0], red[NSW], red[2 * NSW]};
+        for (int k = 1; k < NSW; k++) {
+            r[0] += red[k];
+            r[1] += red[NSW + k];
+            r[2] = rmax(r[2], red[2 * NSW + k]);
+        }
+        for (int q = 0; q < 3; q++) B.gpartial[(long)q * gridDim.x + blockIdx.x] = r[q];   // value-major
+    }
 }
 
 // Scalar logic after a force evaluation: maxF, energyRelChange, beta, the minimum-energy

@@ -193,7 +193,25 @@ void layout(mdn_ctx* ctx) {
 }
 
 int bead_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, BLOCK)); }
-int gather_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, GB)); }
+int gather_size(mdn_ctx* ctx) { return ctx->n_beads >= GB_LARGE_BEADS ? 128 : 64; }
+int gather_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, gather_size(ctx))); }
+// the gather kernels at the context's beads-per-block
+template <typename R>
+void launch_gather_plain(mdn_ctx* ctx, int bmask, const R* slots, long lo, long hi, const double* base, double* out) {
+    const dim3 g(gather_blocks(ctx)), b(BLOCK);
+    if (gather_size(ctx) == 128)
+        hipLaunchKernelGGL((k_gather_plain<R, 128>), g, b, 0, ctx->stream, ctx->sys, bmask, slots, lo, hi, base, out);
+    else
+        hipLaunchKernelGGL((k_gather_plain<R, 64>), g, b, 0, ctx->stream, ctx->sys, bmask, slots, lo, hi, base, out);
+}
+template <typename R>
+void launch_cg_gather(mdn_ctx* ctx, const CgState* st, const CgBufs& B) {
+    const dim3 g(gather_blocks(ctx)), b(BLOCK);
+    if (gather_size(ctx) == 128)
+        hipLaunchKernelGGL((k_cg_gather<R, 128>), g, b, 0, ctx->stream, ctx->sys, st, B);
+    else
+        hipLaunchKernelGGL((k_cg_gather<R, 64>), g, b, 0, ctx->stream, ctx->sys, st, B);
+}
 
 void refresh_ptrs(mdn_ctx* ctx) {
     DevSys& s = ctx->sys;
@@ -319,11 +337,9 @@ int eval_slots(mdn_ctx* ctx, const double* x, int prec) {
 // slots with id in [lo, hi)
 int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec) {
     if (prec == MDN_PREC_MIXED)
-        hipLaunchKernelGGL(k_gather_plain<float>, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                           bmask, (const float*)ctx->slots.p, lo, hi, base, out);
+        launch_gather_plain<float>(ctx, bmask, (const float*)ctx->slots.p, lo, hi, base, out);
     else
-        hipLaunchKernelGGL(k_gather_plain<double>, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                           bmask, (const double*)ctx->slots.p, lo, hi, base, out);
+        launch_gather_plain<double>(ctx, bmask, (const double*)ctx->slots.p, lo, hi, base, out);
     HIPCHK(hipGetLastError());
     return MDN_OK;
 }
@@ -1104,7 +1120,7 @@ static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
     hipLaunchKernelGGL(k_cg_pos, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_ls<R>, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s,
                        (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_gather<R>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
+    launch_cg_gather<R>(ctx, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls, gbk);
 }
 }  // extern "C++"
@@ -1243,12 +1259,8 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
         const CgBufs B = cg_bufs(ctx);
         const int gbk = gather_blocks(ctx);
-        if (prec == MDN_PREC_MIXED)
-            hipLaunchKernelGGL(k_cg_gather<float>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                               (const CgState*)ctx->cg.p, B);
-        else
-            hipLaunchKernelGGL(k_cg_gather<double>, dim3(gbk), dim3(BLOCK), 0, ctx->stream, ctx->sys,
-                               (const CgState*)ctx->cg.p, B);
+        if (prec == MDN_PREC_MIXED) launch_cg_gather<float>(ctx, (const CgState*)ctx->cg.p, B);
+        else launch_cg_gather<double>(ctx, (const CgState*)ctx->cg.p, B);
         hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, ctx->sys, ctx->cg.p, B, ls_blocks(ctx), gbk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
@@ -1525,8 +1537,7 @@ static int bench_forces_t(mdn_ctx* ctx, int32_t iters, double* ms_per_eval, doub
     for (int i = 0; i < iters; i++) {
         hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
                            ctx->bad_count.p, ctx->bad_list.p);
-        hipLaunchKernelGGL(k_gather_plain<R>, dim3(gather_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s, 3,
-                           (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
+        launch_gather_plain<R>(ctx, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
     }
     HIPCHK(hipEventRecord(e1, ctx->stream));
     // dominant kernel alone
