@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 from picgo_amd import abi, mech, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-WORKLOAD = "config1: 2000 filaments x 20 cylinders (42000 beads), 8000 linkers, 4000 motors, 5 um cube"
+WORKLOAD = "single network: 2000 filaments x 20 cylinders (42000 beads), 8000 linkers, 4000 motors, 5 um cube"
 
 
 def algorithmic_bytes(counts, n_beads, sx=8, sf=8, sp=8, idx=4):
@@ -50,7 +50,7 @@ def algorithmic_bytes(counts, n_beads, sx=8, sf=8, sp=8, idx=4):
 
 def cpu_baseline(net, target_s=15.0):
     """Oracle (liboracle_fast.so: the reference's release flags) on one core: force
-    evaluations of the same config-1 network for about target_s seconds."""
+    evaluations of the same 42,000-bead network for about target_s seconds."""
     from oracle import oracle_ffi as of
     o = of.OracleSystem(net.desc, fast=True)
     o.build_nl(net.coord)
@@ -63,24 +63,24 @@ def cpu_baseline(net, target_s=15.0):
         if el >= target_s:
             break
     return {"value": n / el, "unit": "force-evals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} force evaluations of the config-1 network ({el:.1f} s, oracle built "
+            "sample": f"{n} force evaluations of the 42,000-bead network ({el:.1f} s, oracle built "
                       f"-O2 -funroll-loops -flto -ffast-math, single thread)"}
 
 
 def load_traffic(kernel):
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_config1.json")
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_42k_bead_network.json")
     if os.path.exists(p):
         with open(p) as fh:
             return json.load(fh).get("hbm_bytes_per_launch")
     return None
 
 
-STRESS = "config4: 20000 filaments x 20 cylinders (420000 beads), 80000 linkers, 40000 motors, 10 um sphere"
-BATCH = "config3: 256 networks, 20-400 filaments x 5-30 cylinders, 2 linkers + 1 motor per filament"
+STRESS = "successive minimizations: 20000 filaments x 20 cylinders (420000 beads), 80000 linkers, 40000 motors, 10 um sphere"
+BATCH = "batch: 256 networks, 20-400 filaments x 5-30 cylinders, 2 linkers + 1 motor per filament"
 
 
 def run_stress(args):
-    """Successive minimizations of the config-4 network: before each one the beads get seeded
+    """Successive minimizations of the 420,000-bead network: before each one the beads get seeded
     N(0, 5 nm) noise and the neighbor lists are rebuilt; the same noise sequence for both
     precisions, drawn before the timed region (it is the workload's input). Everything between
     the first perturbation and the last result is timed: applying the noise, the neighbor-list
@@ -122,7 +122,7 @@ def run_stress(args):
         "ms_per_step": 1e3 * out["mixed"]["total_s"] / args.stress_count,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator, config 4; seeded N(0, 5 nm) perturbations)",
+        "data": "synthetic (seeded generator; seeded N(0, 5 nm) perturbations between minimizations)",
         "config": {"workload": STRESS, "n_beads": net.n_beads, "minimizations": args.stress_count},
         "mixed": out["mixed"], "fp64": out["fp64"],
         "mixed_vs_fp64_minimizations": out["mixed"]["minimizations_per_s"] / out["fp64"]["minimizations_per_s"],
@@ -131,7 +131,7 @@ def run_stress(args):
 
 
 def run_minimize(args):
-    """One full minimization (reference criterion) of the config-1 or config-4 network from the
+    """One full minimization (reference criterion) of the 42,000- or 420,000-bead network from the
     seeded start, after a 50-energy-call warm-up: the unit the CG kernel profiles are taken on."""
     net = synth.Network.config(args.net)
     ffm = mech.ForceFieldManager(0)
@@ -148,8 +148,8 @@ def run_minimize(args):
         "value": 1.0 / dt, "unit": "minimizations/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == abi.MDN_PREC_FP64 else "f32 term math + f64 excluded volume / accumulation",
-        "data": f"synthetic (seeded generator, config {args.net})",
-        "config": {"workload": f"one minimization of the config-{args.net} network", "n_beads": net.n_beads},
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"one minimization of the {net.n_beads}-bead network", "n_beads": net.n_beads},
         "cg_iterations": res.num_iter, "us_per_cg_iteration": 1e6 * dt / max(1, res.num_iter),
         "line_search_launches": st["ls_launches"],
         "criterion": "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit"),
@@ -158,7 +158,7 @@ def run_minimize(args):
 
 
 def run_batched(args):
-    """The config-3 batch in one persistent kernel (mixed and FP64), against the same systems
+    """The 256-network batch in one persistent kernel (mixed and FP64), against the same systems
     minimized one after another by the single-system minimizer (FP64)."""
     nets = [synth.Network.config(3, index=i) for i in range(args.batch_count)]
     ffms = []
@@ -194,7 +194,7 @@ def run_batched(args):
         "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * out["mixed"]["batch_s"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator, config 3)",
+        "data": "synthetic (seeded generator)",
         "config": {"workload": BATCH, "systems": len(nets), "n_beads_total": int(sum(n.n_beads for n in nets))},
         "batched_mixed": out["mixed"], "batched_fp64": out["fp64"], "sequential_fp64": out["sequential_fp64"],
         "batched_mixed_vs_sequential_fp64": out["mixed"]["minimizations_per_s"] / out["sequential_fp64"]["minimizations_per_s"],
@@ -210,8 +210,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-min", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["config1", "stress", "batched", "minimize"], default="config1",
-                    help="config1: the metric's single network (default); stress: 20,000 x 20 filaments in a "
+    ap.add_argument("--workload", choices=["single", "stress", "batched", "minimize"], default="single",
+                    help="single: the metric's single network (default); stress: 20,000 x 20 filaments in a "
                          "10 um sphere, successive minimizations with 5 nm noise and a neighbor-list rebuild "
                          "each; batched: 256 uneven networks in one persistent kernel; minimize: one "
                          "minimization of the --net network in --prec precision (for profiling)")
@@ -300,7 +300,7 @@ def main():
         "ms_per_step": mixed["us_per_force_eval"] / 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator, config 1)",
+        "data": "synthetic (seeded generator)",
         "config": {"workload": WORKLOAD, "n_beads": net.n_beads, "n_cylinders": net.n_cylinders,
                    "exvol_pairs": n_pairs, "precision_path": "mixed"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -1,8 +1,8 @@
 /*
  * oracle.h -- CPU FP64 restatement of MEDYAN 5.4.0's mechanical-equilibration path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the correctness checker for the gfx950 library
- * (libmdn_mech.so) and the single-core CPU baseline timed beside it. Only tests/, the smoke
+ * TEST INFRASTRUCTURE ONLY. This library is the oracle the gfx950 library (libmdn_mech.so) is
+ * tested against, and the single-core CPU baseline timed beside it. Only tests/, the smoke
  * entry point and the benchmark's cpu_baseline leg may load it; the product never does.
  *
  * It restates, from the mathematics and the cited reference sources read as text (the

@@ -1,7 +1,7 @@
 """Python binding of the CPU oracle (liboracle.so / liboracle_fast.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, as the checker and the CPU baseline. The product (picgo_amd) never imports it.
+cpu_baseline leg, as the oracle and the CPU baseline. The product (picgo_amd) never imports it.
 """
 import ctypes as C
 import os

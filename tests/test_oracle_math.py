@@ -176,7 +176,7 @@ def test_boundary_sphere_closed_form():
 
 # ---------------------------------------------------------------- neighbor lists ------------
 
-def test_nl_brute_equals_cells_on_config1():
+def test_nl_brute_equals_cells_on_the_40k_cylinder_network():
     net = synth.Network.config(1)
     o = of.OracleSystem(net.desc, use_cells=True)
     o.build_nl(net.coord)
