@@ -538,6 +538,32 @@ MDN_HD inline double exvol_energy_forces(D3 c1, D3 c2, D3 c3, D3 c4, double kRep
 // reference's operand order). The invariants a ... JJ are the reference's, bit for bit, so the
 // degenerate geometries (AA, BB, EE, FF or JJ exactly zero) give non-finite values exactly where
 // the reference's do and the Simpson fallbacks still trigger there.
+// The coplanar test without its two normalizations: |n.v3| < 0.1 for the unit normal n of
+// (p2 - p1, p3 - p1) and the unit v3 = (p4 - p1) / |p4 - p1| is (cp.w)^2 < 0.01 |cp|^2 |w|^2 for
+// the unnormalized cp and w (a degenerate cp or w, where the reference's test reads NaN and
+// fails, gives 0 < 0 and fails too). Pairs within rounding of the threshold may be classified
+// differently from the reference's form.
+MDN_HD inline bool are_in_plane_fast(D3 p1, D3 p2, D3 p3, D3 p4) {
+    const double v1[3] = {p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+    const double v2[3] = {p3.x - p1.x, p3.y - p1.y, p3.z - p1.z};
+    const double w[3] = {p4.x - p1.x, p4.y - p1.y, p4.z - p1.z};
+    double cp[3];
+    cross3(v1, v2, cp);
+    const double cw = cp[0] * w[0] + cp[1] * w[1] + cp[2] * w[2];
+    const double cc = cp[0] * cp[0] + cp[1] * cp[1] + cp[2] * cp[2];
+    const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    return cw * cw < 0.01 * (cc * ww);
+}
+// bead 2 moved by d along the unit normal of (p2 - p1, p3 - p1), with one reciprocal square root
+MDN_HD inline D3 move_out_of_plane_fast(D3 p1, D3 p2, D3 p3, double d) {
+    const double v1[3] = {p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+    const double v2[3] = {p3.x - p1.x, p3.y - p1.y, p3.z - p1.z};
+    double n[3];
+    cross3(v1, v2, n);
+    const double sc = d / sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    return D3{p2.x + n[0] * sc, p2.y + n[1] * sc, p2.z + n[2] * sc};
+}
+
 MDN_HD inline double exvol_energy_of_fast(const ExvTerms<double>& t, double kRepScaled) {
     return exvol_energy_of<double>(t, kRepScaled);   // the atan2 differences are in t
 }
@@ -631,7 +657,7 @@ template <typename O> MDN_HD inline bool exvol_forces_from_fast(const ExvTerms<d
 // forms above, on the atan2 terms
 MDN_HD inline bool exvol_forces_fast(D3 c1, D3 c2in, D3 c3, D3 c4, double kRepScaled, D3* out) {
     D3 c2 = c2in;
-    if (are_in_plane(c1, c2, c3, c4)) c2 = move_out_of_plane(c1, c2, c3, 2, 0.01);
+    if (are_in_plane_fast(c1, c2, c3, c4)) c2 = move_out_of_plane_fast(c1, c2, c3, 0.01);
     ExvTerms<double> t;
     exvol_terms<double, D3, true>(c1, c2, c3, c4, t);
     return exvol_forces_from_fast(t, kRepScaled, out);
@@ -651,8 +677,8 @@ MDN_HD inline double exvol_energy_forces_fast(D3 c1, D3 c2, D3 c3, D3 c4, double
     ExvTerms<double> t;
     exvol_terms<double, D3, true>(c1, c2, c3, c4, t);
     const double U = exvol_energy_of_fast(t, kRepScaled);
-    if (are_in_plane(c1, c2, c3, c4)) {
-        const D3 c2s = move_out_of_plane(c1, c2, c3, 2, 0.01);
+    if (are_in_plane_fast(c1, c2, c3, c4)) {
+        const D3 c2s = move_out_of_plane_fast(c1, c2, c3, 0.01);
         exvol_terms<double, D3, true>(c1, c2s, c3, c4, t);
     }
     *fok = exvol_forces_from_fast(t, kRepScaled, out);
