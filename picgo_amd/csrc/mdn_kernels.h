@@ -810,7 +810,7 @@ struct CgBufs {
     double* d[2];       // current / next search direction (dpar)
     double* xmin[2];    // committed / candidate minimum-energy backup (bkpar)
     double* f[2];       // current / next force (fpar)
-    double* fp;         // previous force
+    double* fp;         // previous force (the batched minimizer's; here F_prev is f[fpar])
     double* xh;         // x + (lambda / 2) d, the second trial of a launch after a rejection
     void* slots;        // 3 values of the evaluation's arithmetic type per slot
     double* partial;    // line-search partials, value-major: [value][interaction blocks + bead blocks]
@@ -829,7 +829,7 @@ __device__ inline bool converged(const CgState* st) {
 // ---- line search, bead part: trial coordinates, d.F, the recovery step ------------------
 // Grid: bead blocks. Phase PH_LS: x_next = x + lambda0 d_eff (trial 1, and the speculative next
 // coordinates), x + (lambda0 / 2) d after a rejection (trial 2), and on the first launch of a
-// search the d.F and max-step partials, the materialized direction and F_prev. Phase PH_MOVE:
+// search the d.F and max-step partials and the materialized direction. Phase PH_MOVE:
 // x_next = x + lambda d (moveAlongSearchDir, CGMethod.cpp:350-354; the recovery move,
 // MController.h:318-321). The partials go after the interaction blocks' (index T + block).
 __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __restrict__ st, CgBufs B) {
@@ -838,10 +838,10 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
     // state instead of a chain of dependent ones
     const int done = st->done, phase = st->phase, par = st->parity, dpar = st->dpar, fpar = st->fpar;
     const int ls_active = st->ls_active, dir_pending = st->dir_pending;
-    const double lambda = st->lambda, lam0 = st->ls_lambda, beta = st->dir_beta, maxd = st->max_dist;
+    const double lambda = st->lambda, lam0 = st->ls_lambda, beta = st->dir_beta;
     keep_loaded(done); keep_loaded(phase); keep_loaded(par); keep_loaded(dpar); keep_loaded(fpar);
     keep_loaded(ls_active); keep_loaded(dir_pending);
-    keep_loaded(lambda); keep_loaded(lam0); keep_loaded(beta); keep_loaded(maxd);
+    keep_loaded(lambda); keep_loaded(lam0); keep_loaded(beta);
     if (done) return;
     if (phase != PH_LS && phase != PH_MOVE) return;
     // block b covers the 3 * BLOCK coordinates of beads [BLOCK b, BLOCK (b + 1)), coalesced
@@ -861,7 +861,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
     const bool first = ls_active == 0;
     const bool pend = first && dir_pending;
     const double* __restrict__ fcur = B.f[fpar];
-    double v[2] = {0.0, INFINITY};
+    double v[2] = {0.0, 0.0};
     double* __restrict__ dn = B.d[dpar ^ 1];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -873,15 +873,12 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
         if (!first) B.xh[e] = xc[e] + (lam0 * 0.5) * de;
         if (first) {
             v[0] += de * fv;                          // d.F
-            v[1] = rmin(v[1], maxd / fabs(de));       // MController.h:293-305
-            if (pend) {
-                dn[e] = de;                           // shiftSearchDir materialized
-                B.fp[e] = fv;                         // forcePrev = force
-            }
+            v[1] = rmax(v[1], fabs(de));              // max|d|, for the recovery step
+            if (pend) dn[e] = de;                     // shiftSearchDir materialized
         }
     }
     if (!first) return;   // the partials are read on the first launch of a search only
-    const int op2[2] = {0, 2};
+    const int op2[2] = {0, 1};
     block_reduce_n<2>(v, op2, lds);
     if (threadIdx.x == 0) {
         const long np = s.blk[NFF + 1], k = s.blk[NFF] + blockIdx.x;   // value-major
@@ -1074,6 +1071,9 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     // and force parts decide in k_cg_fin whether it is committed
     const bool cand = num_iter + 1 > force_call_limit / 2;
     double* __restrict__ fnext = B.f[fpar ^ 1];
+    // forcePrev = force after every step and reset (CGMethod.hpp:966-968, 1532-1535), and the
+    // current force buffer is replaced only by a committed gather: F_prev is f[fpar]
+    const double* __restrict__ fprev = B.f[fpar];
     D3 acc = gather_block<R, GB>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
     constexpr int NSW = GB / 64;   // summing waves
     __shared__ double red[3 * NSW];
@@ -1085,7 +1085,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
             for (int q = 0; q < 3; q++) {
                 fnext[3 * b + q] = fv[q];
                 v[0] += fv[q] * fv[q];
-                v[1] += fv[q] * B.fp[3 * b + q];
+                v[1] += fv[q] * fprev[3 * b + q];
                 v[2] = rmax(v[2], fabs(fv[q]));
                 if (cand) B.xmin[bk ^ 1][3 * b + q] = xn[3 * b + q];
             }
@@ -1183,8 +1183,8 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
         sh_bad = *B.bad_count;
     }
     const int T = s.blk[NFF];
-    // E(lambda), E(lambda/2), -, -, E(x), -, d.F, recovery step | F.F, F.F_prev, max|F|
-    double acc[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY, 0.0, 0.0, 0.0};
+    // E(lambda), E(lambda/2), -, -, E(x), -, d.F, max|d| (then the recovery step) | F.F, F.F_prev, max|F|
+    double acc[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int base = 0; base < nls; base += 8 * FIN) {
         double p0[8], p1[8], p2[8];
 #pragma unroll
@@ -1204,7 +1204,7 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
                 acc[4] += p2[j];
             } else {
                 acc[6] += p0[j];
-                acc[7] = rmin(acc[7], p1[j]);
+                acc[7] = rmax(acc[7], p1[j]);
             }
         }
     }
@@ -1227,9 +1227,12 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
     if (Sl.done || (phase0 != PH_LS && phase0 != PH_MOVE)) return;
     // the eight reduced values (the three flags of acc are set from the deferred pass below)
     double r8[8] = {acc[0], acc[1], acc[4], acc[6], acc[7], acc[8], acc[9], acc[10]};
-    const int op[8] = {0, 0, 0, 0, 2, 0, 0, 1};
+    const int op[8] = {0, 0, 0, 0, 1, 0, 0, 1};
     block_reduce_fold<8, FIN>(r8, op, lds);
-    acc[0] = r8[0]; acc[1] = r8[1]; acc[4] = r8[2]; acc[6] = r8[3]; acc[7] = r8[4];
+    acc[0] = r8[0]; acc[1] = r8[1]; acc[4] = r8[2]; acc[6] = r8[3];
+    // the recovery step, min over components of maxDist / |d| (MController.h:293-305), equals
+    // maxDist / max|d| under correctly rounded division: one division instead of one per component
+    acc[7] = Sl.max_dist / r8[4];
     acc[8] = r8[5]; acc[9] = r8[6]; acc[10] = r8[7];
     double dsum[3] = {0.0, 0.0, 0.0};
     int dbad[3] = {0, 0, 0};
