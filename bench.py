@@ -13,9 +13,10 @@ the bonded, crosslink and boundary force fields, FP64 excluded volume, FP64 accu
 FP64 path is timed beside it, and the max force error of the mixed path against FP64 is
 reported. The same JSON line carries the CG minimizations/s of full minimizations from the
 seeded start in both precisions, with the reference criterion (max|F| < 1 pN or |dE/E| < 1e-9)
-and with the force tolerance alone, the roofline of the dominant kernel (HIP events on the
-library's stream) and the single-core CPU baseline (the oracle built with the reference's
-release flags, timed on a bounded sample of the same workload).
+and with the force tolerance alone, the roofline of the force evaluation (its two dependent
+launches, the terms kernel and the gather, timed with HIP events on the library's stream) and the
+single-core CPU baseline (the oracle built with the reference's release flags, timed on a
+bounded sample of the same workload).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-min]
 """
@@ -67,8 +68,8 @@ def cpu_baseline(net, target_s=15.0):
                       f"-O2 -funroll-loops -flto -ffast-math, single thread)"}
 
 
-def load_traffic(kernel):
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel}_42k_bead_network.json")
+def load_traffic(unit):
+    p = os.path.join(ROOT, "profiles", f"traffic_{unit}_42k_bead_network.json")
     if os.path.exists(p):
         with open(p) as fh:
             return json.load(fh).get("hbm_bytes_per_launch")
@@ -291,7 +292,10 @@ def main():
                       f"criterion {mins[f'{name}_{crit}']['criterion']}", file=sys.stderr, flush=True)
 
     mixed, fp64 = evals["mixed"], evals["fp64"]
-    achieved = bytes_alg / (mixed["us_terms_kernel"] * 1e-6) / 1e9
+    # the roofline unit is one force evaluation (the algorithmic bytes are per evaluation): the
+    # terms kernel (slots of the crosslinks, excluded volume and boundary) and the gather (which
+    # recomputes the filament terms' forces and sums every bead's forces), back to back
+    achieved = bytes_alg / (mixed["us_per_force_eval"] * 1e-6) / 1e9
     line = {
         "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
         "value": mixed["force_evals_per_s"],
@@ -304,8 +308,9 @@ def main():
         "config": {"workload": WORKLOAD, "n_beads": net.n_beads, "n_cylinders": net.n_cylinders,
                    "exvol_pairs": n_pairs, "precision_path": "mixed"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("k_forces_terms_mixed"),
-                     "kernel": "k_forces_terms<float>", "algorithmic_bytes": bytes_alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("force_eval_mixed"),
+                     "kernel": "force evaluation: k_forces_terms<float> + k_gather_plain<float, 64, true>",
+                     "algorithmic_bytes": bytes_alg},
         "mixed": mixed, "fp64": fp64,
         "mixed_vs_fp64_force_evals": mixed["force_evals_per_s"] / fp64["force_evals_per_s"],
         "mixed_max_rel_force_error": max_rel_err,

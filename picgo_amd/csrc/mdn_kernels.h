@@ -92,6 +92,18 @@ struct DevSys {
     // term with the bead as bead 3, 2, 1; pmask[bead] bit k marks the pieces the bead has
     void* piece;
     const unsigned char* pmask;
+    // bonded_gather: the minimizer's gather recomputes each bead's bonded pieces from the
+    // coordinates (bonded_at_bead) instead of reading the piece arrays, so its line-search kernel
+    // leaves the filament terms' forces out (force evaluations choose on the host). bnbr[j][bead], j = 0..3: the beads two before, one
+    // before, one after and two after it along its filament (the bead itself where there is
+    // none); bterm[0][bead]: the stretching term with the bead as bead 1, bterm[1][bead]: the
+    // bending term with the bead in the middle (their neighbours along the filament are the
+    // adjacent term indices; out of range where the term does not exist). blk_nb = the
+    // interaction blocks of the other force fields, which are dispatched first.
+    const int* bnbr;
+    const int* bterm;
+    int bonded_gather;
+    int blk_nb;
     // bead-sorted slot layout (linkers, motors, excluded volume, boundary)
     const int* pos;        // [S] slot id -> position
     const int* bstart;     // [n_beads + 1]
@@ -416,9 +428,60 @@ constexpr int NBOND = 5;              // bonded pieces per bead
 // every bead, so the loads below are unconditional (the results are selected, not branched on).
 constexpr long GCHUNK_PAD = GCHUNK_BYTES / 4;   // values (>= 3 * gchunk<R>() for R = float, double)
 
-template <typename R, int GB>
+// The bonded pieces of bead b (the bits of m, in piece order k = 0..4: 0 / 1 the stretching term
+// with the bead as bead 2 / 1, 2 / 3 / 4 the bending term with the bead as bead 3 / 2 / 1)
+// recomputed from the coordinates x and added to acc: the same term functions on the same inputs
+// as in term_ef, so the values the piece arrays would hold (rounded to R, accumulated in FP64).
+// Every load is issued up front and every term evaluated (absent pieces see in-range beads and
+// terms and are dropped by the select), so no load waits behind a branch.
+template <typename R>
+__device__ inline D3 bonded_at_bead(const DevSys& s, const double* __restrict__ x, int b, unsigned m, D3 acc) {
+    const long nb = s.n_beads;
+    const int ns = s.n[FF_STRETCH], nd = s.n[FF_BEND];
+    const int id[5] = {s.bnbr[b], s.bnbr[nb + b], b, s.bnbr[2 * nb + b], s.bnbr[3 * nb + b]};
+    const int t1 = s.bterm[b];        // stretching term (bead, next)
+    const int tc = s.bterm[nb + b];   // bending term (previous, bead, next)
+    D3 c[5];   // along the filament: two before .. two after
+#pragma unroll
+    for (int j = 0; j < 5; j++) c[j] = D3{x[3L * id[j]], x[3L * id[j] + 1], x[3L * id[j] + 2]};
+    double sk[2] = {0.0, 0.0}, sl[2] = {0.0, 0.0}, bk[3] = {0.0, 0.0, 0.0}, bt[3] = {0.0, 0.0, 0.0};
+    if (ns > 0)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int t = min(max(t1 - 1 + j, 0), ns - 1);
+            sk[j] = s.str_k[t];
+            sl[j] = s.str_eql[t];
+        }
+    if (nd > 0)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int t = min(max(tc - 1 + j, 0), nd - 1);
+            bk[j] = s.bend_k[t];
+            bt[j] = s.bend_eqt[t];
+        }
+    D3 o[5][3];
+    stretch_forces<R>(c[1], c[2], sk[0], sl[0], o[0]);       // k = 0: as bead 2 -> o[0][1]
+    stretch_forces<R>(c[2], c[3], sk[1], sl[1], o[1]);       // k = 1: as bead 1 -> o[1][0]
+    bend_forces<R>(c[0], c[1], c[2], bk[0], bt[0], o[2]);    // k = 2: as bead 3 -> o[2][2]
+    bend_forces<R>(c[1], c[2], c[3], bk[1], bt[1], o[3]);    // k = 3: as bead 2 -> o[3][1]
+    bend_forces<R>(c[2], c[3], c[4], bk[2], bt[2], o[4]);    // k = 4: as bead 1 -> o[4][0]
+    const D3 v[5] = {o[0][1], o[1][0], o[2][2], o[3][1], o[4][0]};
+#pragma unroll
+    for (int k = 0; k < NBOND; k++) {
+        const bool on = (m >> k) & 1;
+        acc.x = on ? acc.x + v[k].x : acc.x;
+        acc.y = on ? acc.y + v[k].y : acc.y;
+        acc.z = on ? acc.z + v[k].z : acc.z;
+    }
+    return acc;
+}
+
+// BG: the bonded pieces are recomputed at the coordinates xb (bonded_at_bead) instead of read
+// from the piece arrays (a separate instantiation: the recomputation needs twice the registers).
+template <typename R, int GB, bool BG = false>
 __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restrict__ slots, long lo, long hi,
-                                  bool all, D3 acc, double* lds_raw /* GCHUNK_BYTES */) {
+                                  bool all, D3 acc, double* lds_raw /* GCHUNK_BYTES */,
+                                  const double* __restrict__ xb = nullptr) {
     constexpr int CH = gchunk<R>(), UN = gunroll<R>();
     static_assert(3 * CH <= GCHUNK_PAD, "slot padding");
     R* lds = reinterpret_cast<R*>(lds_raw);
@@ -430,7 +493,7 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
     // bonded pieces in flight first
     const unsigned m = (mine && bmask) ? s.pmask[bc] & ((bmask & 1 ? 0x3u : 0u) | (bmask & 2 ? 0x1cu : 0u)) : 0u;
     R pv[NBOND][3] = {};
-    if (threadIdx.x < GB) {   // the summing waves (uniform branch)
+    if (!BG && threadIdx.x < GB) {   // the summing waves (uniform branch)
         const R* pc = static_cast<const R*>(s.piece);
 #pragma unroll
         for (int k = 0; k < NBOND; k++)
@@ -448,12 +511,16 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
         for (int j = 0; j < UN; j++)
             if (j * BLOCK < n) v[j] = slots[3L * c0 + threadIdx.x + j * BLOCK];
     }
+    if (BG) {
+        if (threadIdx.x < GB) acc = bonded_at_bead<R>(s, xb, bc, m, acc);   // the summing waves
+    } else {
 #pragma unroll
-    for (int k = 0; k < NBOND; k++) {
-        const bool on = (m >> k) & 1;
-        acc.x = on ? acc.x + (double)pv[k][0] : acc.x;
-        acc.y = on ? acc.y + (double)pv[k][1] : acc.y;
-        acc.z = on ? acc.z + (double)pv[k][2] : acc.z;
+        for (int k = 0; k < NBOND; k++) {
+            const bool on = (m >> k) & 1;
+            acc.x = on ? acc.x + (double)pv[k][0] : acc.x;
+            acc.y = on ? acc.y + (double)pv[k][1] : acc.y;
+            acc.z = on ? acc.z + (double)pv[k][2] : acc.z;
+        }
     }
     while (c0 < r1) {   // uniform over the block
         const int c1 = min(c0 + CH, r1);
@@ -537,15 +604,17 @@ __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* 
 }
 
 // force[b] = base[b] + the bonded pieces selected by bmask + b's slots with id in [lo, hi)
-// (base may be null: zero-fill). Grid: gather blocks (GB beads each).
-template <typename R, int GB>
+// (base may be null: zero-fill). The bonded pieces are recomputed at xb when BG, else read from
+// the piece arrays. Grid: gather blocks (GB beads each).
+template <typename R, int GB, bool BG>
 __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, const R* __restrict__ slots, long lo,
-                                                        long hi, const double* __restrict__ base, double* __restrict__ f) {
+                                                        long hi, const double* __restrict__ base, double* __restrict__ f,
+                                                        const double* __restrict__ xb) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     const int b = blockIdx.x * GB + threadIdx.x;
     const bool mine = threadIdx.x < GB && b < s.n_beads;
     D3 acc = (base && mine) ? D3{base[3 * b], base[3 * b + 1], base[3 * b + 2]} : D3{0.0, 0.0, 0.0};
-    acc = gather_block<R, GB>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L);
+    acc = gather_block<R, GB, BG>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L, xb);
     if (!mine) return;
     f[3 * b] = acc.x; f[3 * b + 1] = acc.y; f[3 * b + 2] = acc.z;
 }
@@ -905,8 +974,11 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     const double* __restrict__ xn = B.x[par ^ 1];
     const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
+    // with bonded_gather the filament terms contribute energies only (the gather adds their forces)
+    const bool bonded_e = s.bonded_gather && (ff == FF_STRETCH || ff == FF_BEND);
+    R* const tslots = bonded_e ? nullptr : static_cast<R*>(B.slots);
     if (phase == PH_MOVE) {
-        if (t < s.n[ff]) {
+        if (t < s.n[ff] && !bonded_e) {
             int fbad = 0;
             (void)term_ef<R>(s, ff, t, p1, static_cast<R*>(B.slots), -1, &fbad);
             if (fbad) atomicAdd(B.bad_count, 1);
@@ -918,7 +990,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     double v[3] = {0.0, 0.0, 0.0};
     if (t < s.n[ff]) {
         int fbad = 0;
-        v[0] = term_ef<R>(s, ff, t, p1, static_cast<R*>(B.slots), 0, &fbad);
+        v[0] = term_ef<R>(s, ff, t, p1, tslots, 0, &fbad);
         if (fbad) atomicAdd(B.bad_count, 1);
         if (!first) {   // the first launch of a search evaluates lambda0 alone (see ls_control)
             const Pos p2{B.xh, nullptr, nullptr, 0.0, 0.0, 0};
@@ -1055,7 +1127,7 @@ __device__ inline void ls_control(CgState& S, const double* acc, int& bad_count)
 // ---- gather F into the next force buffer; F.F, F.F_prev, max|F| partials ---------------
 // After a line-search launch the gather is speculative (k_cg_fin uses it when lambda0 is
 // accepted); after a move it is the step's. Grid: bead blocks only.
-template <typename R, int GB>
+template <typename R, int GB, bool BG>
 __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __restrict__ st, CgBufs B) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     // the state fields, loaded before the first branch (one round trip)
@@ -1074,7 +1146,8 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     // forcePrev = force after every step and reset (CGMethod.hpp:966-968, 1532-1535), and the
     // current force buffer is replaced only by a committed gather: F_prev is f[fpar]
     const double* __restrict__ fprev = B.f[fpar];
-    D3 acc = gather_block<R, GB>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
+    D3 acc = gather_block<R, GB, BG>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L,
+                                     B.x[par ^ 1]);
     constexpr int NSW = GB / 64;   // summing waves
     __shared__ double red[3 * NSW];
     if (threadIdx.x < GB) {   // whole waves

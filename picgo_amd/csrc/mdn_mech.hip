@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -73,6 +74,9 @@ struct mdn_ctx {
     DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
     DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
     DBuf<unsigned char> pmask;
+    DBuf<int> bnbr, bterm;   // filament neighbours and term indices per bead (bonded_at_bead)
+    int bonded_gather_mode = -1;   // -1: the defaults below, else both of them
+    int bg_eval = 1;                // force evaluations recompute the bonded pieces in the gather
     // parameter gradients / tangents: per cylinder the bending term it is the second cylinder of
     // (-1: none), per bending term its cylinder (host), per-pair excluded-volume energies, a vector
     DBuf<int> d_cyl_bend;
@@ -190,6 +194,14 @@ void layout(mdn_ctx* ctx) {
     s.slot0[0] = 0;
     for (int ff = 0; ff < NFF; ff++) s.slot0[ff + 1] = s.slot0[ff] + (long)SLOTS_PER_TERM[ff] * s.n[ff];
     s.blk[NFF + 1] = s.blk[NFF] + ceil_div(ctx->n_beads, BLOCK);
+    s.blk_nb = s.blk[4];   // excluded volume, linkers, motors, boundary
+    // Force evaluations have the gather recompute the filament terms' forces from the
+    // coordinates (it saves writing and re-reading the bonded pieces, 5 vectors per bead); the
+    // minimizer's kernels keep the pieces, whose gather measured faster there (the recomputation
+    // lengthens the CG gather, which the line-search kernel's savings do not repay). The same
+    // values either way; a context can fix both choices (bonded_gather_mode).
+    s.bonded_gather = ctx->bonded_gather_mode >= 0 ? ctx->bonded_gather_mode : 0;
+    ctx->bg_eval = ctx->bonded_gather_mode >= 0 ? ctx->bonded_gather_mode : 1;
 }
 
 int bead_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, BLOCK)); }
@@ -197,20 +209,31 @@ int gather_size(mdn_ctx* ctx) { return ctx->n_beads >= GB_LARGE_BEADS ? 128 : 64
 int gather_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, gather_size(ctx))); }
 // the gather kernels at the context's beads-per-block
 template <typename R>
-void launch_gather_plain(mdn_ctx* ctx, int bmask, const R* slots, long lo, long hi, const double* base, double* out) {
+void launch_gather_plain(mdn_ctx* ctx, int bmask, const R* slots, long lo, long hi, const double* base, double* out,
+                         const double* xb) {
     const dim3 g(gather_blocks(ctx)), b(BLOCK);
-    if (gather_size(ctx) == 128)
-        hipLaunchKernelGGL((k_gather_plain<R, 128>), g, b, 0, ctx->stream, ctx->sys, bmask, slots, lo, hi, base, out);
+    const DevSys& s = ctx->sys;
+    const bool l = gather_size(ctx) == 128;
+    if (xb && l)
+        hipLaunchKernelGGL((k_gather_plain<R, 128, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
+    else if (xb)
+        hipLaunchKernelGGL((k_gather_plain<R, 64, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
+    else if (l)
+        hipLaunchKernelGGL((k_gather_plain<R, 128, false>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
     else
-        hipLaunchKernelGGL((k_gather_plain<R, 64>), g, b, 0, ctx->stream, ctx->sys, bmask, slots, lo, hi, base, out);
+        hipLaunchKernelGGL((k_gather_plain<R, 64, false>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
 }
+// the interaction blocks whose forces go through slots or pieces (the filament terms' only
+// when their pieces are wanted, i.e. the gather does not recompute them)
+int force_blocks(const DevSys& s, bool pieces) { return pieces ? s.blk[NFF] : s.blk_nb; }
 template <typename R>
 void launch_cg_gather(mdn_ctx* ctx, const CgState* st, const CgBufs& B) {
     const dim3 g(gather_blocks(ctx)), b(BLOCK);
-    if (gather_size(ctx) == 128)
-        hipLaunchKernelGGL((k_cg_gather<R, 128>), g, b, 0, ctx->stream, ctx->sys, st, B);
-    else
-        hipLaunchKernelGGL((k_cg_gather<R, 64>), g, b, 0, ctx->stream, ctx->sys, st, B);
+    const bool l = gather_size(ctx) == 128, bg = ctx->sys.bonded_gather != 0;
+    if (bg && l) hipLaunchKernelGGL((k_cg_gather<R, 128, true>), g, b, 0, ctx->stream, ctx->sys, st, B);
+    else if (bg) hipLaunchKernelGGL((k_cg_gather<R, 64, true>), g, b, 0, ctx->stream, ctx->sys, st, B);
+    else if (l) hipLaunchKernelGGL((k_cg_gather<R, 128, false>), g, b, 0, ctx->stream, ctx->sys, st, B);
+    else hipLaunchKernelGGL((k_cg_gather<R, 64, false>), g, b, 0, ctx->stream, ctx->sys, st, B);
 }
 
 void refresh_ptrs(mdn_ctx* ctx) {
@@ -224,6 +247,7 @@ void refresh_ptrs(mdn_ctx* ctx) {
     s.ev_b = ctx->ev_b.p; s.ev_ks = ctx->ev_ks.p;
     s.bnd_b = ctx->bnd_b.p; s.bnd_e = ctx->bnd_e.p; s.bnd_krep = ctx->bnd_krep.p; s.bnd_slen = ctx->bnd_slen.p;
     s.piece = (void*)ctx->piece.p; s.pmask = ctx->pmask.p;
+    s.bnbr = ctx->bnbr.p; s.bterm = ctx->bterm.p;
     s.pos = ctx->pos.p; s.bstart = ctx->bstart.p; s.sorted_id = ctx->sorted_id.p;
     s.defer = ctx->defer.p; s.defer_n = ctx->defer_n.p;
     s.n_elem = (int)ctx->elems.size();
@@ -292,12 +316,12 @@ int set_coords_dev(mdn_ctx* ctx, const double* coord) {
 // Term forces + numerical fallback for non-finite excluded-volume pairs, into ctx->slots (in
 // the arithmetic type R), at device coordinates x.
 template <typename R>
-int eval_slots_t(mdn_ctx* ctx, const double* x) {
+int eval_slots_t(mdn_ctx* ctx, const double* x, bool pieces) {
     const DevSys& s = ctx->sys;
     R* slots = reinterpret_cast<R*>(ctx->slots.p);
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    if (s.blk[NFF] > 0)
-        hipLaunchKernelGGL(k_forces_terms<R>, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x, slots,
+    if (force_blocks(s, pieces) > 0)
+        hipLaunchKernelGGL(k_forces_terms<R>, dim3(force_blocks(s, pieces)), dim3(BLOCK), 0, ctx->stream, s, x, slots,
                            ctx->bad_count.p, ctx->bad_list.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -329,17 +353,20 @@ int eval_slots_t(mdn_ctx* ctx, const double* x) {
     }
     return MDN_OK;
 }
-int eval_slots(mdn_ctx* ctx, const double* x, int prec) {
-    return prec == MDN_PREC_MIXED ? eval_slots_t<float>(ctx, x) : eval_slots_t<double>(ctx, x);
+// pieces: also the filament terms' pieces (for a gather that reads them)
+int eval_slots(mdn_ctx* ctx, const double* x, int prec, bool pieces) {
+    return prec == MDN_PREC_MIXED ? eval_slots_t<float>(ctx, x, pieces) : eval_slots_t<double>(ctx, x, pieces);
 }
 
 // force = base + the bonded pieces selected by bmask (bit 0 stretching, bit 1 bending) + the
-// slots with id in [lo, hi)
-int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec) {
+// slots with id in [lo, hi). x: the coordinates the slots were evaluated at (the bonded pieces
+// are recomputed there for force evaluations, bg_eval), or null to read the piece arrays.
+int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec, const double* x) {
+    const double* xb = ctx->bg_eval ? x : nullptr;
     if (prec == MDN_PREC_MIXED)
-        launch_gather_plain<float>(ctx, bmask, (const float*)ctx->slots.p, lo, hi, base, out);
+        launch_gather_plain<float>(ctx, bmask, (const float*)ctx->slots.p, lo, hi, base, out, xb);
     else
-        launch_gather_plain<double>(ctx, bmask, (const double*)ctx->slots.p, lo, hi, base, out);
+        launch_gather_plain<double>(ctx, bmask, (const double*)ctx->slots.p, lo, hi, base, out, xb);
     HIPCHK(hipGetLastError());
     return MDN_OK;
 }
@@ -391,6 +418,10 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
     if (device < 0 || device >= n) return MDN_ERR_ARG;
     mdn_ctx* ctx = new mdn_ctx();
     ctx->device = device;
+    // MDN_BONDED_GATHER=0 / 1 at context creation fixes whether the gathers recompute the
+    // filament terms' forces (force evaluations and the minimizer alike); unset: force
+    // evaluations recompute them, the minimizer reads the pieces
+    if (const char* env = getenv("MDN_BONDED_GATHER")) ctx->bonded_gather_mode = atoi(env) != 0 ? 1 : 0;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return MDN_ERR_HIP;
@@ -443,6 +474,8 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     ctx->d_fil_ends.release();
     ctx->load_out.release();
     ctx->pmask.release();
+    ctx->bnbr.release();
+    ctx->bterm.release();
     ctx->d_cyl_bend.release();
     ctx->ev_U.release();
     ctx->grad_out.release();
@@ -624,6 +657,33 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
             ok = put(bend_b[3 * t + 2], 2) && put(bend_b[3 * t + 1], 3) && put(bend_b[3 * t], 4) && ok;
         if (!ok) return fail(ctx, MDN_ERR_ARG, "a bead appears more than once in the filament bead lists");
     }
+    // per bead, its neighbours along the filament and the indices of the stretching term it
+    // starts and the bending term it is the middle bead of (cylinders and bending terms are
+    // numbered filament after filament, as above)
+    const int nbd = ctx->n_beads;
+    std::vector<int> bnbr(4 * (size_t)nbd), bterm(2 * (size_t)nbd, 0);
+    for (int b = 0; b < nbd; b++)
+        for (int q = 0; q < 4; q++) bnbr[(size_t)q * nbd + b] = b;
+    for (int f = 0; f < d->n_filaments; f++) {
+        const int off = d->fil_bead_offset[f], nf = d->fil_bead_offset[f + 1] - off;
+        for (int j = 0; j < nf; j++) {
+            const int b = d->fil_beads[off + j];
+            const int step[4] = {-2, -1, 1, 2};
+            for (int q = 0; q < 4; q++)
+                if (j + step[q] >= 0 && j + step[q] < nf) bnbr[(size_t)q * nbd + b] = d->fil_beads[off + j + step[q]];
+            bterm[b] = off - f + j;
+            bterm[(size_t)nbd + b] = off - 2 * f + j - 1;
+        }
+    }
+    for (int t = 0; t < n_cyl; t++)   // the numbering the recomputation relies on
+        if (bterm[str_b[2 * t]] != t || bnbr[2 * (size_t)nbd + str_b[2 * t]] != str_b[2 * t + 1])
+            return fail(ctx, MDN_ERR_STATE, "stretching term %d: inconsistent filament tables", t);
+    for (int t = 0; t < n_bend; t++)
+        if (bterm[(size_t)nbd + bend_b[3 * t + 1]] != t || bnbr[(size_t)nbd + bend_b[3 * t + 1]] != bend_b[3 * t] ||
+            bnbr[2 * (size_t)nbd + bend_b[3 * t + 1]] != bend_b[3 * t + 2])
+            return fail(ctx, MDN_ERR_STATE, "bending term %d: inconsistent filament tables", t);
+    HIPCHK(upload(ctx, ctx->bnbr, bnbr));
+    HIPCHK(upload(ctx, ctx->bterm, bterm));
     HIPCHK(upload(ctx, ctx->str_b, str_b)); HIPCHK(upload(ctx, ctx->str_k, str_k)); HIPCHK(upload(ctx, ctx->str_eql, str_eql));
     HIPCHK(upload(ctx, ctx->bend_b, bend_b)); HIPCHK(upload(ctx, ctx->bend_k, bend_k)); HIPCHK(upload(ctx, ctx->bend_eqt, bend_eqt));
     if ((st = set_crosslinks(ctx, d->linkers, d->n_linkers, d->motors, d->n_motors))) return st;
@@ -959,7 +1019,7 @@ static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force,
         HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
         x = ctx->ext.p;
     }
-    if ((st = eval_slots(ctx, x, prec))) return st;
+    if ((st = eval_slots(ctx, x, prec, !ctx->bg_eval))) return st;
     long lo = 0, hi = ctx->sys.slot0[NFF];
     const double* base = nullptr;
     if (ff >= 0) {
@@ -969,7 +1029,7 @@ static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force,
         base = ctx->scratch_f.p;
     }
     const int bmask = ff < 0 ? 3 : ff == FF_STRETCH ? 1 : ff == FF_BEND ? 2 : 0;
-    if ((st = gather(ctx, bmask, lo, hi, base, ctx->f.p, prec))) return st;
+    if ((st = gather(ctx, bmask, lo, hi, base, ctx->f.p, prec, x))) return st;
     HIPCHK(hipMemcpyAsync(force, ctx->f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
@@ -1056,7 +1116,8 @@ int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int
         hipLaunchKernelGGL(k_tangent_terms, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
                            (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
     HIPCHK(hipGetLastError());
-    if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64))) return st;
+    // the tangent pieces are in the piece arrays
+    if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, nullptr))) return st;
     HIPCHK(hipMemcpyAsync(out, ctx->scratch_f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1170,8 +1231,8 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         if ((st = energy_report(ctx, ctx->x[0].p, prec))) return st;
         e_start = ctx->h_eout[6];
     }
-    if ((st = eval_slots(ctx, ctx->x[0].p, prec))) return st;
-    if ((st = gather(ctx, 3, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p, prec))) return st;
+    if ((st = eval_slots(ctx, ctx->x[0].p, prec, !ctx->bg_eval))) return st;
+    if ((st = gather(ctx, 3, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p, prec, ctx->x[0].p))) return st;
     // F.F and maxF on the device (fixed-order reduction), two doubles back
     const int nsb = std::min(VSTAT_BLOCKS, std::max(1, ceil_div(nv, BLOCK)));
     hipLaunchKernelGGL(k_vec_stats, dim3(nsb), dim3(BLOCK), 0, ctx->stream, (const double*)ctx->f.p, (long)nv,
@@ -1251,7 +1312,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         // (same positions, so every other slot is bit-identical), then run this step's gather
         // again and resume.
         n_forcen++;
-        if ((st = eval_slots(ctx, ctx->x[S.parity ^ 1].p, prec))) return st;
+        if ((st = eval_slots(ctx, ctx->x[S.parity ^ 1].p, prec, !ctx->sys.bonded_gather))) return st;
         HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
         S.status = 0;
         S.done = 0;
@@ -1526,10 +1587,11 @@ static int bench_forces_t(mdn_ctx* ctx, int32_t iters, double* ms_per_eval, doub
     const DevSys& s = ctx->sys;
     const double* x = ctx->x[ctx->cur].p;
     R* slots = reinterpret_cast<R*>(ctx->slots.p);
-    const int nb = std::max(1, s.blk[NFF]);
+    const bool pieces = !ctx->bg_eval;
+    const int nb = std::max(1, force_blocks(s, pieces));
     int st;
     // warm-up (and fallback check) once
-    if ((st = eval_slots_t<R>(ctx, x))) return st;
+    if ((st = eval_slots_t<R>(ctx, x, pieces))) return st;
     hipEvent_t e0, e1, k0, k1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&k0)); HIPCHK(hipEventCreate(&k1));
     // whole evaluation: terms + gather
@@ -1537,7 +1599,8 @@ static int bench_forces_t(mdn_ctx* ctx, int32_t iters, double* ms_per_eval, doub
     for (int i = 0; i < iters; i++) {
         hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
                            ctx->bad_count.p, ctx->bad_list.p);
-        launch_gather_plain<R>(ctx, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p);
+        launch_gather_plain<R>(ctx, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p,
+                               pieces ? (const double*)nullptr : x);
     }
     HIPCHK(hipEventRecord(e1, ctx->stream));
     // dominant kernel alone

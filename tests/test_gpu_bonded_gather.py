@@ -1,0 +1,88 @@
+"""GPU: the two ways the filament stretching and bending forces reach the per-bead sums.
+
+Either the terms (or line-search) kernel writes each bonded term's forces into bead-major piece
+arrays and the gather reads them, or the gather recomputes each bead's pieces from the
+coordinates (the same term functions on the same inputs) and the terms and line-search kernels
+leave those forces out. By default force evaluations recompute and the minimizer reads pieces;
+MDN_BONDED_GATHER=0 / 1 at context creation fixes one way for both. The two must agree bit for
+bit: forces and per-force-field forces in both precisions, whole minimization trajectories, the
+Hessian-vector product (whose tangent pieces always go through the piece arrays) and the batched
+minimizer (which always uses them). The recomputed FP64 bonded forces are also checked against
+the oracle bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi, mech, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(gpu, net, mode):
+    os.environ["MDN_BONDED_GATHER"] = str(mode)
+    try:
+        g = gpu()
+    finally:
+        del os.environ["MDN_BONDED_GATHER"]
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    return g
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_recomputed_bonded_forces_equal_pieces(gpu, cfg):
+    net = synth.Network.config(cfg)
+    a, b = _ctx(gpu, net, 0), _ctx(gpu, net, 1)
+    x = net.coord
+    for prec in (abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED):
+        fa, fb = a.compute_forces(x, precision=prec), b.compute_forces(x, precision=prec)
+        print(f"config {cfg} precision {prec}: max |dF| {float(np.abs(fa - fb).max()):.3e}", flush=True)
+        np.testing.assert_array_equal(fa, fb)
+    for ff in range(6):
+        np.testing.assert_array_equal(a.ff_compute_forces(ff, x), b.ff_compute_forces(ff, x))
+    o = of.OracleSystem(net.desc)
+    o.build_nl(x)
+    for ff in (abi.MDN_FF_STRETCH, abi.MDN_FF_BEND):
+        np.testing.assert_array_equal(b.ff_compute_forces(ff, x), o.ff_forces(ff, x))
+    v = np.random.default_rng(cfg).normal(size=x.size)
+
+    def hv(g):   # an unrelaxed network may hold a pair without an analytic tangent: same error
+        try:
+            return g.hvp(v, x)
+        except mech.MdnError as e:
+            return e.code
+    ha, hb = hv(a), hv(b)
+    print(f"config {cfg}: hvp", "error code" if np.isscalar(ha) else "vectors", flush=True)
+    np.testing.assert_array_equal(ha, hb)
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_recomputed_bonded_minimization_equals_pieces(gpu, prec):
+    net = synth.Network.config(1)
+    a, b = _ctx(gpu, net, 0), _ctx(gpu, net, 1)
+    p = abi.default_cg_params(precision=prec, energy_call_limit=200)
+    xa, fa, ra = a.minimize(p, net.coord)
+    xb, fb, rb = b.minimize(p, net.coord)
+    print("precision", prec, "iterations", ra.num_iter, rb.num_iter, "energy calls", ra.num_energy_call,
+          rb.num_energy_call, flush=True)
+    assert (ra.num_iter, ra.num_energy_call, ra.num_force_call, ra.num_safe_mode) == \
+        (rb.num_iter, rb.num_energy_call, rb.num_force_call, rb.num_safe_mode)
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_array_equal(fa, fb)
+    assert ra.energies_after.total == rb.energies_after.total
+
+
+def test_batched_with_recomputing_contexts(gpu):
+    nets = [synth.Network.config(3, index=i) for i in range(3)]
+    sa = [_ctx(gpu, n, 0) for n in nets]
+    sb = [_ctx(gpu, n, 1) for n in nets]
+    p = abi.default_cg_params(energy_call_limit=80)
+    xa, fa, ra = mech.minimize_batched(sa, p, [n.coord for n in nets])
+    xb, fb, rb = mech.minimize_batched(sb, p, [n.coord for n in nets])
+    for i in range(len(nets)):
+        assert (ra[i].num_iter, ra[i].num_energy_call) == (rb[i].num_iter, rb[i].num_energy_call)
+        np.testing.assert_array_equal(xa[i], xb[i])
+        np.testing.assert_array_equal(fa[i], fb[i])
