@@ -92,16 +92,20 @@ struct DevSys {
     // term with the bead as bead 3, 2, 1; pmask[bead] bit k marks the pieces the bead has
     void* piece;
     const unsigned char* pmask;
-    // bonded_gather: the minimizer's gather recomputes each bead's bonded pieces from the
-    // coordinates (bonded_at_bead) instead of reading the piece arrays, so its line-search kernel
-    // leaves the filament terms' forces out (force evaluations choose on the host). bnbr[j][bead], j = 0..3: the beads two before, one
-    // before, one after and two after it along its filament (the bead itself where there is
-    // none); bterm[0][bead]: the stretching term with the bead as bead 1, bterm[1][bead]: the
-    // bending term with the bead in the middle (their neighbours along the filament are the
-    // adjacent term indices; out of range where the term does not exist). blk_nb = the
-    // interaction blocks of the other force fields, which are dispatched first.
+    // Bonded sums: instead of the piece arrays, bead blocks appended to the terms (or
+    // line-search) kernel's grid recompute each bead's bonded pieces from the coordinates
+    // (bonded_at_bead) and store their sum in piece order, bsum[bead] (FP64, 3 values), which the
+    // gather then starts from; the filament terms' own blocks leave their forces out.
+    // bonded_gather selects this for the minimizer's kernels (force evaluations choose on the
+    // host). bnbr[j][bead], j = 0..3: the beads two before, one before, one after and two after
+    // it along its filament (the bead itself where there is none); bterm[0][bead]: the stretching
+    // term with the bead as bead 1, bterm[1][bead]: the bending term with the bead in the middle
+    // (their neighbours along the filament are the adjacent term indices; out of range where the
+    // term does not exist). blk_nb = the interaction blocks of the other force fields, which are
+    // dispatched first.
     const int* bnbr;
     const int* bterm;
+    double* bsum;
     int bonded_gather;
     int blk_nb;
     // bead-sorted slot layout (linkers, motors, excluded volume, boundary)
@@ -476,12 +480,26 @@ __device__ inline D3 bonded_at_bead(const DevSys& s, const double* __restrict__ 
     return acc;
 }
 
-// BG: the bonded pieces are recomputed at the coordinates xb (bonded_at_bead) instead of read
-// from the piece arrays (a separate instantiation: the recomputation needs twice the registers).
+// A bead block: bsum[b] = base[b] (or 0) + bead b's bonded pieces selected by bmask, in piece
+// order, recomputed at the coordinates x. Bead index b = k-th bead block * BLOCK + thread.
+template <typename R>
+__device__ inline void bead_bonded_sum(const DevSys& s, const double* __restrict__ x, int k, int bmask,
+                                       const double* __restrict__ base) {
+    const int b = k * BLOCK + threadIdx.x;
+    if (b >= s.n_beads) return;
+    const unsigned m = bmask ? s.pmask[b] & ((bmask & 1 ? 0x3u : 0u) | (bmask & 2 ? 0x1cu : 0u)) : 0u;
+    D3 acc = base ? D3{base[3L * b], base[3L * b + 1], base[3L * b + 2]} : D3{0.0, 0.0, 0.0};
+    acc = bonded_at_bead<R>(s, x, b, m, acc);
+    s.bsum[3L * b] = acc.x;
+    s.bsum[3L * b + 1] = acc.y;
+    s.bsum[3L * b + 2] = acc.z;
+}
+
+// BG: each bead starts from its bonded sum bsum (base and the bonded pieces, written by the
+// preceding kernel's bead blocks) instead of acc plus the piece arrays.
 template <typename R, int GB, bool BG = false>
 __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restrict__ slots, long lo, long hi,
-                                  bool all, D3 acc, double* lds_raw /* GCHUNK_BYTES */,
-                                  const double* __restrict__ xb = nullptr) {
+                                  bool all, D3 acc, double* lds_raw /* GCHUNK_BYTES */) {
     constexpr int CH = gchunk<R>(), UN = gunroll<R>();
     static_assert(3 * CH <= GCHUNK_PAD, "slot padding");
     R* lds = reinterpret_cast<R*>(lds_raw);
@@ -491,8 +509,12 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
     const bool mine = threadIdx.x < GB && b < s.n_beads;
     const int bc = mine ? b : b0;   // an in-range bead for the unconditional loads
     // bonded pieces in flight first
-    const unsigned m = (mine && bmask) ? s.pmask[bc] & ((bmask & 1 ? 0x3u : 0u) | (bmask & 2 ? 0x1cu : 0u)) : 0u;
+    const unsigned m = (!BG && mine && bmask) ? s.pmask[bc] & ((bmask & 1 ? 0x3u : 0u) | (bmask & 2 ? 0x1cu : 0u)) : 0u;
     R pv[NBOND][3] = {};
+    if (BG && threadIdx.x < GB) {   // the summing waves (uniform branch)
+        const double* q = s.bsum + 3L * bc;
+        acc = mine ? D3{q[0], q[1], q[2]} : D3{0.0, 0.0, 0.0};
+    }
     if (!BG && threadIdx.x < GB) {   // the summing waves (uniform branch)
         const R* pc = static_cast<const R*>(s.piece);
 #pragma unroll
@@ -511,9 +533,7 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
         for (int j = 0; j < UN; j++)
             if (j * BLOCK < n) v[j] = slots[3L * c0 + threadIdx.x + j * BLOCK];
     }
-    if (BG) {
-        if (threadIdx.x < GB) acc = bonded_at_bead<R>(s, xb, bc, m, acc);   // the summing waves
-    } else {
+    if (!BG) {
 #pragma unroll
         for (int k = 0; k < NBOND; k++) {
             const bool on = (m >> k) & 1;
@@ -589,7 +609,13 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
 // Grid: the interaction blocks.
 template <typename R>
 __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* __restrict__ x,
-                                                        R* __restrict__ slots, int* bad_count, int* bad_list) {
+                                                        R* __restrict__ slots, int* bad_count, int* bad_list,
+                                                        int bead0, int bmask, const double* __restrict__ base) {
+    // blocks from bead0 on (when bead0 >= 0) are bead blocks: the bonded sums (bead_bonded_sum)
+    if (bead0 >= 0 && (int)blockIdx.x >= bead0) {
+        bead_bonded_sum<R>(s, x, blockIdx.x - bead0, bmask, base);
+        return;
+    }
     const int ff = ff_of_block(s, blockIdx.x);
     if (ff >= NFF) return;
     const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
@@ -604,17 +630,17 @@ __global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* 
 }
 
 // force[b] = base[b] + the bonded pieces selected by bmask + b's slots with id in [lo, hi)
-// (base may be null: zero-fill). The bonded pieces are recomputed at xb when BG, else read from
-// the piece arrays. Grid: gather blocks (GB beads each).
+// (base may be null: zero-fill). BG: the first two terms come as the bonded sums of the terms
+// kernel's bead blocks (which took base and bmask); else from base and the piece arrays. Grid:
+// gather blocks (GB beads each).
 template <typename R, int GB, bool BG>
 __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, const R* __restrict__ slots, long lo,
-                                                        long hi, const double* __restrict__ base, double* __restrict__ f,
-                                                        const double* __restrict__ xb) {
+                                                        long hi, const double* __restrict__ base, double* __restrict__ f) {
     __shared__ double L[GCHUNK_BYTES / sizeof(double)];
     const int b = blockIdx.x * GB + threadIdx.x;
     const bool mine = threadIdx.x < GB && b < s.n_beads;
-    D3 acc = (base && mine) ? D3{base[3 * b], base[3 * b + 1], base[3 * b + 2]} : D3{0.0, 0.0, 0.0};
-    acc = gather_block<R, GB, BG>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L, xb);
+    D3 acc = (!BG && base && mine) ? D3{base[3 * b], base[3 * b + 1], base[3 * b + 2]} : D3{0.0, 0.0, 0.0};
+    acc = gather_block<R, GB, BG>(s, bmask, slots, lo, hi, lo == 0 && hi == s.slot0[NFF], acc, L);
     if (!mine) return;
     f[3 * b] = acc.x; f[3 * b + 1] = acc.y; f[3 * b + 2] = acc.z;
 }
@@ -968,13 +994,18 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     const int done = st->done, phase = st->phase, par = st->parity, ls_active = st->ls_active;
     const int ecur_valid = st->ecur_valid;
     keep_loaded(done); keep_loaded(phase); keep_loaded(par); keep_loaded(ls_active); keep_loaded(ecur_valid);
-    if (done || (int)blockIdx.x >= s.blk[NFF]) return;   // (a 1-block grid when there are no interactions)
+    if (done) return;
     if (phase != PH_LS && phase != PH_MOVE) return;
+    if ((int)blockIdx.x >= s.blk[NFF]) {   // bead blocks (bonded_gather): the bonded sums at x_next
+        if (s.bonded_gather) bead_bonded_sum<R>(s, B.x[par ^ 1], blockIdx.x - s.blk[NFF], 3, nullptr);
+        return;   // (or the one block of a grid without interactions)
+    }
     const int ff = ff_of_block(s, blockIdx.x);
     const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     const double* __restrict__ xn = B.x[par ^ 1];
     const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
-    // with bonded_gather the filament terms contribute energies only (the gather adds their forces)
+    // with bonded_gather the filament terms contribute energies only (the bead blocks' bonded
+    // sums carry their forces)
     const bool bonded_e = s.bonded_gather && (ff == FF_STRETCH || ff == FF_BEND);
     R* const tslots = bonded_e ? nullptr : static_cast<R*>(B.slots);
     if (phase == PH_MOVE) {
@@ -1146,8 +1177,7 @@ __global__ void __launch_bounds__(BLOCK) k_cg_gather(DevSys s, const CgState* __
     // forcePrev = force after every step and reset (CGMethod.hpp:966-968, 1532-1535), and the
     // current force buffer is replaced only by a committed gather: F_prev is f[fpar]
     const double* __restrict__ fprev = B.f[fpar];
-    D3 acc = gather_block<R, GB, BG>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L,
-                                     B.x[par ^ 1]);
+    D3 acc = gather_block<R, GB, BG>(s, 3, static_cast<const R*>(B.slots), 0, 0, true, D3{0.0, 0.0, 0.0}, L);
     constexpr int NSW = GB / 64;   // summing waves
     __shared__ double red[3 * NSW];
     if (threadIdx.x < GB) {   // whole waves

@@ -75,6 +75,7 @@ struct mdn_ctx {
     DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
     DBuf<unsigned char> pmask;
     DBuf<int> bnbr, bterm;   // filament neighbours and term indices per bead (bonded_at_bead)
+    DBuf<double> bsum;       // per-bead bonded sums, 3 * n_beads
     int bonded_gather_mode = -1;   // -1: the defaults below, else both of them
     int bg_eval = 1;                // force evaluations recompute the bonded pieces in the gather
     // parameter gradients / tangents: per cylinder the bending term it is the second cylinder of
@@ -195,12 +196,11 @@ void layout(mdn_ctx* ctx) {
     for (int ff = 0; ff < NFF; ff++) s.slot0[ff + 1] = s.slot0[ff] + (long)SLOTS_PER_TERM[ff] * s.n[ff];
     s.blk[NFF + 1] = s.blk[NFF] + ceil_div(ctx->n_beads, BLOCK);
     s.blk_nb = s.blk[4];   // excluded volume, linkers, motors, boundary
-    // Force evaluations have the gather recompute the filament terms' forces from the
-    // coordinates (it saves writing and re-reading the bonded pieces, 5 vectors per bead); the
-    // minimizer's kernels keep the pieces, whose gather measured faster there (the recomputation
-    // lengthens the CG gather, which the line-search kernel's savings do not repay). The same
-    // values either way; a context can fix both choices (bonded_gather_mode).
-    s.bonded_gather = ctx->bonded_gather_mode >= 0 ? ctx->bonded_gather_mode : 0;
+    // Force evaluations use the bonded sums (bead blocks in the terms kernel's grid recompute the
+    // filament terms' forces per bead, which saves writing and re-reading five piece vectors per
+    // bead); the minimizer chooses per precision (cg_bonded_sums). The same values either way; a
+    // context can fix every choice (bonded_gather_mode).
+    s.bonded_gather = 0;   // set per precision on the minimizer's copy (cg_sys)
     ctx->bg_eval = ctx->bonded_gather_mode >= 0 ? ctx->bonded_gather_mode : 1;
 }
 
@@ -210,30 +210,31 @@ int gather_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, gath
 // the gather kernels at the context's beads-per-block
 template <typename R>
 void launch_gather_plain(mdn_ctx* ctx, int bmask, const R* slots, long lo, long hi, const double* base, double* out,
-                         const double* xb) {
+                         bool bg) {
     const dim3 g(gather_blocks(ctx)), b(BLOCK);
     const DevSys& s = ctx->sys;
     const bool l = gather_size(ctx) == 128;
-    if (xb && l)
-        hipLaunchKernelGGL((k_gather_plain<R, 128, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
-    else if (xb)
-        hipLaunchKernelGGL((k_gather_plain<R, 64, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
+    if (bg && l)
+        hipLaunchKernelGGL((k_gather_plain<R, 128, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out);
+    else if (bg)
+        hipLaunchKernelGGL((k_gather_plain<R, 64, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out);
     else if (l)
-        hipLaunchKernelGGL((k_gather_plain<R, 128, false>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
+        hipLaunchKernelGGL((k_gather_plain<R, 128, false>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out);
     else
-        hipLaunchKernelGGL((k_gather_plain<R, 64, false>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out, xb);
+        hipLaunchKernelGGL((k_gather_plain<R, 64, false>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out);
 }
-// the interaction blocks whose forces go through slots or pieces (the filament terms' only
-// when their pieces are wanted, i.e. the gather does not recompute them)
-int force_blocks(const DevSys& s, bool pieces) { return pieces ? s.blk[NFF] : s.blk_nb; }
+// The terms kernel's grid: every interaction block (pieces: the filament terms write their
+// pieces), or the other force fields' blocks followed by the bead blocks of the bonded sums.
+int force_blocks(const DevSys& s, bool pieces) { return pieces ? s.blk[NFF] : s.blk_nb + ceil_div(s.n_beads, BLOCK); }
+int bead_block0(const DevSys& s, bool pieces) { return pieces ? -1 : s.blk_nb; }
 template <typename R>
-void launch_cg_gather(mdn_ctx* ctx, const CgState* st, const CgBufs& B) {
+void launch_cg_gather(mdn_ctx* ctx, const DevSys& s, const CgState* st, const CgBufs& B) {
     const dim3 g(gather_blocks(ctx)), b(BLOCK);
-    const bool l = gather_size(ctx) == 128, bg = ctx->sys.bonded_gather != 0;
-    if (bg && l) hipLaunchKernelGGL((k_cg_gather<R, 128, true>), g, b, 0, ctx->stream, ctx->sys, st, B);
-    else if (bg) hipLaunchKernelGGL((k_cg_gather<R, 64, true>), g, b, 0, ctx->stream, ctx->sys, st, B);
-    else if (l) hipLaunchKernelGGL((k_cg_gather<R, 128, false>), g, b, 0, ctx->stream, ctx->sys, st, B);
-    else hipLaunchKernelGGL((k_cg_gather<R, 64, false>), g, b, 0, ctx->stream, ctx->sys, st, B);
+    const bool l = gather_size(ctx) == 128, bg = s.bonded_gather != 0;
+    if (bg && l) hipLaunchKernelGGL((k_cg_gather<R, 128, true>), g, b, 0, ctx->stream, s, st, B);
+    else if (bg) hipLaunchKernelGGL((k_cg_gather<R, 64, true>), g, b, 0, ctx->stream, s, st, B);
+    else if (l) hipLaunchKernelGGL((k_cg_gather<R, 128, false>), g, b, 0, ctx->stream, s, st, B);
+    else hipLaunchKernelGGL((k_cg_gather<R, 64, false>), g, b, 0, ctx->stream, s, st, B);
 }
 
 void refresh_ptrs(mdn_ctx* ctx) {
@@ -247,7 +248,7 @@ void refresh_ptrs(mdn_ctx* ctx) {
     s.ev_b = ctx->ev_b.p; s.ev_ks = ctx->ev_ks.p;
     s.bnd_b = ctx->bnd_b.p; s.bnd_e = ctx->bnd_e.p; s.bnd_krep = ctx->bnd_krep.p; s.bnd_slen = ctx->bnd_slen.p;
     s.piece = (void*)ctx->piece.p; s.pmask = ctx->pmask.p;
-    s.bnbr = ctx->bnbr.p; s.bterm = ctx->bterm.p;
+    s.bnbr = ctx->bnbr.p; s.bterm = ctx->bterm.p; s.bsum = ctx->bsum.p;
     s.pos = ctx->pos.p; s.bstart = ctx->bstart.p; s.sorted_id = ctx->sorted_id.p;
     s.defer = ctx->defer.p; s.defer_n = ctx->defer_n.p;
     s.n_elem = (int)ctx->elems.size();
@@ -316,13 +317,13 @@ int set_coords_dev(mdn_ctx* ctx, const double* coord) {
 // Term forces + numerical fallback for non-finite excluded-volume pairs, into ctx->slots (in
 // the arithmetic type R), at device coordinates x.
 template <typename R>
-int eval_slots_t(mdn_ctx* ctx, const double* x, bool pieces) {
+int eval_slots_t(mdn_ctx* ctx, const double* x, bool pieces, int bmask, const double* base) {
     const DevSys& s = ctx->sys;
     R* slots = reinterpret_cast<R*>(ctx->slots.p);
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
     if (force_blocks(s, pieces) > 0)
         hipLaunchKernelGGL(k_forces_terms<R>, dim3(force_blocks(s, pieces)), dim3(BLOCK), 0, ctx->stream, s, x, slots,
-                           ctx->bad_count.p, ctx->bad_list.p);
+                           ctx->bad_count.p, ctx->bad_list.p, bead_block0(s, pieces), bmask, base);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -353,20 +354,21 @@ int eval_slots_t(mdn_ctx* ctx, const double* x, bool pieces) {
     }
     return MDN_OK;
 }
-// pieces: also the filament terms' pieces (for a gather that reads them)
-int eval_slots(mdn_ctx* ctx, const double* x, int prec, bool pieces) {
-    return prec == MDN_PREC_MIXED ? eval_slots_t<float>(ctx, x, pieces) : eval_slots_t<double>(ctx, x, pieces);
+// pieces: the filament terms' pieces for a gather that reads them; else the bonded sums of
+// the pieces selected by bmask on top of base (null: zero) for a gather that starts from them
+int eval_slots(mdn_ctx* ctx, const double* x, int prec, bool pieces, int bmask = 3, const double* base = nullptr) {
+    return prec == MDN_PREC_MIXED ? eval_slots_t<float>(ctx, x, pieces, bmask, base)
+                                  : eval_slots_t<double>(ctx, x, pieces, bmask, base);
 }
 
 // force = base + the bonded pieces selected by bmask (bit 0 stretching, bit 1 bending) + the
-// slots with id in [lo, hi). x: the coordinates the slots were evaluated at (the bonded pieces
-// are recomputed there for force evaluations, bg_eval), or null to read the piece arrays.
-int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec, const double* x) {
-    const double* xb = ctx->bg_eval ? x : nullptr;
+// slots with id in [lo, hi). bg: the first two terms come as the bonded sums the terms kernel's
+// bead blocks wrote (eval_slots without pieces, with the same bmask and base).
+int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec, bool bg) {
     if (prec == MDN_PREC_MIXED)
-        launch_gather_plain<float>(ctx, bmask, (const float*)ctx->slots.p, lo, hi, base, out, xb);
+        launch_gather_plain<float>(ctx, bmask, (const float*)ctx->slots.p, lo, hi, base, out, bg);
     else
-        launch_gather_plain<double>(ctx, bmask, (const double*)ctx->slots.p, lo, hi, base, out, xb);
+        launch_gather_plain<double>(ctx, bmask, (const double*)ctx->slots.p, lo, hi, base, out, bg);
     HIPCHK(hipGetLastError());
     return MDN_OK;
 }
@@ -418,9 +420,9 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
     if (device < 0 || device >= n) return MDN_ERR_ARG;
     mdn_ctx* ctx = new mdn_ctx();
     ctx->device = device;
-    // MDN_BONDED_GATHER=0 / 1 at context creation fixes whether the gathers recompute the
-    // filament terms' forces (force evaluations and the minimizer alike); unset: force
-    // evaluations recompute them, the minimizer reads the pieces
+    // MDN_BONDED_GATHER=0 / 1 at context creation fixes whether the filament terms' forces go
+    // through per-bead bonded sums (1) or the piece arrays (0), for force evaluations and the
+    // minimizer alike; unset: the measured defaults (layout, cg_bonded_sums)
     if (const char* env = getenv("MDN_BONDED_GATHER")) ctx->bonded_gather_mode = atoi(env) != 0 ? 1 : 0;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
@@ -476,6 +478,7 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     ctx->pmask.release();
     ctx->bnbr.release();
     ctx->bterm.release();
+    ctx->bsum.release();
     ctx->d_cyl_bend.release();
     ctx->ev_U.release();
     ctx->grad_out.release();
@@ -684,6 +687,7 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
             return fail(ctx, MDN_ERR_STATE, "bending term %d: inconsistent filament tables", t);
     HIPCHK(upload(ctx, ctx->bnbr, bnbr));
     HIPCHK(upload(ctx, ctx->bterm, bterm));
+    HIPCHK(ctx->bsum.ensure(3 * (size_t)nbd));
     HIPCHK(upload(ctx, ctx->str_b, str_b)); HIPCHK(upload(ctx, ctx->str_k, str_k)); HIPCHK(upload(ctx, ctx->str_eql, str_eql));
     HIPCHK(upload(ctx, ctx->bend_b, bend_b)); HIPCHK(upload(ctx, ctx->bend_k, bend_k)); HIPCHK(upload(ctx, ctx->bend_eqt, bend_eqt));
     if ((st = set_crosslinks(ctx, d->linkers, d->n_linkers, d->motors, d->n_motors))) return st;
@@ -1019,7 +1023,6 @@ static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force,
         HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
         x = ctx->ext.p;
     }
-    if ((st = eval_slots(ctx, x, prec, !ctx->bg_eval))) return st;
     long lo = 0, hi = ctx->sys.slot0[NFF];
     const double* base = nullptr;
     if (ff >= 0) {
@@ -1029,7 +1032,9 @@ static int forces_impl(mdn_ctx* ctx, int ff, const double* coord, double* force,
         base = ctx->scratch_f.p;
     }
     const int bmask = ff < 0 ? 3 : ff == FF_STRETCH ? 1 : ff == FF_BEND ? 2 : 0;
-    if ((st = gather(ctx, bmask, lo, hi, base, ctx->f.p, prec, x))) return st;
+    const bool bg = ctx->bg_eval != 0;
+    if ((st = eval_slots(ctx, x, prec, !bg, bmask, base))) return st;
+    if ((st = gather(ctx, bmask, lo, hi, base, ctx->f.p, prec, bg))) return st;
     HIPCHK(hipMemcpyAsync(force, ctx->f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
@@ -1117,7 +1122,7 @@ int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int
                            (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
     HIPCHK(hipGetLastError());
     // the tangent pieces are in the piece arrays
-    if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, nullptr))) return st;
+    if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, false))) return st;
     HIPCHK(hipMemcpyAsync(out, ctx->scratch_f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1173,15 +1178,32 @@ static CgBufs cg_bufs(mdn_ctx* ctx) {
     return B;
 }
 
+// The minimizer's kernels use the bonded sums (bonded_gather) in FP64 and, in the mixed path,
+// below CG_MIXED_BONDED_SUMS_BEADS beads: measured per CG iteration against the piece arrays,
+// config 1 FP64 32.3 -> 30.6 us and mixed 30.2 -> 29.3 us, 420,000 beads FP64 74.9 -> 73.1 us but
+// mixed 65.6 -> 67.9 us (there the FP32 pieces are cheap to move and the bead blocks are not).
+static const int CG_MIXED_BONDED_SUMS_BEADS = 100000;
+static bool cg_bonded_sums(const mdn_ctx* ctx, int prec) {
+    if (ctx->bonded_gather_mode >= 0) return ctx->bonded_gather_mode != 0;
+    return prec == MDN_PREC_FP64 || ctx->n_beads < CG_MIXED_BONDED_SUMS_BEADS;
+}
+// the system record the minimizer's kernels of precision prec take
+static DevSys cg_sys(const mdn_ctx* ctx, int prec) {
+    DevSys s = ctx->sys;
+    s.bonded_gather = cg_bonded_sums(ctx, prec) ? 1 : 0;
+    return s;
+}
+
 extern "C++" {
 template <typename R>
-static void launch_slot(mdn_ctx* ctx, const CgBufs& B) {
-    const DevSys& s = ctx->sys;
+static void launch_slot(mdn_ctx* ctx, const DevSys& s, const CgBufs& B) {
     const int nls = ls_blocks(ctx), gbk = gather_blocks(ctx);
     hipLaunchKernelGGL(k_cg_pos, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_ls<R>, dim3(std::max(1, s.blk[NFF])), dim3(BLOCK), 0, ctx->stream, s,
+    // the interaction blocks, then (bonded_gather) the bead blocks of the bonded sums
+    const int lsg = s.blk[NFF] + (s.bonded_gather ? ceil_div(s.n_beads, BLOCK) : 0);
+    hipLaunchKernelGGL(k_cg_ls<R>, dim3(std::max(1, lsg)), dim3(BLOCK), 0, ctx->stream, s,
                        (const CgState*)ctx->cg.p, B);
-    launch_cg_gather<R>(ctx, (const CgState*)ctx->cg.p, B);
+    launch_cg_gather<R>(ctx, s, (const CgState*)ctx->cg.p, B);
     hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls, gbk);
 }
 }  // extern "C++"
@@ -1197,10 +1219,11 @@ static int cg_graph(mdn_ctx* ctx, int prec) {
     ctx->graph_exec[prec] = nullptr;
     ctx->graph[prec] = nullptr;
     CgBufs B = cg_bufs(ctx);
+    const DevSys s = cg_sys(ctx, prec);
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < CG_BATCH; i++) {
-        if (prec == MDN_PREC_MIXED) launch_slot<float>(ctx, B);
-        else launch_slot<double>(ctx, B);
+        if (prec == MDN_PREC_MIXED) launch_slot<float>(ctx, s, B);
+        else launch_slot<double>(ctx, s, B);
     }
     HIPCHK(hipStreamEndCapture(ctx->stream, &ctx->graph[prec]));
     HIPCHK(hipGraphInstantiate(&ctx->graph_exec[prec], ctx->graph[prec], nullptr, nullptr, 0));
@@ -1232,7 +1255,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         e_start = ctx->h_eout[6];
     }
     if ((st = eval_slots(ctx, ctx->x[0].p, prec, !ctx->bg_eval))) return st;
-    if ((st = gather(ctx, 3, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p, prec, ctx->x[0].p))) return st;
+    if ((st = gather(ctx, 3, 0, ctx->sys.slot0[NFF], nullptr, ctx->f.p, prec, ctx->bg_eval != 0))) return st;
     // F.F and maxF on the device (fixed-order reduction), two doubles back
     const int nsb = std::min(VSTAT_BLOCKS, std::max(1, ceil_div(nv, BLOCK)));
     hipLaunchKernelGGL(k_vec_stats, dim3(nsb), dim3(BLOCK), 0, ctx->stream, (const double*)ctx->f.p, (long)nv,
@@ -1312,7 +1335,8 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         // (same positions, so every other slot is bit-identical), then run this step's gather
         // again and resume.
         n_forcen++;
-        if ((st = eval_slots(ctx, ctx->x[S.parity ^ 1].p, prec, !ctx->sys.bonded_gather))) return st;
+        const DevSys cs = cg_sys(ctx, prec);
+        if ((st = eval_slots(ctx, ctx->x[S.parity ^ 1].p, prec, !cs.bonded_gather))) return st;
         HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
         S.status = 0;
         S.done = 0;
@@ -1320,9 +1344,9 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
         const CgBufs B = cg_bufs(ctx);
         const int gbk = gather_blocks(ctx);
-        if (prec == MDN_PREC_MIXED) launch_cg_gather<float>(ctx, (const CgState*)ctx->cg.p, B);
-        else launch_cg_gather<double>(ctx, (const CgState*)ctx->cg.p, B);
-        hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, ctx->sys, ctx->cg.p, B, ls_blocks(ctx), gbk);
+        if (prec == MDN_PREC_MIXED) launch_cg_gather<float>(ctx, cs, (const CgState*)ctx->cg.p, B);
+        else launch_cg_gather<double>(ctx, cs, (const CgState*)ctx->cg.p, B);
+        hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, cs, ctx->cg.p, B, ls_blocks(ctx), gbk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
     }
@@ -1591,23 +1615,22 @@ static int bench_forces_t(mdn_ctx* ctx, int32_t iters, double* ms_per_eval, doub
     const int nb = std::max(1, force_blocks(s, pieces));
     int st;
     // warm-up (and fallback check) once
-    if ((st = eval_slots_t<R>(ctx, x, pieces))) return st;
+    if ((st = eval_slots_t<R>(ctx, x, pieces, 3, nullptr))) return st;
     hipEvent_t e0, e1, k0, k1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&k0)); HIPCHK(hipEventCreate(&k1));
     // whole evaluation: terms + gather
     HIPCHK(hipEventRecord(e0, ctx->stream));
     for (int i = 0; i < iters; i++) {
         hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
-                           ctx->bad_count.p, ctx->bad_list.p);
-        launch_gather_plain<R>(ctx, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p,
-                               pieces ? (const double*)nullptr : x);
+                           ctx->bad_count.p, ctx->bad_list.p, bead_block0(s, pieces), 3, (const double*)nullptr);
+        launch_gather_plain<R>(ctx, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p, !pieces);
     }
     HIPCHK(hipEventRecord(e1, ctx->stream));
     // dominant kernel alone
     HIPCHK(hipEventRecord(k0, ctx->stream));
     for (int i = 0; i < iters; i++)
         hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
-                           ctx->bad_count.p, ctx->bad_list.p);
+                           ctx->bad_count.p, ctx->bad_list.p, bead_block0(s, pieces), 3, (const double*)nullptr);
     HIPCHK(hipEventRecord(k1, ctx->stream));
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventSynchronize(k1));

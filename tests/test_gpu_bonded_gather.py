@@ -1,14 +1,15 @@
 """GPU: the two ways the filament stretching and bending forces reach the per-bead sums.
 
 Either the terms (or line-search) kernel writes each bonded term's forces into bead-major piece
-arrays and the gather reads them, or the gather recomputes each bead's pieces from the
-coordinates (the same term functions on the same inputs) and the terms and line-search kernels
-leave those forces out. By default force evaluations recompute and the minimizer reads pieces;
-MDN_BONDED_GATHER=0 / 1 at context creation fixes one way for both. The two must agree bit for
-bit: forces and per-force-field forces in both precisions, whole minimization trajectories, the
-Hessian-vector product (whose tangent pieces always go through the piece arrays) and the batched
-minimizer (which always uses them). The recomputed FP64 bonded forces are also checked against
-the oracle bit for bit.
+arrays and the gather reads them, or bead blocks of that kernel recompute each bead's pieces from
+the coordinates (the same term functions on the same inputs) and store their sum, which the
+gather starts from, while the filament terms' own blocks leave their forces out. By default
+force evaluations and the FP64 minimizer use the sums; MDN_BONDED_GATHER=0 / 1 at context
+creation fixes one way for everything. The two must agree bit for bit: forces and per-force-field
+forces in both precisions (per-force-field evaluations carry the caller's base forces through the
+sums), whole minimization trajectories, the Hessian-vector product (whose tangent pieces always
+go through the piece arrays) and the batched minimizer (which always uses them). The recomputed
+FP64 bonded forces are also checked against the oracle bit for bit.
 """
 import os
 
@@ -41,8 +42,12 @@ def test_recomputed_bonded_forces_equal_pieces(gpu, cfg):
         fa, fb = a.compute_forces(x, precision=prec), b.compute_forces(x, precision=prec)
         print(f"config {cfg} precision {prec}: max |dF| {float(np.abs(fa - fb).max()):.3e}", flush=True)
         np.testing.assert_array_equal(fa, fb)
+    base = np.random.default_rng(10 + cfg).normal(scale=50.0, size=x.size)
     for ff in range(6):
         np.testing.assert_array_equal(a.ff_compute_forces(ff, x), b.ff_compute_forces(ff, x))
+        # accumulating into given forces, as ForceField::computeForces does
+        np.testing.assert_array_equal(a.ff_compute_forces(ff, x, force=base.copy()),
+                                      b.ff_compute_forces(ff, x, force=base.copy()))
     o = of.OracleSystem(net.desc)
     o.build_nl(x)
     for ff in (abi.MDN_FF_STRETCH, abi.MDN_FF_BEND):
