@@ -108,6 +108,15 @@ struct DevSys {
     double* bsum;
     int bonded_gather;
     int blk_nb;
+    // bead blocks (ceil(n_beads / BLOCK)) and the columns of a line-search slot's partials:
+    // [0, blk[NFF]) interaction blocks, [blk[NFF], + nbb) the line search's bead blocks,
+    // [blk[NFF] + nbb, + nbb) the bead kernel's (k_cg_pos)
+    int nbb;
+    int lsp;
+    // with bonded_gather, where the line search takes the filament terms' energies: 0 their own
+    // blocks (energies only), 1 the bead blocks (each bead the terms it starts), whose grid then
+    // leaves the filament terms' blocks out
+    int bead_energy;
     // bead-sorted slot layout (linkers, motors, excluded volume, boundary)
     const int* pos;        // [S] slot id -> position
     const int* bstart;     // [n_beads + 1]
@@ -976,9 +985,66 @@ __global__ void __launch_bounds__(BLOCK) k_cg_pos(DevSys s, const CgState* __res
     const int op2[2] = {0, 1};
     block_reduce_n<2>(v, op2, lds);
     if (threadIdx.x == 0) {
-        const long np = s.blk[NFF + 1], k = s.blk[NFF] + blockIdx.x;   // value-major
+        const long np = s.lsp, k = s.blk[NFF] + s.nbb + blockIdx.x;   // value-major
         B.partial[k] = v[0];
         B.partial[np + k] = v[1];
+    }
+}
+
+// The energies of the filament terms bead b starts (the stretching term (b, next) and the
+// bending term centred on b), so that over all beads every term counts once; both evaluated and
+// selected (absent terms see in-range indices and are dropped). The same term functions as
+// term_ef.
+template <typename R>
+__device__ inline double bead_bonded_energy(const DevSys& s, const double* __restrict__ x, int b) {
+    const long nb = s.n_beads;
+    const int ns = s.n[FF_STRETCH], nd = s.n[FF_BEND];
+    const unsigned m = s.pmask[b];
+    const int ip = s.bnbr[nb + b], in = s.bnbr[2 * nb + b];
+    const int t1 = min(max(s.bterm[b], 0), max(ns - 1, 0)), tc = min(max(s.bterm[nb + b], 0), max(nd - 1, 0));
+    auto X = [x](int i) { return D3{x[3L * i], x[3L * i + 1], x[3L * i + 2]}; };
+    const D3 cp = X(ip), c0 = X(b), cn = X(in);
+    const double sk = ns > 0 ? s.str_k[t1] : 0.0, sl = ns > 0 ? s.str_eql[t1] : 0.0;
+    const double bk = nd > 0 ? s.bend_k[tc] : 0.0, bt = nd > 0 ? s.bend_eqt[tc] : 0.0;
+    const double es = (double)stretch_energy<R>(c0, cn, sk, sl);
+    const double eb = (double)bend_energy<R>(cp, c0, cn, bk, bt);
+    double e = 0.0;
+    e = (m & 0x02u) ? e + es : e;   // bead 1 of a stretching term
+    e = (m & 0x08u) ? e + eb : e;   // bead 2 of a bending term
+    return e;
+}
+
+// A bead block of the line search (bonded_gather): the bonded sums at x_next (the speculative or
+// the move's forces) and, in phase PH_LS with energies, the block partials of the filament terms' energies at
+// the trial points the interaction blocks evaluate (x_next; x + (lambda / 2) d after a
+// rejection; x when the current energy is not known), in the columns after the interaction
+// blocks'.
+template <typename R>
+__device__ inline void bead_ls(const DevSys& s, const CgBufs& B, int j, bool move, bool first, bool need_e0, int par,
+                               bool energies, double* lds) {
+    const int b = j * BLOCK + threadIdx.x;
+    const bool on = b < s.n_beads;
+    const double* __restrict__ xn = B.x[par ^ 1];
+    if (on) {
+        const D3 acc = bonded_at_bead<R>(s, xn, b, s.pmask[b], D3{0.0, 0.0, 0.0});
+        s.bsum[3L * b] = acc.x;
+        s.bsum[3L * b + 1] = acc.y;
+        s.bsum[3L * b + 2] = acc.z;
+    }
+    if (move || !energies) return;   // block-uniform
+    double v[3] = {0.0, 0.0, 0.0};
+    if (on) {
+        v[0] = bead_bonded_energy<R>(s, xn, b);
+        if (!first) v[1] = bead_bonded_energy<R>(s, B.xh, b);
+        if (need_e0) v[2] = bead_bonded_energy<R>(s, B.x[par], b);
+    }
+    const int op3[3] = {0, 0, 0};
+    block_reduce_n<3>(v, op3, lds);
+    if (threadIdx.x == 0) {
+        const long np = s.lsp, k = s.blk[NFF] + j;   // value-major
+        B.partial[k] = v[0];
+        B.partial[np + k] = v[1];
+        B.partial[2 * np + k] = v[2];
     }
 }
 
@@ -996,28 +1062,31 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     keep_loaded(done); keep_loaded(phase); keep_loaded(par); keep_loaded(ls_active); keep_loaded(ecur_valid);
     if (done) return;
     if (phase != PH_LS && phase != PH_MOVE) return;
-    if ((int)blockIdx.x >= s.blk[NFF]) {   // bead blocks (bonded_gather): the bonded sums at x_next
-        if (s.bonded_gather) bead_bonded_sum<R>(s, B.x[par ^ 1], blockIdx.x - s.blk[NFF], 3, nullptr);
-        return;   // (or the one block of a grid without interactions)
+    const bool first = ls_active == 0;
+    const bool need_e0 = first && !ecur_valid;
+    // with bonded_gather bead blocks follow the interaction blocks and carry the filament terms'
+    // forces (and, with bead_energy, their energies: then the filament terms' blocks are left out)
+    const int nint = (s.bonded_gather && s.bead_energy) ? s.blk_nb : s.blk[NFF];
+    if ((int)blockIdx.x >= nint) {   // (or the one block of a grid without interactions)
+        if (s.bonded_gather)
+            bead_ls<R>(s, B, blockIdx.x - nint, phase == PH_MOVE, first, need_e0, par, s.bead_energy != 0, lds);
+        return;
     }
     const int ff = ff_of_block(s, blockIdx.x);
     const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     const double* __restrict__ xn = B.x[par ^ 1];
     const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
-    // with bonded_gather the filament terms contribute energies only (the bead blocks' bonded
-    // sums carry their forces)
+    // with bonded_gather (and not bead_energy) the filament terms' blocks take energies only
     const bool bonded_e = s.bonded_gather && (ff == FF_STRETCH || ff == FF_BEND);
     R* const tslots = bonded_e ? nullptr : static_cast<R*>(B.slots);
     if (phase == PH_MOVE) {
         if (t < s.n[ff] && !bonded_e) {
             int fbad = 0;
-            (void)term_ef<R>(s, ff, t, p1, static_cast<R*>(B.slots), -1, &fbad);
+            (void)term_ef<R>(s, ff, t, p1, tslots, -1, &fbad);
             if (fbad) atomicAdd(B.bad_count, 1);
         }
         return;
     }
-    const bool first = ls_active == 0;
-    const bool need_e0 = first && !ecur_valid;
     double v[3] = {0.0, 0.0, 0.0};
     if (t < s.n[ff]) {
         int fbad = 0;
@@ -1035,7 +1104,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     const int op3[3] = {0, 0, 0};
     block_reduce_n<3>(v, op3, lds);
     if (threadIdx.x == 0) {
-        const long np = s.blk[NFF + 1], k = blockIdx.x;   // value-major
+        const long np = s.lsp, k = blockIdx.x;   // value-major
         B.partial[k] = v[0];
         B.partial[np + k] = v[1];
         B.partial[2 * np + k] = v[2];
@@ -1268,13 +1337,14 @@ __device__ inline void gather_control(CgState& S, const double* acc, int bad_cou
 }
 
 // One block after every slot: the line-search decision (phase PH_LS: the k_cg_ls and k_cg_pos
-// partials in a fixed order, the deferred Simpson energies, ls_control) and, when the step is accepted at the
+// partials in a fixed order (the columns of DevSys::lsp), the deferred Simpson energies,
+// ls_control) and, when the step is accepted at the
 // speculative trial or a move just ran (phase PH_MOVE), the post-step control on the gather
 // partials (gather_control). The kernel is a chain of latencies, so every independent global
 // read (the state, the deferred-pair and bad-pair counts, both sets of partials) is issued at
 // the start, the eleven sums and extrema are reduced in one pass, and thread 0 runs the scalar
 // control on a register copy of the state and writes it back.
-__global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int nls, int ngb) {
+__global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ st, CgBufs B, int ngb) {
     constexpr int NW = sizeof(CgState) / sizeof(int);
     static_assert(NW <= FIN, "the state is staged one word per thread");
     __shared__ CgState Sl;
@@ -1285,30 +1355,35 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
         sh_ndef = __hip_atomic_load(s.defer_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sh_bad = *B.bad_count;
     }
-    const int T = s.blk[NFF];
+    const int T = s.blk[NFF], np = s.lsp;
+    // energy columns: the interaction blocks in the line search's grid, then (bonded_gather) its
+    // bead blocks at T; the bead kernel's columns at T + nbb
+    const bool be = s.bonded_gather && s.bead_energy;
+    const int ne0 = be ? s.blk_nb : T, neb = be ? s.nbb : 0;
+    const int ne = ne0 + neb, p0c = T + s.nbb;
     // E(lambda), E(lambda/2), -, -, E(x), -, d.F, max|d| (then the recovery step) | F.F, F.F_prev, max|F|
     double acc[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int base = 0; base < nls; base += 8 * FIN) {
-        double p0[8], p1[8], p2[8];
+    // both column ranges' loads in flight before any sum (one round trip when they fit)
+    const int nc = ne > s.nbb ? ne : s.nbb;
+    for (int base = 0; base < nc; base += 8 * FIN) {
+        double p0[8], p1[8], p2[8], q0[8], q1[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int bk = base + threadIdx.x + j * FIN;
-            p0[j] = bk < nls ? B.partial[bk] : 0.0;
-            p1[j] = bk < nls ? B.partial[(long)nls + bk] : 0.0;
-            p2[j] = bk < T ? B.partial[2L * nls + bk] : 0.0;
+            const int c = base + threadIdx.x + j * FIN;
+            const long col = c < ne0 ? c : T + (c - ne0);
+            p0[j] = c < ne ? B.partial[col] : 0.0;
+            p1[j] = c < ne ? B.partial[np + col] : 0.0;
+            p2[j] = c < ne ? B.partial[2L * np + col] : 0.0;
+            q0[j] = c < s.nbb ? B.partial[p0c + c] : 0.0;
+            q1[j] = c < s.nbb ? B.partial[np + p0c + c] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int bk = base + threadIdx.x + j * FIN;
-            if (bk >= nls) continue;
-            if (bk < T) {
-                acc[0] += p0[j];
-                acc[1] += p1[j];
-                acc[4] += p2[j];
-            } else {
-                acc[6] += p0[j];
-                acc[7] = rmax(acc[7], p1[j]);
-            }
+            acc[0] += p0[j];
+            acc[1] += p1[j];
+            acc[4] += p2[j];
+            acc[6] += q0[j];
+            acc[7] = rmax(acc[7], q1[j]);
         }
     }
     for (int base = 0; base < ngb; base += 4 * FIN) {

@@ -195,6 +195,8 @@ void layout(mdn_ctx* ctx) {
     s.slot0[0] = 0;
     for (int ff = 0; ff < NFF; ff++) s.slot0[ff + 1] = s.slot0[ff] + (long)SLOTS_PER_TERM[ff] * s.n[ff];
     s.blk[NFF + 1] = s.blk[NFF] + ceil_div(ctx->n_beads, BLOCK);
+    s.nbb = std::max(1, ceil_div(ctx->n_beads, BLOCK));
+    s.lsp = s.blk[NFF] + 2 * s.nbb;
     s.blk_nb = s.blk[4];   // excluded volume, linkers, motors, boundary
     // Force evaluations use the bonded sums (bead blocks in the terms kernel's grid recompute the
     // filament terms' forces per bead, which saves writing and re-reading five piece vectors per
@@ -263,8 +265,9 @@ int ensure_eval_buffers(mdn_ctx* ctx) {
     const DevSys& s = ctx->sys;
     // padded by one gather chunk: the gather's chunk loads are unconditional
     HIPCHK(ctx->slots.ensure(3 * (size_t)std::max(1L, s.slot0[NFF]) + GCHUNK_PAD));
-    // line-search partials (interaction blocks, then bead blocks), then the CG gather's
-    const long nls = (long)s.blk[NFF] + bead_blocks(ctx);
+    // line-search partials (interaction blocks, then the line search's and the bead kernel's bead
+    // blocks; this also covers the evaluation partials), then the CG gather's
+    const long nls = (long)s.lsp;
     HIPCHK(ctx->partial.ensure((size_t)(nls + gather_blocks(ctx) + 1) * PSTRIDE));
     HIPCHK(ctx->pos.ensure((size_t)std::max(1L, s.slot0[NFF])));
     HIPCHK(ctx->sorted_id.ensure((size_t)std::max(1L, s.slot0[NFF])));
@@ -1165,7 +1168,7 @@ int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n
 // CGMethod::minimize on the device
 // ----------------------------------------------------------------------------------------
 // Line-search partial blocks: the interaction blocks (k_cg_ls), then the bead blocks (k_cg_pos).
-static int ls_blocks(mdn_ctx* ctx) { return ctx->sys.blk[NFF] + bead_blocks(ctx); }
+static int ls_blocks(mdn_ctx* ctx) { return ctx->sys.lsp; }
 
 static CgBufs cg_bufs(mdn_ctx* ctx) {
     CgBufs B;
@@ -1178,33 +1181,37 @@ static CgBufs cg_bufs(mdn_ctx* ctx) {
     return B;
 }
 
-// The minimizer's kernels use the bonded sums (bonded_gather) in FP64 and, in the mixed path,
-// below CG_MIXED_BONDED_SUMS_BEADS beads: measured per CG iteration against the piece arrays,
-// config 1 FP64 32.3 -> 30.6 us and mixed 30.2 -> 29.3 us, 420,000 beads FP64 74.9 -> 73.1 us but
-// mixed 65.6 -> 67.9 us (there the FP32 pieces are cheap to move and the bead blocks are not).
-static const int CG_MIXED_BONDED_SUMS_BEADS = 100000;
+// The minimizer's kernels use the bonded sums (bonded_gather); below CG_BEAD_ENERGY_BEADS beads
+// the line search keeps the filament terms' own blocks for their energies, from there on the bead
+// blocks take the energies too and those blocks leave the grid (bead_energy). Measured per CG
+// iteration (same process, piece arrays -> this choice): 42,000 beads FP64 32.2 -> 31.2-32.3 us,
+// mixed 30.5 -> 29.6 us (bead-block energies there: 32.9 and 30.8 us); 420,000 beads FP64 75.3 ->
+// 68.4 us, mixed 65.5 -> 63.0 us (own energy blocks there: 73.1 and 67.9 us).
+static const int CG_BEAD_ENERGY_BEADS = 100000;
 static bool cg_bonded_sums(const mdn_ctx* ctx, int prec) {
-    if (ctx->bonded_gather_mode >= 0) return ctx->bonded_gather_mode != 0;
-    return prec == MDN_PREC_FP64 || ctx->n_beads < CG_MIXED_BONDED_SUMS_BEADS;
+    (void)prec;
+    return ctx->bonded_gather_mode >= 0 ? ctx->bonded_gather_mode != 0 : true;
 }
 // the system record the minimizer's kernels of precision prec take
 static DevSys cg_sys(const mdn_ctx* ctx, int prec) {
     DevSys s = ctx->sys;
     s.bonded_gather = cg_bonded_sums(ctx, prec) ? 1 : 0;
+    s.bead_energy = ctx->n_beads >= CG_BEAD_ENERGY_BEADS ? 1 : 0;
     return s;
 }
 
 extern "C++" {
 template <typename R>
 static void launch_slot(mdn_ctx* ctx, const DevSys& s, const CgBufs& B) {
-    const int nls = ls_blocks(ctx), gbk = gather_blocks(ctx);
+    const int gbk = gather_blocks(ctx);
     hipLaunchKernelGGL(k_cg_pos, dim3(bead_blocks(ctx)), dim3(BLOCK), 0, ctx->stream, s, (const CgState*)ctx->cg.p, B);
-    // the interaction blocks, then (bonded_gather) the bead blocks of the bonded sums
-    const int lsg = s.blk[NFF] + (s.bonded_gather ? ceil_div(s.n_beads, BLOCK) : 0);
+    // all interaction blocks, then (bonded_gather) the bead blocks; with bead_energy the filament
+    // terms' blocks are left out
+    const int lsg = s.bonded_gather ? (s.bead_energy ? s.blk_nb : s.blk[NFF]) + s.nbb : s.blk[NFF];
     hipLaunchKernelGGL(k_cg_ls<R>, dim3(std::max(1, lsg)), dim3(BLOCK), 0, ctx->stream, s,
                        (const CgState*)ctx->cg.p, B);
     launch_cg_gather<R>(ctx, s, (const CgState*)ctx->cg.p, B);
-    hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, nls, gbk);
+    hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, s, ctx->cg.p, B, gbk);
 }
 }  // extern "C++"
 
@@ -1346,7 +1353,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         const int gbk = gather_blocks(ctx);
         if (prec == MDN_PREC_MIXED) launch_cg_gather<float>(ctx, cs, (const CgState*)ctx->cg.p, B);
         else launch_cg_gather<double>(ctx, cs, (const CgState*)ctx->cg.p, B);
-        hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, cs, ctx->cg.p, B, ls_blocks(ctx), gbk);
+        hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, cs, ctx->cg.p, B, gbk);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
     }

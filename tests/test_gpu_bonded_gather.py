@@ -4,8 +4,9 @@ Either the terms (or line-search) kernel writes each bonded term's forces into b
 arrays and the gather reads them, or bead blocks of that kernel recompute each bead's pieces from
 the coordinates (the same term functions on the same inputs) and store their sum, which the
 gather starts from, while the filament terms' own blocks leave their forces out. By default
-force evaluations and the FP64 minimizer use the sums; MDN_BONDED_GATHER=0 / 1 at context
-creation fixes one way for everything. The two must agree bit for bit: forces and per-force-field
+force evaluations and the minimizer use the sums (from 100,000 beads the line search's bead
+blocks take the filament terms' energies as well); MDN_BONDED_GATHER=0 / 1 at context creation
+fixes one way for everything. The two must agree bit for bit: forces and per-force-field
 forces in both precisions (per-force-field evaluations carry the caller's base forces through the
 sums), whole minimization trajectories, the Hessian-vector product (whose tangent pieces always
 go through the piece arrays) and the batched minimizer (which always uses them). The recomputed
@@ -23,11 +24,13 @@ pytestmark = pytest.mark.gpu
 
 
 def _ctx(gpu, net, mode):
-    os.environ["MDN_BONDED_GATHER"] = str(mode)
+    """A context with MDN_BONDED_GATHER=mode at creation (None: the defaults)."""
+    if mode is not None:
+        os.environ["MDN_BONDED_GATHER"] = str(mode)
     try:
         g = gpu()
     finally:
-        del os.environ["MDN_BONDED_GATHER"]
+        os.environ.pop("MDN_BONDED_GATHER", None)
     g.upload(net.desc, net.coord)
     g.reset_neighbor_lists(net.coord)
     return g
@@ -78,6 +81,24 @@ def test_recomputed_bonded_minimization_equals_pieces(gpu, prec):
     np.testing.assert_array_equal(xa, xb)
     np.testing.assert_array_equal(fa, fb)
     assert ra.energies_after.total == rb.energies_after.total
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_large_network_minimization_equals_pieces(gpu, prec):
+    """From 100,000 beads the line search's bead blocks also take the filament terms' energies
+    (their own blocks leave the grid): the 420,000-bead network's trajectory with the defaults
+    must be the piece-array trajectory, bit for bit (the line-search energies are summed in
+    another order, which moves no decision here)."""
+    net = synth.Network.config(4)
+    a, b = _ctx(gpu, net, 0), _ctx(gpu, net, None)
+    p = abi.default_cg_params(precision=prec, energy_call_limit=40)
+    xa, fa, ra = a.minimize(p, net.coord)
+    xb, fb, rb = b.minimize(p, net.coord)
+    print("precision", prec, "iterations", ra.num_iter, rb.num_iter, flush=True)
+    assert (ra.num_iter, ra.num_energy_call, ra.num_force_call) == (rb.num_iter, rb.num_energy_call, rb.num_force_call)
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(a.compute_forces(net.coord, precision=prec), b.compute_forces(net.coord, precision=prec))
 
 
 def test_batched_with_recomputing_contexts(gpu):
