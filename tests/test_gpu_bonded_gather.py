@@ -17,8 +17,11 @@ import os
 import numpy as np
 import pytest
 
+import ctypes as C
+
 from oracle import oracle_ffi as of
 from picgo_amd import abi, mech, synth
+from tests.hand_systems import HandSystem
 
 pytestmark = pytest.mark.gpu
 
@@ -112,3 +115,68 @@ def test_batched_with_recomputing_contexts(gpu):
         assert (ra[i].num_iter, ra[i].num_energy_call) == (rb[i].num_iter, rb[i].num_energy_call)
         np.testing.assert_array_equal(xa[i], xb[i])
         np.testing.assert_array_equal(fa[i], fb[i])
+
+
+def _relabelled(desc, coord, seed):
+    """The same system with its beads renumbered by a random permutation: bead b becomes perm[b]
+    (coordinates moved accordingly, filament bead lists mapped), so a filament's beads are no
+    longer consecutive in the bead array. Cylinders keep their numbering."""
+    n = desc.n_beads
+    perm = np.random.default_rng(seed).permutation(n).astype(np.int32)
+    m = desc.fil_bead_offset[desc.n_filaments]
+    fb = np.array([perm[desc.fil_beads[i]] for i in range(m)], dtype=np.int32)
+    x = np.empty_like(coord)
+    x.reshape(-1, 3)[perm] = coord.reshape(-1, 3)
+    d = abi.SystemDesc.from_buffer_copy(desc)
+    d.fil_beads = fb.ctypes.data_as(C.POINTER(C.c_int32))
+    d._keep = fb
+    return d, x, perm
+
+
+def _short_filaments():
+    """Filaments of 2, 3, 4 and 5 beads (1 to 4 cylinders), bent, far from each other and from
+    the boundary."""
+    P = np.array
+    fils = []
+    for j, nb in enumerate((2, 3, 4, 5, 2, 3)):
+        base = P([400.0 + 350.0 * j, 600.0, 700.0])
+        fils.append([base + P([100.0 * k, 15.0 * k * k, 5.0 * k]) for k in range(nb)])
+    return HandSystem(fils, [])
+
+
+@pytest.mark.parametrize("case", ["network", "short"])
+def test_relabelled_beads(gpu, case):
+    """Bonded sums use per-bead neighbour tables built from the filament bead lists: with the beads
+    renumbered, every force and energy must be the original one at the renumbered bead, bit for
+    bit, in both layouts and precisions (and the stretching and bending forces the oracle's)."""
+    if case == "network":
+        net = synth.Network.config(0)
+        desc, coord = net.desc, net.coord
+    else:
+        hs = _short_filaments()
+        desc, coord = hs.desc, hs.coord
+    d2, x2, perm = _relabelled(desc, coord, 5)
+    for mode in (0, 1):
+        a, b = _ctx_desc(gpu, desc, coord, mode), _ctx_desc(gpu, d2, x2, mode)
+        for prec in (abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED):
+            fa = a.compute_forces(coord, precision=prec).reshape(-1, 3)
+            fb = b.compute_forces(x2, precision=prec).reshape(-1, 3)
+            np.testing.assert_array_equal(fb[perm], fa)
+            ea, eb = a.compute_individual_energies(coord, precision=prec), b.compute_individual_energies(x2, precision=prec)
+            assert list(ea.ff) == list(eb.ff)
+        o = of.OracleSystem(d2)
+        o.build_nl(x2)
+        for ff in (abi.MDN_FF_STRETCH, abi.MDN_FF_BEND):
+            np.testing.assert_array_equal(b.ff_compute_forces(ff, x2), o.ff_forces(ff, x2))
+        print(f"{case} mode {mode}: forces and energies identical after renumbering", flush=True)
+
+
+def _ctx_desc(gpu, desc, coord, mode):
+    os.environ["MDN_BONDED_GATHER"] = str(mode)
+    try:
+        g = gpu()
+    finally:
+        os.environ.pop("MDN_BONDED_GATHER", None)
+    g.upload(desc, coord)
+    g.reset_neighbor_lists(coord)
+    return g
