@@ -16,14 +16,16 @@
 //   interaction, bead of the interaction), which is the order the reference adds into `force`.
 //   Interaction kernels write through pos[] (slot id -> bead-sorted position). The filament
 //   stretching and bending forces, which come first in that order, go to bead-major pieces
-//   piece[5][n_beads] instead (coalesced writes: consecutive cylinders have consecutive beads);
-//   the gather adds a bead's pieces, then streams its contiguous slot range, so it reproduces
-//   the reference's per-bead summation order exactly and needs no atomics.
+//   piece[5][n_beads] instead (coalesced writes: consecutive cylinders have consecutive beads),
+//   or are recomputed per bead by bead blocks and summed in the same order into bsum[n_beads]
+//   (the bonded sums, DevSys::bonded_gather); the gather starts from a bead's pieces or bonded
+//   sum, then streams its contiguous slot range, so it reproduces the reference's per-bead
+//   summation order exactly and needs no atomics.
 //
-// Grid layout of the interaction kernels: blocks [blk[ff], blk[ff+1]) hold force field ff's
-// interactions (each range padded to whole blocks, so a block belongs to one force field), then
-// blocks [blk[6], blk[7]) hold the beads. The line-search kernel repeats the interaction blocks
-// once per trial step length.
+// Grid layout of the interaction kernels: blocks [blk[k], blk[k+1]) hold force field dff[k]'s
+// interactions (each range padded to whole blocks, so a block belongs to one force field; the
+// longest-running force fields first, the filament terms last), then, where the bonded sums are
+// used, bead blocks.
 //
 // Reductions are deterministic: block tree reduction, block partials in fixed slots, then a
 // one-block finalize kernel reduces the partials with all its threads in a fixed order and runs
@@ -66,8 +68,8 @@ struct DevSys {
     int n_beads;
     int n[NFF];                 // interactions per force field
     // interaction blocks in dispatch order: range k = [blk[k], blk[k + 1]) holds force field
-    // dff[k]; blk[NFF] = all interaction blocks, blk[NFF + 1] = + bead blocks. fblk[ff] = first
-    // block of force field ff.
+    // dff[k]; blk[NFF] = all interaction blocks, blk[NFF + 1] = + one block per BLOCK beads.
+    // fblk[ff] = first block of force field ff.
     int blk[NFF + 2];
     int dff[NFF];
     int fblk[NFF];

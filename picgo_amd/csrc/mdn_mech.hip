@@ -77,7 +77,7 @@ struct mdn_ctx {
     DBuf<int> bnbr, bterm;   // filament neighbours and term indices per bead (bonded_at_bead)
     DBuf<double> bsum;       // per-bead bonded sums, 3 * n_beads
     int bonded_gather_mode = -1;   // -1: the defaults below, else both of them
-    int bg_eval = 1;                // force evaluations recompute the bonded pieces in the gather
+    int bg_eval = 1;                // force evaluations use the bonded sums (bead blocks)
     // parameter gradients / tangents: per cylinder the bending term it is the second cylinder of
     // (-1: none), per bending term its cylinder (host), per-pair excluded-volume energies, a vector
     DBuf<int> d_cyl_bend;
