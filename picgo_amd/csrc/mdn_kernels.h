@@ -617,9 +617,11 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
 
 // Slots of every interaction at x (ForceFieldManager::computeForces before the gather).
 // Excluded-volume pairs needing the numerical fallback are listed in bad_list.
-// Grid: the interaction blocks.
+// Grid: the interaction blocks. The mixed instantiation is held to 4 waves per SIMD (126 VGPRs,
+// no spills; measured 1-2 % faster per evaluation at 42,000 and 420,000 beads), the FP64 one
+// keeps 3 (at 4 it spills and was slower at 42,000 beads).
 template <typename R>
-__global__ void __launch_bounds__(BLOCK) k_forces_terms(DevSys s, const double* __restrict__ x,
+__global__ void __launch_bounds__(BLOCK, sizeof(R) == 4 ? 4 : 1) k_forces_terms(DevSys s, const double* __restrict__ x,
                                                         R* __restrict__ slots, int* bad_count, int* bad_list,
                                                         int bead0, int bmask, const double* __restrict__ base) {
     // blocks from bead0 on (when bead0 >= 0) are bead blocks: the bonded sums (bead_bonded_sum)
