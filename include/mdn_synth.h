@@ -50,17 +50,18 @@ typedef struct mdn_synth_params {
 
 typedef struct mdn_synth mdn_synth;
 
-/* Fill `p` with the preset of benchmark configuration `config` (0..4); for config 3
- * `index` selects one of the 256 batched networks (seed 3 + index). `dense` != 0 selects the
- * 100 um^-3 density variant of config 3 (the default 0.1 um^-3 is as written). */
+/* Fill `p` with preset `config` (0: 50 x 10 filaments in a 2 um cube; 1: 2,000 x 20 in a 5 um
+ * cube; 2: as 1 with per-cylinder parameters; 3: one network of the batch of 256; 4: 20,000 x 20
+ * in a 10 um sphere); for the batch `index` selects one of the 256 networks (seed 3 + index) and
+ * `dense` != 0 selects the 100 um^-3 density variant (the default 0.1 um^-3 is as written). */
 int mdn_synth_preset(int32_t config, int32_t index, int32_t dense, mdn_synth_params* p);
 
 int mdn_synth_create(const mdn_synth_params* p, mdn_synth** out);
 void mdn_synth_destroy(mdn_synth* s);
 const mdn_system_desc* mdn_synth_desc(const mdn_synth* s);
 const double* mdn_synth_coord(const mdn_synth* s);
-/* Add N(0, sigma) noise to every coordinate (config 4's per-minimization perturbation),
- * drawn from the generator's stream. */
+/* Add N(0, sigma) noise to every coordinate (the 420,000-bead network's per-minimization
+ * perturbation), drawn from the generator's stream. */
 int mdn_synth_perturb(mdn_synth* s, double* coord, double sigma);
 
 #ifdef __cplusplus

@@ -28,4 +28,4 @@ def test_adapter_runs_minimization():
     pairs, e0, e1, iters, mx, conv, nsf, loaded = out.stdout.split()
     assert int(pairs) > 0 and int(conv) == 1
     assert float(e1) < float(e0) and float(mx) < 1.0 and int(iters) > 10
-    assert int(nsf) == 20 and int(loaded) > 0   # config 0: 20 linkers; filament ends near the walls
+    assert int(nsf) == 20 and int(loaded) > 0   # the 550-bead network: 20 linkers; filament ends near the walls

@@ -1,6 +1,6 @@
 """GPU: parameter gradients dE/dtheta (mdn_param_grad) and Hessian-vector products (mdn_hvp) on
-the config-2 network (config-1 geometry, lognormal per-cylinder stretching k, rest length and
-bending k), near the relaxed state of an FP64 minimization (the unrelaxed network has overlapping
+the 42,000-bead network with per-cylinder parameters (lognormal per-cylinder stretching k, rest
+length and bending k), near the relaxed state of an FP64 minimization (the unrelaxed network has overlapping
 cylinders with forces up to 1e13 pN, where finite differences say nothing).
 
 Central finite differences with h = 1e-4 relative to the perturbed quantity (1e-4 x the mean rest

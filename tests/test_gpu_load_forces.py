@@ -1,6 +1,6 @@
 """GPU: boundary load forces on the filament ends (mdn_boundary_load_forces) against the oracle
-(pinned on closed forms in tests/test_load_forces.py): the hand geometries, the config-0 network
-in its cube after a short minimization, and a config-0-sized network with one- to three-cylinder
+(pinned on closed forms in tests/test_load_forces.py): the hand geometries, the 550-bead network
+in its cube after a short minimization, and a network of that size with one- to three-cylinder
 filaments in a sphere. The device's exp differs from the host's by at most an ulp, so the bar is
 1e-13 relative to the largest load."""
 import numpy as np

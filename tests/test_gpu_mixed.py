@@ -8,7 +8,7 @@ Bars (the FP64 path itself is pinned to the oracle by test_gpu_parity.py):
   unrelaxed networks (forces up to 1e14 pN from overlapping cylinders): normwise relative force
     error max|dF| / max|F| <= 1e-6 (the target is 1e-5; measured 2e-8 .. 7e-8);
   total energy: relative 1e-6;
-  relaxed network (config 1 after an FP64 minimization, max|F| ~ 50 pN, per-term forces much
+  relaxed network (the 42,000-bead network after an FP64 minimization, max|F| ~ 50 pN, per-term forces much
     larger): absolute force error <= 1e-2 pN, i.e. 1 % of the 1 pN force tolerance, outside the
     beads of excluded-volume pairs with normalized triple product below 1e-3 (measured 1e-4 pN;
     FP32 rounding of the large cancelling bonded terms);
@@ -66,7 +66,7 @@ def test_mixed_force_error_relaxed(gpu):
     m = well_conditioned(g, net, x)
     dF = np.abs(f32 - f64)
     err = float(dF[m].max())
-    print(f"relaxed config 1: max|F| {np.abs(f64).max():.3f} pN, mixed abs error {err:.3e} pN outside "
+    print(f"relaxed 42,000-bead network: max|F| {np.abs(f64).max():.3f} pN, mixed abs error {err:.3e} pN outside "
           f"{int((~m).sum())} coordinates on pairs with tau < 1e-3 (max there {float(dF.max()):.3e})", flush=True)
     assert err <= 1e-2 and np.all(np.isfinite(f32))
 
@@ -109,7 +109,7 @@ def test_mixed_minimize_force_tolerance(gpu):
     p = abi.default_cg_params(precision=abi.MDN_PREC_MIXED, energy_rel_tol=0.0)
     x, f, res = g.minimize(p, net.coord)
     fx = g.compute_forces(x)
-    print(f"config 1 force-only: {res.num_iter} iterations, mixed max|F| {res.grad_inf_norm:.4f}, "
+    print(f"42,000-bead network, force-only: {res.num_iter} iterations, mixed max|F| {res.grad_inf_norm:.4f}, "
           f"FP64 max|F| {np.abs(fx).max():.4f}", flush=True)
     assert res.converged_grad and not res.call_limit_exceeded
     assert res.grad_inf_norm < 1.0

@@ -88,7 +88,7 @@ def test_minimize_trajectory_prefix(gpu, ncalls):
 def test_minimize_trajectory_prefix_other_methods(gpu, method):
     """The descent-direction variants of CGMethod (steepest descent: beta = 0; Fletcher-Reeves:
     beta = F.F / F_prev.F_prev, CGMethod.hpp:1322-1332) follow the oracle's trajectory for 60
-    energy calls on config 0, like Polak-Ribiere above, and a full minimization converges by the
+    energy calls on the 550-bead network, like Polak-Ribiere above, and a full minimization converges by the
     reference criterion at a state whose oracle forces confirm it."""
     net, g, o, _ = _pair(gpu, 0)
     p = abi.default_cg_params(method=method, energy_call_limit=60)

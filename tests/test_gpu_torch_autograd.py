@@ -1,5 +1,5 @@
 """GPU: the libtorch autograd functions (picgo_amd/_torch_mech.so, csrc/torch_mech.cpp) on the
-config-2 network near its relaxed state: torch.autograd gives dE/dx = -F (mdn_forces), dE/dtheta
+42,000-bead network with per-cylinder parameters near its relaxed state: torch.autograd gives dE/dx = -F (mdn_forces), dE/dtheta
 (mdn_param_grad), and a double backward gives H v (mdn_hvp); the parameter gradient through
 torch matches central differences of the energy (h = 1e-4 relative)."""
 import numpy as np
