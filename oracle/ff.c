@@ -254,6 +254,7 @@ typedef struct {
     double a, b, c, d, e, F, g, h, I;
     double AA, BB, CC, DD, EE, FF, GG, HH, JJ;
     double ATG1, ATG2, ATG3, ATG4;
+    double r14, r23, r24; /* |c4 - c1|^2, |c2 - c3|^2, |c4 - c2|^2 */
 } exv_terms;
 
 static void exvol_terms(const double* c1, const double* c2, const double* c3, const double* c4,
@@ -272,6 +273,10 @@ static void exvol_terms(const double* c1, const double* c2, const double* c3, co
     double sqmag_D = (x * x + y * y + z * z);
     x = c3[0] - c2[0]; y = c3[1] - c2[1]; z = c3[2] - c2[2];
     double sqmag_E = (x * x + y * y + z * z);
+    t->r14 = sqmag_D;
+    t->r23 = sqmag_E;
+    x = c4[0] - c2[0]; y = c4[1] - c2[1]; z = c4[2] - c2[2];
+    t->r24 = (x * x + y * y + z * z);
     double vA[3], vB[3], vC[3], cp[3];
     for (int k = 0; k < 3; k++) {
         vA[k] = c1[k] - c2[k];
@@ -480,9 +485,71 @@ double or_exvol_energy(const double* coord, const int* beadSet, const double* kr
     return U;
 }
 
+/* Sensitivities of one edge term X psi, psi = (atan(p / Q) - atan(q / Q)) / Q, Q^2 = P, to its
+ * radicand and its two arguments: o = {psi, X dpsi/dP, X dpsi/dp, X dpsi/dq} with
+ * dpsi/dp = 1 / (P + p^2), dpsi/dq = -1 / (P + q^2), dpsi/dP = -(psi + p dpsi/dp + q dpsi/dq) / (2P);
+ * Pp = P + p^2 and Pq = P + q^2 are passed as products of squared lengths. */
+static void edge_sens(double X, double Q, double atg, double p, double q, double Pp, double Pq,
+                      double o[4]) {
+    const double iQ = 1 / Q;
+    o[0] = atg * iQ;
+    o[2] = X / Pp;
+    o[3] = -X / Pq;
+    o[1] = -(X * o[0] + p * o[2] + q * o[3]) * (iQ * iQ) / 2;
+}
+
+/* Forces -dU/dc_k of the closed-form pair energy U = (K / 2) S / JJ, S = sum over the four edge
+ * terms (CylinderExclVolRepulsion.cpp:353-435 defines U), by the chain rule through the six
+ * scalars a = u1.u1, b = u2.u2, c = u3.u3, d = u1.u2, e = u1.u3, F = u2.u3 of u1 = c2 - c1,
+ * u2 = c4 - c3, u3 = c1 - c3. The edge terms (radicand P, weight X, arguments p, q):
+ *   A: P = ac - e^2, X = CC, p = a + e, q = e;  E: P = a r14 - (d - e)^2, X = GG, p = a + e - d,
+ *   q = e - d;  B: P = bc - F^2, X = DD, p = F, q = F - b;  H: P = b r23 - (d + F)^2, X = HH,
+ *   p = d + F, q = d + F - b;
+ * with P + p^2, P + q^2 = a r23, a c (A); a r24, a r14 (E); b c, b r14 (B); b r23, b r24 (H).
+ * Then dU/ds = (K / (2 JJ)) (dS/ds - (S / JJ) dJJ/ds), JJ = -(abc - aF^2 - cd^2 + 2deF - be^2),
+ * and dU/du1 = 2 U_a u1 + U_d u2 + U_e u3, dU/du2 = 2 U_b u2 + U_d u1 + U_F u3,
+ * dU/du3 = 2 U_c u3 + U_e u1 + U_F u2; f1 = G1 - G3, f2 = -G1, f3 = G2 + G3, f4 = -G2. */
+static void exvol_pair_forces(const exv_terms* t, double K, const double* u1, const double* u2,
+                              const double* u3, double fo[4][3]) {
+    const double a = t->a, b = t->b, c = t->c, d = t->d, e = t->e, F = t->F;
+    const double g = d - e, h = d + F;
+    double A[4], E[4], B[4], H[4];
+    edge_sens(t->CC, t->AA, t->ATG1, a + e, e, a * t->r23, a * c, A);
+    edge_sens(t->GG, t->EE, t->ATG2, a + e - d, e - d, a * t->r24, a * t->r14, E);
+    edge_sens(t->DD, t->BB, t->ATG3, F, F - b, b * c, b * t->r14, B);
+    edge_sens(t->HH, t->FF, t->ATG4, d + F, d + F - b, b * t->r23, b * t->r24, H);
+    const double S = t->CC * A[0] + t->GG * E[0] + t->DD * B[0] + t->HH * H[0];
+    const double dSa = -F * A[0] + c * A[1] + A[2] + (F - b) * E[0] + t->r14 * E[1] + E[2] - b * H[0] + b * H[1];
+    const double dSb = -a * E[0] + a * E[1] + e * B[0] + c * B[1] - B[3] - (a + e) * H[0] + t->r23 * H[1] - H[3];
+    const double dSc = a * (A[1] + E[1]) + b * (B[1] + H[1]);
+    const double dSd = e * A[0] + (2 * d - e) * E[0] - 2 * g * E[1] - E[2] - E[3] - F * B[0] + (2 * d + F) * H[0] -
+                       2 * h * H[1] + H[2] + H[3];
+    const double dSe = d * A[0] - 2 * e * A[1] + A[2] + A[3] - d * E[0] + 2 * g * E[1] + E[2] + E[3] + b * B[0] -
+                       b * H[0] + 2 * b * H[1];
+    const double dSF = -a * A[0] + a * E[0] - 2 * a * E[1] - d * B[0] - 2 * F * B[1] + B[2] + B[3] + d * H[0] -
+                       2 * h * H[1] + H[2] + H[3];
+    const double w = K / (2 * t->JJ), r = S / t->JJ;
+    const double Ua = w * (dSa - r * (F * F - b * c));
+    const double Ub = w * (dSb - r * (e * e - a * c));
+    const double Uc = w * (dSc - r * (d * d - a * b));
+    const double Ud = w * (dSd - r * (2 * (c * d - e * F)));
+    const double Ue = w * (dSe - r * (2 * (b * e - d * F)));
+    const double UF = w * (dSF - r * (2 * (a * F - d * e)));
+    for (int k = 0; k < 3; k++) {
+        const double G1 = 2 * Ua * u1[k] + Ud * u2[k] + Ue * u3[k];
+        const double G2 = 2 * Ub * u2[k] + Ud * u1[k] + UF * u3[k];
+        const double G3 = 2 * Uc * u3[k] + Ue * u1[k] + UF * u2[k];
+        fo[0][k] = G1 - G3;
+        fo[1][k] = -G1;
+        fo[2][k] = G2 + G3;
+        fo[3][k] = -G2;
+    }
+}
+
 void or_exvol_forces(const double* coord, double* f, const int* beadSet, const double* krep,
                      const double* eqLengths, int nint) {
-    /* CylinderExclVolRepulsion.cpp:578-882 */
+    /* CylinderExclVolRepulsion.cpp:578-882: the coplanar shift of bead 2 (:610-630), the closed
+     * form's gradient, the Simpson forceN fallback for a non-finite or > 1e15 component (:859-872) */
     exv_terms t;
     for (int i = 0; i < nint; i++) {
         double kRepScaled = krep[i] * eqLengths[2 * i] * eqLengths[2 * i + 1];
@@ -500,96 +567,13 @@ void or_exvol_forces(const double* coord, double* f, const int* beadSet, const d
             c2 = newc2;
         }
         exvol_terms(c1, c2, c3, c4, &t);
-        double a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
-        double AA = t.AA, BB = t.BB, CC = t.CC, DD = t.DD, EE = t.EE, FF = t.FF, GG = t.GG,
-               HH = t.HH, JJ = t.JJ;
-        double ATG1 = t.ATG1, ATG2 = t.ATG2, ATG3 = t.ATG3, ATG4 = t.ATG4;
-        double invJJ = 1 / JJ;
-
-        const double sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
-        const double enFactor = kRepScaled * invJJ / 2;
-
-        const double A1 = AA * AA / (AA * AA + (a + e) * (a + e));
-        const double A2 = AA * AA / (AA * AA + e * e);
-        const double E1 = EE * EE / (EE * EE + (a + e - d) * (a + e - d));
-        const double E2 = EE * EE / (EE * EE + (e - d) * (e - d));
-        const double B1 = BB * BB / (BB * BB + F * F);
-        const double B2 = BB * BB / (BB * BB + (F - b) * (F - b));
-        const double F1 = FF * FF / (FF * FF + (d + F) * (d + F));
-        const double F2 = FF * FF / (FF * FF + (d + F - b) * (d + F - b));
-
-        const double blockA_C = ATG1 / AA;
-        const double blockA_A = -(ATG1 * CC) / (AA * AA) + (-A1 * (a + e) + A2 * e) * CC / (AA * AA * AA);
-        const double blockA_e = ((A1 - A2) * CC) / (AA * AA);
-        const double blockA_a = (A1 * CC) / (AA * AA);
-
-        const double blockE_G = ATG2 / EE;
-        const double blockE_E = -(ATG2 * GG) / (EE * EE) + (-E1 * (a + e - d) + E2 * (e - d)) * GG / (EE * EE * EE);
-        const double blockE_e_minus_d = ((E1 - E2) * GG) / (EE * EE);
-        const double blockE_a = (E1 * GG) / (EE * EE);
-
-        const double blockB_D = ATG3 / BB;
-        const double blockB_B = -(ATG3 * DD) / (BB * BB) + (-B1 * F + B2 * (F - b)) * DD / (BB * BB * BB);
-        const double blockB_f_minus_b = ((B1 - B2) * DD) / (BB * BB);
-        const double blockB_b = (B1 * DD) / (BB * BB);
-
-        const double blockF_H = ATG4 / FF;
-        const double blockF_F = -(ATG4 * HH) / (FF * FF) + (-F1 * (d + F) + F2 * (d + F - b)) * HH / (FF * FF * FF);
-        const double blockF_d_plus_f_minus_b = ((F1 - F2) * HH) / (FF * FF);
-        const double blockF_b = (F1 * HH) / (FF * FF);
-
-        const double block_11 = blockA_C * (-2 * F) + blockA_A * (c / AA) + blockA_a * 2 +
-                                blockE_G * (2 * (F - b)) + blockE_E * ((b + c - 2 * F) / EE) + blockE_a * 2 +
-                                blockF_H * (-2 * b) + blockF_F * (b / FF);
-        const double block_12 = blockA_C * e + blockE_G * (2 * d - e) + blockE_E * ((e - d) / EE) +
-                                blockE_e_minus_d * (-1) + blockB_D * (-F) + blockF_H * (2 * d + F) +
-                                blockF_F * (-(d + F) / FF) + blockF_d_plus_f_minus_b;
-        const double block_13 = blockA_C * d + blockA_A * (-e / AA) + blockA_e + blockE_G * (-d) +
-                                blockE_E * (-(e - d) / EE) + blockE_e_minus_d + blockB_D * b +
-                                blockF_H * (-b) + blockF_F * (b / FF);
-        const double block_21 = block_12;
-        const double block_22 = blockE_G * (-2 * a) + blockE_E * (a / EE) + blockB_D * (2 * e) +
-                                blockB_B * (c / BB) + blockB_f_minus_b * (-2) + blockB_b * 2 +
-                                blockF_H * (-2 * (a + e)) + blockF_F * ((a + c + 2 * e) / FF) +
-                                blockF_d_plus_f_minus_b * (-2) + blockF_b * 2;
-        const double block_23 = blockA_C * (-a) + blockE_G * a + blockE_E * (-a / EE) +
-                                blockB_D * (-d) + blockB_B * (-F / BB) + blockB_f_minus_b +
-                                blockF_H * d + blockF_F * (-(d + F) / FF) + blockF_d_plus_f_minus_b;
-        const double block_31 = block_13;
-        const double block_32 = block_23;
-        const double block_33 = blockA_A * (a / AA) + blockE_E * (a / EE) + blockB_B * (b / BB) +
-                                blockF_F * (b / FF);
-
-        const double JJ_11 = 2 * (-b * c + F * F);
-        const double JJ_12 = 2 * (c * d - e * F);
-        const double JJ_13 = 2 * (b * e - d * F);
-        const double JJ_21 = 2 * (c * d - e * F);
-        const double JJ_22 = 2 * (-a * c + e * e);
-        const double JJ_23 = 2 * (-d * e + a * F);
-        const double JJ_31 = 2 * (b * e - d * F);
-        const double JJ_32 = 2 * (-d * e + a * F);
-        const double JJ_33 = 2 * (d * d - a * b);
-
-        double deriv[4][3];
-        deriv[0][0] = enFactor * (-invJJ * sumBlock * (JJ_31 - JJ_11) + (block_31 - block_11));
-        deriv[0][1] = enFactor * (-invJJ * sumBlock * (JJ_32 - JJ_12) + (block_32 - block_12));
-        deriv[0][2] = enFactor * (-invJJ * sumBlock * (JJ_33 - JJ_13) + (block_33 - block_13));
-        deriv[1][0] = enFactor * (-invJJ * sumBlock * (JJ_11) + (block_11));
-        deriv[1][1] = enFactor * (-invJJ * sumBlock * (JJ_12) + (block_12));
-        deriv[1][2] = enFactor * (-invJJ * sumBlock * (JJ_13) + (block_13));
-        deriv[2][0] = enFactor * (-invJJ * sumBlock * (-JJ_31 - JJ_21) + (-block_31 - block_21));
-        deriv[2][1] = enFactor * (-invJJ * sumBlock * (-JJ_32 - JJ_22) + (-block_32 - block_22));
-        deriv[2][2] = enFactor * (-invJJ * sumBlock * (-JJ_33 - JJ_23) + (-block_33 - block_23));
-        deriv[3][0] = enFactor * (-invJJ * sumBlock * (JJ_21) + (block_21));
-        deriv[3][1] = enFactor * (-invJJ * sumBlock * (JJ_22) + (block_22));
-        deriv[3][2] = enFactor * (-invJJ * sumBlock * (JJ_23) + (block_23));
-
-        double fo[4][3];
-        for (int bi = 0; bi < 4; bi++)
-            for (int k = 0; k < 3; k++)
-                fo[bi][k] = -(deriv[bi][0] * (c2[k] - c1[k]) + deriv[bi][1] * (c4[k] - c3[k]) +
-                              deriv[bi][2] * (c1[k] - c3[k]));
-
+        double u1[3], u2[3], u3[3], fo[4][3];
+        for (int k = 0; k < 3; k++) {
+            u1[k] = c2[k] - c1[k];
+            u2[k] = c4[k] - c3[k];
+            u3[k] = c1[k] - c3[k];
+        }
+        exvol_pair_forces(&t, kRepScaled, u1, u2, u3, fo);
         if (bad3(fo[0]) || bad3(fo[1]) || bad3(fo[2]) || bad3(fo[3])) {
             or_exvol_forceN(coord, f, beadSet, krep, eqLengths, i, 0);
         } else {
@@ -598,6 +582,13 @@ void or_exvol_forces(const double* coord, double* f, const int* beadSet, const d
             }
         }
     }
+}
+
+/* The coplanar shift of bead i in {1, 2} by d along the unit normal of (p2 - p1, p3 - p1)
+ * (movePointOutOfPlane, MathFunctions.h:1221-1278), for tests. */
+void or_move_out_of_plane(const double* p1, const double* p2, const double* p3, double* newp, int i,
+                          double d) {
+    move_out_of_plane(p1, p2, p3, newp, i, d);
 }
 
 /* ======================= BoundaryCylinderRepulsionExp ================================= */

@@ -72,6 +72,10 @@ void or_exvol_forceN(const double* coord, double* f, const int* beadSet, const d
                      const double* eqLengths, int i, int movebeads);
 /* MathFunctions.h:1101-1135 areInPlane (|n.v| < 0.1 after normalization) */
 int or_are_in_plane(const double* p1, const double* p2, const double* p3, const double* p4);
+/* movePointOutOfPlane (MathFunctions.h:1221-1278): point i in {1, 2} moved by d along the unit
+ * normal of (p2 - p1, p3 - p1), exposed for tests */
+void or_move_out_of_plane(const double* p1, const double* p2, const double* p3, double* newp, int i,
+                          double d);
 
 /* BoundaryCylinderRepulsionExp::energy / forces (BoundaryCylinderRepulsionExp.cpp:27-118) */
 double or_boundary_energy(const double* coord, const int* beadSet, const double* krep,

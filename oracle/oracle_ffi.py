@@ -52,6 +52,7 @@ def lib(fast=False):
         L.or_exvol_forceN.argtypes = [dp, dp, ip, dp, dp, C.c_int, C.c_int]
         L.or_are_in_plane.restype = C.c_int
         L.or_are_in_plane.argtypes = [dp, dp, dp, dp]
+        L.or_move_out_of_plane.argtypes = [dp, dp, dp, dp, C.c_int, C.c_double]
         bep = C.POINTER(abi.BoundaryElement)
         L.or_boundary_energy.restype = C.c_double
         L.or_boundary_energy.argtypes = [dp, ip, dp, dp, ip, bep, C.c_int]
@@ -173,6 +174,14 @@ def exvol_forceN(coord, beadSet, krep, eqlen, i, movebeads=0):
 def are_in_plane(p1, p2, p3, p4):
     a = [_arr(x, np.float64) for x in (p1, p2, p3, p4)]
     return bool(lib().or_are_in_plane(*[_p(x) for x in a]))
+
+
+def move_out_of_plane(p1, p2, p3, i, d):
+    """movePointOutOfPlane: point i (1 or 2) moved by d along the unit normal of (p2 - p1, p3 - p1)."""
+    a = [_arr(x, np.float64) for x in (p1, p2, p3)]
+    out = np.zeros(3)
+    lib().or_move_out_of_plane(*[_p(x) for x in a], _p(out), int(i), float(d))
+    return out
 
 
 def plane_element(point, normal, krep, slen):
@@ -324,9 +333,12 @@ class OracleSystem:
         self._L.or_ff_forces(self._h, ff, _p(c), _p(f))
         return f
 
-    def stretch_forces_of(self, motor=False):
-        n = self._L.or_system_ncyl(self._h)  # placeholder size guard
-        return self._L.or_system_stretchforce(self._h, 1 if motor else 0), n
+    def stretch_forces_of(self, n, motor=False):
+        """The linker (or motor) stretchForce values k (L - L0) of the last force evaluation
+        (LinkerStretchingHarmonic.cpp:291), n of them."""
+        if n == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(self._L.or_system_stretchforce(self._h, 1 if motor else 0), shape=(n,)).copy()
 
     def load_forces(self, coord, mon_size, num_mon, stride):
         """Boundary load forces on the filament ends: (plus, minus), each [n_filaments, stride]

@@ -162,6 +162,14 @@ int fail(mdn_ctx* c, int code, const char* fmt, ...) {
                         __FILE__, __LINE__);                                                    \
     } while (0)
 
+// Device-to-host read ordered after everything queued on the context's stream (a blocking
+// hipMemcpy runs on the null stream, which does not wait for the non-blocking context stream).
+hipError_t d2h(mdn_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->stream);
+}
+
 bool call_limit_host(const CgState& S) {
     return (S.energy_call_limit != 0 && S.num_energy_call > S.energy_call_limit) ||
            (S.force_call_limit != 0 && S.num_force_call > S.force_call_limit);
@@ -333,7 +341,7 @@ int eval_slots_t(mdn_ctx* ctx, const double* x, bool pieces, int bmask, const do
     int nbad = ctx->h_int[0];
     if (nbad == 0) return MDN_OK;
     std::vector<int> bad(nbad);
-    HIPCHK(hipMemcpy(bad.data(), ctx->bad_list.p, sizeof(int) * nbad, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(ctx, bad.data(), ctx->bad_list.p, sizeof(int) * nbad));
     std::sort(bad.begin(), bad.end());
     const int G = 240;
     HIPCHK(ctx->fallback_partial.ensure(6 * G));
@@ -837,6 +845,10 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
     int st = need_system(ctx, false);
     if (st) return st;
     HIPCHK(hipSetDevice(ctx->device));
+    // The rebuild reallocates the pair, excluded-volume and boundary tables that the system
+    // record and the captured minimizer graphs point at; until the new layout is complete
+    // (relayout), nothing may use them: a failure on the way leaves the lists unbuilt.
+    ctx->nl_built = false;
     if ((st = set_coords_dev(ctx, coord))) return st;
     const double* x = ctx->x[ctx->cur].p;
     NLDev n = nl_view(ctx);
@@ -870,11 +882,11 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
     HIPCHK(hipMemcpyAsync(ctx->h_int + 1, ctx->nl_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->h_int[1] != 0) {
-        ctx->nl_built = false;
         return fail(ctx, MDN_ERR_NL_BOUNDS, "%d cylinder midpoints outside the neighbor-list grid [0, %g)x[0, %g)x[0, %g)",
                     ctx->h_int[1], ctx->grid[0], ctx->grid[1], ctx->grid[2]);
     }
     const long np = ctx->h_int[0];
+    if (np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many excluded-volume pairs (%ld)", np);
     HIPCHK(ctx->pairs.ensure(2 * (size_t)std::max(1L, np)));
     hipLaunchKernelGGL(k_nl_pairs<1>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, ctx->mid.p, ctx->bin.p,
                        ctx->bin_start.p, ctx->members.p, ctx->nl_count.p, ctx->nl_off.p, ctx->pairs.p);
@@ -916,7 +928,6 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
                            ctx->bnd_e.p, ctx->bnd_krep.p, ctx->bnd_slen.p);
     ctx->n_bint = nbi;
     // new layout
-    if (np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many excluded-volume pairs");
     ctx->sys.n[FF_EXVOL] = (int)np;
     ctx->sys.n[FF_BOUNDARY] = nbi;
     if ((st = relayout(ctx))) return st;
@@ -976,7 +987,7 @@ int mdn_nl_pairs(mdn_ctx* ctx, int32_t* out, int64_t cap, int64_t* n_pairs) {
     if (n_pairs) *n_pairs = ctx->n_pairs;
     if (cap < ctx->n_pairs) return fail(ctx, MDN_ERR_CAPACITY, "need room for %ld pairs", ctx->n_pairs);
     if (ctx->n_pairs > 0)
-        HIPCHK(hipMemcpy(out, ctx->pairs.p, sizeof(int) * 2 * ctx->n_pairs, hipMemcpyDeviceToHost));
+        HIPCHK(d2h(ctx, out, ctx->pairs.p, sizeof(int) * 2 * ctx->n_pairs));
     return MDN_OK;
 }
 
@@ -987,7 +998,7 @@ int mdn_boundary_list(mdn_ctx* ctx, int32_t* out_beads, int64_t cap, int32_t* ou
     if (n_interactions) *n_interactions = ctx->n_bint;
     if (cap < ctx->n_bint) return fail(ctx, MDN_ERR_CAPACITY, "need room for %d interactions", ctx->n_bint);
     if (ctx->n_bint > 0 && out_beads)
-        HIPCHK(hipMemcpy(out_beads, ctx->bnd_b.p, sizeof(int) * ctx->n_bint, hipMemcpyDeviceToHost));
+        HIPCHK(d2h(ctx, out_beads, ctx->bnd_b.p, sizeof(int) * ctx->n_bint));
     if (out_nneighbors)
         for (size_t e = 0; e < ctx->bnd_nneighbors.size(); e++) out_nneighbors[e] = ctx->bnd_nneighbors[e];
     return MDN_OK;
@@ -1056,9 +1067,9 @@ int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor
     int st = need_system(ctx, true);
     if (st) return st;
     if (linker_out && ctx->sys.n[FF_LINKER])
-        HIPCHK(hipMemcpy(linker_out, ctx->lnk_sf.p, sizeof(double) * ctx->sys.n[FF_LINKER], hipMemcpyDeviceToHost));
+        HIPCHK(d2h(ctx, linker_out, ctx->lnk_sf.p, sizeof(double) * ctx->sys.n[FF_LINKER]));
     if (motor_out && ctx->sys.n[FF_MOTOR])
-        HIPCHK(hipMemcpy(motor_out, ctx->mot_sf.p, sizeof(double) * ctx->sys.n[FF_MOTOR], hipMemcpyDeviceToHost));
+        HIPCHK(d2h(ctx, motor_out, ctx->mot_sf.p, sizeof(double) * ctx->sys.n[FF_MOTOR]));
     return MDN_OK;
 }
 
@@ -1334,8 +1345,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
             }
             k ^= 1;
         }
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
+        HIPCHK(d2h(ctx, &S, ctx->cg.p, sizeof(CgState)));
         if (!(S.done && S.status == ST_NEED_FORCEN)) break;
         // The step's forces need the numerical excluded-volume fallback (forceN,
         // CylinderExclVolRepulsion.cpp:984-1254): recompute them at the new coordinates with it
@@ -1355,7 +1365,9 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         else launch_cg_gather<double>(ctx, cs, (const CgState*)ctx->cg.p, B);
         hipLaunchKernelGGL(k_cg_fin, dim3(1), dim3(FIN), 0, ctx->stream, cs, ctx->cg.p, B, gbk);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpy(&S, ctx->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
+        // the state after this step's finalize (and the host state's upload above) is complete
+        // before the pinned copy is overwritten
+        HIPCHK(d2h(ctx, &S, ctx->cg.p, sizeof(CgState)));
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
@@ -1433,7 +1445,11 @@ static int launch_batched(mdn_ctx* L, int n, const BatchParams& P) {
     int per_cu = 0, n_cu = 0;
     HIPCHK_CTX(L, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0));
     HIPCHK_CTX(L, hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, L->device));
-    const int grid = std::max(1, std::min(n, std::max(1, per_cu) * std::max(1, n_cu)));
+    int grid = std::max(1, std::min(n, std::max(1, per_cu) * std::max(1, n_cu)));
+    // MDN_BATCH_MAX_WORKGROUPS caps the persistent grid, so that every workgroup pulls several
+    // systems from the queue (the schedule a batch larger than the resident workgroups runs)
+    if (const char* env = getenv("MDN_BATCH_MAX_WORKGROUPS"))
+        if (atoi(env) > 0) grid = std::min(grid, atoi(env));
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, L->stream, L->batch_queue.p, 0);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, L->stream, (const BatchItem*)L->batch_items.p,
                        (const int*)L->batch_order.p, n, L->batch_queue.p, P);
@@ -1553,16 +1569,16 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     for (int i = 0; i < n; i++) {
         mdn_ctx* c = ctxs[i];
         CgState S;
-        HIPCHK_CTX(L, hipMemcpy(&S, c->cg.p, sizeof(CgState), hipMemcpyDeviceToHost));
-        HIPCHK_CTX(L, hipMemcpy(rep.data(), c->eout.p, sizeof(double) * 27, hipMemcpyDeviceToHost));
+        HIPCHK_CTX(L, d2h(L, &S, c->cg.p, sizeof(CgState)));
+        HIPCHK_CTX(L, d2h(L, rep.data(), c->eout.p, sizeof(double) * 27));
         c->last_cg = S;
         c->cur = S.parity;
         const bool ok = S.status == 0 && S.error_flags == 0;
         fill_result(S, rep.data(), rep.data() + 9, ok, &results[i]);
         const int nv = 3 * c->n_beads;
-        HIPCHK_CTX(L, hipMemcpy(coords[i], c->x[c->cur].p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+        HIPCHK_CTX(L, d2h(L, coords[i], c->x[c->cur].p, sizeof(double) * nv));
         if (forces_out && forces_out[i])
-            HIPCHK_CTX(L, hipMemcpy(forces_out[i], (S.fpar ? c->f2 : c->f).p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+            HIPCHK_CTX(L, d2h(L, forces_out[i], (S.fpar ? c->f2 : c->f).p, sizeof(double) * nv));
         int code = MDN_OK;
         if (S.status != 0) {
             code = (S.status == ST_NEED_FORCEN || S.status == 9) ? MDN_ERR_UNSUPPORTED : S.status;

@@ -2,9 +2,11 @@
 // gfx950 device code.
 //
 // Every function is a template over the arithmetic type R:
-//   R = double : the FP64 path. Expressions keep the operand order of the cited reference
-//                kernels so that, built with -ffp-contract=off, results round exactly like the
-//                reference's IEEE double arithmetic (transcendental calls aside).
+//   R = double : the FP64 path. The stretching, bending, crosslink and boundary expressions keep
+//                the operand order of the cited reference kernels so that, built with
+//                -ffp-contract=off, results round exactly like the reference's IEEE double
+//                arithmetic (transcendental calls aside). The excluded-volume forces are this
+//                file's own gradient of the closed-form energy (mdn_exvol_grad.inc).
 //   R = float  : the mixed-precision path (stretching, bending, crosslinks, boundary; the kernels
 //                evaluate excluded volume with R = double in both paths). Absolute coordinates
 //                stay FP64; every bond vector / coordinate difference is formed in FP64 and only
@@ -268,7 +270,8 @@ template <typename R> struct ExvTerms {
     R a, b, c, d, e, F, g, h, I;
     R AA, BB, CC, DD, EE, FF, GG, HH, JJ;
     R ATG1, ATG2, ATG3, ATG4;
-    V3<R> u1, u2, u3;   // c2 - c1, c4 - c3, c1 - c3 (the force assembly's basis vectors)
+    R r14, r23, r24;    // |c4 - c1|^2, |c2 - c3|^2, |c4 - c2|^2 (the force sensitivities' denominators)
+    V3<R> u1, u2, u3;   // c2 - c1, c4 - c3, c1 - c3 (the scalars' gradient directions)
 };
 
 // The pair invariants (CylinderExclVolRepulsion.cpp:353-397) from five FP64 difference vectors:
@@ -288,6 +291,10 @@ template <typename R, typename P, bool FAST = false> MDN_HD inline void exvol_te
     t.I = dot(u2, w14);
     const R sqmag_D = (-w14.x * -w14.x + -w14.y * -w14.y + -w14.z * -w14.z);   // |c1 - c4|^2
     const R sqmag_E = (-w32.x * -w32.x + -w32.y * -w32.y + -w32.z * -w32.z);   // |c3 - c2|^2
+    const V3<R> w24 = dif<R>(c4, c2);
+    t.r14 = sqmag_D;
+    t.r23 = sqmag_E;
+    t.r24 = dot(w24, w24);
     const R a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
     R ac = sqrt(a * c);
     R bc = sqrt(b * c);
@@ -394,103 +401,13 @@ template <typename R> MDN_HD inline bool bad_component(R v) {
     return a != a || a > (R)1e15;   // NaN, +-inf or beyond 1e15 (checkNaN_INF, CylinderExclVolRepulsion.cpp:862-864)
 }
 
-// Analytic pair forces (CylinderExclVolRepulsion.cpp:610-858) on the 4 beads, from the
-// invariants of the (possibly shifted) geometry. Returns false when a component is non-finite or
-// > 1e15, in which case the reference switches to the 10^4 x 10^4 Simpson forceN.
+// Pair forces -dU/dc_k on the 4 beads from the invariants of the (possibly shifted) geometry,
+// by the chain rule through the six scalars (derivation in mdn_exvol_grad.inc). Returns false
+// when a component is non-finite or > 1e15, in which case the reference switches to the
+// 10^4 x 10^4 Simpson forceN (CylinderExclVolRepulsion.cpp:859-872).
 template <typename R, typename O>
-MDN_HD inline bool exvol_forces_from(const ExvTerms<R>& t, R kRepScaled, O* out) {
-    const R a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
-    const R AA = t.AA, BB = t.BB, CC = t.CC, DD = t.DD, EE = t.EE, FF = t.FF, GG = t.GG, HH = t.HH, JJ = t.JJ;
-    const R ATG1 = t.ATG1, ATG2 = t.ATG2, ATG3 = t.ATG3, ATG4 = t.ATG4;
-    const R invJJ = 1 / JJ;
-    const R sumBlock = CC / AA * ATG1 + GG / EE * ATG2 + DD / BB * ATG3 + HH / FF * ATG4;
-    const R enFactor = kRepScaled * invJJ / 2;
-
-    const R A1 = AA * AA / (AA * AA + (a + e) * (a + e));
-    const R A2 = AA * AA / (AA * AA + e * e);
-    const R E1 = EE * EE / (EE * EE + (a + e - d) * (a + e - d));
-    const R E2 = EE * EE / (EE * EE + (e - d) * (e - d));
-    const R B1 = BB * BB / (BB * BB + F * F);
-    const R B2 = BB * BB / (BB * BB + (F - b) * (F - b));
-    const R F1 = FF * FF / (FF * FF + (d + F) * (d + F));
-    const R F2 = FF * FF / (FF * FF + (d + F - b) * (d + F - b));
-
-    const R blockA_C = ATG1 / AA;
-    const R blockA_A = -(ATG1 * CC) / (AA * AA) + (-A1 * (a + e) + A2 * e) * CC / (AA * AA * AA);
-    const R blockA_e = ((A1 - A2) * CC) / (AA * AA);
-    const R blockA_a = (A1 * CC) / (AA * AA);
-    const R blockE_G = ATG2 / EE;
-    const R blockE_E = -(ATG2 * GG) / (EE * EE) + (-E1 * (a + e - d) + E2 * (e - d)) * GG / (EE * EE * EE);
-    const R blockE_e_minus_d = ((E1 - E2) * GG) / (EE * EE);
-    const R blockE_a = (E1 * GG) / (EE * EE);
-    const R blockB_D = ATG3 / BB;
-    const R blockB_B = -(ATG3 * DD) / (BB * BB) + (-B1 * F + B2 * (F - b)) * DD / (BB * BB * BB);
-    const R blockB_f_minus_b = ((B1 - B2) * DD) / (BB * BB);
-    const R blockB_b = (B1 * DD) / (BB * BB);
-    const R blockF_H = ATG4 / FF;
-    const R blockF_F = -(ATG4 * HH) / (FF * FF) + (-F1 * (d + F) + F2 * (d + F - b)) * HH / (FF * FF * FF);
-    const R blockF_d_plus_f_minus_b = ((F1 - F2) * HH) / (FF * FF);
-    const R blockF_b = (F1 * HH) / (FF * FF);
-
-    const R block_11 = blockA_C * (-2 * F) + blockA_A * (c / AA) + blockA_a * 2 +
-                       blockE_G * (2 * (F - b)) + blockE_E * ((b + c - 2 * F) / EE) + blockE_a * 2 +
-                       blockF_H * (-2 * b) + blockF_F * (b / FF);
-    const R block_12 = blockA_C * e + blockE_G * (2 * d - e) + blockE_E * ((e - d) / EE) +
-                       blockE_e_minus_d * (-1) + blockB_D * (-F) + blockF_H * (2 * d + F) +
-                       blockF_F * (-(d + F) / FF) + blockF_d_plus_f_minus_b;
-    const R block_13 = blockA_C * d + blockA_A * (-e / AA) + blockA_e + blockE_G * (-d) +
-                       blockE_E * (-(e - d) / EE) + blockE_e_minus_d + blockB_D * b +
-                       blockF_H * (-b) + blockF_F * (b / FF);
-    const R block_21 = block_12;
-    const R block_22 = blockE_G * (-2 * a) + blockE_E * (a / EE) + blockB_D * (2 * e) +
-                       blockB_B * (c / BB) + blockB_f_minus_b * (-2) + blockB_b * 2 +
-                       blockF_H * (-2 * (a + e)) + blockF_F * ((a + c + 2 * e) / FF) +
-                       blockF_d_plus_f_minus_b * (-2) + blockF_b * 2;
-    const R block_23 = blockA_C * (-a) + blockE_G * a + blockE_E * (-a / EE) + blockB_D * (-d) +
-                       blockB_B * (-F / BB) + blockB_f_minus_b + blockF_H * d +
-                       blockF_F * (-(d + F) / FF) + blockF_d_plus_f_minus_b;
-    const R block_31 = block_13;
-    const R block_32 = block_23;
-    const R block_33 = blockA_A * (a / AA) + blockE_E * (a / EE) + blockB_B * (b / BB) + blockF_F * (b / FF);
-
-    const R JJ_11 = 2 * (-b * c + F * F);
-    const R JJ_12 = 2 * (c * d - e * F);
-    const R JJ_13 = 2 * (b * e - d * F);
-    const R JJ_21 = 2 * (c * d - e * F);
-    const R JJ_22 = 2 * (-a * c + e * e);
-    const R JJ_23 = 2 * (-d * e + a * F);
-    const R JJ_31 = 2 * (b * e - d * F);
-    const R JJ_32 = 2 * (-d * e + a * F);
-    const R JJ_33 = 2 * (d * d - a * b);
-
-    R dv[4][3];
-    dv[0][0] = enFactor * (-invJJ * sumBlock * (JJ_31 - JJ_11) + (block_31 - block_11));
-    dv[0][1] = enFactor * (-invJJ * sumBlock * (JJ_32 - JJ_12) + (block_32 - block_12));
-    dv[0][2] = enFactor * (-invJJ * sumBlock * (JJ_33 - JJ_13) + (block_33 - block_13));
-    dv[1][0] = enFactor * (-invJJ * sumBlock * (JJ_11) + (block_11));
-    dv[1][1] = enFactor * (-invJJ * sumBlock * (JJ_12) + (block_12));
-    dv[1][2] = enFactor * (-invJJ * sumBlock * (JJ_13) + (block_13));
-    dv[2][0] = enFactor * (-invJJ * sumBlock * (-JJ_31 - JJ_21) + (-block_31 - block_21));
-    dv[2][1] = enFactor * (-invJJ * sumBlock * (-JJ_32 - JJ_22) + (-block_32 - block_22));
-    dv[2][2] = enFactor * (-invJJ * sumBlock * (-JJ_33 - JJ_23) + (-block_33 - block_23));
-    dv[3][0] = enFactor * (-invJJ * sumBlock * (JJ_21) + (block_21));
-    dv[3][1] = enFactor * (-invJJ * sumBlock * (JJ_22) + (block_22));
-    dv[3][2] = enFactor * (-invJJ * sumBlock * (JJ_23) + (block_23));
-
-    const R u1[3] = {t.u1.x, t.u1.y, t.u1.z};
-    const R u2[3] = {t.u2.x, t.u2.y, t.u2.z};
-    const R u3[3] = {t.u3.x, t.u3.y, t.u3.z};
-    bool ok = true;
-    for (int bi = 0; bi < 4; bi++) {
-        R f[3];
-        for (int k = 0; k < 3; k++) {
-            f[k] = -(dv[bi][0] * u1[k] + dv[bi][1] * u2[k] + dv[bi][2] * u3[k]);
-            ok = ok && !bad_component<R>(f[k]);
-        }
-        put3(out[bi], f[0], f[1], f[2]);
-    }
-    return ok;
-}
+MDN_HD inline bool exvol_forces_from(const ExvTerms<R>& t, R K, O* out)
+#include "mdn_exvol_grad.inc"
 
 // Forces only: the coplanar shift of c2 (CylinderExclVolRepulsion.cpp:610-630) applies.
 template <typename R, typename P, typename O>
@@ -526,18 +443,16 @@ MDN_HD inline double exvol_energy_forces(D3 c1, D3 c2, D3 c3, D3 c4, double kRep
 }
 
 // ---- the mixed path's excluded volume: the same closed form in fewer FP64 operations -------
-// The reference form (above) spends most of its issue slots on eight atan calls and about fifty
-// divisions. With AA > 0 both angles of atan(p / AA) - atan(q / AA) lie in (-pi/2, pi/2), so
+// The energy form above spends most of its issue slots on eight atan calls. With AA > 0 both
+// angles of atan(p / AA) - atan(q / AA) lie in (-pi/2, pi/2), so
 //     atan(p / AA) - atan(q / AA) = atan2((p - q) AA, AA^2 + p q)
 // (tangent of a difference; the cosine of the difference has the sign of AA^2 + p q): four
 // atan2 instead of eight atan, with p - q (= a or b) entering exactly instead of through the
-// cancellation of two angles. In the force blocks the divisions by AA, BB, EE, FF and their
-// powers become products with four reciprocals, the eight cos^2 = 1 / (1 + tan^2) factors share
-// one division per pair of angles, and the expressions contract to FMAs. In exact arithmetic
-// every quantity equals the reference's; the rounding differs (the FP64 path keeps the
-// reference's operand order). The invariants a ... JJ are the reference's, bit for bit, so the
-// degenerate geometries (AA, BB, EE, FF or JJ exactly zero) give non-finite values exactly where
-// the reference's do and the Simpson fallbacks still trigger there.
+// cancellation of two angles. The force gradient is the same as the FP64 path's with FMA
+// contraction. In exact arithmetic every quantity equals the FP64 path's; the rounding differs.
+// The invariants a ... JJ are the reference's, bit for bit, so the degenerate geometries (AA,
+// BB, EE, FF or JJ exactly zero) give non-finite values exactly where the reference's do and the
+// Simpson fallbacks still trigger there.
 // The coplanar test without its two normalizations: |n.v3| < 0.1 for the unit normal n of
 // (p2 - p1, p3 - p1) and the unit v3 = (p4 - p1) / |p4 - p1| is (cp.w)^2 < 0.01 |cp|^2 |w|^2 for
 // the unnormalized cp and w (a degenerate cp or w, where the reference's test reads NaN and
@@ -568,89 +483,11 @@ MDN_HD inline double exvol_energy_of_fast(const ExvTerms<double>& t, double kRep
     return exvol_energy_of<double>(t, kRepScaled);   // the atan2 differences are in t
 }
 
-// the force blocks of CylinderExclVolRepulsion.cpp:668-858 with the reciprocals; a block that
-// the reference only uses divided by AA (blockA_A and its E, B, F kin) carries that factor
-template <typename O> MDN_HD inline bool exvol_forces_from_fast(const ExvTerms<double>& t, double kRepScaled, O* out) {
+// the same gradient with FMA contraction (the mixed path)
+template <typename O> MDN_HD inline bool exvol_forces_from_fast(const ExvTerms<double>& t, double K, O* out) {
 #pragma clang fp contract(fast)
-    const double a = t.a, b = t.b, c = t.c, d = t.d, e = t.e, F = t.F;
-    const double CC = t.CC, DD = t.DD, GG = t.GG, HH = t.HH;
-    const double iA = 1.0 / t.AA, iB = 1.0 / t.BB, iE = 1.0 / t.EE, iF = 1.0 / t.FF, iJJ = 1.0 / t.JJ;
-    const double iA2 = iA * iA, iB2 = iB * iB, iE2 = iE * iE, iF2 = iF * iF;
-    const double ATG1 = t.ATG1, ATG2 = t.ATG2, ATG3 = t.ATG3, ATG4 = t.ATG4;
-    const double sumBlock = CC * iA * ATG1 + GG * iE * ATG2 + DD * iB * ATG3 + HH * iF * ATG4;
-    const double enFactor = kRepScaled * iJJ * 0.5;
-
-    // cos^2 of the eight angles, 1 / (1 + tan^2), two per division
-    const double tA1 = (a + e) * iA, tA2 = e * iA, tE1 = (a + e - d) * iE, tE2 = (e - d) * iE;
-    const double tB1 = F * iB, tB2 = (F - b) * iB, tF1 = (d + F) * iF, tF2 = (d + F - b) * iF;
-    const double pA1 = 1 + tA1 * tA1, pA2 = 1 + tA2 * tA2, pE1 = 1 + tE1 * tE1, pE2 = 1 + tE2 * tE2;
-    const double pB1 = 1 + tB1 * tB1, pB2 = 1 + tB2 * tB2, pF1 = 1 + tF1 * tF1, pF2 = 1 + tF2 * tF2;
-    const double rA = 1.0 / (pA1 * pA2), rE = 1.0 / (pE1 * pE2), rB = 1.0 / (pB1 * pB2), rF = 1.0 / (pF1 * pF2);
-    const double A1 = pA2 * rA, A2 = pA1 * rA, E1 = pE2 * rE, E2 = pE1 * rE;
-    const double B1 = pB2 * rB, B2 = pB1 * rB, F1 = pF2 * rF, F2 = pF1 * rF;
-
-    const double bA_C = ATG1 * iA;
-    const double bA_An = (-(ATG1 * CC) + (-A1 * (a + e) + A2 * e) * CC * iA) * iA2 * iA;   // blockA_A / AA
-    const double bA_e = (A1 - A2) * CC * iA2;
-    const double bA_a = A1 * CC * iA2;
-    const double bE_G = ATG2 * iE;
-    const double bE_En = (-(ATG2 * GG) + (-E1 * (a + e - d) + E2 * (e - d)) * GG * iE) * iE2 * iE;
-    const double bE_emd = (E1 - E2) * GG * iE2;
-    const double bE_a = E1 * GG * iE2;
-    const double bB_D = ATG3 * iB;
-    const double bB_Bn = (-(ATG3 * DD) + (-B1 * F + B2 * (F - b)) * DD * iB) * iB2 * iB;
-    const double bB_fmb = (B1 - B2) * DD * iB2;
-    const double bB_b = B1 * DD * iB2;
-    const double bF_H = ATG4 * iF;
-    const double bF_Fn = (-(ATG4 * HH) + (-F1 * (d + F) + F2 * (d + F - b)) * HH * iF) * iF2 * iF;
-    const double bF_dfb = (F1 - F2) * HH * iF2;
-    const double bF_b = F1 * HH * iF2;
-
-    const double block_11 = bA_C * (-2 * F) + bA_An * c + bA_a * 2 + bE_G * (2 * (F - b)) +
-                            bE_En * (b + c - 2 * F) + bE_a * 2 + bF_H * (-2 * b) + bF_Fn * b;
-    const double block_12 = bA_C * e + bE_G * (2 * d - e) + bE_En * (e - d) - bE_emd + bB_D * (-F) +
-                            bF_H * (2 * d + F) - bF_Fn * (d + F) + bF_dfb;
-    const double block_13 = bA_C * d - bA_An * e + bA_e - bE_G * d - bE_En * (e - d) + bE_emd + bB_D * b -
-                            bF_H * b + bF_Fn * b;
-    const double block_22 = bE_G * (-2 * a) + bE_En * a + bB_D * (2 * e) + bB_Bn * c - 2 * bB_fmb + 2 * bB_b +
-                            bF_H * (-2 * (a + e)) + bF_Fn * (a + c + 2 * e) - 2 * bF_dfb + 2 * bF_b;
-    const double block_23 = -a * bA_C + a * bE_G - a * bE_En - d * bB_D - F * bB_Bn + bB_fmb + d * bF_H -
-                            (d + F) * bF_Fn + bF_dfb;
-    const double block_33 = a * bA_An + a * bE_En + b * bB_Bn + b * bF_Fn;
-    const double block_21 = block_12, block_31 = block_13, block_32 = block_23;
-
-    const double JJ_11 = 2 * (-b * c + F * F), JJ_12 = 2 * (c * d - e * F), JJ_13 = 2 * (b * e - d * F);
-    const double JJ_21 = JJ_12, JJ_22 = 2 * (-a * c + e * e), JJ_23 = 2 * (-d * e + a * F);
-    const double JJ_31 = JJ_13, JJ_32 = JJ_23, JJ_33 = 2 * (d * d - a * b);
-    const double sj = -iJJ * sumBlock;
-
-    double dv[4][3];
-    dv[0][0] = enFactor * (sj * (JJ_31 - JJ_11) + (block_31 - block_11));
-    dv[0][1] = enFactor * (sj * (JJ_32 - JJ_12) + (block_32 - block_12));
-    dv[0][2] = enFactor * (sj * (JJ_33 - JJ_13) + (block_33 - block_13));
-    dv[1][0] = enFactor * (sj * JJ_11 + block_11);
-    dv[1][1] = enFactor * (sj * JJ_12 + block_12);
-    dv[1][2] = enFactor * (sj * JJ_13 + block_13);
-    dv[2][0] = enFactor * (sj * (-JJ_31 - JJ_21) + (-block_31 - block_21));
-    dv[2][1] = enFactor * (sj * (-JJ_32 - JJ_22) + (-block_32 - block_22));
-    dv[2][2] = enFactor * (sj * (-JJ_33 - JJ_23) + (-block_33 - block_23));
-    dv[3][0] = enFactor * (sj * JJ_21 + block_21);
-    dv[3][1] = enFactor * (sj * JJ_22 + block_22);
-    dv[3][2] = enFactor * (sj * JJ_23 + block_23);
-
-    const double u1[3] = {t.u1.x, t.u1.y, t.u1.z};
-    const double u2[3] = {t.u2.x, t.u2.y, t.u2.z};
-    const double u3[3] = {t.u3.x, t.u3.y, t.u3.z};
-    bool ok = true;
-    for (int bi = 0; bi < 4; bi++) {
-        double f[3];
-        for (int k = 0; k < 3; k++) {
-            f[k] = -(dv[bi][0] * u1[k] + dv[bi][1] * u2[k] + dv[bi][2] * u3[k]);
-            ok = ok && !bad_component<double>(f[k]);
-        }
-        put3(out[bi], f[0], f[1], f[2]);
-    }
-    return ok;
+    typedef double R;
+#include "mdn_exvol_grad.inc"
 }
 
 // forces only (the coplanar shift applies), energy only, energy and forces: as the reference

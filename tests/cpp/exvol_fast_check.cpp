@@ -1,15 +1,20 @@
-// Host-side check of the mixed path's excluded-volume form (exvol_*_fast in
-// picgo_amd/csrc/mdn_terms.h) against the reference-order FP64 form of the same closed form
-// (CylinderExclVolRepulsion.cpp:330-858), both measured against the reference form evaluated in
-// x87 extended precision (64-bit significand) on the same FP64 coordinate differences, on seeded
-// random pairs of 108 nm cylinders. Built host-only. Prints per class (by the normalized triple
-// product tau, the closed form's conditioning ~1/tau^2, and the coplanar-shifted pairs)
-//     "<class> <cases> <energy: fast-vs-ref max> <force: fast-vs-ref max>
-//      <energy err 99th pct / max, ref> <same, fast> <force err 99th pct / max, ref> <same, fast>"
-// (errors relative to |E| and to the pair's max |F|), then "flags <cases> <mismatches>" (the
-// deferred / non-finite flags and the entry points' agreement) and "degenerate <cases>
-// <mismatches>" on exactly degenerate integer geometries (parallel, an end on the other line,
-// crossing, identical), where both forms must flag the same pairs for the Simpson fallbacks.
+// Host-side checks of the excluded-volume pair arithmetic (picgo_amd/csrc/mdn_terms.h, gradient
+// in mdn_exvol_grad.inc) on seeded random pairs of 108 nm cylinders. Built host-only.
+//
+// 1. The gradient formula itself: the forces of the extended-precision (x87, 64-bit significand)
+//    instantiation against Richardson-extrapolated central differences of the closed-form energy
+//    (CylinderExclVolRepulsion.cpp:353-435) in the same precision, on pairs outside the coplanar
+//    test (where the force is -dE/dx). Prints "gradient_fd <cases> <max relative error>".
+// 2. Rounding: the FP64 path's form (atan differences, no contraction) and the mixed path's form
+//    (atan2 differences, FMA contraction) against the extended-precision evaluation of the same
+//    closed form and gradient. Prints per class (by the normalized triple product tau, the closed
+//    form's conditioning ~1/tau^2, and the coplanar-shifted pairs)
+//     "<class> <cases> <energy: fast-vs-fp64 max> <force: fast-vs-fp64 max>
+//      <energy err 99th pct / max, fp64> <same, fast> <force err 99th pct / max, fp64> <same, fast>"
+//    (errors relative to |E| and to the pair's max |F|), then "flags <cases> <mismatches>" (the
+//    deferred / non-finite flags and the entry points' agreement) and "degenerate <cases>
+//    <mismatches>" on exactly degenerate integer geometries (parallel, an end on the other line,
+//    crossing, identical), where both forms must flag the same pairs for the Simpson fallbacks.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -17,6 +22,13 @@
 #include <cstdio>
 #include <random>
 #include <vector>
+
+// the extended-precision instantiation must find the long double overloads of the functions the
+// term code calls unqualified (otherwise the double ones of the global namespace are used)
+using std::atan;
+using std::atan2;
+using std::fabs;
+using std::sqrt;
 
 #include "../../picgo_amd/csrc/mdn_terms.h"
 
@@ -38,7 +50,7 @@ static D3 add(D3 a, D3 b, double s) { return D3{a.x + s * b.x, a.y + s * b.y, a.
 struct Cls {
     const char* name;
     int n = 0;
-    double e_err = 0.0, f_err = 0.0;   // fast vs reference form
+    double e_err = 0.0, f_err = 0.0;   // fast vs the FP64 path's form
     std::vector<double> er_ref, er_fast, fr_ref, fr_fast;   // each vs extended precision
     void add(double e_rel, double f_rel, double e_ref, double e_fast, double f_ref, double f_fast) {
         n++;
@@ -112,6 +124,43 @@ static bool compare(const D3* x, double ks, Res* r, bool* flag_mismatch) {
     return true;
 }
 
+// closed-form energy of a pair in extended precision
+static long double energy_ext(const D3* x, double ks) {
+    ExvTerms<long double> t;
+    exvol_terms<long double>(x[0], x[1], x[2], x[3], t);
+    return exvol_energy_of<long double>(t, (long double)ks);
+}
+
+// forces of the extended-precision gradient vs -dE/dx by Richardson-extrapolated central
+// differences (step 2^-10 nm and 2^-11 nm, exact in FP64 at these coordinates); relative to the
+// pair's max |F|
+static double gradient_fd_error(const D3* x, double ks) {
+    ExvTerms<long double> t;
+    exvol_terms<long double>(x[0], x[1], x[2], x[3], t);
+    D3 f[4];
+    (void)exvol_forces_from<long double>(t, (long double)ks, f);
+    double sc = 0.0, err = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const double fk[3] = {f[k].x, f[k].y, f[k].z};
+        for (int q = 0; q < 3; q++) {
+            long double dd[2];
+            for (int r = 0; r < 2; r++) {
+                const double h = r ? 0x1p-11 : 0x1p-10;
+                D3 xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+                double* cp = q == 0 ? &xp[k].x : q == 1 ? &xp[k].y : &xp[k].z;
+                double* cm = q == 0 ? &xm[k].x : q == 1 ? &xm[k].y : &xm[k].z;
+                *cp += h;
+                *cm -= h;
+                dd[r] = (energy_ext(xp, ks) - energy_ext(xm, ks)) / (2.0L * h);
+            }
+            const long double g = (4.0L * dd[1] - dd[0]) / 3.0L;
+            sc = std::fmax(sc, std::fabs(fk[q]));
+            err = std::fmax(err, (double)std::fabs((long double)fk[q] + g));
+        }
+    }
+    return err / sc;
+}
+
 int main() {
     const double ks = 1e5 * 108.0 * 108.0;
     Cls well{"tau_ge_0.2"}, mid{"tau_0.01_0.2"}, low{"tau_1e-3_0.01"}, shifted{"coplanar_shifted"};
@@ -157,9 +206,26 @@ int main() {
         n_deg++;
         n_deg_mm += mm;
     }
+    // the gradient formula against the energy's finite differences (tau >= 0.01, outside the
+    // coplanar test)
+    int n_fd = 0;
+    double fd_err = 0.0;
+    while (n_fd < 2000) {
+        x[0] = D3{(double)(int)U(-20, 20), (double)(int)U(-20, 20), (double)(int)U(-20, 20)};
+        const D3 a = rnd_dir();
+        x[1] = add(x[0], a, 108.0);
+        const D3 m2 = add(add(x[0], a, 54.0), rnd_dir(), U(15, 100));
+        const D3 b = rnd_dir();
+        x[2] = add(m2, b, -54.0);
+        x[3] = add(m2, b, 54.0);
+        if (tau_of(x) < 0.01 || are_in_plane(x[0], x[1], x[2], x[3])) continue;
+        fd_err = std::fmax(fd_err, gradient_fd_error(x, ks));
+        n_fd++;
+    }
     n_cases = well.n + mid.n + low.n + shifted.n;
     for (const Cls* c : {&well, &mid, &low, &shifted}) c->print();
     std::printf("flags %d %d\n", n_cases, n_flag_mismatch);
     std::printf("degenerate %d %d\n", n_deg, n_deg_mm);
+    std::printf("gradient_fd %d %.3e\n", n_fd, fd_err);
     return 0;
 }

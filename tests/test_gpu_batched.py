@@ -5,12 +5,17 @@ Bars:
     CGMethod::minimize trajectory (same iteration / energy-call / force-call counts, coordinates
     within rtol 1e-8: only the rounding order of the reductions differs), through the
     safe-mode entries and recovery moves these dense networks start with;
-  full batch: every system stops by the reference criterion without an error; the reported
+  full batch (all 256 networks): every system stops by the reference criterion without an error; the reported
     max|F| is the FP64 max|F| at the returned coordinates (FP64 batch: exactly; mixed batch:
     within 1e-3 pN), the reported final energy is the FP64 energy there, and it does not exceed
     the start energy;
-  dense networks (excluded volume, safe mode, recovery moves): the same checks.
+  dense networks (excluded volume, safe mode, recovery moves): the same checks;
+  queue: with the persistent grid capped below the number of systems (every workgroup pulls
+    several systems in turn, reusing its shared state), results are bit-identical to the
+    uncapped batch and still follow the oracle's trajectory prefixes.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -48,9 +53,15 @@ def test_batched_trajectory_prefix_vs_oracle(gpu, ncalls):
         np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
 
 
+@pytest.fixture(scope="module")
+def batch256(gpu):
+    """The full batch: 256 networks of 20-400 filaments x 5-30 cylinders (about 1.03 M beads)."""
+    return _batch(gpu, 256)
+
+
 @pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
-def test_batched_config3_converges(gpu, prec):
-    nets, ffms = _batch(gpu, 64)
+def test_batched_config3_converges(batch256, prec):
+    nets, ffms = batch256
     p = abi.default_cg_params(precision=prec)
     xs, fs, res = mech.minimize_batched(ffms, p, [n.coord for n in nets])
     lead_sched = mech.last_batch_schedule(ffms[0])
@@ -95,3 +106,43 @@ def test_batched_rejects_bad_input(gpu):
     g.upload(nets[1].desc, nets[1].coord)   # no neighbor lists yet
     with pytest.raises(mech.MdnError):
         mech.minimize_batched([ffms[0], g], p, [nets[0].coord, nets[1].coord])
+
+
+def _capped(fn, groups):
+    os.environ["MDN_BATCH_MAX_WORKGROUPS"] = str(groups)
+    try:
+        return fn()
+    finally:
+        os.environ.pop("MDN_BATCH_MAX_WORKGROUPS", None)
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_batched_queue_multiple_pulls(gpu, prec):
+    """40 systems on 3 workgroups: each workgroup minimizes about 13 systems one after another.
+    Every system's result is the one the uncapped grid (one system per workgroup) gives, bit for
+    bit, and the dense systems among them follow the oracle's trajectory prefix."""
+    nets, ffms = _batch(gpu, 34)
+    dn, dffms = _batch(gpu, 6, dense=True)
+    nets, ffms = nets + dn, ffms + dffms
+    coords = [n.coord for n in nets]
+    p = abi.default_cg_params(precision=prec, energy_call_limit=60)
+    xa, fa, ra = mech.minimize_batched(ffms, p, coords)
+    xb, fb, rb = _capped(lambda: mech.minimize_batched(ffms, p, coords), 3)
+    for i in range(len(nets)):
+        assert (ra[i].num_iter, ra[i].num_energy_call, ra[i].num_force_call) == \
+            (rb[i].num_iter, rb[i].num_energy_call, rb[i].num_force_call), i
+        np.testing.assert_array_equal(xa[i], xb[i])
+        np.testing.assert_array_equal(fa[i], fb[i])
+        assert ra[i].energies_after.total == rb[i].energies_after.total
+    print(f"precision {prec}: 40 systems on 3 workgroups identical to one system per workgroup", flush=True)
+    if prec != abi.MDN_PREC_FP64:
+        return
+    q = abi.default_cg_params(energy_call_limit=22)
+    xc, _, rc = _capped(lambda: mech.minimize_batched(dffms, q, [n.coord for n in dn]), 2)
+    for i, net in enumerate(dn):
+        o = of.OracleSystem(net.desc)
+        o.build_nl(net.coord)
+        st, xo, fo, ro = o.minimize(q, net.coord)
+        assert (rc[i].num_iter, rc[i].num_energy_call, rc[i].num_force_call) == \
+            (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+        np.testing.assert_allclose(xc[i], xo, rtol=1e-8, atol=0)

@@ -59,6 +59,24 @@ def test_forces_per_ff(gpu, cfg):
 
 
 @pytest.mark.parametrize("cfg", [0, 1])
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_crosslink_stretch_forces(gpu, cfg, prec):
+    """assignallforcemags: every linker's and motor's stretchForce = k (L - L0) of the last force
+    evaluation (ForceFieldManager.cpp:347-351, LinkerStretchingHarmonic.cpp:291), bit-identical to
+    the oracle's (the tension is formed in FP64 in both precisions)."""
+    net, g, o, _ = _pair(gpu, cfg)
+    x = net.coord + np.random.default_rng(7).normal(0.0, 3.0, net.coord.size)   # stretched springs
+    g.compute_forces(x, precision=prec)
+    o.forces(x)
+    nl, nm = net.desc.n_linkers, net.desc.n_motors
+    lg, mg = g.stretch_forces(nl, nm)
+    lo, mo = o.stretch_forces_of(nl), o.stretch_forces_of(nm, motor=True)
+    assert np.abs(lo).max() > 0
+    np.testing.assert_array_equal(lg, lo)
+    np.testing.assert_array_equal(mg, mo)
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
 def test_energies(gpu, cfg):
     net, g, o, _ = _pair(gpu, cfg)
     rep = g.compute_individual_energies(net.coord)

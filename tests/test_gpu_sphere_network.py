@@ -1,0 +1,123 @@
+"""GPU: the 420,000-bead network (20,000 filaments x 20 cylinders, 80,000 linkers, 40,000 motors)
+inside a 10 um-radius spherical boundary, at its full size.
+
+Bars:
+  neighbor lists (excluded-volume pairs, boundary interactions with the sphere element): identical
+    to the oracle's cell list, element for element, at the start and after a perturbation;
+  forces per force field: stretching / bending / linker / motor bit-identical to the oracle in
+    FP64; excluded volume and the sphere boundary |dF| <= 1e-12 * max|F|; energies 1e-12 relative;
+    the mixed path's forces within 1e-6 * max|F| of FP64;
+  successive minimizations (seeded N(0, 5 nm) bead noise, neighbor lists rebuilt before each):
+    every minimization stops by the reference criterion without an error, the reported max|F| is
+    the FP64 max|F| at the returned coordinates (FP64: exactly; mixed: within 1e-2 pN), the
+    returned forces are the FP64 ones (FP64: 1e-9 of max|F|; mixed: 1e-2 pN outside the beads of
+    excluded-volume pairs with tau < 1e-3), the reported final energy is the FP64 energy there
+    (1e-12 relative) and not above the start energy.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle_ffi as of
+from picgo_amd import abi, synth
+from tests.exvol_conditioning import well_conditioned
+
+pytestmark = pytest.mark.gpu
+
+FF_EXACT = (abi.MDN_FF_STRETCH, abi.MDN_FF_BEND, abi.MDN_FF_LINKER, abi.MDN_FF_MOTOR)
+
+
+@pytest.fixture(scope="module")
+def sphere(gpu):
+    net = synth.Network.config(4)
+    assert net.desc.n_boundary == 1 and net.desc.boundary[0].kind == abi.MDN_BND_SPHERE
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    n = g.reset_neighbor_lists(net.coord)
+    o = of.OracleSystem(net.desc)
+    o.build_nl(net.coord)
+    print(f"420k-bead sphere network: {n} excluded-volume pairs", flush=True)
+    return net, g, o
+
+
+def _same_lists(g, o, nel):
+    np.testing.assert_array_equal(g.neighbor_pairs(), o.pairs())
+    beads, nn = g.boundary_interactions(nel)
+    np.testing.assert_array_equal(beads, o.boundary_list())
+    np.testing.assert_array_equal(nn, o.boundary_nneighbors(nel))
+    return len(beads)
+
+
+def test_sphere_network_neighbor_lists_identical(sphere):
+    net, g, o = sphere
+    assert o.n_pairs > 50000
+    nb = _same_lists(g, o, net.desc.n_boundary)
+    assert nb > 1000   # the sphere element has neighbours
+    print(f"{o.n_pairs} pairs and {nb} boundary interactions identical", flush=True)
+
+
+def test_sphere_network_forces_and_energies(sphere):
+    net, g, o = sphere
+    x = net.coord
+    for ff in range(abi.MDN_NUM_FF):
+        fg, fo = g.ff_compute_forces(ff, x), o.ff_forces(ff, x)
+        if ff in FF_EXACT:
+            np.testing.assert_array_equal(fg, fo, err_msg=abi.FF_NAMES[ff])
+        else:
+            scale = max(1.0, float(np.abs(fo).max()))
+            err = float(np.abs(fg - fo).max())
+            print(f"{abi.FF_NAMES[ff]}: max|dF| {err:.3e} of {scale:.3e}", flush=True)
+            assert err <= 1e-12 * scale, abi.FF_NAMES[ff]
+    fb = o.ff_forces(abi.MDN_FF_BOUNDARY, x)
+    assert float(np.abs(fb).max()) > 0   # the sphere repels some beads
+    ft, fto = g.compute_forces(x), o.forces(x)
+    scale = float(np.abs(fto).max())
+    assert float(np.abs(ft - fto).max()) <= 1e-12 * scale
+    rep = g.compute_individual_energies(x)
+    _, repo = o.energy(x)
+    for ff in range(abi.MDN_NUM_FF):
+        assert rep.ff[ff] == pytest.approx(repo.ff[ff], rel=1e-12, abs=1e-300), abi.FF_NAMES[ff]
+    assert rep.total == pytest.approx(repo.total, rel=1e-12)
+    fm = g.compute_forces(x, precision=abi.MDN_PREC_MIXED)
+    assert float(np.abs(fm - ft).max()) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_sphere_network_successive_minimizations(gpu, prec):
+    """Three minimizations as the stress workload runs them: seeded 5 nm noise on every bead, the
+    neighbor lists rebuilt, then CGMethod::minimize to the reference criterion."""
+    net = synth.Network.config(4)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    rng = np.random.default_rng(4)
+    x = net.coord.copy()
+    bar = 0.0 if prec == abi.MDN_PREC_FP64 else 1e-2
+    for k in range(3):
+        x = x + rng.normal(0.0, 5.0, x.size)
+        n = g.reset_neighbor_lists(x)
+        if k == 0:   # the rebuilt lists at perturbed coordinates
+            o = of.OracleSystem(net.desc)
+            o.build_nl(x)
+            assert n == o.n_pairs
+            _same_lists(g, o, net.desc.n_boundary)
+        x, f, r = g.minimize(abi.default_cg_params(precision=prec), x)
+        fx = g.compute_forces(x)
+        print(f"precision {prec} minimization {k}: {n} pairs, {r.num_iter} iterations, "
+              f"max|F| {r.grad_inf_norm:.4f} (FP64 {float(np.abs(fx).max()):.4f}), "
+              f"E {r.energies_before.total:.6e} -> {r.energies_after.total:.6e}", flush=True)
+        assert r.error_flags == 0 and not r.call_limit_exceeded, k
+        assert r.converged_grad or r.converged_energy, k
+        assert abs(float(np.abs(fx).max()) - r.grad_inf_norm) <= bar, k
+        dF = np.abs(f - fx)
+        if prec == abi.MDN_PREC_FP64:
+            assert float(dF.max()) <= 1e-9 * float(np.abs(fx).max()), k
+        else:
+            # the returned forces are the mixed path's: within 1e-2 pN of FP64 outside the beads
+            # of excluded-volume pairs with tau < 1e-3, where both FP64 forms of the closed form
+            # carry rounding errors of order eps / tau^2 of their own (test_exvol_fast.py)
+            m = well_conditioned(g, net, x)
+            print(f"  mixed forces: max|dF| {float(dF[m].max()):.3e} pN outside {int((~m).sum())} coordinates "
+                  f"on pairs with tau < 1e-3 (max there {float(dF.max()):.3e})", flush=True)
+            assert float(dF[m].max()) <= 1e-2, k
+        assert r.energies_after.total == pytest.approx(g.compute_energy(x), rel=1e-12), k
+        assert r.energies_after.total <= r.energies_before.total * (1 + 1e-12), k

@@ -61,3 +61,34 @@ def test_theta_gradient_vs_finite_differences(sysx):
             fd = (float(autograd.energy(g, x, tp)) - float(autograd.energy(g, x, tm))) / (2 * h)
             assert abs(fd - float(gt[kind, c])) <= 1e-5 * abs(float(gt[kind, c])) + 1e-6, (kind, c)
     autograd.energy(g, x, th0)   # restore the parameters in the context
+
+
+def test_two_parameter_sets_share_one_backward(sysx):
+    """Two energies of one context at different theta, summed, then one backward pass: each term's
+    dE/dx, dE/dtheta and Hessian-vector product are taken at its own theta (every backward call
+    reinstalls the parameters its forward ran with), identical to separate backward passes."""
+    net, g, x0 = sysx
+    th0 = autograd.theta0(net.desc)
+    th1 = th0 * torch.exp(0.1 * torch.from_numpy(np.random.default_rng(5).standard_normal(tuple(th0.shape))))
+    v = torch.from_numpy(np.random.default_rng(6).standard_normal(x0.size))
+
+    def separate(th):
+        x = torch.tensor(x0, dtype=torch.float64, requires_grad=True)
+        t = th.clone().requires_grad_(True)
+        gx, gt = torch.autograd.grad(autograd.energy(g, x, t), (x, t), create_graph=True)
+        (hv,) = torch.autograd.grad(gx, x, grad_outputs=v)
+        return gx.detach(), gt.detach(), hv
+
+    gx0, gt0, hv0 = separate(th0)
+    gx1, gt1, hv1 = separate(th1)
+    x = torch.tensor(x0, dtype=torch.float64, requires_grad=True)
+    ta, tb = th0.clone().requires_grad_(True), th1.clone().requires_grad_(True)
+    E = autograd.energy(g, x, ta) + autograd.energy(g, x, tb)
+    gx, gta, gtb = torch.autograd.grad(E, (x, ta, tb), create_graph=True)
+    (hv,) = torch.autograd.grad(gx, x, grad_outputs=v)
+    np.testing.assert_array_equal(gta.detach().numpy(), gt0.numpy())
+    np.testing.assert_array_equal(gtb.detach().numpy(), gt1.numpy())
+    np.testing.assert_array_equal(gx.detach().numpy(), (gx0 + gx1).numpy())
+    np.testing.assert_array_equal(hv.numpy(), (hv0 + hv1).numpy())
+    assert float(np.abs(gt1.numpy() - gt0.numpy()).max()) > 0   # the two sets differ
+    autograd.energy(g, torch.tensor(x0), th0)   # restore the parameters in the context

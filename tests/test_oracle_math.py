@@ -116,6 +116,110 @@ def test_exvol_forces_invariants():
                         lambda x: of.exvol_forces(x, [0, 3, 6, 9], [1e5], [108.0, 108.0]), rng)
 
 
+def richardson_gradient(energy, c, h=2.0 ** -8):
+    """dE/dc by Richardson-extrapolated central differences (steps h and h / 2)."""
+    g = np.zeros_like(c)
+    for i in range(c.size):
+        d = []
+        for s in (h, h / 2):
+            cp, cm = c.copy(), c.copy()
+            cp[i] += s
+            cm[i] -= s
+            d.append((energy(cp) - energy(cm)) / (2 * s))
+        g[i] = (4 * d[1] - d[0]) / 3
+    return g
+
+
+def random_pair(rng):
+    """Two 108 nm cylinders with closest approach 15-100 nm (the neighbor list's range)."""
+    a, b, off = (v / np.linalg.norm(v) for v in rng.normal(size=(3, 3)))
+    c1 = rng.uniform(-20, 20, 3)
+    m = c1 + 54 * a + off * rng.uniform(15, 100)
+    return np.concatenate([c1, c1 + 108 * a, m - 54 * b, m + 54 * b])
+
+
+def coplanar_measure(c):
+    """|n.v| of the reference's coplanar test (areInPlane, MathFunctions.h:1101-1135)."""
+    c1, c2, c3, c4 = c.reshape(4, 3)
+    n = np.cross(c2 - c1, c3 - c1)
+    v = c4 - c1
+    return abs(n @ v) / np.linalg.norm(n) / np.linalg.norm(v)
+
+
+def test_exvol_forces_are_the_energy_gradient():
+    """Outside the coplanar test the pair forces are -dE/dx of the closed-form energy, to the
+    accuracy of the differences (a dropped term or a wrong sign in the gradient is off by 1e-3 or
+    more)."""
+    rng = np.random.default_rng(22)
+    bs, kk, eq = [0, 3, 6, 9], [1e5], [108.0, 97.0]
+    n = 0
+    while n < 40:
+        c = random_pair(rng)
+        if coplanar_measure(c) < 0.15:
+            continue
+        f = of.exvol_forces(c, bs, kk, eq)
+        g = richardson_gradient(lambda x: of.exvol_energy(x, bs, kk, eq), c)
+        assert np.abs(f + g).max() <= 1e-7 * np.abs(f).max()
+        n += 1
+
+
+def test_exvol_coplanar_shift_hand_cases():
+    """movePointOutOfPlane (MathFunctions.h:1221-1278): bead 2 (or 1) moves by d along the unit
+    normal of (c2 - c1) x (c3 - c1), whatever the lengths, and the normal's sign follows the order
+    of the two vectors."""
+    c1, c2, c3 = np.zeros(3), np.array([100.0, 0, 0]), np.array([0, 100.0, 0])
+    np.testing.assert_array_equal(of.move_out_of_plane(c1, c2, c3, 2, 0.01), [100.0, 0.0, 0.01])
+    np.testing.assert_array_equal(of.move_out_of_plane(c1, c2, c3, 1, 0.01), [0.0, 0.0, 0.01])
+    np.testing.assert_array_equal(of.move_out_of_plane(c1, np.array([3.0, 0, 0]), np.array([0, 5.0, 0]), 2, 0.01),
+                                  [3.0, 0.0, 0.01])
+    np.testing.assert_array_equal(of.move_out_of_plane(c1, c2, np.array([0, -7.0, 0]), 2, 0.01), [100.0, 0.0, -0.01])
+    # a tilted plane: the shift is perpendicular to both in-plane vectors, of length d
+    rng = np.random.default_rng(23)
+    for _ in range(20):
+        p1, p2, p3 = rng.normal(0, 50, (3, 3))
+        moved = of.move_out_of_plane(p1, p2, p3, 2, 0.01)
+        s = moved - p2
+        assert np.linalg.norm(s) == pytest.approx(0.01, rel=1e-12)
+        assert abs(s @ (p2 - p1)) <= 1e-12 * np.linalg.norm(p2 - p1) and abs(s @ (p3 - p1)) <= 1e-12 * np.linalg.norm(p3 - p1)
+        assert s @ np.cross(p2 - p1, p3 - p1) > 0
+
+
+def test_exvol_coplanar_branch_is_the_gradient_at_the_shifted_geometry():
+    """Inside the coplanar test (|n.v| < 0.1) the reference takes the forces at a geometry with
+    bead 2 moved 0.01 nm out of the plane of (c1, c2, c3) (CylinderExclVolRepulsion.cpp:610-630)
+    and adds them to the original beads. So the oracle's forces there must equal -dE/dx of the
+    closed-form energy evaluated at the shifted geometry (differences in every coordinate of the
+    four beads, bead 2 at its shifted position), and differ from -dE/dx at the original geometry by
+    the shift's effect. Pairs with |n.v| in [0.01, 0.1) keep the energy conditioned well enough
+    for the differences."""
+    rng = np.random.default_rng(24)
+    bs, kk, eq = [0, 3, 6, 9], [1e5], [108.0, 108.0]
+    n = 0
+    while n < 30:
+        c = random_pair(rng)
+        # tilt the second cylinder toward the plane of (c1, c2, c3) until it falls inside the test
+        c1, c2, c3, c4 = c.reshape(4, 3)
+        nrm = np.cross(c2 - c1, c3 - c1)
+        nrm /= np.linalg.norm(nrm)
+        target = rng.uniform(0.01, 0.09)
+        v = c4 - c1
+        v_in = v - (v @ nrm) * nrm
+        c4 = c1 + np.linalg.norm(v) * (v_in / np.linalg.norm(v_in) * np.sqrt(1 - target ** 2) + target * nrm)
+        c = np.concatenate([c1, c2, c3, c4])
+        if not of.are_in_plane(c1, c2, c3, c4) or coplanar_measure(c) < 0.01:
+            continue
+        f = of.exvol_forces(c, bs, kk, eq)
+        shifted = c.copy()
+        shifted[3:6] = of.move_out_of_plane(c1, c2, c3, 2, 0.01)
+        g_shift = richardson_gradient(lambda x: of.exvol_energy(x, bs, kk, eq), shifted)
+        g_orig = richardson_gradient(lambda x: of.exvol_energy(x, bs, kk, eq), c)
+        scale = np.abs(f).max()
+        err_shift = np.abs(f + g_shift).max()
+        assert err_shift <= 1e-6 * scale, (err_shift / scale, coplanar_measure(c))
+        assert np.abs(f + g_orig).max() >= 10 * max(err_shift, 1e-9 * scale)
+        n += 1
+
+
 def test_exvol_parallel_segments_use_simpson():
     """Exactly parallel, coplanar segments make the closed form 0/0; the energy then comes from
     the 400 x 400 Simpson rule (energyN), which must agree with a converged quadrature to the
