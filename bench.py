@@ -1,24 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the mechanical-equilibration hot path on one MI355X.
 
-Workload: the single large network (2,000 filaments x 20 cylinders = 40,000 cylinders /
-42,000 beads in a 5 um cube, 8,000 linkers, 4,000 motors, seeded synthetic coordinates),
-neighbor list built once from the start coordinates, as the reference freezes it per
-minimization.
+Default workload (the stress case): successive minimizations of the 420,000-bead network (20,000
+filaments x 20 cylinders = 400,000 cylinders inside a 10 um-radius sphere, 80,000 linkers, 40,000
+motors). Before each minimization every bead gets seeded N(0, 5 nm) noise and the neighbor lists
+are rebuilt; then CGMethod::minimize runs to the reference criterion (max|F| < 1 pN or
+|dE/E| < 1e-9). A "step" is one such minimization: --steps K times K of them (default 100, the
+workload's count) after --warmup W untimed ones (the sequence then restarts from the seeded start
+state). The coordinates stay in HBM for the whole sequence: each step displaces the device
+coordinates by the seeded noise, generated in place (mdn_perturb), rebuilds the lists from them
+(mdn_nl_build) and minimizes them in place (mdn_minimize with no host buffer); only the result
+statistics cross the host link.
 
-A "step" is one force evaluation (ForceFieldManager::computeForces: six force fields, the
-gather into the 3 x 42,000 force vector) with the coordinates resident in HBM. `value` is the
-mixed-precision path's force-evaluations per second over the K timed steps (FP32 term math for
-the bonded, crosslink and boundary force fields, FP64 excluded volume, FP64 accumulation); the
-FP64 path is timed beside it, and the max force error of the mixed path against FP64 is
-reported. The same JSON line carries the CG minimizations/s of full minimizations from the
-seeded start in both precisions, with the reference criterion (max|F| < 1 pN or |dE/E| < 1e-9)
-and with the force tolerance alone, the roofline of the force evaluation (its two dependent
-launches, the terms kernel and the gather, timed with HIP events on the library's stream) and the
-single-core CPU baseline (the oracle built with the reference's release flags, timed on a
-bounded sample of the same workload).
+`value` = the mixed-precision path's minimizations/s (FP32 term arithmetic, FP64 excluded volume
+and FP64 reductions); the FP64 path runs the same sequence beside it. The line also carries:
+  force_only       the first few minimizations of the sequence with the energy criterion off, so
+                   both paths stop at the same force tolerance (1 pN);
+  roofline         the force evaluation of the 420,000-bead network (the terms kernel and the
+                   gather, back to back, timed with HIP events on the library's stream), with the
+                   HBM traffic of both kernels from the committed rocprofv3 counter summary;
+  roofline_cg_iteration   the algorithmic bytes of a CG iteration over its measured time;
+  cpu_baseline     the oracle (built with the reference's release flags) on one host core: a
+                   bounded sample of the first minimization, scaled to minimizations/s;
+  network_42k      the 42,000-bead network (5 um cube): force evaluations/s and whole
+                   minimizations in both precisions;
+  batch_256        the 256-network batch in the persistent batched kernel.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-min]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
+       python bench.py --workload single|batched|minimize ...   (one part alone, for profiling)
 """
 import argparse
 import json
@@ -33,101 +42,315 @@ sys.path.insert(0, ROOT)
 
 from picgo_amd import abi, mech, synth  # noqa: E402
 
+METRIC = "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol"
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-WORKLOAD = "single network: 2000 filaments x 20 cylinders (42000 beads), 8000 linkers, 4000 motors, 5 um cube"
+STRESS = ("successive minimizations: 20000 filaments x 20 cylinders (420000 beads) in a 10 um-radius sphere, "
+          "80000 linkers, 40000 motors; seeded 5 nm bead noise and a neighbor-list rebuild before each")
+SINGLE = "single network: 2000 filaments x 20 cylinders (42000 beads), 8000 linkers, 4000 motors, 5 um cube"
+BATCH = "batch: 256 networks, 20-400 filaments x 5-30 cylinders, 2 linkers + 1 motor per filament"
+DTYPE = "f32 term math + f64 excluded volume / reductions"
+
+TABLES = {"FilamentStretching": (2, 2), "FilamentBending": (3, 2), "LinkerStretching": (4, 4),
+          "MotorGhostStretching": (4, 4), "CylinderExclVolume": (4, 3), "BoundaryCylinderRepulsion": (1, 2)}
 
 
-def algorithmic_bytes(counts, n_beads, sx=8, sf=8, sp=8, idx=4):
-    """Bytes a force evaluation must move at minimum: coordinates in, forces out, each
-    interaction table read once: (bead indices, parameters) = stretch (2,2), bend (3,2),
-    linker / motor (4,4), excluded volume (4,3), boundary (1,2)."""
-    per = {"FilamentStretching": (2, 2), "FilamentBending": (3, 2), "LinkerStretching": (4, 4),
-           "MotorGhostStretching": (4, 4), "CylinderExclVolume": (4, 3), "BoundaryCylinderRepulsion": (1, 2)}
-    b = 3 * n_beads * (sx + sf)
-    for name, (ni, npar) in per.items():
-        b += counts[name] * (idx * ni + sp * npar)
-    return b
+def table_bytes(counts, idx=4, sp=8):
+    """Each interaction table read once: (bead indices, parameters) per interaction."""
+    return sum(counts[name] * (idx * ni + sp * npar) for name, (ni, npar) in TABLES.items())
 
 
-def cpu_baseline(net, target_s=15.0):
-    """Oracle (liboracle_fast.so: the reference's release flags) on one core: force
-    evaluations of the same 42,000-bead network for about target_s seconds."""
-    from oracle import oracle_ffi as of
-    o = of.OracleSystem(net.desc, fast=True)
-    o.build_nl(net.coord)
-    o.forces(net.coord)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        o.forces(net.coord)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= target_s:
-            break
-    return {"value": n / el, "unit": "force-evals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} force evaluations of the 42,000-bead network ({el:.1f} s, oracle built "
-                      f"-O2 -funroll-loops -flto -ffast-math, single thread)"}
+def force_eval_bytes(counts, n_beads):
+    """Algorithmic bytes of a force evaluation: coordinates in, forces out, the tables."""
+    return 3 * n_beads * (8 + 8) + table_bytes(counts)
 
 
-def load_traffic(unit):
-    p = os.path.join(ROOT, "profiles", f"traffic_{unit}_42k_bead_network.json")
+def energy_eval_bytes(counts, n_beads):
+    return 3 * n_beads * 8 + table_bytes(counts)
+
+
+def cg_iteration_bytes(counts, n_beads, energies_per_iter):
+    """A CG iteration: one force evaluation, its energy evaluations, and the vector passes
+    (read x, d, F, F_prev; write x, d, F_prev)."""
+    return (force_eval_bytes(counts, n_beads) + energies_per_iter * energy_eval_bytes(counts, n_beads)
+            + 7 * 3 * n_beads * 8)
+
+
+def load_traffic(unit, net_name):
+    p = os.path.join(ROOT, "profiles", f"traffic_{unit}_{net_name}.json")
     if os.path.exists(p):
         with open(p) as fh:
             return json.load(fh).get("hbm_bytes_per_launch")
     return None
 
 
-STRESS = "successive minimizations: 20000 filaments x 20 cylinders (420000 beads), 80000 linkers, 40000 motors, 10 um sphere"
-BATCH = "batch: 256 networks, 20-400 filaments x 5-30 cylinders, 2 linkers + 1 motor per filament"
+def criterion(res):
+    return "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit")
 
 
-def run_stress(args):
-    """Successive minimizations of the 420,000-bead network: before each one the beads get seeded
-    N(0, 5 nm) noise and the neighbor lists are rebuilt; the same noise sequence for both
-    precisions, drawn before the timed region (it is the workload's input). Everything between
-    the first perturbation and the last result is timed: applying the noise, the neighbor-list
-    rebuild, the minimization with its start and end energy reports, the coordinates back."""
-    net = synth.Network.config(4)
-    ffm = mech.ForceFieldManager(0)
-    ffm.upload(net.desc, net.coord)
-    ffm.reset_neighbor_lists(net.coord)
-    rng = np.random.default_rng(4)
-    noise = [rng.normal(0.0, 5.0, net.coord.size) for _ in range(args.stress_count)]
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------------------------
+# the stress sequence (default)
+# ---------------------------------------------------------------------------------------------
+
+NOISE_SEED, NOISE_SIGMA = 4, 5.0
+
+
+def run_sequence(ffm, net, steps, params):
+    """The first `steps` minimizations of the sequence from the seeded start state, with the
+    coordinates resident in HBM: minimization k starts from the previous result displaced by the
+    seeded N(0, 5 nm) noise of stream k (generated in place on the device, mdn_perturb), with the
+    neighbor lists rebuilt there. Returns per-minimization statistics and the wall times."""
+    ffm.set_coords(net.coord)
+    stats, t_nl = [], 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ffm.perturb(NOISE_SEED, k, NOISE_SIGMA)
+        t1 = time.perf_counter()
+        n_pairs = ffm.reset_neighbor_lists(None)
+        t_nl += time.perf_counter() - t1
+        _, _, res = ffm.minimize(params, None, want_forces=False)
+        if res.error_flags or not (res.converged_grad or res.converged_energy or res.call_limit_exceeded):
+            raise RuntimeError(f"minimization {k} failed: flags {res.error_flags}")
+        stats.append((res.num_iter, res.num_energy_call, res.num_force_call, criterion(res), n_pairs,
+                      res.grad_inf_norm, res.energies_after.total))
+    total = time.perf_counter() - t0
+    return stats, total, t_nl
+
+
+def summarize(stats, total, t_nl, steps):
+    iters = [s[0] for s in stats]
+    crit = {"grad": 0, "energy": 0, "limit": 0}
+    for s in stats:
+        crit[s[3]] += 1
+    return {"minimizations_per_s": steps / total, "total_s": total, "ms_per_minimization": 1e3 * total / steps,
+            "nl_rebuild_ms_mean": 1e3 * t_nl / steps, "cg_iterations_total": int(sum(iters)),
+            "cg_iterations_mean": float(np.mean(iters)), "cg_iterations_median": float(np.median(iters)),
+            "energy_calls_total": int(sum(s[1] for s in stats)), "force_calls_total": int(sum(s[2] for s in stats)),
+            "us_per_cg_iteration": 1e6 * (total - t_nl) / max(1, sum(iters)), "criteria": crit,
+            "exvol_pairs_last": int(stats[-1][4]), "final_max_abs_force_pN": float(stats[-1][5]),
+            "final_energy_fp64": float(stats[-1][6])}
+
+
+def stress_part(args, net, ffm):
     out = {}
     for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
         p = abi.default_cg_params(precision=prec)
-        ffm.minimize(abi.default_cg_params(precision=prec, energy_call_limit=20), net.coord)   # warm
-        x = net.coord.copy()
-        iters, crit, t_nl, pairs = [], {"grad": 0, "energy": 0, "limit": 0}, 0.0, []
-        t0 = time.perf_counter()
-        for k in range(args.stress_count):
-            x = x + noise[k]
+        if args.warmup > 0:   # untimed: the first W minimizations of the sequence
+            run_sequence(ffm, net, args.warmup, p)
+        stats, total, t_nl = run_sequence(ffm, net, args.steps, p)
+        out[name] = summarize(stats, total, t_nl, args.steps)
+        log(f"stress {name}: {args.steps} minimizations in {total:.3f} s ({out[name]['minimizations_per_s']:.2f}/s), "
+            f"{out[name]['cg_iterations_total']} CG iterations, criteria {out[name]['criteria']}")
+    fo = {}
+    if args.force_only > 0:
+        for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+            p = abi.default_cg_params(precision=prec, energy_rel_tol=0.0)
+            stats, total, t_nl = run_sequence(ffm, net, args.force_only, p)
+            fo[name] = summarize(stats, total, t_nl, args.force_only)
+            log(f"force-only {name}: {args.force_only} minimizations in {total:.3f} s, "
+                f"{fo[name]['cg_iterations_total']} CG iterations, criteria {fo[name]['criteria']}")
+        fo["mixed_vs_fp64_minimizations"] = fo["mixed"]["minimizations_per_s"] / fo["fp64"]["minimizations_per_s"]
+        fo["minimizations"] = args.force_only
+    return out, fo
+
+
+def force_evals(ffm, steps, warmup):
+    ev = {}
+    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+        ffm.bench_forces(max(1, warmup), prec)
+        ms_eval, ms_kernel = ffm.bench_forces(steps, prec)
+        ev[name] = {"force_evals_per_s": 1000.0 / ms_eval, "us_per_force_eval": 1e3 * ms_eval,
+                    "us_terms_kernel": 1e3 * ms_kernel}
+    ev["mixed_vs_fp64_force_evals"] = ev["mixed"]["force_evals_per_s"] / ev["fp64"]["force_evals_per_s"]
+    return ev
+
+
+def cpu_baseline_stress(net, noise_host0, fp64_iters_mean, target_s):
+    """The oracle (liboracle_fast.so, the reference's release flags) on one core: the first
+    minimization of the sequence, bounded by an energy-call limit sized to about target_s, then
+    scaled to whole minimizations with the FP64 GPU path's mean iteration count (the same
+    algorithm to the same criterion)."""
+    from oracle import oracle_ffi as of
+    o = of.OracleSystem(net.desc, fast=True)
+    x = net.coord + noise_host0
+    o.build_nl(x)
+    t0 = time.perf_counter()
+    o.forces(x)
+    t_force = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.energy(x)
+    t_energy = time.perf_counter() - t0
+    limit = max(10, int(target_s / (t_force + 2.0 * t_energy) * 2.0))
+    t0 = time.perf_counter()
+    st, _, _, res = o.minimize(abi.default_cg_params(energy_call_limit=limit), x)
+    el = time.perf_counter() - t0
+    s_per_iter = el / max(1, res.num_iter)
+    return {"value": 1.0 / (s_per_iter * fp64_iters_mean), "unit": "minimizations/s", "cores": 1, "kind": "port",
+            "sample": (f"first minimization of the 420,000-bead sequence stopped after {res.num_energy_call} energy "
+                       f"calls ({res.num_iter} CG iterations, {el:.1f} s, single thread, oracle built -O2 "
+                       f"-funroll-loops -flto -ffast-math): {1e3 * s_per_iter:.1f} ms per CG iteration, scaled by "
+                       f"the {fp64_iters_mean:.0f} iterations per minimization of the FP64 GPU path"),
+            "force_evals_per_s": 1.0 / t_force, "energy_evals_per_s": 1.0 / t_energy,
+            "host_nproc": os.cpu_count()}
+
+
+# ---------------------------------------------------------------------------------------------
+# the 42,000-bead network and the 256-network batch
+# ---------------------------------------------------------------------------------------------
+
+def single_network(args):
+    net = synth.Network.config(1)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    n_pairs = ffm.reset_neighbor_lists(net.coord)
+    counts = ffm.counts()
+    ev = force_evals(ffm, 200, 20)
+    f64 = ffm.compute_forces(net.coord)
+    f32 = ffm.compute_forces(net.coord, precision=abi.MDN_PREC_MIXED)
+    out = {"workload": SINGLE, "n_beads": net.n_beads, "exvol_pairs": n_pairs, "force_eval": ev,
+           "mixed_max_rel_force_error": float(np.abs(f32 - f64).max() / np.abs(f64).max())}
+    B = force_eval_bytes(counts, net.n_beads)
+    out["force_eval_roofline_frac"] = B / (ev["mixed"]["us_per_force_eval"] * 1e-6) / 1e9 / HBM_PEAK_GBS
+    mins = {}
+    for crit, ert in (("reference", 1e-9), ("force_only", 0.0)):
+        for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+            ffm.minimize(abi.default_cg_params(precision=prec, energy_call_limit=50), net.coord)   # warm
             t1 = time.perf_counter()
-            pairs.append(ffm.reset_neighbor_lists(x))
-            t_nl += time.perf_counter() - t1
-            x, f, res = ffm.minimize(p, x, want_forces=False)
-            iters.append(res.num_iter)
-            crit["grad" if res.converged_grad else ("energy" if res.converged_energy else "limit")] += 1
-            if k % 10 == 0:
-                print(f"# stress {name}: minimization {k} {res.num_iter} iterations, "
-                      f"{time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
-        total = time.perf_counter() - t0
-        out[name] = {"minimizations_per_s": args.stress_count / total, "total_s": total,
-                     "nl_rebuild_ms_mean": 1e3 * t_nl / args.stress_count, "cg_iterations_total": int(sum(iters)),
-                     "cg_iterations_first": iters[0], "cg_iterations_median": float(np.median(iters)),
-                     "us_per_cg_iteration": 1e6 * (total - t_nl) / max(1, sum(iters)), "criteria": crit,
-                     "exvol_pairs_last": int(pairs[-1]), "final_energy_fp64": res.energies_after.total}
-    line = {
-        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
-        "value": out["mixed"]["minimizations_per_s"], "unit": "minimizations/s",
-        "n_gpus": 1, "steps": args.stress_count, "warmup": 1,
-        "ms_per_step": 1e3 * out["mixed"]["total_s"] / args.stress_count,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator; seeded N(0, 5 nm) perturbations between minimizations)",
-        "config": {"workload": STRESS, "n_beads": net.n_beads, "minimizations": args.stress_count},
-        "mixed": out["mixed"], "fp64": out["fp64"],
-        "mixed_vs_fp64_minimizations": out["mixed"]["minimizations_per_s"] / out["fp64"]["minimizations_per_s"],
-    }
+            x, f, res = ffm.minimize(abi.default_cg_params(precision=prec, energy_rel_tol=ert), net.coord)
+            tmin = time.perf_counter() - t1
+            mins[f"{name}_{crit}"] = {
+                "minimizations_per_s": 1.0 / tmin, "us_per_cg_iteration": 1e6 * tmin / max(1, res.num_iter),
+                "cg_iterations": res.num_iter, "energy_calls": res.num_energy_call, "criterion": criterion(res),
+                "final_max_abs_force_pN": res.grad_inf_norm,
+                "fp64_max_abs_force_at_result_pN": float(np.abs(ffm.compute_forces(x)).max())}
+    out["minimization"] = mins
+    out["mixed_vs_fp64_minimizations"] = (mins["mixed_reference"]["minimizations_per_s"] /
+                                          mins["fp64_reference"]["minimizations_per_s"])
+    out["mixed_vs_fp64_minimizations_force_only"] = (mins["mixed_force_only"]["minimizations_per_s"] /
+                                                     mins["fp64_force_only"]["minimizations_per_s"])
+    log(f"42k network: force eval {ev['mixed']['us_per_force_eval']:.1f} us mixed / "
+        f"{ev['fp64']['us_per_force_eval']:.1f} us FP64")
+    ffm.close()
+    return out
+
+
+def batch_256(args):
+    nets = [synth.Network.config(3, index=i) for i in range(args.batch_count)]
+    ffms = []
+    for n in nets:
+        g = mech.ForceFieldManager(0)
+        g.upload(n.desc, n.coord)
+        g.reset_neighbor_lists(n.coord)
+        ffms.append(g)
+    coords = [n.coord for n in nets]
+    out = {"workload": BATCH, "systems": len(nets), "n_beads_total": int(sum(n.n_beads for n in nets))}
+    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+        p = abi.default_cg_params(precision=prec)
+        mech.minimize_batched(ffms, p, coords)   # schedule tuning + warm-up
+        t0 = time.perf_counter()
+        xs, fs, res = mech.minimize_batched(ffms, p, coords, want_forces=False)
+        dt = time.perf_counter() - t0
+        nt, srt, _, tune = mech.last_batch_schedule(ffms[0])
+        big = int(np.argmax([n.n_beads for n in nets]))
+        t1 = time.perf_counter()
+        mech.minimize_batched([ffms[big]], p, [coords[big]], want_forces=False)
+        solo = time.perf_counter() - t1
+        out[name] = {"minimizations_per_s": len(nets) / dt, "batch_ms": 1e3 * dt,
+                     "largest_network_alone_ms": 1e3 * solo, "batch_over_largest_alone": dt / solo,
+                     "converged": sum(1 for r in res if r.converged_grad or r.converged_energy),
+                     "cg_iterations_max": max(r.num_iter for r in res),
+                     "schedule": {"workgroup": nt, "largest_first": srt, "tuning_ms": tune}}
+        log(f"batch {name}: {dt * 1e3:.1f} ms (largest network alone {solo * 1e3:.1f} ms)")
+    p = abi.default_cg_params()
+    t0 = time.perf_counter()
+    for g, c in zip(ffms, coords):
+        g.minimize(p, c, want_forces=False)
+    dseq = time.perf_counter() - t0
+    out["sequential_fp64_minimizations_per_s"] = len(nets) / dseq
+    out["batched_mixed_vs_sequential_fp64"] = out["mixed"]["minimizations_per_s"] / out["sequential_fp64_minimizations_per_s"]
+    for g in ffms:
+        g.close()
+    return out
+
+
+def base_line(value, unit, steps, warmup, ms_per_step, config):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": DTYPE, "data": "synthetic (seeded generator)", "config": config}
+
+
+def run_stress(args):
+    net = synth.Network.config(4)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    n_pairs0 = ffm.reset_neighbor_lists(net.coord)
+    counts = ffm.counts()
+    log(f"420k network: {n_pairs0} excluded-volume pairs at the start; counts {counts}")
+    # the force evaluation at the start state (its roofline)
+    ev = force_evals(ffm, 200, 20)
+    log(f"force eval {ev['mixed']['us_per_force_eval']:.1f} us mixed / {ev['fp64']['us_per_force_eval']:.1f} us FP64")
+    seq, fo = stress_part(args, net, ffm)
+    mixed, fp64 = seq["mixed"], seq["fp64"]
+
+    B_F = force_eval_bytes(counts, net.n_beads)
+    achieved = B_F / (ev["mixed"]["us_per_force_eval"] * 1e-6) / 1e9
+    n_e = mixed["energy_calls_total"] / max(1, mixed["cg_iterations_total"])
+    B_it = cg_iteration_bytes(counts, net.n_beads, n_e)
+    ach_it = B_it / (mixed["us_per_cg_iteration"] * 1e-6) / 1e9
+    line = base_line(mixed["minimizations_per_s"], "minimizations/s", args.steps, args.warmup,
+                     mixed["ms_per_minimization"],
+                     {"workload": f"{STRESS}; {args.steps} minimizations timed", "n_beads": net.n_beads,
+                      "n_cylinders": net.n_cylinders, "minimizations": args.steps, "precision_path": "mixed"})
+    line["data"] = "synthetic (seeded generator; seeded N(0, 5 nm) bead noise generated in place in HBM)"
+    line["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": load_traffic("force_eval_mixed", "420k_bead_network"),
+                        "kernel": "force evaluation of the 420,000-bead network: k_forces_terms<float> + "
+                                  "k_gather_plain<float, 128, true>",
+                        "algorithmic_bytes": B_F}
+    line["roofline_cg_iteration"] = {"bound": "hbm", "achieved": ach_it, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": ach_it / HBM_PEAK_GBS, "algorithmic_bytes": B_it,
+                                     "energy_evals_per_iteration": n_e,
+                                     "us_per_cg_iteration": mixed["us_per_cg_iteration"]}
+    line["mixed"], line["fp64"] = mixed, fp64
+    line["mixed_vs_fp64_minimizations"] = mixed["minimizations_per_s"] / fp64["minimizations_per_s"]
+    line["mixed_vs_fp64_us_per_cg_iteration"] = fp64["us_per_cg_iteration"] / mixed["us_per_cg_iteration"]
+    if fo:
+        line["force_only"] = fo
+    line["force_eval"] = ev
+    f64 = ffm.compute_forces(net.coord)
+    f32 = ffm.compute_forces(net.coord, precision=abi.MDN_PREC_MIXED)
+    line["mixed_max_rel_force_error"] = float(np.abs(f32 - f64).max() / np.abs(f64).max())
+    ffm.close()
+    if not args.no_extras:
+        line["network_42k"] = single_network(args)
+        line["batch_256"] = batch_256(args)
+    if not args.no_cpu:
+        first = mech.perturbation(NOISE_SEED, 0, NOISE_SIGMA, 3 * net.n_beads)
+        line["cpu_baseline"] = cpu_baseline_stress(net, first, fp64["cg_iterations_mean"], args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------------------------------------
+# single parts (profiling)
+# ---------------------------------------------------------------------------------------------
+
+def run_single(args):
+    out = single_network(args)
+    line = base_line(out["force_eval"]["mixed"]["force_evals_per_s"], "force-evals/s", 200, 20,
+                     out["force_eval"]["mixed"]["us_per_force_eval"] / 1e3, {"workload": SINGLE})
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+def run_batched(args):
+    out = batch_256(args)
+    line = base_line(out["mixed"]["minimizations_per_s"], "minimizations/s", 1, 1, out["mixed"]["batch_ms"],
+                     {"workload": BATCH})
+    line.update(out)
     print(json.dumps(line), flush=True)
 
 
@@ -144,81 +367,41 @@ def run_minimize(args):
     x, f, res = ffm.minimize(abi.default_cg_params(precision=prec), net.coord, want_forces=False)
     dt = time.perf_counter() - t0
     st = ffm.last_min_stats()
-    line = {
-        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
-        "value": 1.0 / dt, "unit": "minimizations/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": 1e3 * dt, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if prec == abi.MDN_PREC_FP64 else "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": f"one minimization of the {net.n_beads}-bead network", "n_beads": net.n_beads},
-        "cg_iterations": res.num_iter, "us_per_cg_iteration": 1e6 * dt / max(1, res.num_iter),
-        "line_search_launches": st["ls_launches"],
-        "criterion": "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit"),
-    }
+    line = base_line(1.0 / dt, "minimizations/s", 1, 1, 1e3 * dt,
+                     {"workload": f"one minimization of the {net.n_beads}-bead network", "n_beads": net.n_beads})
+    line.update({"cg_iterations": res.num_iter, "us_per_cg_iteration": 1e6 * dt / max(1, res.num_iter),
+                 "line_search_launches": st["ls_launches"], "criterion": criterion(res)})
     print(json.dumps(line), flush=True)
 
 
-def run_batched(args):
-    """The 256-network batch in one persistent kernel (mixed and FP64), against the same systems
-    minimized one after another by the single-system minimizer (FP64)."""
-    nets = [synth.Network.config(3, index=i) for i in range(args.batch_count)]
-    ffms = []
-    for n in nets:
-        g = mech.ForceFieldManager(0)
-        g.upload(n.desc, n.coord)
-        g.reset_neighbor_lists(n.coord)
-        ffms.append(g)
-    coords = [n.coord for n in nets]
-    out = {}
-    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
-        p = abi.default_cg_params(precision=prec)
-        mech.minimize_batched(ffms, p, coords)   # schedule tuning + warm-up
-        t0 = time.perf_counter()
-        xs, fs, res = mech.minimize_batched(ffms, p, coords)
-        dt = time.perf_counter() - t0
-        nt, srt, _, tune = mech.last_batch_schedule(ffms[0])
-        out[name] = {"minimizations_per_s": len(nets) / dt, "batch_s": dt,
-                     "converged": sum(1 for r in res if r.converged_grad or r.converged_energy),
-                     "cg_iterations_max": max(r.num_iter for r in res),
-                     "cg_iterations_total": int(sum(r.num_iter for r in res)),
-                     "schedule": {"workgroup": nt, "largest_first": srt, "tuning_ms": tune}}
-        print(f"# batched {name}: {dt * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    p = abi.default_cg_params()
-    t0 = time.perf_counter()
-    for g, c in zip(ffms, coords):
-        g.minimize(p, c)
-    dseq = time.perf_counter() - t0
-    out["sequential_fp64"] = {"minimizations_per_s": len(nets) / dseq, "total_s": dseq}
-    line = {
-        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
-        "value": out["mixed"]["minimizations_per_s"], "unit": "minimizations/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 * out["mixed"]["batch_s"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": BATCH, "systems": len(nets), "n_beads_total": int(sum(n.n_beads for n in nets))},
-        "batched_mixed": out["mixed"], "batched_fp64": out["fp64"], "sequential_fp64": out["sequential_fp64"],
-        "batched_mixed_vs_sequential_fp64": out["mixed"]["minimizations_per_s"] / out["sequential_fp64"]["minimizations_per_s"],
-    }
-    print(json.dumps(line), flush=True)
+def run_force_evals(args):
+    """Force evaluations of the --net network in --prec precision (the unit of the force-kernel
+    profiles): 20 warm-up evaluations, then --steps timed ones."""
+    net = synth.Network.config(args.net)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    ffm.reset_neighbor_lists(net.coord)
+    prec = abi.MDN_PREC_MIXED if args.prec == "mixed" else abi.MDN_PREC_FP64
+    ffm.bench_forces(20, prec)
+    ms, ms_k = ffm.bench_forces(args.steps, prec)
+    print(json.dumps({"workload": f"force evaluations of the {net.n_beads}-bead network", "precision": args.prec,
+                      "us_per_force_eval": 1e3 * ms, "us_terms_kernel": 1e3 * ms_k, "evaluations": args.steps}),
+          flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100, help="minimizations of the stress sequence timed")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed minimizations before the timed sequence")
+    ap.add_argument("--force-only", type=int, default=3,
+                    help="minimizations of the sequence also run with the energy criterion off")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-min", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the 42,000-bead network and the 256-network batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["single", "stress", "batched", "minimize"], default="single",
-                    help="single: the metric's single network (default); stress: 20,000 x 20 filaments in a "
-                         "10 um sphere, successive minimizations with 5 nm noise and a neighbor-list rebuild "
-                         "each; batched: 256 uneven networks in one persistent kernel; minimize: one "
-                         "minimization of the --net network in --prec precision (for profiling)")
-    ap.add_argument("--net", type=int, choices=[1, 4], default=1)
-    ap.add_argument("--prec", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--stress-count", type=int, default=100)
+    ap.add_argument("--workload", choices=["stress", "single", "batched", "minimize", "forces"], default="stress")
+    ap.add_argument("--net", type=int, choices=[1, 4], default=4)
+    ap.add_argument("--prec", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run 20 device copies of 64 MiB (FETCH_SIZE / WRITE_SIZE calibration)")
@@ -236,96 +419,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "stress":
-        return run_stress(args)
-    if args.workload == "batched":
-        return run_batched(args)
-    if args.workload == "minimize":
-        return run_minimize(args)
-
-    net = synth.Network.config(1)
-    ffm = mech.ForceFieldManager(0)
-    ffm.upload(net.desc, net.coord)
-    n_pairs = ffm.reset_neighbor_lists(net.coord)
-    counts = ffm.counts()
-    print(f"# {WORKLOAD}; {n_pairs} excluded-volume pairs; counts {counts}", file=sys.stderr, flush=True)
-    bytes_alg = algorithmic_bytes(counts, net.n_beads)
-
-    # force evaluations: warm-up, then exactly K timed evaluations per precision (the library
-    # times them with HIP events on its stream and synchronizes at the end)
-    evals = {}
-    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
-        ffm.bench_forces(max(1, args.warmup), prec)
-        t0 = time.perf_counter()
-        ms_eval, ms_kernel = ffm.bench_forces(args.steps, prec)
-        wall = time.perf_counter() - t0
-        evals[name] = {"force_evals_per_s": 1000.0 / ms_eval, "us_per_force_eval": 1e3 * ms_eval,
-                       "us_terms_kernel": 1e3 * ms_kernel, "wall_s": wall}
-        print(f"# {name}: force eval {ms_eval * 1e3:.1f} us (terms kernel {ms_kernel * 1e3:.1f} us), "
-              f"{args.steps} steps in {wall:.3f} s wall", file=sys.stderr, flush=True)
-    # max force error of the mixed path against FP64 at the same coordinates
-    f64 = ffm.compute_forces(net.coord)
-    f32 = ffm.compute_forces(net.coord, precision=abi.MDN_PREC_MIXED)
-    max_rel_err = float(np.abs(f32 - f64).max() / np.abs(f64).max())
-
-    mins = {}
-    if not args.no_min:
-        # the reference criterion (max|F| < 1 pN or |dE/E| < 1e-9), and the force tolerance alone
-        for crit, ert in (("reference", 1e-9), ("force_only", 0.0)):
-            for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
-                p = abi.default_cg_params(precision=prec, energy_rel_tol=ert)
-                ffm.minimize(abi.default_cg_params(precision=prec, energy_call_limit=50), net.coord)   # warm
-                t1 = time.perf_counter()
-                x, f, res = ffm.minimize(p, net.coord)
-                tmin = time.perf_counter() - t1
-                fx = ffm.compute_forces(x)   # FP64 forces at the result
-                mins[f"{name}_{crit}"] = {
-                    "minimizations_per_s": 1.0 / tmin, "minimization_s": tmin,
-                    "us_per_cg_iteration": 1e6 * tmin / max(1, res.num_iter),
-                    "cg_iterations": res.num_iter, "energy_calls": res.num_energy_call,
-                    "force_calls": res.num_force_call, "safe_mode": res.num_safe_mode,
-                    "criterion": "grad" if res.converged_grad else ("energy" if res.converged_energy else "limit"),
-                    "final_max_abs_force_pN": res.grad_inf_norm,
-                    "fp64_max_abs_force_at_result_pN": float(np.abs(fx).max()),
-                    "final_energy_fp64": res.energies_after.total}
-                print(f"# minimization {name} ({crit}): {tmin:.3f} s, {res.num_iter} iterations, "
-                      f"criterion {mins[f'{name}_{crit}']['criterion']}", file=sys.stderr, flush=True)
-
-    mixed, fp64 = evals["mixed"], evals["fp64"]
-    # the roofline unit is one force evaluation (the algorithmic bytes are per evaluation): the
-    # terms kernel (slots of the crosslinks, excluded volume and boundary) and the gather (which
-    # recomputes the filament terms' forces and sums every bead's forces), back to back
-    achieved = bytes_alg / (mixed["us_per_force_eval"] * 1e-6) / 1e9
-    line = {
-        "metric": "force-evals/sec & CG minimizations/sec, mixed vs FP64 GPU, same force tol",
-        "value": mixed["force_evals_per_s"],
-        "unit": "force-evals/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": mixed["us_per_force_eval"] / 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 term math + f64 excluded volume / accumulation",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": WORKLOAD, "n_beads": net.n_beads, "n_cylinders": net.n_cylinders,
-                   "exvol_pairs": n_pairs, "precision_path": "mixed"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic("force_eval_mixed"),
-                     "kernel": "force evaluation: k_forces_terms<float> + k_gather_plain<float, 64, true>",
-                     "algorithmic_bytes": bytes_alg},
-        "mixed": mixed, "fp64": fp64,
-        "mixed_vs_fp64_force_evals": mixed["force_evals_per_s"] / fp64["force_evals_per_s"],
-        "mixed_max_rel_force_error": max_rel_err,
-    }
-    if mins:
-        line["minimization"] = mins
-        line["minimizations_per_s"] = mins["mixed_reference"]["minimizations_per_s"]
-        line["mixed_vs_fp64_minimizations"] = (mins["mixed_reference"]["minimizations_per_s"] /
-                                               mins["fp64_reference"]["minimizations_per_s"])
-        line["mixed_vs_fp64_minimizations_force_only"] = (mins["mixed_force_only"]["minimizations_per_s"] /
-                                                          mins["fp64_force_only"]["minimizations_per_s"])
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(net, args.cpu_seconds)
-        line["cpu_baseline"]["host_nproc"] = os.cpu_count()
-    print(json.dumps(line), flush=True)
+    if args.steps < 1:
+        sys.exit("--steps must be at least 1")
+    {"stress": run_stress, "single": run_single, "batched": run_batched, "minimize": run_minimize,
+     "forces": run_force_evals}[args.workload](args)
 
 
 if __name__ == "__main__":

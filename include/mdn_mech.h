@@ -238,7 +238,10 @@ int mdn_boundary_load_forces(mdn_ctx* ctx, const double* coord, int32_t n_types,
 /* CGMethod::minimize (CGMethod.hpp:826-1759) driven by MController::run
  * (MController.h:108-211): minimizes from `coord` (host, in/out), writes the final forces to
  * force_out (3*n_beads, may be NULL) and the statistics to res. The neighbor list must have
- * been built (mdn_nl_build) and stays frozen during the minimization, as in the reference. */
+ * been built (mdn_nl_build) and stays frozen during the minimization, as in the reference.
+ * coord may be NULL: the minimization then starts from the context's current device
+ * coordinates (the last upload, mdn_set_coords, mdn_displace or minimization result) and leaves
+ * its result there, with no coordinate transfer over the host link (mdn_get_coords reads it). */
 int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* params, double* coord, double* force_out,
                  mdn_min_result* res);
 
@@ -263,6 +266,20 @@ int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
 int mdn_set_coords(mdn_ctx* ctx, const double* coord);
 int mdn_get_coords(mdn_ctx* ctx, double* coord);
 
+/* Displace the current device coordinates in place: x += delta, with delta a DEVICE buffer of
+ * n_var = 3 * n_beads doubles on the context's device (the caller orders its writes of delta
+ * before the call). The perturbation between successive minimizations (a seeded Gaussian bead
+ * displacement) applied without a coordinate round trip over the host link; the neighbor lists
+ * are not rebuilt (mdn_nl_build with coord NULL rebuilds them at the displaced coordinates). */
+int mdn_displace(mdn_ctx* ctx, const double* delta, int32_t n_var);
+
+/* Displace the current device coordinates by seeded Gaussian noise, x_i += sigma z_i with z_i
+ * standard normal, generated on the device from (seed, stream, i) alone (SplitMix64 counters,
+ * Box-Muller; picgo_amd.mech.perturbation reproduces it on the host): the seeded bead
+ * perturbation between successive minimizations, with no buffer and no host transfer. The
+ * neighbor lists are not rebuilt. */
+int mdn_perturb(mdn_ctx* ctx, uint64_t seed, uint64_t stream, double sigma);
+
 /* Mixed precision (MDN_PREC_MIXED): the bending and boundary terms and the force components of
  * the harmonic springs (stretching, linkers, motors) are computed in FP32 from bond vectors and
  * coordinate differences formed in FP64 (absolute coordinates never enter FP32 arithmetic; spring
@@ -278,6 +295,7 @@ int mdn_get_coords(mdn_ctx* ctx, double* coord);
  * energies_before / energies_after stay FP64). */
 int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
 int mdn_energy_mixed(mdn_ctx* ctx, const double* coord, mdn_energy_report* out);
+
 
 /* ---- differentiation ---------------------------------------------------------------------- */
 

@@ -16,6 +16,8 @@ MDN_ERR_NONFINITE = 4
 MDN_ERR_LINESEARCH = 5
 MDN_ERR_NL_BOUNDS = 6
 MDN_ERR_RECOVERY = 7
+MDN_ERR_UNSUPPORTED = 9
+MDN_ERR_CAPACITY = 10
 
 FF_NAMES = ("FilamentStretching", "FilamentBending", "LinkerStretching",
             "MotorGhostStretching", "CylinderExclVolume", "BoundaryCylinderRepulsion")

@@ -707,6 +707,35 @@ __device__ inline int run_defers(const DevSys& s, double* lds, double* sum, int*
     return run_defers_n<NTH>(s, nsh, lds, sum, badc, ncode);
 }
 
+// Seeded Gaussian displacement, counter-based so every coordinate's value depends only on
+// (seed, stream, index): SplitMix64's output function (Steele, Lea and Flood, OOPSLA 2014) turns
+// the counters into two uniforms per coordinate, u1 in (0, 1] and u2 in [0, 1), and the
+// Box-Muller transform gives x[i] += sigma sqrt(-2 ln u1) cos(2 pi u2).
+//   key = mix(seed + G (stream + 1)),  r(j) = mix(key + G (j + 1)),  G = 0x9E3779B97F4A7C15,
+//   u1 = ((r(2i) >> 11) + 1) 2^-53,     u2 = (r(2i + 1) >> 11) 2^-53.
+__host__ __device__ inline unsigned long long splitmix_mix(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__global__ void __launch_bounds__(BLOCK) k_perturb(double* __restrict__ x, long n, unsigned long long seed,
+                                                  unsigned long long stream, double sigma) {
+    const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long G = 0x9E3779B97F4A7C15ULL;
+    const unsigned long long key = splitmix_mix(seed + G * (stream + 1));
+    const unsigned long long r1 = splitmix_mix(key + G * (2ULL * (unsigned long long)i + 1));
+    const unsigned long long r2 = splitmix_mix(key + G * (2ULL * (unsigned long long)i + 2));
+    const double u1 = (double)((r1 >> 11) + 1) * 0x1p-53, u2 = (double)(r2 >> 11) * 0x1p-53;
+    x[i] += sigma * (sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+// x[i] += d[i], i < n (a displacement of the device coordinates)
+__global__ void __launch_bounds__(BLOCK) k_axpy_inplace(double* __restrict__ x, const double* __restrict__ d, long n) {
+    const long i = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) x[i] += d[i];
+}
+
 // v.v and max|v| of a vector (the minimizer's start: F.F and maxF): grid-stride block partials,
 // value-major, then one block in a fixed order into out[0..1].
 __global__ void __launch_bounds__(BLOCK) k_vec_stats(const double* __restrict__ v, long n, double* __restrict__ partial) {

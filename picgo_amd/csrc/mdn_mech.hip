@@ -219,10 +219,9 @@ int gather_size(mdn_ctx* ctx) { return ctx->n_beads >= GB_LARGE_BEADS ? 128 : 64
 int gather_blocks(mdn_ctx* ctx) { return std::max(1, ceil_div(ctx->n_beads, gather_size(ctx))); }
 // the gather kernels at the context's beads-per-block
 template <typename R>
-void launch_gather_plain(mdn_ctx* ctx, int bmask, const R* slots, long lo, long hi, const double* base, double* out,
-                         bool bg) {
+void launch_gather_plain(mdn_ctx* ctx, const DevSys& s, int bmask, const R* slots, long lo, long hi, const double* base,
+                         double* out, bool bg) {
     const dim3 g(gather_blocks(ctx)), b(BLOCK);
-    const DevSys& s = ctx->sys;
     const bool l = gather_size(ctx) == 128;
     if (bg && l)
         hipLaunchKernelGGL((k_gather_plain<R, 128, true>), g, b, 0, ctx->stream, s, bmask, slots, lo, hi, base, out);
@@ -377,9 +376,9 @@ int eval_slots(mdn_ctx* ctx, const double* x, int prec, bool pieces, int bmask =
 // bead blocks wrote (eval_slots without pieces, with the same bmask and base).
 int gather(mdn_ctx* ctx, int bmask, long lo, long hi, const double* base, double* out, int prec, bool bg) {
     if (prec == MDN_PREC_MIXED)
-        launch_gather_plain<float>(ctx, bmask, (const float*)ctx->slots.p, lo, hi, base, out, bg);
+        launch_gather_plain<float>(ctx, ctx->sys, bmask, (const float*)ctx->slots.p, lo, hi, base, out, bg);
     else
-        launch_gather_plain<double>(ctx, bmask, (const double*)ctx->slots.p, lo, hi, base, out, bg);
+        launch_gather_plain<double>(ctx, ctx->sys, bmask, (const double*)ctx->slots.p, lo, hi, base, out, bg);
     HIPCHK(hipGetLastError());
     return MDN_OK;
 }
@@ -788,6 +787,31 @@ int mdn_set_coords(mdn_ctx* ctx, const double* coord) {
     return MDN_OK;
 }
 
+int mdn_displace(mdn_ctx* ctx, const double* delta, int32_t n_var) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    if (!delta || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    HIPCHK(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(k_axpy_inplace, dim3(std::max(1, ceil_div(n_var, BLOCK))), dim3(BLOCK), 0, ctx->stream,
+                       ctx->x[ctx->cur].p, delta, (long)n_var);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+int mdn_perturb(mdn_ctx* ctx, uint64_t seed, uint64_t stream, double sigma) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    if (!(sigma >= 0.0)) return fail(ctx, MDN_ERR_ARG, "sigma must be >= 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    const long n = 3L * ctx->n_beads;
+    hipLaunchKernelGGL(k_perturb, dim3(std::max(1, ceil_div(n, BLOCK))), dim3(BLOCK), 0, ctx->stream,
+                       ctx->x[ctx->cur].p, n, (unsigned long long)seed, (unsigned long long)stream, sigma);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
 int mdn_get_coords(mdn_ctx* ctx, double* coord) {
     int st = need_system(ctx, false);
     if (st) return st;
@@ -1175,6 +1199,7 @@ int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n
     return forces_impl(ctx, -1, coord, force, n_var, MDN_PREC_MIXED);
 }
 
+
 // ----------------------------------------------------------------------------------------
 // CGMethod::minimize on the device
 // ----------------------------------------------------------------------------------------
@@ -1252,7 +1277,7 @@ static int cg_graph(mdn_ctx* ctx, int prec) {
 int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* force_out, mdn_min_result* res) {
     int st = need_system(ctx, true);
     if (st) return st;
-    if (!p || !coord || !res) return fail(ctx, MDN_ERR_ARG, "null argument");
+    if (!p || !res) return fail(ctx, MDN_ERR_ARG, "null argument");
     if (p->precision != MDN_PREC_FP64 && p->precision != MDN_PREC_MIXED)
         return fail(ctx, MDN_ERR_ARG, "unknown precision %d", p->precision);
     const int prec = p->precision;
@@ -1260,9 +1285,13 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     HIPCHK(hipSetDevice(ctx->device));
     memset(res, 0, sizeof(*res));
     const int nv = 3 * ctx->n_beads;
-    // startMinimization + initial energy / forces (CGMethod.hpp:876-896)
+    // startMinimization + initial energy / forces (CGMethod.hpp:876-896); the start is in x[0]
+    if (coord)
+        HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
+    else if (ctx->cur != 0)   // the context's device coordinates
+        HIPCHK(hipMemcpyAsync(ctx->x[0].p, ctx->x[ctx->cur].p, sizeof(double) * nv, hipMemcpyDeviceToDevice,
+                              ctx->stream));
     ctx->cur = 0;
-    HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
     // the reported energies are FP64 in both precisions; the minimizer's own energies and forces
     // are evaluated in the requested precision
     if ((st = energy_report(ctx, ctx->x[0].p, MDN_PREC_FP64))) return st;
@@ -1402,7 +1431,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         fill_report(ctx->h_eout, &res->energies_after);
         res->num_energy_call += 1;
     }
-    HIPCHK(hipMemcpyAsync(coord, xres, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
+    if (coord) HIPCHK(hipMemcpyAsync(coord, xres, sizeof(double) * nv, hipMemcpyDeviceToHost, ctx->stream));
     if (force_out)
         HIPCHK(hipMemcpyAsync(force_out, (S.fpar ? ctx->f2 : ctx->f).p, sizeof(double) * nv, hipMemcpyDeviceToHost,
                               ctx->stream));
@@ -1646,7 +1675,7 @@ static int bench_forces_t(mdn_ctx* ctx, int32_t iters, double* ms_per_eval, doub
     for (int i = 0; i < iters; i++) {
         hipLaunchKernelGGL(k_forces_terms<R>, dim3(nb), dim3(BLOCK), 0, ctx->stream, s, x, slots,
                            ctx->bad_count.p, ctx->bad_list.p, bead_block0(s, pieces), 3, (const double*)nullptr);
-        launch_gather_plain<R>(ctx, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p, !pieces);
+        launch_gather_plain<R>(ctx, s, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, ctx->f.p, !pieces);
     }
     HIPCHK(hipEventRecord(e1, ctx->stream));
     // dominant kernel alone

@@ -31,6 +31,7 @@ EXPORTED = [
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
+    "mdn_displace", "mdn_perturb",
 ]
 
 
@@ -83,6 +84,8 @@ def lib():
         L.mdn_hvp.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
         L.mdn_set_params.argtypes = [C.c_void_p, dp, dp, dp]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
+        L.mdn_displace.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        L.mdn_perturb.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double]
         _lib = L
     return _lib
 
@@ -195,13 +198,24 @@ class ForceFieldManager:
 
     def minimize(self, params, coord, want_forces=True):
         """CGMethod::minimize from ``coord`` with the neighbor lists frozen.
-        Returns (coord_out, force_out or None, MinResult)."""
-        x = _f64(coord).copy()
-        f = np.zeros_like(x) if want_forces else None
+        Returns (coord_out, force_out or None, MinResult). ``coord`` None: start from the
+        context's device coordinates and leave the result there (coord_out is None)."""
+        x = None if coord is None else _f64(coord).copy()
+        f = np.zeros(3 * self.n_beads) if want_forces else None
         res = abi.MinResult()
-        self._chk(self._L.mdn_minimize(self._h, C.byref(params), x.ctypes.data_as(dp),
+        self._chk(self._L.mdn_minimize(self._h, C.byref(params), None if x is None else x.ctypes.data_as(dp),
                                        f.ctypes.data_as(dp) if want_forces else None, C.byref(res)))
         return x, f, res
+
+    def perturb(self, seed, stream, sigma):
+        """x_i += sigma z_i with seeded standard normals z generated on the device (mdn_perturb;
+        ``perturbation`` gives the same values on the host)."""
+        self._chk(self._L.mdn_perturb(self._h, int(seed), int(stream), float(sigma)))
+
+    def displace(self, delta_dev_ptr):
+        """x += delta on the device coordinates; ``delta_dev_ptr``: address of 3 * n_beads doubles
+        in device memory (mdn_displace)."""
+        self._chk(self._L.mdn_displace(self._h, C.c_void_p(delta_dev_ptr), 3 * self.n_beads))
 
     def update_crosslinks(self, linkers, motors):
         """LinkerStretching / MotorGhostStretching::vectorize alone (mdn_crosslinks_update): new
@@ -278,6 +292,28 @@ class ForceFieldManager:
             self.close()
         except Exception:
             pass
+
+
+def perturbation(seed, stream, sigma, n):
+    """The seeded Gaussian displacement mdn_perturb adds to n coordinates, on the host: SplitMix64
+    counters (key = mix(seed + G (stream + 1)), r(j) = mix(key + G (j + 1))) give u1 = ((r(2i) >> 11)
+    + 1) 2^-53 and u2 = (r(2i + 1) >> 11) 2^-53, and z_i = sqrt(-2 ln u1) cos(2 pi u2). The device
+    evaluates log, sqrt and cos with its own math library, so values agree to a few ulps."""
+    G = np.uint64(0x9E3779B97F4A7C15)
+
+    def mix(z):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+    with np.errstate(over="ignore"):
+        key = mix(np.uint64(seed) + G * np.uint64(stream + 1))
+        i = np.arange(n, dtype=np.uint64)
+        r1 = mix(key + G * (np.uint64(2) * i + np.uint64(1)))
+        r2 = mix(key + G * (np.uint64(2) * i + np.uint64(2)))
+    u1 = ((r1 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * 2.0 ** -53
+    u2 = (r2 >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return sigma * (np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2))
 
 
 def minimize_batched(systems, params, coords, want_forces=True):

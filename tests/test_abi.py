@@ -1,9 +1,11 @@
-"""The C-ABI libraries load on a CPU-only host and export every function their headers declare."""
+"""The C-ABI libraries load on a CPU-only host and export every function their headers declare;
+the host side of the seeded perturbation generator."""
 import ctypes as C
 import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from picgo_amd import abi, mech, synth
@@ -106,3 +108,18 @@ def test_synth_sphere_config():
     r = ((x - 10000.0) ** 2).sum(1) ** 0.5
     assert r.max() <= 10000.0 - 25.0
     assert net.desc.n_boundary == 1 and net.desc.boundary[0].kind == abi.MDN_BND_SPHERE
+
+
+def test_host_perturbation_generator():
+    """picgo_amd.mech.perturbation (the host replica of mdn_perturb): standard normal per
+    coordinate (mean, variance, fourth moment), deterministic, independent across streams."""
+    from picgo_amd import mech
+    n = 400000
+    z = mech.perturbation(4, 0, 1.0, n)
+    assert abs(z.mean()) < 5 / np.sqrt(n)
+    assert abs(z.var() - 1.0) < 5 * np.sqrt(2.0 / n)
+    assert abs((z ** 4).mean() - 3.0) < 0.05
+    np.testing.assert_array_equal(z, mech.perturbation(4, 0, 1.0, n))
+    w = mech.perturbation(4, 1, 1.0, n)
+    assert abs(np.corrcoef(z, w)[0, 1]) < 5 / np.sqrt(n)
+    np.testing.assert_allclose(mech.perturbation(4, 0, 5.0, 10), 5.0 * z[:10], rtol=1e-15)

@@ -1,0 +1,80 @@
+"""GPU: the device-resident entry points that the successive-minimization workload uses.
+
+* mdn_perturb: the device's seeded Gaussian displacement equals the host generator
+  (picgo_amd.mech.perturbation) to a few ulps of the noise (the device's log / cos / sqrt), so the
+  displaced coordinates agree to 1e-9 nm.
+* mdn_minimize with no host buffer: starts from the device coordinates and leaves its result
+  there; bit-identical (counts, coordinates, forces) to the same minimization from a host buffer.
+* mdn_displace: x += delta from a device buffer, bit-identical to the host sum.
+* mdn_nl_build with no host buffer: the lists of the displaced device coordinates, identical to
+  those built from the same coordinates passed from the host.
+"""
+import numpy as np
+import pytest
+import torch
+
+from picgo_amd import abi, mech, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net1(gpu):
+    net = synth.Network.config(1)
+    g = gpu()
+    g.upload(net.desc, net.coord)
+    g.reset_neighbor_lists(net.coord)
+    return net, g
+
+
+@pytest.mark.parametrize("stream", [0, 7])
+def test_perturb_matches_host_generator(net1, stream):
+    net, g = net1
+    g.set_coords(net.coord)
+    g.perturb(4, stream, 5.0)
+    x = g.get_coords()
+    ref = net.coord + mech.perturbation(4, stream, 5.0, net.coord.size)
+    err = float(np.abs(x - ref).max())
+    print(f"stream {stream}: max |device - host| {err:.3e} nm", flush=True)
+    assert err <= 1e-9
+    assert float(np.abs(x - net.coord).std()) == pytest.approx(5.0 * np.sqrt(1 - 2 / np.pi), rel=0.01)
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_minimize_from_device_coordinates(net1, prec):
+    net, g = net1
+    p = abi.default_cg_params(precision=prec, energy_call_limit=120)
+    g.set_coords(net.coord)
+    g.perturb(4, 1, 5.0)
+    x0 = g.get_coords()
+    g.reset_neighbor_lists(x0)
+    xh, fh, rh = g.minimize(p, x0)
+    g.set_coords(x0)
+    _, fd, rd = g.minimize(p, None)
+    xd = g.get_coords()
+    assert (rh.num_iter, rh.num_energy_call, rh.num_force_call) == (rd.num_iter, rd.num_energy_call, rd.num_force_call)
+    np.testing.assert_array_equal(xd, xh)
+    np.testing.assert_array_equal(fd, fh)
+    assert rd.energies_after.total == rh.energies_after.total
+    # a second minimization continues from the first one's device result
+    _, _, r2 = g.minimize(p, None)
+    xh2, _, rh2 = g.minimize(p, xh)
+    np.testing.assert_array_equal(g.get_coords(), xh2)
+    assert r2.num_iter == rh2.num_iter
+
+
+def test_displace_and_rebuild_from_device_coordinates(net1):
+    net, g = net1
+    delta = np.random.default_rng(3).normal(0.0, 5.0, net.coord.size)
+    d_dev = torch.from_numpy(delta).to("cuda:0")
+    torch.cuda.synchronize()
+    g.set_coords(net.coord)
+    g.displace(d_dev.data_ptr())
+    x = g.get_coords()
+    np.testing.assert_array_equal(x, net.coord + delta)
+    n_dev = g.reset_neighbor_lists(None)
+    pairs_dev = g.neighbor_pairs()
+    n_host = g.reset_neighbor_lists(net.coord + delta)
+    assert n_dev == n_host
+    np.testing.assert_array_equal(pairs_dev, g.neighbor_pairs())
+    g.reset_neighbor_lists(net.coord)
