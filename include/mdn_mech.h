@@ -296,6 +296,20 @@ int mdn_perturb(mdn_ctx* ctx, uint64_t seed, uint64_t stream, double sigma);
 int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n_var);
 int mdn_energy_mixed(mdn_ctx* ctx, const double* coord, mdn_energy_report* out);
 
+/* ---- force evaluation without allocation --------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The forces of
+ * mdn_forces (precision MDN_PREC_FP64) or mdn_forces_mixed (MDN_PREC_MIXED), zero-filled then
+ * every force field accumulated, bit-identical to them, with coordinates and forces in DEVICE
+ * memory of the context's device (3 * n_beads doubles each). The evaluation's scratch (slot
+ * vectors, bonded sums, the numerical-fallback list and partial sums) lives in a file-scope
+ * device array of MDN_NOALLOC_SCRATCH_MIB MiB that each device holds until the process ends; a
+ * system whose evaluation needs more is refused with MDN_ERR_CAPACITY. The call allocates
+ * nothing (no hipMalloc / hipFree) and returns after the forces are written. All calls on one
+ * GPU share that array: calls that overlap on one GPU (other threads, other contexts) race. */
+#define MDN_NOALLOC_SCRATCH_MIB 48
+int mdn_forces_noalloc(mdn_ctx* ctx, int32_t precision, const double* coord_dev, double* force_dev,
+                       int32_t n_var);
 
 /* ---- differentiation ---------------------------------------------------------------------- */
 

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,7 +22,30 @@
 
 using namespace mdn;
 
+// Scratch of the force evaluation without allocation (mdn_forces_noalloc): the slot vectors,
+// the per-bead bonded sums, the excluded-volume pairs that need the numerical fallback and that
+// fallback's partial sums. Every device the library runs on holds its own copy for the life of
+// the process. The name is a host-side stand-in, not a device address: host code uses the
+// address hipGetSymbolAddress returns for the current device (scratch_dev below), never the name.
+constexpr size_t MDN_SCRATCH_BYTES = (size_t)MDN_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[MDN_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[MAX_DEVICES] = {};   // device address of `scratch`, per device
+
+// Looks the array's address up on `device` (current) once; every later call returns it.
+hipError_t scratch_lookup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_dev[device] = static_cast<unsigned char*>(p);
+    return e;
+}
 
 // stored slots per interaction (filament stretching / bending forces are computed by the gather)
 constexpr int SLOTS_PER_TERM[NFF] = {0, 0, 4, 4, 4, 1};
@@ -435,6 +459,11 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
     // minimizer alike; unset: the measured defaults (layout, cg_bonded_sums)
     if (const char* env = getenv("MDN_BONDED_GATHER")) ctx->bonded_gather_mode = atoi(env) != 0 ? 1 : 0;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MDN_ERR_HIP;
+    }
+    if (scratch_lookup(device) != hipSuccess) {   // the no-allocation entry point's array
+        (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return MDN_ERR_HIP;
     }
@@ -1199,6 +1228,100 @@ int mdn_forces_mixed(mdn_ctx* ctx, const double* coord, double* force, int32_t n
     return forces_impl(ctx, -1, coord, force, n_var, MDN_PREC_MIXED);
 }
 
+}  // extern "C"
+
+// ----------------------------------------------------------------------------------------
+// Force evaluation without allocation
+// ----------------------------------------------------------------------------------------
+namespace {
+
+// Byte offsets of the evaluation's buffers in `scratch` (each 256-byte aligned): the fallback
+// count and flag, the fallback partials (6 per block of the 240-block Simpson rule), the list of
+// pairs that need it, the bonded sums (3 doubles per bead) and last the slot vectors (3 values of
+// R per slot, padded by one gather chunk, which the gather reads unconditionally).
+struct ScratchLayout {
+    size_t bad_count, ok, partial, bad_list, bsum, slots, total;
+};
+constexpr int FORCEN_BLOCKS = 240;
+size_t up256(size_t v) { return (v + 255) & ~(size_t)255; }
+ScratchLayout scratch_layout(const DevSys& s, int n_beads, size_t rsize) {
+    ScratchLayout L;
+    L.bad_count = 0;
+    L.ok = 256;
+    L.partial = 512;
+    L.bad_list = up256(L.partial + sizeof(double) * 6 * FORCEN_BLOCKS);
+    L.bsum = up256(L.bad_list + sizeof(int) * (size_t)std::max(1, s.n[FF_EXVOL]));
+    L.slots = up256(L.bsum + sizeof(double) * 3 * (size_t)std::max(1, n_beads));
+    L.total = L.slots + rsize * (3 * (size_t)std::max(1L, s.slot0[NFF]) + GCHUNK_PAD);
+    return L;
+}
+
+template <typename R>
+int forces_noalloc_t(mdn_ctx* ctx, unsigned char* scr, const double* x, double* out) {
+    DevSys s = ctx->sys;   // the same tables, with the bonded sums in the scratch
+    const ScratchLayout L = scratch_layout(s, ctx->n_beads, sizeof(R));
+    int* bad_count = reinterpret_cast<int*>(scr + L.bad_count);
+    int* ok = reinterpret_cast<int*>(scr + L.ok);
+    double* partial = reinterpret_cast<double*>(scr + L.partial);
+    int* bad_list = reinterpret_cast<int*>(scr + L.bad_list);
+    s.bsum = reinterpret_cast<double*>(scr + L.bsum);
+    R* slots = reinterpret_cast<R*>(scr + L.slots);
+    // the previous call's count stays in the array: clear it (a kernel, not a memset node)
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, bad_count, 0);
+    hipLaunchKernelGGL(k_forces_terms<R>, dim3(force_blocks(s, false)), dim3(BLOCK), 0, ctx->stream, s, x, slots,
+                       bad_count, bad_list, bead_block0(s, false), 3, (const double*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(d2h(ctx, ctx->h_int, bad_count, sizeof(int)));
+    const int nbad = ctx->h_int[0];
+    // pairs whose closed-form forces are non-finite: the Simpson forceN rule, retried once with
+    // beads 1 and 2 moved out of plane (CylinderExclVolRepulsion.cpp:1198-1204); each pair writes
+    // only its own slots, so the order does not matter
+    for (int i = 0; i < nbad; i++) {
+        HIPCHK(d2h(ctx, ctx->h_int + 1, bad_list + i, sizeof(int)));
+        const int t = ctx->h_int[1];
+        int done = 0;
+        for (int mv = 0; mv < 2 && !done; mv++) {
+            hipLaunchKernelGGL(k_exvol_forceN_rows, dim3(FORCEN_BLOCKS), dim3(BLOCK), 0, ctx->stream, s, x, t, mv,
+                               partial);
+            hipLaunchKernelGGL(k_exvol_forceN_finish<R>, dim3(1), dim3(64), 0, ctx->stream, s, x, t, mv,
+                               FORCEN_BLOCKS, (const double*)partial, slots, ok);
+            HIPCHK(hipGetLastError());
+            HIPCHK(d2h(ctx, ctx->h_int + 2, ok, sizeof(int)));
+            done = ctx->h_int[2];
+        }
+        if (!done)
+            return fail(ctx, MDN_ERR_NONFINITE,
+                        "excluded-volume pair %d: force non-finite after the numerical fallback", t);
+    }
+    launch_gather_plain<R>(ctx, s, 3, (const R*)slots, 0L, s.slot0[NFF], (const double*)nullptr, out, true);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdn_forces_noalloc(mdn_ctx* ctx, int32_t precision, const double* coord_dev, double* force_dev, int32_t n_var) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!coord_dev || !force_dev) return fail(ctx, MDN_ERR_ARG, "null coordinate or force buffer");
+    if (n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    if (precision != MDN_PREC_FP64 && precision != MDN_PREC_MIXED)
+        return fail(ctx, MDN_ERR_ARG, "unknown precision %d", precision);
+    const ScratchLayout L =
+        scratch_layout(ctx->sys, ctx->n_beads, precision == MDN_PREC_MIXED ? sizeof(float) : sizeof(double));
+    if (L.total > MDN_SCRATCH_BYTES)
+        return fail(ctx, MDN_ERR_CAPACITY,
+                    "the evaluation needs %zu bytes of scratch; mdn_forces_noalloc has %zu (%d MiB): use mdn_forces",
+                    L.total, MDN_SCRATCH_BYTES, MDN_NOALLOC_SCRATCH_MIB);
+    HIPCHK(hipSetDevice(ctx->device));
+    unsigned char* scr = ctx->device < MAX_DEVICES ? g_scratch_dev[ctx->device] : nullptr;
+    if (!scr) return fail(ctx, MDN_ERR_STATE, "no scratch address for device %d", ctx->device);
+    return precision == MDN_PREC_MIXED ? forces_noalloc_t<float>(ctx, scr, coord_dev, force_dev)
+                                       : forces_noalloc_t<double>(ctx, scr, coord_dev, force_dev);
+}
 
 // ----------------------------------------------------------------------------------------
 // CGMethod::minimize on the device

@@ -31,7 +31,7 @@ EXPORTED = [
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
-    "mdn_displace", "mdn_perturb",
+    "mdn_forces_noalloc", "mdn_displace", "mdn_perturb",
 ]
 
 
@@ -84,6 +84,7 @@ def lib():
         L.mdn_hvp.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
         L.mdn_set_params.argtypes = [C.c_void_p, dp, dp, dp]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
+        L.mdn_forces_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
         L.mdn_displace.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
         L.mdn_perturb.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double]
         _lib = L
