@@ -134,9 +134,14 @@ def summarize(stats, total, t_nl, steps):
             "final_energy_fp64": float(stats[-1][6])}
 
 
+def precisions(args):
+    both = (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED))
+    return [b for b in both if args.stress_prec in ("both", b[0])]
+
+
 def stress_part(args, net, ffm):
     out = {}
-    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+    for name, prec in precisions(args):
         p = abi.default_cg_params(precision=prec)
         if args.warmup > 0:   # untimed: the first W minimizations of the sequence
             run_sequence(ffm, net, args.warmup, p)
@@ -146,13 +151,14 @@ def stress_part(args, net, ffm):
             f"{out[name]['cg_iterations_total']} CG iterations, criteria {out[name]['criteria']}")
     fo = {}
     if args.force_only > 0:
-        for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+        for name, prec in precisions(args):
             p = abi.default_cg_params(precision=prec, energy_rel_tol=0.0)
             stats, total, t_nl = run_sequence(ffm, net, args.force_only, p)
             fo[name] = summarize(stats, total, t_nl, args.force_only)
             log(f"force-only {name}: {args.force_only} minimizations in {total:.3f} s, "
                 f"{fo[name]['cg_iterations_total']} CG iterations, criteria {fo[name]['criteria']}")
-        fo["mixed_vs_fp64_minimizations"] = fo["mixed"]["minimizations_per_s"] / fo["fp64"]["minimizations_per_s"]
+        if len(fo) == 2:
+            fo["mixed_vs_fp64_minimizations"] = fo["mixed"]["minimizations_per_s"] / fo["fp64"]["minimizations_per_s"]
         fo["minimizations"] = args.force_only
     return out, fo
 
@@ -290,9 +296,13 @@ def run_stress(args):
     counts = ffm.counts()
     log(f"420k network: {n_pairs0} excluded-volume pairs at the start; counts {counts}")
     # the force evaluation at the start state (its roofline)
-    ev = force_evals(ffm, 200, 20)
-    log(f"force eval {ev['mixed']['us_per_force_eval']:.1f} us mixed / {ev['fp64']['us_per_force_eval']:.1f} us FP64")
+    if args.stress_prec == "both":
+        ev = force_evals(ffm, 200, 20)
+        log(f"force eval {ev['mixed']['us_per_force_eval']:.1f} us mixed / {ev['fp64']['us_per_force_eval']:.1f} us FP64")
     seq, fo = stress_part(args, net, ffm)
+    if args.stress_prec != "both":   # one precision alone (profiling): its statistics only
+        print(json.dumps({"workload": STRESS, "precision": args.stress_prec, **seq[args.stress_prec]}), flush=True)
+        return
     mixed, fp64 = seq["mixed"], seq["fp64"]
 
     B_F = force_eval_bytes(counts, net.n_beads)
@@ -403,6 +413,8 @@ def main():
     ap.add_argument("--net", type=int, choices=[1, 4], default=4)
     ap.add_argument("--prec", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--stress-prec", choices=["both", "fp64", "mixed"], default="both",
+                    help="run the stress sequence in one precision only and print its statistics (profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run 20 device copies of 64 MiB (FETCH_SIZE / WRITE_SIZE calibration)")
     args = ap.parse_args()

@@ -150,6 +150,7 @@ struct mdn_ctx {
     int batch_nt = 256, batch_sorted = 1, batch_tuned = 0;
     double batch_tune_ms[4] = {0, 0, 0, 0};
 
+    bool use_graph = true;   // replay the CG slots as a captured hipGraph (MDN_CG_GRAPH=0: launch them directly)
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // per precision (MDN_PREC_*)
     hipGraph_t graph[2] = {nullptr, nullptr};
     long sys_gen = 0, graph_gen[2] = {-1, -1};
@@ -458,6 +459,10 @@ int mdn_ctx_create(int device, mdn_ctx** out) {
     // through per-bead bonded sums (1) or the piece arrays (0), for force evaluations and the
     // minimizer alike; unset: the measured defaults (layout, cg_bonded_sums)
     if (const char* env = getenv("MDN_BONDED_GATHER")) ctx->bonded_gather_mode = atoi(env) != 0 ? 1 : 0;
+    // MDN_CG_GRAPH=0 at context creation: the minimizer launches each batch of slots directly
+    // instead of replaying a captured graph (the same kernels and decisions; for tools that do
+    // not follow graph launches)
+    if (const char* env = getenv("MDN_CG_GRAPH")) ctx->use_graph = atoi(env) != 0;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return MDN_ERR_HIP;
@@ -1469,7 +1474,9 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     HIPCHK(hipMemsetAsync(ctx->d[0].p, 0, sizeof(double) * nv, ctx->stream));   // searchDir.assign(numDof, 0)
     HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    if ((st = cg_graph(ctx, prec))) return st;
+    if (ctx->use_graph && (st = cg_graph(ctx, prec))) return st;
+    const DevSys slot_sys = cg_sys(ctx, prec);
+    const CgBufs slot_bufs = cg_bufs(ctx);
 
     // Batches are replayed back to back; the host polls the done flag of batch i while batch
     // i+1 runs, so the device never idles waiting for the host.
@@ -1486,7 +1493,15 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         int k = 0;
         long here = 0;
         while (!done && launched < max_batches) {
-            HIPCHK(hipGraphLaunch(ctx->graph_exec[prec], ctx->stream));
+            if (ctx->use_graph) {
+                HIPCHK(hipGraphLaunch(ctx->graph_exec[prec], ctx->stream));
+            } else {
+                for (int i = 0; i < CG_BATCH; i++) {
+                    if (prec == MDN_PREC_MIXED) launch_slot<float>(ctx, slot_sys, slot_bufs);
+                    else launch_slot<double>(ctx, slot_sys, slot_bufs);
+                }
+                HIPCHK(hipGetLastError());
+            }
             launched++;
             here++;
             HIPCHK(hipMemcpyAsync(&h_done[k], &ctx->cg.p->done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -78,3 +78,26 @@ def test_displace_and_rebuild_from_device_coordinates(net1):
     assert n_dev == n_host
     np.testing.assert_array_equal(pairs_dev, g.neighbor_pairs())
     g.reset_neighbor_lists(net.coord)
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_direct_slot_launches_equal_graph_replay(gpu, prec):
+    """MDN_CG_GRAPH=0 (a context that launches the minimizer's slots directly instead of replaying
+    the captured graph): the same kernels and decisions, so the same minimization bit for bit."""
+    import os
+    net = synth.Network.config(1)
+    os.environ["MDN_CG_GRAPH"] = "0"
+    try:
+        d = gpu()
+    finally:
+        os.environ.pop("MDN_CG_GRAPH", None)
+    g = gpu()
+    for c in (d, g):
+        c.upload(net.desc, net.coord)
+        c.reset_neighbor_lists(net.coord)
+    p = abi.default_cg_params(precision=prec, energy_call_limit=150)
+    xd, fd, rd = d.minimize(p, net.coord)
+    xg, fg, rg = g.minimize(p, net.coord)
+    assert (rd.num_iter, rd.num_energy_call, rd.num_force_call) == (rg.num_iter, rg.num_energy_call, rg.num_force_call)
+    np.testing.assert_array_equal(xd, xg)
+    np.testing.assert_array_equal(fd, fg)
