@@ -329,6 +329,12 @@ int mdn_param_grad(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out
  * 354-500). MDN_ERR_UNSUPPORTED when an excluded-volume pair needs the numerical force fallback. */
 int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var);
 
+/* mdn_param_grad and mdn_hvp with the mixed path's arithmetic: FP32 term arithmetic (dual numbers
+ * for the tangents) for the stretching, bending and crosslink force fields, FP64 excluded volume,
+ * boundary and accumulation (the precision split of mdn_forces_mixed). */
+int mdn_param_grad_mixed(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out);
+int mdn_hvp_mixed(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var);
+
 /* Replace the per-cylinder parameters (any pointer may be null: unchanged): stretching k and
  * rest length L0 (the excluded-volume kRepScaled of the current neighbor list is rebuilt) per
  * cylinder, bending k per cylinder (applied to the bending term whose second cylinder it is). */

@@ -5,6 +5,11 @@
 //       vectorizeAllForceFields, computeEnergy (+inf sentinel), computeIndividualEnergies,
 //       computeForces (zero-fill + accumulate), assignallforcemags, computeLoadForces, plus
 //       resetNeighborLists.
+//   mdn::ForceField         ~ medyan::ForceField (ForceField.h:34-99): getName, vectorize,
+//       computeEnergy (-1 sentinel), computeForces (accumulating), assignforcemags,
+//       computeLoadForces, getNeighborLists; one adapter per force field on the device
+//       (GpuFilamentStretching, GpuFilamentBending, GpuLinkerStretching, GpuMotorGhostStretching,
+//       GpuCylinderExclVolume, GpuBoundaryCylinderRepulsion), from ForceFieldManager::forceFields.
 //   mdn::CGMethod           ~ medyan::CGMethod::minimize (CGMethod.hpp:1746-1759) with the
 //       ConjugateGradientParams fields (CGMethod.hpp:44-63).
 //
@@ -12,6 +17,7 @@
 #pragma once
 
 #include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -36,6 +42,29 @@ inline const char* ff_name(int ff) {
                                             "BoundaryCylinderRepulsion"};
     return (ff >= 0 && ff < MDN_NUM_FF) ? names[ff] : "?";
 }
+
+class ForceFieldManager;
+
+// The reference's force-field interface (ForceField.h:34-99). The six force fields of a system
+// live in one device context (vectorized together by ForceFieldManager::vectorizeAllForceFields
+// and resetNeighborLists), so vectorize() here only checks that the context holds a system, and a
+// force field's energy is read from the context's per-force-field energy evaluation (which
+// evaluates all six at once: callers that need the total use ForceFieldManager::computeEnergy).
+class ForceField {
+public:
+    virtual ~ForceField() = default;
+    virtual std::string getName() = 0;
+    virtual void vectorize() = 0;
+    // this force field's energy at coord, or -1 when it is not finite even after the numerical
+    // fallback (ForceField.h:55-58)
+    virtual double computeEnergy(double* coord) = 0;
+    // this force field's forces at coord ADDED to f (3 * n_beads values)
+    virtual void computeForces(double* coord, double* f) = 0;
+    virtual void computeLoadForces() {}
+    virtual void assignforcemags() {}
+    // the neighbor lists live on the device (ForceFieldManager::resetNeighborLists)
+    virtual std::vector<void*> getNeighborLists() { return {}; }
+};
 
 class ForceFieldManager {
 public:
@@ -117,13 +146,103 @@ public:
     mdn_ctx* ctx() const { return ctx_; }
     int numVar() const { return nvar_; }
 
+    // the force fields in ForceFieldManager order (MController.cpp:70-178), as ForceField
+    // adapters on this manager's context
+    inline const std::vector<std::unique_ptr<ForceField>>& forceFields();
+    // one force field's energy and (accumulated) forces
+    double ffEnergy(int ff, const double* coord) {
+        mdn_energy_report r{};
+        check(mdn_energy(ctx_, coord, &r));
+        return r.ff[ff];
+    }
+    void checkVectorized() {
+        int64_t counts[2 + MDN_NUM_FF];
+        check(mdn_system_counts(ctx_, counts));
+        if (counts[0] <= 0) throw Error(MDN_ERR_STATE, "no system vectorized (vectorizeAllForceFields)");
+    }
+
 private:
     void check(int st) const {
         if (st != MDN_OK) throw Error(st, mdn_last_error(ctx_));
     }
     mdn_ctx* ctx_ = nullptr;
     int nvar_ = 0;
+    std::vector<std::unique_ptr<ForceField>> ffs_;
 };
+
+// A force field of the device context (MDN_FF_*).
+class DeviceForceField : public ForceField {
+public:
+    DeviceForceField(ForceFieldManager& m, int ff) : m_(m), ff_(ff) {}
+    std::string getName() override { return ff_name(ff_); }
+    void vectorize() override { m_.checkVectorized(); }
+    double computeEnergy(double* coord) override { return m_.ffEnergy(ff_, coord); }
+    void computeForces(double* coord, double* f) override { m_.computeForces(ff_, coord, f, m_.numVar()); }
+
+protected:
+    ForceFieldManager& m_;
+    int ff_;
+};
+
+struct GpuFilamentStretching : DeviceForceField {
+    explicit GpuFilamentStretching(ForceFieldManager& m) : DeviceForceField(m, MDN_FF_STRETCH) {}
+};
+struct GpuFilamentBending : DeviceForceField {
+    explicit GpuFilamentBending(ForceFieldManager& m) : DeviceForceField(m, MDN_FF_BEND) {}
+};
+// LinkerStretching / MotorGhostStretching: assignforcemags reads each crosslink's stretchForce of
+// the last force evaluation (LinkerStretchingHarmonic.cpp:291) into stretchForce().
+struct GpuCrosslinkStretching : DeviceForceField {
+    GpuCrosslinkStretching(ForceFieldManager& m, int ff) : DeviceForceField(m, ff) {}
+    void assignforcemags() override {
+        std::vector<double> l, mo;
+        m_.assignallforcemags(l, mo);
+        sf_ = ff_ == MDN_FF_LINKER ? l : mo;
+    }
+    const std::vector<double>& stretchForce() const { return sf_; }
+
+private:
+    std::vector<double> sf_;
+};
+struct GpuLinkerStretching : GpuCrosslinkStretching {
+    explicit GpuLinkerStretching(ForceFieldManager& m) : GpuCrosslinkStretching(m, MDN_FF_LINKER) {}
+};
+struct GpuMotorGhostStretching : GpuCrosslinkStretching {
+    explicit GpuMotorGhostStretching(ForceFieldManager& m) : GpuCrosslinkStretching(m, MDN_FF_MOTOR) {}
+};
+struct GpuCylinderExclVolume : DeviceForceField {
+    explicit GpuCylinderExclVolume(ForceFieldManager& m) : DeviceForceField(m, MDN_FF_EXVOL) {}
+};
+// BoundaryCylinderRepulsion: computeLoadForces for filaments of one type with the given monomer
+// size and monomers per cylinder (BoundaryCylinderRepulsion.cpp:122-187), into loadForcesP/M().
+struct GpuBoundaryCylinderRepulsion : DeviceForceField {
+    explicit GpuBoundaryCylinderRepulsion(ForceFieldManager& m, int nFilaments = 0, double monomerSize = 2.7,
+                                          int cylinderNumMon = 40)
+        : DeviceForceField(m, MDN_FF_BOUNDARY), nfil_(nFilaments), msize_(monomerSize), nmon_(cylinderNumMon) {}
+    void computeLoadForces() override {
+        if (nfil_ > 0) m_.computeLoadForces(nfil_, {msize_}, {nmon_}, p_, m_load_);
+    }
+    const std::vector<double>& loadForcesP() const { return p_; }
+    const std::vector<double>& loadForcesM() const { return m_load_; }
+
+private:
+    int nfil_;
+    double msize_;
+    int nmon_;
+    std::vector<double> p_, m_load_;
+};
+
+inline const std::vector<std::unique_ptr<ForceField>>& ForceFieldManager::forceFields() {
+    if (ffs_.empty()) {
+        ffs_.emplace_back(new GpuFilamentStretching(*this));
+        ffs_.emplace_back(new GpuFilamentBending(*this));
+        ffs_.emplace_back(new GpuLinkerStretching(*this));
+        ffs_.emplace_back(new GpuMotorGhostStretching(*this));
+        ffs_.emplace_back(new GpuCylinderExclVolume(*this));
+        ffs_.emplace_back(new GpuBoundaryCylinderRepulsion(*this));
+    }
+    return ffs_;
+}
 
 // ConjugateGradientParams (CGMethod.hpp:44-63) with MController's values (MController.cpp:34-63)
 struct ConjugateGradientParams {

@@ -1480,23 +1480,27 @@ __device__ inline DualD3 pos_dual(const double* __restrict__ x, const double* __
     return DualD3{ld3(x, b), ld3(v, b)};
 }
 
-// d/de of interaction t's forces at x + e v (forward-mode dual arithmetic, FP64) into the
-// pieces / slots; their gather is (dF/dx) v = -H v. *fbad: an excluded-volume pair whose
-// analytic force is non-finite (the reference would switch to forceN, which has no tangent here).
+// d/de of interaction t's forces at x + e v (forward-mode dual arithmetic) into the pieces /
+// slots (FP64); their gather is (dF/dx) v = -H v. RT = DualD: FP64 throughout; RT = DualF: the
+// mixed path's tangents, the stretching, bending and crosslink arithmetic in FP32 duals (bond
+// vectors, lengths and extensions formed in FP64 as in the mixed forces), excluded volume and the
+// boundary in FP64. *fbad: an excluded-volume pair whose analytic force is non-finite (the
+// reference would switch to forceN, which has no tangent here).
+template <typename RT>
 __device__ inline void term_tangent(const DevSys& s, int ff, int t, const double* __restrict__ x,
                                     const double* __restrict__ v, double* __restrict__ slots, int* fbad) {
     DualD3 out[4];
     switch (ff) {
     case FF_STRETCH: {
         const int b1 = s.str_b[2 * t], b2 = s.str_b[2 * t + 1];
-        stretch_forces<DualD>(pos_dual(x, v, b1), pos_dual(x, v, b2), s.str_k[t], s.str_eql[t], out);
+        stretch_forces<RT>(pos_dual(x, v, b1), pos_dual(x, v, b2), s.str_k[t], s.str_eql[t], out);
         put_piece<double>(s, 1, b1, out[0].d);
         put_piece<double>(s, 0, b2, out[1].d);
         return;
     }
     case FF_BEND: {
         const int b1 = s.bend_b[3 * t], b2 = s.bend_b[3 * t + 1], b3 = s.bend_b[3 * t + 2];
-        bend_forces<DualD>(pos_dual(x, v, b1), pos_dual(x, v, b2), pos_dual(x, v, b3), s.bend_k[t], s.bend_eqt[t], out);
+        bend_forces<RT>(pos_dual(x, v, b1), pos_dual(x, v, b2), pos_dual(x, v, b3), s.bend_k[t], s.bend_eqt[t], out);
         put_piece<double>(s, 4, b1, out[0].d);
         put_piece<double>(s, 3, b2, out[1].d);
         put_piece<double>(s, 2, b3, out[2].d);
@@ -1508,7 +1512,7 @@ __device__ inline void term_tangent(const DevSys& s, int ff, int t, const double
         const int* b = lk ? s.lnk_b : s.mot_b;
         const double k = lk ? s.lnk_k[t] : s.mot_k[t], eql = lk ? s.lnk_eql[t] : s.mot_eql[t];
         const double p1 = lk ? s.lnk_p1[t] : s.mot_p1[t], p2 = lk ? s.lnk_p2[t] : s.mot_p2[t];
-        (void)xlink_forces<DualD>(pos_dual(x, v, b[4 * t]), pos_dual(x, v, b[4 * t + 1]), pos_dual(x, v, b[4 * t + 2]),
+        (void)xlink_forces<RT>(pos_dual(x, v, b[4 * t]), pos_dual(x, v, b[4 * t + 1]), pos_dual(x, v, b[4 * t + 2]),
                                   pos_dual(x, v, b[4 * t + 3]), k, eql, p1, p2, out);
         const long id = s.slot0[ff] + 4L * t;
         for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q].d);
@@ -1532,6 +1536,7 @@ __device__ inline void term_tangent(const DevSys& s, int ff, int t, const double
     }
 }
 
+template <typename RT>
 __global__ void __launch_bounds__(BLOCK) k_tangent_terms(DevSys s, const double* __restrict__ x,
                                                          const double* __restrict__ v, double* __restrict__ slots,
                                                          int* bad_count) {
@@ -1540,7 +1545,7 @@ __global__ void __launch_bounds__(BLOCK) k_tangent_terms(DevSys s, const double*
     const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     if (t >= s.n[ff]) return;
     int fbad = 0;
-    term_tangent(s, ff, t, x, v, slots, &fbad);
+    term_tangent<RT>(s, ff, t, x, v, slots, &fbad);
     if (fbad) atomicAdd(bad_count, 1);
 }
 
@@ -1576,15 +1581,18 @@ __global__ void __launch_bounds__(BLOCK) k_ev_pair_energyN(DevSys s, const doubl
 //     cylinder c adds U / L0_c to dE/dL0_c; those pairs are the ones in the incidence list of c's
 //     first bead with the bead as the pair's bead 1 or bead 3 (ascending pair order);
 //   bending term whose second cylinder is c (its k, FilamentBending.cpp:29-60): dE/dk = U / k.
-// out[c], out[n_cyl + c], out[2 n_cyl + c].
+// out[c], out[n_cyl + c], out[2 n_cyl + c]. R = float: the mixed path's arithmetic (the
+// extension formed in FP64 and the stretching derivatives rounded to FP32 as the mixed stretching
+// energy is, FP32 bending energy, FP64 excluded-volume energies).
+template <typename R>
 __global__ void k_param_grad(DevSys s, int n_cyl, const int* __restrict__ cyl_b1, const int* __restrict__ cyl_bend,
                              const double* __restrict__ x, const double* __restrict__ U, double* __restrict__ out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_cyl) return;
     const V3<double> r = dif<double>(ld3(x, s.str_b[2 * c + 1]), ld3(x, s.str_b[2 * c]));
     const double dl = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) - s.str_eql[c];
-    out[c] = 0.5 * dl * dl;
-    double gl = -s.str_k[c] * dl;
+    out[c] = (double)(R)(0.5 * dl * dl);
+    double gl = (double)(R)(-s.str_k[c] * dl);
     const int b = cyl_b1[c];
     const long e0 = s.slot0[FF_EXVOL], e1 = s.slot0[FF_EXVOL + 1];
     for (int q = s.bstart[b]; q < s.bstart[b + 1]; q++) {
@@ -1595,8 +1603,8 @@ __global__ void k_param_grad(DevSys s, int n_cyl, const int* __restrict__ cyl_b1
     }
     out[n_cyl + c] = gl;
     const int tb = cyl_bend[c];
-    out[2 * n_cyl + c] = tb >= 0 ? bend_energy<double>(ld3(x, s.bend_b[3 * tb]), ld3(x, s.bend_b[3 * tb + 1]),
-                                                       ld3(x, s.bend_b[3 * tb + 2]), 1.0, s.bend_eqt[tb])
+    out[2 * n_cyl + c] = tb >= 0 ? (double)bend_energy<R>(ld3(x, s.bend_b[3 * tb]), ld3(x, s.bend_b[3 * tb + 1]),
+                                                          ld3(x, s.bend_b[3 * tb + 2]), 1.0, s.bend_eqt[tb])
                                  : 0.0;
 }
 

@@ -1131,7 +1131,7 @@ int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor
     return MDN_OK;
 }
 
-int mdn_param_grad(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out) {
+static int param_grad_impl(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out, int prec) {
     int st = need_system(ctx, true);
     if (st) return st;
     if (!out || n_out != 3L * ctx->n_cyl) return fail(ctx, MDN_ERR_ARG, "n_out must be 3*n_cylinders = %d", 3 * ctx->n_cyl);
@@ -1165,17 +1165,31 @@ int mdn_param_grad(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out
                     return fail(ctx, MDN_ERR_NONFINITE, "excluded-volume pair %d: energy non-finite after the Simpson fallback", t);
         }
     }
-    if (ctx->n_cyl > 0)
-        hipLaunchKernelGGL(k_param_grad, dim3(ceil_div(ctx->n_cyl, BLOCK)), dim3(BLOCK), 0, ctx->stream, s, ctx->n_cyl,
-                           (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x, (const double*)ctx->ev_U.p,
-                           ctx->grad_out.p);
+    if (ctx->n_cyl > 0) {
+        if (prec == MDN_PREC_MIXED)
+            hipLaunchKernelGGL(k_param_grad<float>, dim3(ceil_div(ctx->n_cyl, BLOCK)), dim3(BLOCK), 0, ctx->stream, s,
+                               ctx->n_cyl, (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x,
+                               (const double*)ctx->ev_U.p, ctx->grad_out.p);
+        else
+            hipLaunchKernelGGL(k_param_grad<double>, dim3(ceil_div(ctx->n_cyl, BLOCK)), dim3(BLOCK), 0, ctx->stream, s,
+                               ctx->n_cyl, (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x,
+                               (const double*)ctx->ev_U.p, ctx->grad_out.p);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, ctx->grad_out.p, sizeof(double) * 3 * ctx->n_cyl, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
 }
 
-int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var) {
+int mdn_param_grad(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out) {
+    return param_grad_impl(ctx, coord, out, n_out, MDN_PREC_FP64);
+}
+
+int mdn_param_grad_mixed(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out) {
+    return param_grad_impl(ctx, coord, out, n_out, MDN_PREC_MIXED);
+}
+
+static int hvp_impl(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var, int prec) {
     int st = need_system(ctx, true);
     if (st) return st;
     if (!v || !out || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
@@ -1189,9 +1203,14 @@ int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int
     HIPCHK(ctx->tan_v.ensure(n_var));
     HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    if (s.blk[NFF] > 0)
-        hipLaunchKernelGGL(k_tangent_terms, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
-                           (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
+    if (s.blk[NFF] > 0) {
+        if (prec == MDN_PREC_MIXED)
+            hipLaunchKernelGGL(k_tangent_terms<DualF>, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
+                               (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
+        else
+            hipLaunchKernelGGL(k_tangent_terms<DualD>, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
+                               (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
+    }
     HIPCHK(hipGetLastError());
     // the tangent pieces are in the piece arrays
     if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, false))) return st;
@@ -1203,6 +1222,14 @@ int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int
                     ctx->h_int[0]);
     for (int i = 0; i < n_var; i++) out[i] = -out[i];   // H v = -(dF/dx) v
     return MDN_OK;
+}
+
+int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var) {
+    return hvp_impl(ctx, coord, v, out, n_var, MDN_PREC_FP64);
+}
+
+int mdn_hvp_mixed(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var) {
+    return hvp_impl(ctx, coord, v, out, n_var, MDN_PREC_MIXED);
 }
 
 int mdn_set_params(mdn_ctx* ctx, const double* kstr, const double* eql, const double* kbend) {

@@ -53,6 +53,7 @@ template <typename T> struct Dual {
     MDN_HD Dual() : v(0), d(0) {}
     MDN_HD Dual(T x) : v(x), d(0) {}
     MDN_HD Dual(T x, T dx) : v(x), d(dx) {}
+    template <typename U> MDN_HD explicit Dual(const Dual<U>& o) : v((T)o.v), d((T)o.d) {}
     MDN_HD friend Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
     MDN_HD friend Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
     MDN_HD friend Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
@@ -84,6 +85,7 @@ template <typename T> struct Dual {
     MDN_HD friend Dual fabs(Dual a) { return a.v < 0 ? -a : a; }
 };
 typedef Dual<double> DualD;
+typedef Dual<float> DualF;   // the mixed path's tangents
 
 // a position and its tangent
 struct DualD3 { D3 v, d; };
@@ -93,11 +95,15 @@ template <typename R> MDN_HD inline V3<R> dif(DualD3 a, DualD3 b) {
 // the FP64 companion type of an arithmetic type (spring lengths, boundary distances)
 template <typename R> struct Wide { typedef double T; };
 template <> struct Wide<DualD> { typedef DualD T; };
+template <> struct Wide<DualF> { typedef DualD T; };
 
 // force outputs: D3 for the value paths, DualD3 (value + derivative) for the tangent path
 MDN_HD inline void put3(D3& o, double x, double y, double z) { o = D3{x, y, z}; }
 MDN_HD inline void put3(D3& o, float x, float y, float z) { o = D3{(double)x, (double)y, (double)z}; }
 MDN_HD inline void put3(DualD3& o, DualD x, DualD y, DualD z) { o = DualD3{D3{x.v, y.v, z.v}, D3{x.d, y.d, z.d}}; }
+MDN_HD inline void put3(DualD3& o, DualF x, DualF y, DualF z) {
+    o = DualD3{D3{(double)x.v, (double)y.v, (double)z.v}, D3{(double)x.d, (double)y.d, (double)z.d}};
+}
 
 // ---------------- FilamentStretchingHarmonic (FilamentStretchingHarmonic.cpp:185-248) -------
 // The harmonic springs (filament stretching, linkers, motors): the length L and the extension

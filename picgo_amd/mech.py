@@ -31,7 +31,7 @@ EXPORTED = [
     "mdn_bench_forces", "mdn_bench_energy", "mdn_set_coords", "mdn_get_coords", "mdn_forces_mixed",
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
-    "mdn_forces_noalloc", "mdn_displace", "mdn_perturb",
+    "mdn_forces_noalloc", "mdn_displace", "mdn_perturb", "mdn_param_grad_mixed", "mdn_hvp_mixed",
 ]
 
 
@@ -82,6 +82,8 @@ def lib():
         L.mdn_last_batch_schedule.argtypes = [C.c_void_p, i32p, dp]
         L.mdn_param_grad.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.mdn_hvp.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
+        L.mdn_hvp_mixed.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
+        L.mdn_param_grad_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.mdn_set_params.argtypes = [C.c_void_p, dp, dp, dp]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
         L.mdn_forces_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
@@ -250,21 +252,23 @@ class ForceFieldManager:
         self._chk(self._L.mdn_last_min_stats(self._h, out))
         return {"ls_launches": out[0], "forceN_steps": out[1], "gathers": out[2], "iterations": out[3]}
 
-    def param_grad(self, coord=None):
+    def param_grad(self, coord=None, precision=abi.MDN_PREC_FP64):
         """dE/dtheta for theta = (stretching k, rest length L0, bending k) per cylinder; returns
-        an array of shape (3, n_cylinders) (mdn_param_grad)."""
+        an array of shape (3, n_cylinders) (mdn_param_grad, mdn_param_grad_mixed)."""
         nc = self.counts()["n_cylinders"]
         out = np.zeros(3 * nc)
         cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
-        self._chk(self._L.mdn_param_grad(self._h, cp, out.ctypes.data_as(dp), out.size))
+        fn = self._L.mdn_param_grad_mixed if precision == abi.MDN_PREC_MIXED else self._L.mdn_param_grad
+        self._chk(fn(self._h, cp, out.ctypes.data_as(dp), out.size))
         return out.reshape(3, nc)
 
-    def hvp(self, v, coord=None):
-        """Hessian-vector product H v = -(dF/dx) v (mdn_hvp)."""
+    def hvp(self, v, coord=None, precision=abi.MDN_PREC_FP64):
+        """Hessian-vector product H v = -(dF/dx) v (mdn_hvp, mdn_hvp_mixed)."""
         vv = _f64(v)
         out = np.zeros(3 * self.n_beads)
         cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
-        self._chk(self._L.mdn_hvp(self._h, cp, vv.ctypes.data_as(dp), out.ctypes.data_as(dp), out.size))
+        fn = self._L.mdn_hvp_mixed if precision == abi.MDN_PREC_MIXED else self._L.mdn_hvp
+        self._chk(fn(self._h, cp, vv.ctypes.data_as(dp), out.ctypes.data_as(dp), out.size))
         return out
 
     def set_params(self, kstr=None, eql=None, kbend=None):

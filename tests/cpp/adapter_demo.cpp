@@ -1,7 +1,8 @@
 // C++ caller of the drop-in adapters (include/mdn_mech.hpp): the MController-shaped sequence
 // serialize/vectorize -> reset neighbor lists -> minimize -> stretch forces and boundary load
-// forces, on the seeded 50-filament network. Prints "pairs energy_before energy_after iterations
-// max_force converged n_linker_stretch_forces n_loaded_filament_ends".
+// forces, on the seeded 50-filament network, and the per-force-field adapters (ForceField).
+// Prints "pairs energy_before energy_after iterations max_force converged n_linker_stretch_forces
+// n_loaded_filament_ends force_field_adapters_ok".
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -23,6 +24,23 @@ int main() {
         double e0 = ffm.computeEnergy(coord.data());
         std::vector<double> f(coord.size());
         ffm.computeForces(coord.data(), f.data(), (int)f.size());
+        // the per-force-field adapters (ForceField interface): their forces accumulated one after
+        // another equal the manager's, their energies sum to its total, in the manager's order
+        std::vector<double> facc(coord.size(), 0.0);
+        double esum = 0;
+        int ff_ok = 1, k = 0;
+        for (const auto& ffp : ffm.forceFields()) {
+            ffp->vectorize();
+            ff_ok &= ffp->getName() == mdn::ff_name(k++);
+            esum += ffp->computeEnergy(coord.data());
+            ffp->computeForces(coord.data(), facc.data());
+        }
+        double fdiff = 0, fmax = 0;
+        for (size_t i = 0; i < f.size(); i++) {
+            fdiff = std::fmax(fdiff, std::fabs(facc[i] - f[i]));
+            fmax = std::fmax(fmax, std::fabs(f[i]));
+        }
+        ff_ok &= fdiff <= 1e-12 * fmax && std::fabs(esum - e0) <= 1e-12 * std::fabs(e0);
         mdn::CGMethod cg;
         mdn::ConjugateGradientParams p;
         std::vector<double> force;
@@ -35,8 +53,11 @@ int main() {
         const int stride = ffm.computeLoadForces(d->n_filaments, {2.7}, {40}, lp, lm);
         int loaded = 0;
         for (int fi = 0; fi < d->n_filaments; fi++) loaded += (lp[(size_t)fi * stride] > 0) + (lm[(size_t)fi * stride] > 0);
-        std::printf("%ld %.10e %.10e %d %.6e %d %zu %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
-                    (int)res.converged(), lsf.size(), loaded);
+        auto* lk = dynamic_cast<mdn::GpuLinkerStretching*>(ffm.forceFields()[MDN_FF_LINKER].get());
+        lk->assignforcemags();
+        ff_ok &= lk->stretchForce() == lsf;
+        std::printf("%ld %.10e %.10e %d %.6e %d %zu %d %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
+                    (int)res.converged(), lsf.size(), loaded, ff_ok);
     } catch (const mdn::Error& e) {
         std::printf("error %d %s\n", e.code, e.what());
         mdn_synth_destroy(net);

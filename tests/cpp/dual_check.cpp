@@ -61,6 +61,10 @@ int main() {
     const double h = 1e-3;   // truncation error <= 1e-8 here; smaller h only adds rounding noise
     const int N = 2000;
     Err es, eb, ex, ev, evp, ep;   // evp: pairs inside the coplanar test
+    Err fs, fb, fb0, fx;           // FP32 tangents (the mixed path's) against the FP64 ones
+                                   // (fb0: bending with theta0 != 0)
+    DualD3 o32[4];
+    D3 t32[4];
     D3 x[4], v[4], fd[4], t[4];
     DualD3 p[4], o[4];
     for (int i = 0; i < N; i++) {   // stretching
@@ -73,6 +77,9 @@ int main() {
         stretch_forces<DualD>(p[0], p[1], kk, l0, o);
         tangents(o, 2, t);
         es.add_case(t, fd, 2);
+        stretch_forces<DualF>(p[0], p[1], kk, l0, o32);
+        tangents(o32, 2, t32);
+        fs.add_case(t, t32, 2);
     }
     for (int i = 0; i < N; i++) {   // bending, theta0 = 0 and theta0 != 0
         x[0] = D3{U(-50, 50), U(-50, 50), U(-50, 50)};
@@ -86,6 +93,9 @@ int main() {
         bend_forces<DualD>(p[0], p[1], p[2], kb, eqt, o);
         tangents(o, 3, t);
         eb.add_case(t, fd, 3);
+        bend_forces<DualF>(p[0], p[1], p[2], kb, eqt, o32);
+        tangents(o32, 3, t32);
+        (eqt == 0.0 ? fb : fb0).add_case(t, t32, 3);
     }
     for (int i = 0; i < N; i++) {   // linkers / motors
         for (int k = 0; k < 4; k += 2) {
@@ -100,6 +110,9 @@ int main() {
         (void)xlink_forces<DualD>(p[0], p[1], p[2], p[3], kk, l0, p1, p2, o);
         tangents(o, 4, t);
         ex.add_case(t, fd, 4);
+        (void)xlink_forces<DualF>(p[0], p[1], p[2], p[3], kk, l0, p1, p2, o32);
+        tangents(o32, 4, t32);
+        fx.add_case(t, t32, 4);
     }
     int n_inplane = 0;
     for (int i = 0; i < N;) {   // excluded volume
@@ -164,5 +177,9 @@ int main() {
     std::printf("exvol %d %.3e\n", ev.n, ev.worst);
     std::printf("exvol_inplane %d %.3e\n", evp.n, evp.worst);
     std::printf("boundary %d %.3e\n", ep.n, ep.worst);
+    std::printf("stretch_fp32 %d %.3e\n", fs.n, fs.worst);
+    std::printf("bend_fp32 %d %.3e\n", fb.n, fb.worst);
+    std::printf("bend_theta0_fp32 %d %.3e\n", fb0.n, fb0.worst);
+    std::printf("xlink_fp32 %d %.3e\n", fx.n, fx.worst);
     return 0;
 }

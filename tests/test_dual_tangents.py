@@ -3,7 +3,9 @@ picgo_amd/csrc/mdn_terms.h instantiated with dual numbers against central differ
 FP64 value instantiation on seeded random geometries of every force field (tests/cpp/dual_check.cpp).
 Excluded-volume pairs are well conditioned (normalized triple product >= 0.2) or inside the
 coplanar test away from its threshold; the closed form's rounding noise near coplanarity, not the
-tangent, sets the bar there."""
+tangent, sets the bar there. The mixed path's FP32 tangents (stretching, bending with theta0 = 0,
+crosslinks) are held to 1e-5 of the FP64 ones; bending with theta0 != 0 to 1e-3 (FP32 cancellation
+in sin(theta) near straight triples)."""
 import os
 import subprocess
 
@@ -22,8 +24,16 @@ def test_dual_tangents_match_central_differences():
         f = line.split()
         res[f[0]] = (int(f[1]), float(f[2]))
     print(res)
-    assert set(res) == {"stretch", "bend", "xlink", "exvol", "exvol_inplane", "boundary"}
+    assert set(res) == {"stretch", "bend", "xlink", "exvol", "exvol_inplane", "boundary", "stretch_fp32",
+                        "bend_fp32", "bend_theta0_fp32", "xlink_fp32"}
     for name, (n, err) in res.items():
         assert n >= (100 if name == "exvol_inplane" else 1000), name
-        bar = {"exvol": 1e-5, "exvol_inplane": 1e-4}.get(name, 1e-6)   # central-difference noise
+        # central-difference noise; the FP32 tangents against the FP64 ones: FP32 rounding. With a
+        # nonzero equilibrium angle the bending force divides by sin(theta) = sqrt(1 - cos^2), which
+        # cancels in FP32 for nearly straight triples (a relative error of order eps / sin^2, up to
+        # 2e-4 on these random angles); the networks here use theta0 = 0, where that term is absent
+        bar = {"exvol": 1e-5, "exvol_inplane": 1e-4}.get(name, 1e-6)
+        if name.endswith("_fp32"):
+            bar = 1e-3 if name == "bend_theta0_fp32" else 1e-5
+            assert err > 0, name   # the arithmetic is FP32 indeed
         assert err <= bar, (name, err)

@@ -12,7 +12,8 @@ length for coordinates) pin:
   dE/dtheta against (E(theta + h) - E(theta - h)) / 2h for sampled cylinders and every kind.
 Exact identities (no finite-difference error) pin the tangent arithmetic further: H v is linear in
 v, and u.Hv = v.Hu up to the coplanar-shift pairs, whose force is not -dE/dx by construction
-(CylinderExclVolRepulsion.cpp:610-630).
+(CylinderExclVolRepulsion.cpp:610-630). The mixed path's HVP and dE/dtheta (FP32 term arithmetic)
+are held to 1e-5 of the FP64 ones.
 """
 import numpy as np
 import pytest
@@ -190,3 +191,27 @@ def test_param_grad_vs_finite_differences(relaxed):
         g.set_params(kstr, eql, kb)
     # cylinder 0 (the first of filament 0) is no bending term's second cylinder
     assert grad[2, 0] == 0.0
+
+
+def test_mixed_hvp_and_param_grad(relaxed):
+    """The mixed path's Hessian-vector products (FP32 dual arithmetic for the stretching, bending
+    and crosslink terms, FP64 excluded volume, boundary and accumulation) and parameter gradients
+    against the FP64 ones, on the same 64 seeded unit vectors: normwise max|dHv| / max|Hv| <= 1e-5
+    (the FP32 rounding of the term arithmetic), dE/dtheta 1e-5 relative per component (above a
+    floor of 1e-6 of the kind's largest value)."""
+    net, g, x = relaxed
+    rng = np.random.default_rng(64)
+    worst = 0.0
+    for _ in range(64):
+        v = _unit(rng, x.size)
+        h64 = g.hvp(v, x)
+        h32 = g.hvp(v, x, precision=abi.MDN_PREC_MIXED)
+        worst = max(worst, float(np.abs(h32 - h64).max()) / float(np.abs(h64).max()))
+    g64 = g.param_grad(x)
+    g32 = g.param_grad(x, precision=abi.MDN_PREC_MIXED)
+    rel = [float((np.abs(g32[k] - g64[k]) / np.maximum(np.abs(g64[k]), 1e-6 * np.abs(g64[k]).max())).max())
+           for k in range(3)]
+    print(f"mixed HVP: max normwise relative error {worst:.3e} over 64 vectors; "
+          f"mixed dE/dtheta relative error (k, L0, kbend) {rel}", flush=True)
+    assert 0.0 < worst <= 1e-5
+    assert max(rel) <= 1e-5
