@@ -138,6 +138,16 @@ template <typename R> MDN_HD inline R bend_energy(D3 c1, D3 c2, D3 c3, double kd
     V3<R> r1 = dif<R>(c2, c1), r2 = dif<R>(c3, c2);
     R L1 = sqrt(dot(r1, r1));
     R L2 = sqrt(dot(r2, r2));
+    if constexpr (sizeof(R) == 4) {
+        // FP32 with theta0 = 0: 1 - cos(theta) as half the squared chord of the unit bond vectors,
+        // |r1 / L1 - r2 / L2|^2 / 2, which does not cancel for nearly straight triples (1 - x does:
+        // a relative error of eps / theta^2 there)
+        if (are_equal_zero(eqtd)) {
+            const R i1 = 1 / L1, i2 = 1 / L2;
+            const R dx = r1.x * i1 - r2.x * i2, dy = r1.y * i1 - r2.y * i2, dz = r1.z * i1 - r2.z * i2;
+            return kbend * ((dx * dx + dy * dy + dz * dz) / 2);
+        }
+    }
     R L1L2 = L1 * L2;
     R l1l2 = dot(r1, r2);
     R x = l1l2 / L1L2;

@@ -13,7 +13,7 @@ length for coordinates) pin:
 Exact identities (no finite-difference error) pin the tangent arithmetic further: H v is linear in
 v, and u.Hv = v.Hu up to the coplanar-shift pairs, whose force is not -dE/dx by construction
 (CylinderExclVolRepulsion.cpp:610-630). The mixed path's HVP and dE/dtheta (FP32 term arithmetic)
-are held to 1e-5 of the FP64 ones.
+are held to 1e-5 of the FP64 ones (normwise).
 """
 import numpy as np
 import pytest
@@ -209,9 +209,15 @@ def test_mixed_hvp_and_param_grad(relaxed):
         worst = max(worst, float(np.abs(h32 - h64).max()) / float(np.abs(h64).max()))
     g64 = g.param_grad(x)
     g32 = g.param_grad(x, precision=abi.MDN_PREC_MIXED)
-    rel = [float((np.abs(g32[k] - g64[k]) / np.maximum(np.abs(g64[k]), 1e-6 * np.abs(g64[k]).max())).max())
-           for k in range(3)]
-    print(f"mixed HVP: max normwise relative error {worst:.3e} over 64 vectors; "
-          f"mixed dE/dtheta relative error (k, L0, kbend) {rel}", flush=True)
+    norm = [float(np.abs(g32[k] - g64[k]).max() / np.abs(g64[k]).max()) for k in range(3)]
+    comp = [float((np.abs(g32[k] - g64[k]) / np.abs(g64[k]).clip(1e-300)).max()) for k in (0, 2)]
+    print(f"mixed HVP: max normwise relative error {worst:.3e} over 64 vectors; mixed dE/dtheta normwise "
+          f"relative error (k, L0, kbend) {norm}, per component (k, kbend) {comp}", flush=True)
     assert 0.0 < worst <= 1e-5
-    assert max(rel) <= 1e-5
+    assert max(norm) <= 1e-5
+    # per component: dE/dk = (L - L0)^2 / 2 keeps FP32 relative accuracy; dE/dkbend = 1 - cos(theta)
+    # (half the squared chord of the unit bond vectors in FP32, no cancellation) carries the FP32
+    # rounding of the bond directions, a relative error of eps / theta (measured 9e-4 on the
+    # straightest triples of the relaxed network, where 1 - x in FP32 gave 0.12); dE/dL0 =
+    # -k (L - L0) + sum U / L0 can cancel between its two parts, so it is held normwise only
+    assert comp[0] <= 1e-6 and comp[1] <= 1e-2
