@@ -347,10 +347,13 @@ int mdn_set_params(mdn_ctx* ctx, const double* kstr, const double* eql, const do
  * semantics as mdn_minimize per system; only the rounding order of the block reductions
  * differs), then pulls the next. ctxs[i] must have its system uploaded and its neighbor lists
  * built, all on one device; ctxs[0] is the lead context (its stream runs the batch and it keeps
- * the tuned schedule). coords[i] (3 * n_beads of system i) is the start and receives the result;
+ * the tuned schedule). coords[i] (3 * n_beads of system i) is the start and receives the result
+ * (coords[i] NULL: the start is the context's device coordinates, which receive the result, with
+ * no coordinate transfer over the host link);
  * forces_out may be null, or forces_out[i] may be null; results[i] is filled for every system.
- * The schedule (workgroup size 256 or 512, given or largest-first order) is chosen on the first
- * call for a given (n, precision, total beads) by timing a short run of each candidate.
+ * The schedule (workgroup size 128, 256 or 512 threads, given or largest-first order) is chosen on
+ * the first call for a given (n, precision, total beads) by timing a complete run of each candidate
+ * from the given coordinates.
  * Returns MDN_OK when every system succeeded, else the status of the first failing system (its
  * context's mdn_last_error and the lead's carry the message). No reference counterpart for the
  * batching itself: the reference minimizes one system per call. */
@@ -358,10 +361,11 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
                          double* const* coords, double* const* forces_out, mdn_min_result* results);
 
 /* The schedule of the last batched minimization on a lead context: out3 = [workgroup size,
- * largest-first order (1) or given order (0), tuned in that call (1) or reused (0)]; tune_ms4
- * (may be null) = the tuning times of the candidates (256 given, 256 sorted, 512 given,
- * 512 sorted) in ms. */
-int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms4);
+ * largest-first order (1) or given order (0), tuned in that call (1) or reused (0)]; tune_ms (may
+ * be null, MDN_BATCH_CANDIDATES values) = the tuning times in ms of the candidates 128 given,
+ * 128 largest first, 256 given, 256 largest first, 512 given, 512 largest first. */
+#define MDN_BATCH_CANDIDATES 6
+int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms);
 
 #ifdef __cplusplus
 }

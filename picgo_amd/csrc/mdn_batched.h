@@ -337,6 +337,21 @@ __global__ void __launch_bounds__(NT) k_cg_batched(const BatchItem* __restrict__
     }
 }
 
+// The final CG state and the energy report (27 values) of every system of a batch, gathered into
+// contiguous buffers for one read-back: states[i], reports[27 i ...]. Grid-stride over
+// (system, word).
+__global__ void __launch_bounds__(256) k_batch_collect(const BatchItem* __restrict__ items, int n,
+                                                       CgState* __restrict__ states, double* __restrict__ reports) {
+    constexpr int NW = sizeof(CgState) / sizeof(int);
+    static_assert(sizeof(CgState) % sizeof(int) == 0, "CgState in whole words");
+    const long total = (long)n * (NW + 27);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / (NW + 27)), w = (int)(k % (NW + 27));
+        if (w < NW) reinterpret_cast<int*>(states + i)[w] = reinterpret_cast<const int*>(items[i].st)[w];
+        else reports[27L * i + (w - NW)] = items[i].report[w - NW];
+    }
+}
+
 __global__ void k_fill_int(int* p, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }

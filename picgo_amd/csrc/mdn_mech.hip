@@ -145,10 +145,12 @@ struct mdn_ctx {
 
     // batched minimizer (the lead context of a batch owns these)
     DBuf<BatchItem> batch_items;
+    DBuf<CgState> batch_states;   // the batch's final states and reports, gathered for one read-back
+    DBuf<double> batch_reports;
     DBuf<int> batch_order, batch_queue;
     long batch_key = -1;          // (systems, precision, beads) of the tuned schedule
     int batch_nt = 256, batch_sorted = 1, batch_tuned = 0;
-    double batch_tune_ms[4] = {0, 0, 0, 0};
+    double batch_tune_ms[MDN_BATCH_CANDIDATES] = {0, 0, 0, 0, 0, 0};
 
     bool use_graph = true;   // replay the CG slots as a captured hipGraph (MDN_CG_GRAPH=0: launch them directly)
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // per precision (MDN_PREC_*)
@@ -534,6 +536,8 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     ctx->defer.release();
     ctx->defer_n.release();
     ctx->batch_items.release();
+    ctx->batch_states.release();
+    ctx->batch_reports.release();
     ctx->batch_order.release();
     ctx->batch_queue.release();
     ctx->cg.release();
@@ -1652,8 +1656,11 @@ static int launch_batched(mdn_ctx* L, int n, const BatchParams& P) {
 }
 
 static int run_batched(mdn_ctx* L, int n, const BatchParams& P, int prec, int nt) {
-    if (prec == MDN_PREC_MIXED) return nt == 512 ? launch_batched<float, 512>(L, n, P) : launch_batched<float, 256>(L, n, P);
-    return nt == 512 ? launch_batched<double, 512>(L, n, P) : launch_batched<double, 256>(L, n, P);
+    if (prec == MDN_PREC_MIXED)
+        return nt == 512 ? launch_batched<float, 512>(L, n, P)
+               : nt == 128 ? launch_batched<float, 128>(L, n, P) : launch_batched<float, 256>(L, n, P);
+    return nt == 512 ? launch_batched<double, 512>(L, n, P)
+           : nt == 128 ? launch_batched<double, 128>(L, n, P) : launch_batched<double, 256>(L, n, P);
 }
 
 extern "C" {
@@ -1669,7 +1676,7 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     long beads = 0;
     for (int i = 0; i < n; i++) {
         mdn_ctx* c = ctxs[i];
-        if (!c || !coords[i]) return fail(L, MDN_ERR_ARG, "system %d: null context or coordinates", i);
+        if (!c) return fail(L, MDN_ERR_ARG, "system %d: null context", i);
         if (c->device != L->device) return fail(L, MDN_ERR_ARG, "system %d is on another device", i);
         for (int j = 0; j < i; j++)
             if (ctxs[j] == c) return fail(L, MDN_ERR_ARG, "system %d repeats context %d", i, j);
@@ -1684,9 +1691,14 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     std::vector<long> size(n);
     for (int i = 0; i < n; i++) {
         mdn_ctx* c = ctxs[i];
+        const size_t bytes = sizeof(double) * 3 * c->n_beads;
+        if (coords[i])
+            HIPCHK_CTX(L, hipMemcpyAsync(c->x[0].p, coords[i], bytes, hipMemcpyHostToDevice, L->stream));
+        else if (c->cur != 0)   // the context's device coordinates
+            HIPCHK_CTX(L, hipMemcpyAsync(c->x[0].p, c->x[c->cur].p, bytes, hipMemcpyDeviceToDevice, L->stream));
         c->cur = 0;
-        HIPCHK_CTX(L, hipMemcpyAsync(c->x[0].p, coords[i], sizeof(double) * 3 * c->n_beads, hipMemcpyHostToDevice,
-                                     L->stream));
+        // the start, kept on the device for the schedule tuning runs (each starts from it)
+        HIPCHK_CTX(L, hipMemcpyAsync(c->ext.p, c->x[0].p, bytes, hipMemcpyDeviceToDevice, L->stream));
         items[i].s = c->sys;
         items[i].B = cg_bufs(c);
         items[i].st = c->cg.p;
@@ -1714,19 +1726,23 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     for (int i = 0; i < n; i++) order_given[i] = order_sorted[i] = i;
     std::stable_sort(order_sorted.begin(), order_sorted.end(), [&](int a, int b) { return size[a] > size[b]; });
     int st;
-    // schedule: workgroup size x system order, chosen once per (systems, precision, beads) by
-    // timing a short run of each candidate (force-call limit 20) on this batch
+    // schedule: workgroup size (128, 256, 512) x system order (given, largest first), chosen once
+    // per (systems, precision, beads) by timing each candidate on this batch, run to completion
+    // from the caller's coordinates: the batch time is set by its slowest systems, whose iteration
+    // counts only a complete run shows (a short run measures the start-up instead)
     const long key = ((long)n * 2 + p->precision) * 1000003L + beads;
     if (L->batch_key != key) {
-        BatchParams T = P;
-        T.limit_override = 20;
+        const BatchParams T = P;
         hipEvent_t e0, e1;
         HIPCHK_CTX(L, hipEventCreate(&e0));
         HIPCHK_CTX(L, hipEventCreate(&e1));
         float best = 1e30f;
-        const int nts[2] = {256, 512};
-        for (int c = 0; c < 4; c++) {
+        const int nts[3] = {128, 256, 512};
+        for (int c = 0; c < MDN_BATCH_CANDIDATES; c++) {
             const int nt = nts[c >> 1], sorted = c & 1;
+            for (int i = 0; i < n; i++)   // every candidate starts from the given coordinates
+                HIPCHK_CTX(L, hipMemcpyAsync(ctxs[i]->x[0].p, ctxs[i]->ext.p, sizeof(double) * 3 * ctxs[i]->n_beads,
+                                             hipMemcpyDeviceToDevice, L->stream));
             HIPCHK_CTX(L, hipMemcpyAsync(L->batch_order.p, sorted ? order_sorted.data() : order_given.data(),
                                          sizeof(int) * n, hipMemcpyHostToDevice, L->stream));
             HIPCHK_CTX(L, hipEventRecord(e0, L->stream));
@@ -1746,10 +1762,10 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
         (void)hipEventDestroy(e1);
         L->batch_key = key;
         L->batch_tuned = 1;
-        // the tuning runs moved the coordinates: start again from the caller's
+        // the tuning runs moved the coordinates: start again from the given ones
         for (int i = 0; i < n; i++)
-            HIPCHK_CTX(L, hipMemcpyAsync(ctxs[i]->x[0].p, coords[i], sizeof(double) * 3 * ctxs[i]->n_beads,
-                                         hipMemcpyHostToDevice, L->stream));
+            HIPCHK_CTX(L, hipMemcpyAsync(ctxs[i]->x[0].p, ctxs[i]->ext.p, sizeof(double) * 3 * ctxs[i]->n_beads,
+                                         hipMemcpyDeviceToDevice, L->stream));
     } else {
         L->batch_tuned = 0;
     }
@@ -1757,22 +1773,39 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
                                  sizeof(int) * n, hipMemcpyHostToDevice, L->stream));
     if ((st = run_batched(L, n, P, p->precision, L->batch_nt))) return st;
     HIPCHK_CTX(L, hipStreamSynchronize(L->stream));
-    // results
-    int first_err = MDN_OK;
-    std::vector<double> rep(27);
+    // results: every system's state and report gathered on the device and read back at once, then
+    // the coordinates (and forces) of every system queued with one synchronization
+    HIPCHK_CTX(L, L->batch_states.ensure((size_t)n));
+    HIPCHK_CTX(L, L->batch_reports.ensure(27 * (size_t)n));
+    hipLaunchKernelGGL(k_batch_collect, dim3(std::min(1024, std::max(1, ceil_div(60L * n, 256)))), dim3(256), 0,
+                       L->stream, (const BatchItem*)L->batch_items.p, n, L->batch_states.p, L->batch_reports.p);
+    HIPCHK_CTX(L, hipGetLastError());
+    std::vector<CgState> states(n);
+    std::vector<double> reports(27 * (size_t)n);
+    HIPCHK_CTX(L, hipMemcpyAsync(states.data(), L->batch_states.p, sizeof(CgState) * n, hipMemcpyDeviceToHost,
+                                 L->stream));
+    HIPCHK_CTX(L, d2h(L, reports.data(), L->batch_reports.p, sizeof(double) * 27 * n));
     for (int i = 0; i < n; i++) {
         mdn_ctx* c = ctxs[i];
-        CgState S;
-        HIPCHK_CTX(L, d2h(L, &S, c->cg.p, sizeof(CgState)));
-        HIPCHK_CTX(L, d2h(L, rep.data(), c->eout.p, sizeof(double) * 27));
+        const CgState& S = states[i];
+        const int nv = 3 * c->n_beads;
+        if (coords[i])
+            HIPCHK_CTX(L, hipMemcpyAsync(coords[i], c->x[S.parity].p, sizeof(double) * nv, hipMemcpyDeviceToHost,
+                                         L->stream));
+        if (forces_out && forces_out[i])
+            HIPCHK_CTX(L, hipMemcpyAsync(forces_out[i], (S.fpar ? c->f2 : c->f).p, sizeof(double) * nv,
+                                         hipMemcpyDeviceToHost, L->stream));
+    }
+    HIPCHK_CTX(L, hipStreamSynchronize(L->stream));
+    int first_err = MDN_OK;
+    for (int i = 0; i < n; i++) {
+        mdn_ctx* c = ctxs[i];
+        const CgState& S = states[i];
+        const double* rep = reports.data() + 27L * i;
         c->last_cg = S;
         c->cur = S.parity;
         const bool ok = S.status == 0 && S.error_flags == 0;
-        fill_result(S, rep.data(), rep.data() + 9, ok, &results[i]);
-        const int nv = 3 * c->n_beads;
-        HIPCHK_CTX(L, d2h(L, coords[i], c->x[c->cur].p, sizeof(double) * nv));
-        if (forces_out && forces_out[i])
-            HIPCHK_CTX(L, d2h(L, forces_out[i], (S.fpar ? c->f2 : c->f).p, sizeof(double) * nv));
+        fill_result(S, rep, rep + 9, ok, &results[i]);
         int code = MDN_OK;
         if (S.status != 0) {
             code = (S.status == ST_NEED_FORCEN || S.status == 9) ? MDN_ERR_UNSUPPORTED : S.status;
@@ -1797,14 +1830,14 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
 
 // The schedule of the last batched minimization on a lead context: [workgroup size,
 // largest-first order (1) or given order (0), tuned in that call (1) or reused (0)], and the
-// tuning times (ms) of the four candidates (256 given, 256 sorted, 512 given, 512 sorted).
-int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms4) {
+// tuning times (ms) of the candidates (128, 256, 512 threads, each in given then sorted order).
+int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms) {
     if (!ctx || !out3) return MDN_ERR_ARG;
     out3[0] = ctx->batch_nt;
     out3[1] = ctx->batch_sorted;
     out3[2] = ctx->batch_tuned;
-    if (tune_ms4)
-        for (int c = 0; c < 4; c++) tune_ms4[c] = ctx->batch_tune_ms[c];
+    if (tune_ms)
+        for (int c = 0; c < MDN_BATCH_CANDIDATES; c++) tune_ms[c] = ctx->batch_tune_ms[c];
     return MDN_OK;
 }
 

@@ -325,13 +325,15 @@ def minimize_batched(systems, params, coords, want_forces=True):
     """CGMethod::minimize of many independent systems in one persistent kernel
     (mdn_minimize_batched). ``systems``: ForceFieldManager objects with uploaded systems and
     built neighbor lists (the first one is the batch's lead context); ``coords``: one start
-    vector per system. Returns (list of coordinates, list of forces or None, list of MinResult)."""
+    vector per system (an entry None, or coords None: that system starts from its device
+    coordinates and keeps the result there). Returns (list of coordinates or None, list of forces
+    or None, list of MinResult)."""
     L = lib()
     n = len(systems)
-    xs = [_f64(c).copy() for c in coords]
-    fs = [np.zeros_like(x) for x in xs] if want_forces else None
+    xs = [None if c is None else _f64(c).copy() for c in (coords if coords is not None else [None] * n)]
+    fs = [np.zeros(3 * s.n_beads) for s in systems] if want_forces else None
     ctxs = (C.c_void_p * n)(*[s._h.value for s in systems])
-    xp = (dp * n)(*[x.ctypes.data_as(dp) for x in xs])
+    xp = (dp * n)(*[None if x is None else x.ctypes.data_as(dp) for x in xs])
     fp = (dp * n)(*[f.ctypes.data_as(dp) for f in fs]) if want_forces else None
     res = (abi.MinResult * n)()
     st = L.mdn_minimize_batched(ctxs, n, C.byref(params), xp, fp, res)
@@ -343,7 +345,7 @@ def minimize_batched(systems, params, coords, want_forces=True):
 def last_batch_schedule(lead):
     """(workgroup size, largest-first order, tuned in the last call, tuning times in ms)."""
     out = (C.c_int32 * 3)()
-    ms = (C.c_double * 4)()
+    ms = (C.c_double * 6)()
     st = lib().mdn_last_batch_schedule(lead._h, out, ms)
     if st != 0:
         raise MdnError(st, "mdn_last_batch_schedule")

@@ -146,3 +146,22 @@ def test_batched_queue_multiple_pulls(gpu, prec):
         assert (rc[i].num_iter, rc[i].num_energy_call, rc[i].num_force_call) == \
             (ro.num_iter, ro.num_energy_call, ro.num_force_call)
         np.testing.assert_allclose(xc[i], xo, rtol=1e-8, atol=0)
+
+
+def test_batched_from_device_coordinates(gpu):
+    """Systems whose coordinates entry is None start from their contexts' device coordinates and
+    keep the result there: the same minimizations, bit for bit, as from host buffers."""
+    nets, ffms = _batch(gpu, 12, dense=True)
+    p = abi.default_cg_params(energy_call_limit=40)
+    xh, fh, rh = mech.minimize_batched(ffms, p, [n.coord for n in nets])
+    for g, n in zip(ffms, nets):
+        g.set_coords(n.coord)
+    coords = [None if i % 2 else n.coord for i, n in enumerate(nets)]   # half device-resident
+    xd, fd, rd = mech.minimize_batched(ffms, p, coords)
+    for i, g in enumerate(ffms):
+        x = g.get_coords() if coords[i] is None else xd[i]
+        if coords[i] is None:
+            assert xd[i] is None
+        assert (rh[i].num_iter, rh[i].num_energy_call) == (rd[i].num_iter, rd[i].num_energy_call), i
+        np.testing.assert_array_equal(x, xh[i])
+        np.testing.assert_array_equal(fd[i], fh[i])
