@@ -94,6 +94,19 @@ public:
         check(mdn_nl_build(ctx_, coord.data(), &n));
         return (long)n;
     }
+    // the same from the context's device coordinates (the last upload or minimization result)
+    long resetNeighborLists() {
+        int64_t n = 0;
+        check(mdn_nl_build(ctx_, nullptr, &n));
+        return (long)n;
+    }
+    // the device coordinates: read back, replace, displace by seeded Gaussian bead noise
+    void getCoords(std::vector<double>& coord) {
+        coord.resize((size_t)nvar_);
+        check(mdn_get_coords(ctx_, coord.data()));
+    }
+    void setCoords(const std::vector<double>& coord) { check(mdn_set_coords(ctx_, coord.data())); }
+    void perturb(uint64_t seed, uint64_t stream, double sigma) { check(mdn_perturb(ctx_, seed, stream, sigma)); }
     EnergyReport computeIndividualEnergies(const double* coord) {
         mdn_energy_report r{};
         check(mdn_energy(ctx_, coord, &r));
@@ -289,6 +302,16 @@ public:
         if (outForce) outForce->assign(coord.size(), 0.0);
         int st = mdn_minimize(ffm.ctx(), &cp, coord.data(), outForce ? outForce->data() : nullptr, &res);
         if (st == MDN_ERR_LINESEARCH) return res;   // MController runs its diagnosis and throws
+        if (st != MDN_OK) throw Error(st, mdn_last_error(ffm.ctx()));
+        return res;
+    }
+    // minimize from the context's device coordinates and leave the result there (no coordinate
+    // transfer: successive minimizations between chemistry steps that change no topology)
+    MinimizationResult minimize(ForceFieldManager& ffm, const ConjugateGradientParams& p) {
+        MinimizationResult res{};
+        mdn_cg_params cp = p.c();
+        int st = mdn_minimize(ffm.ctx(), &cp, nullptr, nullptr, &res);
+        if (st == MDN_ERR_LINESEARCH) return res;
         if (st != MDN_OK) throw Error(st, mdn_last_error(ffm.ctx()));
         return res;
     }

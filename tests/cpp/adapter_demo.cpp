@@ -1,8 +1,9 @@
 // C++ caller of the drop-in adapters (include/mdn_mech.hpp): the MController-shaped sequence
 // serialize/vectorize -> reset neighbor lists -> minimize -> stretch forces and boundary load
 // forces, on the seeded 50-filament network, and the per-force-field adapters (ForceField).
-// Prints "pairs energy_before energy_after iterations max_force converged n_linker_stretch_forces
-// n_loaded_filament_ends force_field_adapters_ok".
+// Then the device-resident calls. Prints "pairs energy_before energy_after iterations max_force
+// converged n_linker_stretch_forces n_loaded_filament_ends force_field_adapters_ok
+// device_resident_ok".
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -56,8 +57,30 @@ int main() {
         auto* lk = dynamic_cast<mdn::GpuLinkerStretching*>(ffm.forceFields()[MDN_FF_LINKER].get());
         lk->assignforcemags();
         ff_ok &= lk->stretchForce() == lsf;
-        std::printf("%ld %.10e %.10e %d %.6e %d %zu %d %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
-                    (int)res.converged(), lsf.size(), loaded, ff_ok);
+        // device-resident minimization (no coordinate transfer) of a displaced state: the same
+        // iterations and bit-identical coordinates as the host-buffer call on the same state
+        std::vector<double> xp(coord), xh;
+        for (size_t i = 0; i < xp.size(); i++) xp[i] += 2.0 * std::sin(0.7 * (double)i);
+        xh = xp;
+        ffm.resetNeighborLists(xh);
+        auto rh = cg.minimize(ffm, p, xh, nullptr);
+        ffm.setCoords(xp);
+        ffm.resetNeighborLists();
+        auto rd = cg.minimize(ffm, p);
+        std::vector<double> xd;
+        ffm.getCoords(xd);
+        int dev_ok = xd == xh && rd.num_iter == rh.num_iter && rd.converged();
+        // seeded bead noise generated on the device: unit variance per coordinate
+        ffm.setCoords(xd);
+        ffm.perturb(3, 1, 1.0);
+        std::vector<double> xn;
+        ffm.getCoords(xn);
+        double ss = 0;
+        for (size_t i = 0; i < xn.size(); i++) ss += (xn[i] - xd[i]) * (xn[i] - xd[i]);
+        const double rms = std::sqrt(ss / (double)xn.size());
+        dev_ok &= rms > 0.9 && rms < 1.1;
+        std::printf("%ld %.10e %.10e %d %.6e %d %zu %d %d %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
+                    (int)res.converged(), lsf.size(), loaded, ff_ok, dev_ok);
     } catch (const mdn::Error& e) {
         std::printf("error %d %s\n", e.code, e.what());
         mdn_synth_destroy(net);

@@ -26,8 +26,9 @@ def test_adapter_runs_minimization():
     exe = EXE if os.path.exists(EXE) else build_demo()
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
-    pairs, e0, e1, iters, mx, conv, nsf, loaded, ff_ok = out.stdout.split()
+    pairs, e0, e1, iters, mx, conv, nsf, loaded, ff_ok, dev_ok = out.stdout.split()
     assert int(pairs) > 0 and int(conv) == 1
     assert float(e1) < float(e0) and float(mx) < 1.0 and int(iters) > 10
     assert int(nsf) == 20 and int(loaded) > 0   # the 550-bead network: 20 linkers; filament ends near the walls
     assert int(ff_ok) == 1   # the six ForceField adapters add up to the manager's forces and energy
+    assert int(dev_ok) == 1  # device-resident minimization = host-buffer one; device noise of unit variance
