@@ -118,6 +118,12 @@ typedef struct mdn_energy_report {
 } mdn_energy_report;
 
 enum { MDN_CG_STEEPEST = 0, MDN_CG_FLETCHER_REEVES = 1, MDN_CG_POLAK_RIBIERE = 2 };
+/* ConjugateGradientParams::lineSearchAlgorithm (CGMethod.hpp:51, :1319-1331): BACKTRACKING, or
+ * QUADRATIC (quadraticLineSearchV2 + quadraticoptimization, CGMethod.hpp:522-764, outside safe
+ * mode; safe mode backtracks with slope 0 in both). The quadratic search here evaluates every
+ * trial at coord + lambda * searchDir; the reference's version moves coord in place and evaluates
+ * a line-search buffer it does not refresh, so its values depend on earlier calls. */
+enum { MDN_LS_BACKTRACKING = 0, MDN_LS_QUADRATIC = 1 };
 enum { MDN_PREC_FP64 = 0, MDN_PREC_MIXED = 1 };
 
 /* ConjugateGradientParams (CGMethod.hpp:44-63) plus the precision selector. */
@@ -132,6 +138,8 @@ typedef struct mdn_cg_params {
     int32_t energy_call_limit;    /* 0 = no limit                                           */
     int32_t report_on_call_limit; /* restore minimum-energy coordinates on call limit       */
     int32_t precision;            /* MDN_PREC_*                                             */
+    int32_t line_search;          /* MDN_LS_* (0 = BACKTRACKING, the reference's default)   */
+    int32_t reserved;             /* 0                                                      */
 } mdn_cg_params;
 
 enum { MDN_MIN_ERR_LINESEARCH = 1 };   /* MinimizationResult::errorLineSearch */

@@ -269,10 +269,12 @@ struct ConjugateGradientParams {
     bool reportOnCallLimit = true;
     bool tryToRecoverInLineSearchError = true;
     int precision = MDN_PREC_FP64;
+    int lineSearchAlgorithm = MDN_LS_BACKTRACKING;   // "BACKTRACKING" / "QUADRATIC" (CGMethod.hpp:51)
 
     mdn_cg_params c() const {
         mdn_cg_params p{};
         p.method = descentSearchMethod;
+        p.line_search = lineSearchAlgorithm;
         p.try_recover = tryToRecoverInLineSearchError;
         p.max_dist = maxDist;
         p.lambda_max = lambdaMax;

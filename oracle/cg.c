@@ -1,6 +1,7 @@
 /*
- * cg.c -- oracle restatement of CGMethod::minimize with the backtracking line search, and
- * of the MController wiring with the recovery move. TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * cg.c -- oracle restatement of CGMethod::minimize with the backtracking and quadratic line
+ * searches, and of the MController wiring with the recovery move. TEST INFRASTRUCTURE ONLY (see
+ * oracle.h).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -91,6 +92,98 @@ static ls_result backtracking(cg_state* st, const double* coord, const or_cg_cal
     return res;
 }
 
+/* quadraticLineSearchV2 (CGMethod.hpp:522-656) and quadraticoptimization (:660-764), with every
+ * trial evaluated at coord + lambda * searchDir (the reference moves coord in place and evaluates
+ * coordLineSearch, which it does not refresh). Two guards the reference lacks, both on paths where
+ * it would evaluate a non-finite or endless trial: a non-finite parabola vertex returns the middle
+ * point x1 without an evaluation (the reference evaluates it, then returns x1 through its "out of
+ * range" branch), and the parabola iterations stop at QUAD_ITER_CAP, returning x1. */
+#define QUADTOL_LAMBDA 1e-3   /* LAMBDAQUADTOL (CGMethod.hpp:106) */
+#define QUAD_ITER_CAP 64
+
+static double quad_opt(cg_state* st, const double* coord, const or_cg_callbacks* cb, int* numEnergyCall,
+                       double x0, double x1, double x2, double y0, double y1, double y2) {
+    const double btlambda = x1;
+    for (int iter = 1;; iter++) {
+        if (iter > QUAD_ITER_CAP) return x1;
+        const double d = (x0 - x1) * (x0 - x2) * (x1 - x2);
+        const double a = y0 * (x1 - x2) - y1 * (x0 - x2) + y2 * (x0 - x1);
+        const double b = -(y0 * (x1 + x2) * (x1 - x2) - y1 * (x0 + x2) * (x0 - x2) + y2 * (x0 + x1) * (x0 - x1));
+        const double lq = -b / (2 * a);
+        if (!isfinite(lq)) return x1;
+        if (fabs(lq - x1) / x1 < QUADTOL_LAMBDA) return lq;
+        if (a / d < 0) return btlambda;   /* the model has no minimum */
+        for (int i = 0; i < st->ndof; i++) st->coordLS[i] = coord[i] + lq * st->searchDir[i];
+        const double eq = cb->energy(cb->user, st->coordLS);
+        ++*numEnergyCall;
+        if (fabs(eq - y1) / y1 < QUADTOL_LAMBDA) return lq;
+        if (lq < x1) {
+            if (eq < y0 && eq < y1) { x1 = lq; y1 = eq; }
+            else { x0 = lq; y0 = eq; }
+        } else if (x1 < lq) {
+            if (eq < y1 && eq < y2) { x1 = lq; y1 = eq; }
+            else { x2 = lq; y2 = eq; }
+        } else {
+            return x1;
+        }
+    }
+}
+
+static ls_result quadratic(cg_state* st, const double* coord, const or_cg_callbacks* cb, double MAXDIST,
+                           double maxForce, double LAMBDAMAX) {
+    ls_result res = {0.0, 0, 0.0};
+    if (maxForce == 0.0) return res;
+    double lambda = rmin(LAMBDAMAX, MAXDIST / maxForce);
+    const double currentEnergy = cb->energy(cb->user, coord);
+    ++res.numEnergyCall;
+    const double slope = dot_n(st->searchDir, st->force, st->ndof);
+    double energyLambda = 0.0, prevEnergy = 0.0, prevLambda = 0.0;
+    /* the recorded (lambda, energy) list: its first entry and the largest energy of the entries
+     * after it, which is all the quadratic phase reads of it */
+    double lv0 = 0.0, ev0 = 0.0, emax = -INFINITY;
+    int n = 0, iter = 0, conv = 0;
+    while (!conv) {
+        iter++;
+        for (int i = 0; i < st->ndof; i++) st->coordLS[i] = coord[i] + lambda * st->searchDir[i];
+        energyLambda = cb->energy(cb->user, st->coordLS);
+        ++res.numEnergyCall;
+        const double idealEnergyChange = -BACKTRACKSLOPE * lambda * slope;
+        if (idealEnergyChange > 0) {   /* the reference exits the program; a failed search here */
+            lambda = 0.0;
+            break;
+        }
+        if (energyLambda - currentEnergy <= idealEnergyChange) conv = 1;
+        else {
+            prevLambda = lambda;
+            prevEnergy = energyLambda;
+            lambda *= LAMBDAREDUCE;
+        }
+        double l, e;
+        if (lambda <= 0.0 || lambda <= LAMBDATOL) {
+            conv = 1;
+            lambda = 0.0;
+            l = lambda; e = currentEnergy;
+        } else if (conv) {
+            l = lambda; e = energyLambda;
+        } else {
+            l = prevLambda; e = prevEnergy;
+        }
+        if (n++ == 0) { lv0 = l; ev0 = e; }
+        else emax = rmax(emax, e);
+    }
+    /* quadstatus: some entry after the first holds more energy than the accepted trial; the
+     * reference's loop leaves its index at 0, so the third point is the first entry (the first
+     * trial) */
+    if (lambda > 0.0 && iter > 1 && energyLambda < currentEnergy && energyLambda < emax) {
+        const double lq = quad_opt(st, coord, cb, &res.numEnergyCall, 0.0, lambda, lv0, currentEnergy, energyLambda, ev0);
+        lambda = (lq <= 0.0 || lq <= LAMBDATOL) ? 0.0 : lq;
+    }
+    st->totalEnergy = lambda > 0 ? energyLambda : currentEnergy;
+    res.lambda = lambda;
+    res.newEnergy = energyLambda;
+    return res;
+}
+
 static int call_limit_exceeded(const mdn_min_result* r) {
     return (r->energy_call_limit != 0 && r->num_energy_call > r->energy_call_limit) ||
            (r->force_call_limit != 0 && r->num_force_call > r->force_call_limit);
@@ -147,8 +240,11 @@ int or_cg_minimize(const mdn_cg_params* p, double* coord, int ndof, int nvar,
     while (!call_limit_exceeded(res) && !converged(res)) {
         double beta = 0, newGrad = 0, prevGrad = 0;
         numIter++;
-        ls_result lsr = backtracking(&st, coord, cb, p->max_dist, res->grad_inf_norm,
-                                     p->lambda_max, safeMode ? 0 : BACKTRACKSLOPE);
+        /* CGMethod.hpp:1319-1331: QUADRATIC backtracks with slope 0 in safe mode */
+        ls_result lsr = (p->line_search == MDN_LS_QUADRATIC && !safeMode)
+                            ? quadratic(&st, coord, cb, p->max_dist, res->grad_inf_norm, p->lambda_max)
+                            : backtracking(&st, coord, cb, p->max_dist, res->grad_inf_norm, p->lambda_max,
+                                           safeMode ? 0 : BACKTRACKSLOPE);
         const double lambda = lsr.lambda;
         res->num_energy_call += lsr.numEnergyCall;
         if (lambda == 0) {

@@ -27,6 +27,7 @@ MDN_FF_STRETCH, MDN_FF_BEND, MDN_FF_LINKER, MDN_FF_MOTOR, MDN_FF_EXVOL, MDN_FF_B
 MDN_BND_PLANE, MDN_BND_SPHERE = 0, 1
 MDN_CG_STEEPEST, MDN_CG_FLETCHER_REEVES, MDN_CG_POLAK_RIBIERE = 0, 1, 2
 MDN_PREC_FP64, MDN_PREC_MIXED = 0, 1
+MDN_LS_BACKTRACKING, MDN_LS_QUADRATIC = 0, 1
 MDN_MIN_ERR_LINESEARCH = 1
 
 c_int32_p = C.POINTER(C.c_int32)
@@ -71,7 +72,7 @@ class CgParams(C.Structure):
                 ("lambda_max", C.c_double), ("grad_tol", C.c_double),
                 ("energy_rel_tol", C.c_double), ("force_call_limit", C.c_int32),
                 ("energy_call_limit", C.c_int32), ("report_on_call_limit", C.c_int32),
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("line_search", C.c_int32), ("reserved", C.c_int32)]
 
 
 def default_cg_params(**kw):

@@ -27,6 +27,7 @@ struct BatchItem {
 struct BatchParams {
     int method, try_recover, force_call_limit, energy_call_limit, report_on_call_limit;
     int limit_override;   // > 0: force-call limit of every system (schedule tuning runs)
+    int line_search;      // MDN_LS_*
     double max_dist, lambda_max, grad_tol, energy_rel_tol;
 };
 
@@ -257,6 +258,7 @@ __device__ inline void blk_minimize(const BatchItem& it, const BatchParams& P, C
     if (threadIdx.x == 0) {
         S = CgState{};
         S.method = P.method;
+        S.ls_algo = P.line_search;
         S.try_recover = P.try_recover;
         S.force_call_limit = P.limit_override > 0 ? P.limit_override : max(P.force_call_limit, 3 * nb);
         S.energy_call_limit = P.energy_call_limit;
@@ -291,9 +293,10 @@ __device__ inline void blk_minimize(const BatchItem& it, const BatchParams& P, C
         sh_bad = 0;
     }
     __syncthreads();
-    // CG slots; every slot either advances the call counters or ends the line search, so the
-    // bound below is never reached by a correct state machine (it guards the exit)
-    const long max_slots = 4L * ((long)S.force_call_limit + 16) + 128;
+    // CG slots; a step costs at most one launch per two halvings from lambda0 to the tolerance,
+    // the parabola fits of the quadratic search and a move, so the bound below is never reached
+    // by a correct state machine (it guards the exit)
+    const long max_slots = slots_per_force_call(S) * ((long)S.force_call_limit + 16) + 128;
     for (long k = 0; k < max_slots && !S.done; k++) {
         if (S.phase == PH_LS) blk_ls<R, NT>(s, B, S, red, sh_bad);
         if (!S.done && S.phase == PH_MOVE) blk_move<R, NT>(s, B, S, sh_bad);

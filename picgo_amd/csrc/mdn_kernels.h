@@ -154,6 +154,12 @@ struct CgState {
     // minimum-energy backup (CGMethod.hpp:794-814)
     double total_energy, minimum_e;
     int have_min_e;
+    // the quadratic line search (MDN_LS_QUADRATIC): line_search = ls_algo; q_phase marks a launch
+    // that evaluates the parabola's vertex; the recorded (lambda, energy) list as its first entry
+    // and the largest later energy; the parabola's points; the backtracking step and energy;
+    // q_stale: the step is not the accepted trial, so E(x_new) must be evaluated
+    int ls_algo, q_phase, q_nrec, q_iter, q_stale, q_pad;
+    double q_lv0, q_ev0, q_emax, q_x0, q_x1, q_x2, q_y0, q_y1, q_y2, q_bt, q_ebt;
     // launches that did work (statistics)
     int n_ls_launch, n_move_launch, n_gather_launch;
     int reserved[2];
@@ -1155,81 +1161,9 @@ __device__ inline bool converged_s(const CgState& S) {
 // Scalar logic after the trial energies: backtrackingLineSearch (CGMethod.hpp:238-373) over the
 // evaluated step lengths, and the failure handling of CGMethod::minimize (:1337-1367).
 // acc = {E(lambda), E(lambda/2), bad, bad2, E(x), bad0, d.F, recovery step}
-__device__ inline void ls_control(CgState& S, const double* acc, int& bad_count) {
-    const bool first = S.ls_active == 0;
-    const double E1 = acc[2] > 0.0 ? INFINITY : acc[0];   // computeEnergy: -1 sentinel -> +inf
-    const double E2 = acc[3] > 0.0 ? INFINITY : acc[1];
-    const double lam0 = S.ls_lambda;
-    S.n_ls_launch += 1;
-    if (first) {
-        if (S.dir_pending) {
-            if (!S.dir_reset && acc[6] <= 0) {
-                // d.F <= 0 after shiftSearchDir(beta): resetCGParams (CGMethod.hpp:1505-1507);
-                // redo this line-search launch with d = F
-                S.dir_beta = 0.0;
-                S.dir_reset = 1;
-                bad_count = 0;
-                return;
-            }
-            S.dpar ^= 1;
-            S.dir_pending = 0;
-            if (S.dir_reset) S.last_fdotf = S.fdotf;   // lastFDotF = F.F
-            S.dir_reset = 0;
-        }
-        S.num_iter += 1;             // numIter++ at the top of the loop (CGMethod.hpp:1297)
-        S.rec_step = acc[7];
-        S.ls_trials = 0;
-        if (S.grad_inf_norm != 0.0) {   // maxForce == 0 returns lambda = 0 at once (:252-256)
-            if (!S.ecur_valid) S.ls_cur = acc[5] > 0.0 ? INFINITY : acc[4];
-            S.num_energy_call += 1;
-            S.ls_slope = acc[6];
-            S.ls_reduced = (S.safe_mode ? 0.0 : 0.4) * acc[6];
-            S.ls_active = 1;
-        }
-    }
-    bool conv = false;
-    int acc_trial = -1;
-    double lambda = lam0, energyLambda = 0.0;
-    if (first && S.grad_inf_norm == 0.0) {
-        conv = true;
-        lambda = 0.0;
-    }
-    // The first launch of a search evaluates lambda0 alone (usually accepted: its forces are
-    // the speculative ones); after a rejection each launch evaluates the next NTRIAL halvings.
-    // The trial sequence is the reference's either way.
-    const int ntrial = first ? 1 : NTRIAL;
-    for (int k = 0; k < ntrial && !conv; k++) {
-        energyLambda = k == 0 ? E1 : E2;
-        S.num_energy_call += 1;
-        S.ls_trials += 1;
-        const double ideal = -S.ls_reduced * lambda;
-        if (lambda < 0 || ideal > 0) { conv = true; lambda = 0.0; }
-        const double change = energyLambda - S.ls_cur;
-        if (change <= ideal) { conv = true; acc_trial = k; }
-        else lambda *= 0.5;
-        if (lambda <= 0.0 || lambda <= 1e-8) {
-            conv = true;
-            acc_trial = -1;
-            lambda = 0.0;
-            energyLambda = S.ls_cur;
-        }
-    }
-    const bool spec = bad_count == 0;
-    bad_count = 0;
-    if (!conv) {                 // keep halving in the next launch
-        S.ls_lambda = lambda;
-        return;
-    }
-    S.total_energy = energyLambda;   // TotalEnergy = energyLambda (CGMethod.hpp:368)
-    S.ls_active = 0;
-    if (lambda != 0.0) {
-        S.lambda = lambda;
-        S.new_energy = energyLambda;
-        S.recovery_move = 0;
-        // accepted at the first trial with usable speculative forces: gather directly
-        S.phase = (acc_trial == 0 && spec) ? PH_GATHER : PH_MOVE;
-        return;
-    }
+// A search that ends with lambda = 0 (CGMethod.hpp:1335-1365): recovery on -> safe mode, or the
+// recovery move when already in safe mode; recovery off -> the line-search error.
+__device__ inline void ls_failed(CgState& S) {
     if (S.try_recover) {
         if (S.safe_mode) {
             if (S.rec_step == INFINITY) {   // MController.h:312-316 throws
@@ -1255,6 +1189,179 @@ __device__ inline void ls_control(CgState& S, const double* acc, int& bad_count)
     }
     S.error_flags |= 1u;
     S.done = 1;
+}
+
+// ---- the quadratic line search (quadraticLineSearchV2 + quadraticoptimization,
+// CGMethod.hpp:522-764), every trial at x + lambda d; the same guards as the oracle's: a
+// non-finite vertex returns x1 unevaluated, and at most QUAD_ITER_CAP parabola fits.
+constexpr int QUAD_ITER_CAP = 64;
+constexpr double LAMBDAQUADTOL = 1e-3;   // CGMethod.hpp:106
+
+// the search's end at step lam (spec: this launch's speculative forces are at lam)
+__device__ inline void quad_finish(CgState& S, double lam, bool spec) {
+    S.q_phase = 0;
+    S.ls_active = 0;
+    if (lam <= 0.0 || lam <= 1e-8) lam = 0.0;
+    S.total_energy = lam > 0.0 ? S.q_ebt : S.ls_cur;   // TotalEnergy (CGMethod.hpp:649-652)
+    if (lam == 0.0) {
+        ls_failed(S);
+        return;
+    }
+    S.lambda = lam;
+    S.new_energy = S.q_ebt;        // res.newEnergy = the backtracking phase's energy (:655)
+    S.recovery_move = 0;
+    S.q_stale = 1;
+    S.phase = spec ? PH_GATHER : PH_MOVE;
+}
+
+// The next parabola fit through (x0, y0), (x1, y1), (x2, y2): ends the search, or sets up the
+// launch that evaluates the vertex.
+__device__ inline void quad_next(CgState& S) {
+    if (++S.q_iter > QUAD_ITER_CAP) {
+        quad_finish(S, S.q_x1, false);
+        return;
+    }
+    const double x0 = S.q_x0, x1 = S.q_x1, x2 = S.q_x2, y0 = S.q_y0, y1 = S.q_y1, y2 = S.q_y2;
+    const double d = (x0 - x1) * (x0 - x2) * (x1 - x2);
+    const double a = y0 * (x1 - x2) - y1 * (x0 - x2) + y2 * (x0 - x1);
+    const double b = -(y0 * (x1 + x2) * (x1 - x2) - y1 * (x0 + x2) * (x0 - x2) + y2 * (x0 + x1) * (x0 - x1));
+    const double lq = -b / (2 * a);
+    if (!isfinite(lq)) { quad_finish(S, x1, false); return; }
+    if (fabs(lq - x1) / x1 < LAMBDAQUADTOL) { quad_finish(S, lq, false); return; }
+    if (a / d < 0) { quad_finish(S, S.q_bt, false); return; }   // the model has no minimum
+    S.ls_lambda = lq;   // trial 1 of the next launch
+    S.q_phase = 1;
+}
+
+// The vertex's energy eq (trial 1 of this launch): accept it, or replace a bracket point.
+__device__ inline void quad_eval(CgState& S, double eq, bool spec) {
+    const double lq = S.ls_lambda;
+    S.num_energy_call += 1;
+    if (fabs(eq - S.q_y1) / S.q_y1 < LAMBDAQUADTOL) { quad_finish(S, lq, spec); return; }
+    if (lq < S.q_x1) {
+        if (eq < S.q_y0 && eq < S.q_y1) { S.q_x1 = lq; S.q_y1 = eq; }
+        else { S.q_x0 = lq; S.q_y0 = eq; }
+    } else if (S.q_x1 < lq) {
+        if (eq < S.q_y1 && eq < S.q_y2) { S.q_x1 = lq; S.q_y1 = eq; }
+        else { S.q_x2 = lq; S.q_y2 = eq; }
+    } else {
+        quad_finish(S, S.q_x1, false);
+        return;
+    }
+    quad_next(S);
+}
+
+// An upper bound on the minimizer slots per force call (the guard on its exit): the first launch
+// of a search, one launch per two of the at most 60 halvings of lambda0 <= 1e10 down to 1e-8,
+// the parabola fits of the quadratic search, a move.
+MDN_HD inline long slots_per_force_call(const CgState& S) { return 32 + (S.ls_algo == 1 ? QUAD_ITER_CAP : 0); }
+
+__device__ inline void ls_control(CgState& S, const double* acc, int& bad_count) {
+    const bool first = S.ls_active == 0;
+    const double E1 = acc[2] > 0.0 ? INFINITY : acc[0];   // computeEnergy: -1 sentinel -> +inf
+    const double E2 = acc[3] > 0.0 ? INFINITY : acc[1];
+    const double lam0 = S.ls_lambda;
+    S.n_ls_launch += 1;
+    if (S.q_phase) {   // a launch that evaluated the parabola's vertex (trial 1)
+        const bool spec = bad_count == 0;
+        bad_count = 0;
+        quad_eval(S, E1, spec);
+        return;
+    }
+    if (first) {
+        if (S.dir_pending) {
+            if (!S.dir_reset && acc[6] <= 0) {
+                // d.F <= 0 after shiftSearchDir(beta): resetCGParams (CGMethod.hpp:1505-1507);
+                // redo this line-search launch with d = F
+                S.dir_beta = 0.0;
+                S.dir_reset = 1;
+                bad_count = 0;
+                return;
+            }
+            S.dpar ^= 1;
+            S.dir_pending = 0;
+            if (S.dir_reset) S.last_fdotf = S.fdotf;   // lastFDotF = F.F
+            S.dir_reset = 0;
+        }
+        S.num_iter += 1;             // numIter++ at the top of the loop (CGMethod.hpp:1297)
+        S.rec_step = acc[7];
+        S.ls_trials = 0;
+        S.q_nrec = 0;
+        S.q_emax = -INFINITY;
+        if (S.grad_inf_norm != 0.0) {   // maxForce == 0 returns lambda = 0 at once (:252-256)
+            if (!S.ecur_valid) S.ls_cur = acc[5] > 0.0 ? INFINITY : acc[4];
+            S.num_energy_call += 1;
+            S.ls_slope = acc[6];
+            S.ls_reduced = (S.safe_mode ? 0.0 : 0.4) * acc[6];
+            S.ls_active = 1;
+        }
+    }
+    // QUADRATIC outside safe mode (CGMethod.hpp:1322-1325): the backtracking phase of
+    // quadraticLineSearchV2, its ideal change -BACKTRACKSLOPE * lambda * d.F in that order
+    const bool quad = S.ls_algo == 1 && !S.safe_mode;
+    bool conv = false;
+    int acc_trial = -1;
+    double lambda = lam0, energyLambda = 0.0;
+    if (first && S.grad_inf_norm == 0.0) {
+        conv = true;
+        lambda = 0.0;
+    }
+    // The first launch of a search evaluates lambda0 alone (usually accepted: its forces are
+    // the speculative ones); after a rejection each launch evaluates the next NTRIAL halvings.
+    // The trial sequence is the reference's either way.
+    const int ntrial = first ? 1 : NTRIAL;
+    for (int k = 0; k < ntrial && !conv; k++) {
+        energyLambda = k == 0 ? E1 : E2;
+        S.num_energy_call += 1;
+        S.ls_trials += 1;
+        const double lt = lambda, et = energyLambda;
+        const double ideal = quad ? -0.4 * lambda * S.ls_slope : -S.ls_reduced * lambda;
+        if (lambda < 0 || ideal > 0) { conv = true; lambda = 0.0; }
+        const double change = energyLambda - S.ls_cur;
+        if (change <= ideal) { conv = true; acc_trial = k; }
+        else lambda *= 0.5;
+        if (lambda <= 0.0 || lambda <= 1e-8) {
+            conv = true;
+            acc_trial = -1;
+            lambda = 0.0;
+            energyLambda = S.ls_cur;
+        }
+        if (quad) {   // the recorded entry of this trial (CGMethod.hpp:601-613): the trial, or
+                      // (0, E(x)) when lambda fell to the tolerance
+            const double l = lambda == 0.0 ? 0.0 : lt, e = lambda == 0.0 ? S.ls_cur : et;
+            if (S.q_nrec++ == 0) { S.q_lv0 = l; S.q_ev0 = e; }
+            else S.q_emax = rmax(S.q_emax, e);
+        }
+    }
+    const bool spec = bad_count == 0;
+    bad_count = 0;
+    if (!conv) {                 // keep halving in the next launch
+        S.ls_lambda = lambda;
+        return;
+    }
+    S.total_energy = energyLambda;   // TotalEnergy = energyLambda (CGMethod.hpp:368)
+    S.ls_active = 0;
+    if (lambda != 0.0) {
+        if (quad && S.ls_trials > 1 && energyLambda < S.ls_cur && energyLambda < S.q_emax) {
+            // the parabola phase (CGMethod.hpp:618-646): through (0, E(x)), the accepted trial
+            // and the first entry (the reference's index loop ends at 0)
+            S.ls_active = 1;
+            S.q_x0 = 0.0; S.q_y0 = S.ls_cur;
+            S.q_x1 = lambda; S.q_y1 = energyLambda;
+            S.q_x2 = S.q_lv0; S.q_y2 = S.q_ev0;
+            S.q_bt = lambda; S.q_ebt = energyLambda;
+            S.q_iter = 0;
+            quad_next(S);
+            return;
+        }
+        S.lambda = lambda;
+        S.new_energy = energyLambda;
+        S.recovery_move = 0;
+        // accepted at the first trial with usable speculative forces: gather directly
+        S.phase = (acc_trial == 0 && spec) ? PH_GATHER : PH_MOVE;
+        return;
+    }
+    ls_failed(S);
 }
 
 // ---- gather F into the next force buffer; F.F, F.F_prev, max|F| partials ---------------
@@ -1352,7 +1459,8 @@ __device__ inline void gather_control(CgState& S, const double* acc, int bad_cou
         S.energy_rel_change = fabs(S.new_energy - S.prev_energy) / fabs(S.prev_energy);
         S.prev_energy = S.new_energy;
         S.ls_cur = S.new_energy;      // E(x_new) = the accepted trial energy, bit for bit
-        S.ecur_valid = 1;
+        S.ecur_valid = S.q_stale ? 0 : 1;   // unless the quadratic search moved elsewhere
+        S.q_stale = 0;
         double beta = 0.0;
         if (S.method == 1) beta = FF / S.last_fdotf;
         else if (S.method == 2) beta = rmax(0.0, (FF - FP) / S.last_fdotf);

@@ -1441,6 +1441,8 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         return fail(ctx, MDN_ERR_ARG, "unknown precision %d", p->precision);
     const int prec = p->precision;
     if (p->method < 0 || p->method > 2) return fail(ctx, MDN_ERR_ARG, "unknown descent method %d", p->method);
+    if (p->line_search != MDN_LS_BACKTRACKING && p->line_search != MDN_LS_QUADRATIC)
+        return fail(ctx, MDN_ERR_ARG, "unknown line search %d", p->line_search);
     HIPCHK(hipSetDevice(ctx->device));
     memset(res, 0, sizeof(*res));
     const int nv = 3 * ctx->n_beads;
@@ -1474,6 +1476,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     CgState& S = *ctx->h_cg;
     memset(&S, 0, sizeof(S));
     S.method = p->method;
+    S.ls_algo = p->line_search;
     S.try_recover = p->try_recover;
     S.force_call_limit = std::max(p->force_call_limit, 3 * ctx->n_beads);
     S.energy_call_limit = p->energy_call_limit;
@@ -1511,7 +1514,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
 
     // Batches are replayed back to back; the host polls the done flag of batch i while batch
     // i+1 runs, so the device never idles waiting for the host.
-    const long max_batches = (4L * ((long)S.force_call_limit + 16)) / CG_BATCH + 8;
+    const long max_batches = (slots_per_force_call(S) * ((long)S.force_call_limit + 16)) / CG_BATCH + 8;
     long launched = 0;
     hipEvent_t ev[2];
     HIPCHK(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
@@ -1673,6 +1676,8 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     if (p->precision != MDN_PREC_FP64 && p->precision != MDN_PREC_MIXED)
         return fail(L, MDN_ERR_ARG, "unknown precision %d", p->precision);
     if (p->method < 0 || p->method > 2) return fail(L, MDN_ERR_ARG, "unknown descent method %d", p->method);
+    if (p->line_search != MDN_LS_BACKTRACKING && p->line_search != MDN_LS_QUADRATIC)
+        return fail(L, MDN_ERR_ARG, "unknown line search %d", p->line_search);
     long beads = 0;
     for (int i = 0; i < n; i++) {
         mdn_ctx* c = ctxs[i];
@@ -1708,6 +1713,7 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     }
     BatchParams P{};
     P.method = p->method;
+    P.line_search = p->line_search;
     P.try_recover = p->try_recover;
     P.force_call_limit = p->force_call_limit;
     P.energy_call_limit = p->energy_call_limit;

@@ -53,6 +53,30 @@ def test_batched_trajectory_prefix_vs_oracle(gpu, ncalls):
         np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
 
 
+def test_batched_quadratic_line_search_prefix_vs_oracle(gpu):
+    """The quadratic line search in the persistent kernel (the same ls_control as the
+    single-system minimizer): networks 0-3 of the batch (on three of them the oracle's parabola
+    phase runs within 40 energy calls) follow the oracle's trajectories."""
+    nets, ffms = _batch(gpu, 4)
+    kw = dict(energy_call_limit=40, lambda_max=0.1, max_dist=5.0)
+    p = abi.default_cg_params(line_search=abi.MDN_LS_QUADRATIC, **kw)
+    xs, fs, res = mech.minimize_batched(ffms, p, [n.coord for n in nets])
+    changed = 0
+    for i, net in enumerate(nets):
+        o = of.OracleSystem(net.desc)
+        o.build_nl(net.coord)
+        st, xo, fo, ro = o.minimize(p, net.coord)
+        _, xb, _, _ = o.minimize(abi.default_cg_params(**kw), net.coord)
+        changed += not np.array_equal(xo, xb)
+        rb = res[i]
+        print(f"system {i}: batched {rb.num_iter}/{rb.num_energy_call}/{rb.num_force_call} "
+              f"oracle {ro.num_iter}/{ro.num_energy_call}/{ro.num_force_call} "
+              f"max|dx| {float(np.abs(xs[i] - xo).max()):.3e}", flush=True)
+        assert (rb.num_iter, rb.num_energy_call, rb.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+        np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
+    assert changed >= 3
+
+
 @pytest.fixture(scope="module")
 def batch256(gpu):
     """The full batch: 256 networks of 20-400 filaments x 5-30 cylinders (about 1.03 M beads)."""

@@ -172,3 +172,46 @@ def test_minimize_prefix_through_safe_mode(gpu, ncalls):
     assert (rg.num_iter, rg.num_energy_call, rg.num_force_call, rg.num_safe_mode, rg.num_recovery) == \
         (ro.num_iter, ro.num_energy_call, ro.num_force_call, ro.num_safe_mode, ro.num_recovery)
     np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("ncalls", [40, 120])
+@pytest.mark.parametrize("lam_max,max_dist", [(0.01, 1.0), (0.1, 5.0)])
+def test_minimize_prefix_quadratic_line_search(gpu, ncalls, lam_max, max_dist):
+    """lineSearchAlgorithm QUADRATIC (CGMethod.hpp:1322-1325): the backtracking phase of
+    quadraticLineSearchV2 with slope 0.4, then, after two or more rejections, the parabola fits
+    through (0, E(x)), the accepted trial and the first trial, whose vertices the device evaluates
+    in launches of their own (CGMethod.hpp:660-764). The device follows the oracle's trajectory
+    (counts identical, coordinates 1e-8) on the 550-bead network, where the oracle's own
+    trajectory shows that the parabola phase ran (it differs from the backtracking one)."""
+    net, g, o, _ = _pair(gpu, 0)
+    p = abi.default_cg_params(energy_call_limit=ncalls, line_search=abi.MDN_LS_QUADRATIC, lambda_max=lam_max,
+                              max_dist=max_dist)
+    xg, fg, rg = g.minimize(p, net.coord)
+    st, xo, fo, ro = o.minimize(p, net.coord)
+    _, xb, _, rb = o.minimize(abi.default_cg_params(energy_call_limit=ncalls, lambda_max=lam_max, max_dist=max_dist),
+                              net.coord)
+    print(ncalls, "gpu", rg.num_iter, rg.num_energy_call, rg.num_force_call, "oracle", ro.num_iter,
+          ro.num_energy_call, ro.num_force_call, "backtracking", rb.num_iter, rb.num_energy_call,
+          "max|dx|", float(np.abs(xg - xo).max()), flush=True)
+    assert not np.array_equal(xo, xb)   # the parabola phase changed the oracle's trajectory
+    assert (rg.num_iter, rg.num_energy_call, rg.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+    np.testing.assert_allclose(xg, xo, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_minimize_quadratic_line_search_converges(gpu, prec):
+    """A full minimization with the quadratic line search stops by the reference criterion; the
+    oracle's forces at the result confirm the reported state."""
+    net, g, o, _ = _pair(gpu, 0)
+    p = abi.default_cg_params(line_search=abi.MDN_LS_QUADRATIC, precision=prec)
+    xg, fg, rg = g.minimize(p, net.coord)
+    print(prec, rg.as_dict(), flush=True)
+    assert rg.success() and rg.converged() and not rg.call_limit_exceeded
+    f_at_xg = o.forces(xg)
+    # FP64: rounding; mixed: the FP32 term arithmetic's absolute error at a relaxed state, far
+    # below the 1 pN force tolerance (the bar of the mixed path's relaxed-state checks)
+    atol = 1e-9 * max(1.0, float(np.abs(fg).max())) if prec == abi.MDN_PREC_FP64 else 1e-2
+    np.testing.assert_allclose(fg, f_at_xg, rtol=0, atol=atol)
+    e_at_xg, _ = o.energy(xg)
+    assert rg.energies_after.total == pytest.approx(e_at_xg, rel=1e-12)
+    assert rg.energies_after.total < 0.2 * rg.energies_before.total

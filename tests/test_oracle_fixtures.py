@@ -129,13 +129,16 @@ OPT = np.array([-500.0, -400.0, -300.0, -200.0, -100.0, 0.0, 100.0, 200.0, 300.0
 W = np.array([3, 1, 0.1, 0.5, 1, 10, 20, 50, 100, 0.1])
 
 
+@pytest.mark.parametrize("ls", [abi.MDN_LS_BACKTRACKING, abi.MDN_LS_QUADRATIC], ids=["backtracking", "quadratic"])
 @pytest.mark.parametrize("method", [abi.MDN_CG_STEEPEST, abi.MDN_CG_FLETCHER_REEVES, abi.MDN_CG_POLAK_RIBIERE],
                          ids=["steepest", "fletcher_rieves", "polak_ribiere"])
 @pytest.mark.parametrize("init", ["input_0", "input_1_already_minimized", "input_2"])
-def test_cg_harmonic10(method, init):
+def test_cg_harmonic10(method, init, ls):
     """TestCGMethod.cpp:9-93: 10-D separable harmonic, gradTol 0.1, maxDist 10, lambdaMax 1,
     recovery off; success() and every |F| <= gradTol. input_0 is the reference's
-    vector<floatingpoint>(800, n): 800 entries equal to 10, of which the first 10 are DOFs."""
+    vector<floatingpoint>(800, n): 800 entries equal to 10, of which the first 10 are DOFs. The
+    reference's fixture runs the default (backtracking) line search; the quadratic one must meet
+    the same criteria."""
     n = 10
     if init == "input_0":
         x0, nvar = np.full(800, 10.0), 800
@@ -150,9 +153,105 @@ def test_cg_harmonic10(method, init):
     def force(c, f):
         f[:n] = -W * (c[:n] - OPT)
 
-    p = abi.default_cg_params(method=method, grad_tol=0.1, max_dist=10, lambda_max=1, try_recover=0)
+    p = abi.default_cg_params(method=method, grad_tol=0.1, max_dist=10, lambda_max=1, try_recover=0, line_search=ls)
     st, x, f, res = of.cg_minimize(p, x0, n, energy, force, nvar=nvar)
     assert st == 0
     assert res.success()
     assert np.count_nonzero(np.abs(f) > 0.1) == 0
     np.testing.assert_allclose(x[:n], OPT, atol=0.1 / W.min() + 1e-9)
+
+
+@pytest.mark.parametrize("ratio,step,trials", [(5.0, 1.0, 5), (3.0, 1.0, 4), (8.0, 1.0, 4), (2.2, 1.1, 2)])
+def test_quadratic_line_search_lands_on_the_line_minimizer(ratio, step, trials):
+    """On a quadratic energy the parabola through (0, E0), the accepted backtracking trial and the
+    first trial is the energy along the line itself, so the quadratic search (CGMethod.hpp:522-764,
+    trials at x + lambda d) must step exactly to the line minimizer lambda* = F.F / F.H.F whenever
+    its parabola phase runs. With lambda0 = ratio * lambda* the Armijo rule (slope 0.4) rejects
+    every trial above 1.2 lambda*:
+      ratio 5: 5, 2.5, 1.25 rejected, 0.625 accepted; the vertex is 60 % away, so one evaluation
+               there (within 1e-3 of the accepted energy) returns it: 5 trial evaluations;
+      ratio 3: 3, 1.5 rejected, 0.75 accepted, one evaluation at the vertex: 4;
+      ratio 8: 8, 4, 2 rejected, lambda* accepted; the vertex is within 1e-3 of it and is
+               returned without an evaluation: 4;
+      ratio 2.2: one rejection, 1.1 accepted; the parabola phase needs an entry after the first
+               with more energy than the accepted trial, which a single rejection never leaves,
+               so the backtracking step stays: 2."""
+    rng = np.random.default_rng(31)
+    n = 6
+    h = rng.uniform(1.0, 10.0, n)
+    c = rng.normal(0.0, 5.0, n)
+    x0 = c + rng.normal(0.0, 20.0, n)
+
+    def energy(x):
+        return float(1e6 + 0.5 * np.sum(h * (x - c) ** 2))
+
+    f0 = -h * (x0 - c)
+    lam_star = (f0 @ f0) / (f0 @ (h * f0))
+    calls, forces_at = [], []
+
+    def energy_rec(x):
+        calls.append(x.copy())
+        return energy(x)
+
+    def force(x, f):
+        forces_at.append((x.copy(), len(calls)))
+        f[:] = -h * (x - c)
+
+    lam0 = ratio * lam_star
+    p = abi.default_cg_params(method=abi.MDN_CG_STEEPEST, grad_tol=1e-12, max_dist=1e9, lambda_max=lam0,
+                              try_recover=0, line_search=abi.MDN_LS_QUADRATIC, energy_call_limit=40)
+    st, x, f, res = of.cg_minimize(p, x0, n, energy_rec, force)
+    assert st == 0
+    x1, ncalls = forces_at[1]             # the step's forces: at x0 + lambda d
+    assert ncalls - 2 == trials           # less the start report and the search's current energy
+    np.testing.assert_allclose(x1, x0 + step * lam_star * f0, rtol=1e-12)
+    # a quadratic energy with the search's exact steps converges (steepest descent)
+    assert res.num_iter >= 2
+
+
+def test_quadratic_line_search_refits_near_the_line_minimizer():
+    """On a quadratic energy with a quartic perturbation the parabola through three points is a
+    model, so the search re-fits it around its best point (CGMethod.hpp:660-764) until the vertex
+    or the energy moves by less than 1e-3 relative. The far first trial stays a bracket point, so
+    the re-fits creep from the backtracking step toward the true line minimizer (found here by
+    bisection on the directional derivative) and stop short of it: the step must lie between the
+    two, at lower energy than the backtracking step (a wrong bracket update moves away or past)."""
+    rng = np.random.default_rng(32)
+    n = 5
+    h = rng.uniform(1.0, 3.0, n)
+    c = rng.normal(0.0, 2.0, n)
+    x0 = c + rng.normal(0.0, 3.0, n)
+    eps = 0.005
+
+    def grad(x):
+        return h * (x - c) + 4 * eps * h * (x - c) ** 3
+
+    def energy(x):
+        return float(np.sum(0.5 * h * (x - c) ** 2 + eps * h * (x - c) ** 4))
+
+    def force(x, f):
+        forces.append(x.copy())
+        f[:] = -grad(x)
+
+    f0 = -grad(x0)
+    lo, hi = 0.0, 1.0
+    while -grad(x0 + hi * f0) @ f0 > 0:
+        hi *= 2
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if -grad(x0 + mid * f0) @ f0 > 0 else (lo, mid)
+    lam_true = 0.5 * (lo + hi)
+    for ratio, near in ((3.0, 0.15), (5.0, 0.2)):
+        results = {}
+        for ls in (abi.MDN_LS_BACKTRACKING, abi.MDN_LS_QUADRATIC):
+            forces = []
+            p = abi.default_cg_params(method=abi.MDN_CG_STEEPEST, grad_tol=1e-12, max_dist=1e9,
+                                      lambda_max=ratio * lam_true, try_recover=0, line_search=ls,
+                                      energy_call_limit=60)
+            st, _, _, _ = of.cg_minimize(p, x0, n, energy, force)
+            assert st == 0
+            results[ls] = ((forces[1] - x0) @ f0) / (f0 @ f0)
+        lam_q, lam_bt = results[abi.MDN_LS_QUADRATIC], results[abi.MDN_LS_BACKTRACKING]
+        assert lam_bt < lam_q <= lam_true * (1 + 1e-3), (ratio, lam_bt / lam_true, lam_q / lam_true)
+        assert lam_true - lam_q <= near * lam_true
+        assert energy(x0 + lam_q * f0) < energy(x0 + lam_bt * f0)
