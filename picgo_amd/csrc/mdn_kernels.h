@@ -83,6 +83,11 @@ struct DevSys {
     double* lnk_sf;
     const int* mot_b; const double* mot_k; const double* mot_eql; const double* mot_p1; const double* mot_p2;
     double* mot_sf;
+    // The linker / motor tables (and the tensions *_sf) are stored in the order of the first bead
+    // of each crosslink's first cylinder, so that a wave's table reads, first-cylinder coordinates
+    // and slot writes are close together; *_ord[i] is the crosslink's own index, which places its
+    // slots (so the results do not depend on the order)
+    const int* lnk_ord; const int* mot_ord;
     // excluded volume: beads (4 per pair) and kRepScaled = krep * L0_i * L0_j
     const int* ev_b;   const double* ev_ks;
     // boundary: bead, element, krep, slen
@@ -384,7 +389,7 @@ __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R
         if (slots) {
             double sf = xlink_forces<R>(c1, c2, c3, c4, k, eql, p1, p2, out);
             (lk ? s.lnk_sf : s.mot_sf)[t] = sf;
-            const long id = s.slot0[ff] + 4L * t;
+            const long id = s.slot0[ff] + 4L * (lk ? s.lnk_ord : s.mot_ord)[t];
             for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q]);
         }
         return (double)xlink_energy<R>(c1, c2, c3, c4, k, eql, p1, p2);
@@ -1622,7 +1627,7 @@ __device__ inline void term_tangent(const DevSys& s, int ff, int t, const double
         const double p1 = lk ? s.lnk_p1[t] : s.mot_p1[t], p2 = lk ? s.lnk_p2[t] : s.mot_p2[t];
         (void)xlink_forces<RT>(pos_dual(x, v, b[4 * t]), pos_dual(x, v, b[4 * t + 1]), pos_dual(x, v, b[4 * t + 2]),
                                   pos_dual(x, v, b[4 * t + 3]), k, eql, p1, p2, out);
-        const long id = s.slot0[ff] + 4L * t;
+        const long id = s.slot0[ff] + 4L * (lk ? s.lnk_ord : s.mot_ord)[t];
         for (int q = 0; q < 4; q++) put_slot(s, slots, id + q, out[q].d);
         return;
     }

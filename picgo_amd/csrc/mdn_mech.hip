@@ -93,7 +93,8 @@ struct mdn_ctx {
     double bnd_cutoff = 0, rmax = 0, rmin = 0, grid[3] = {0, 0, 0};
 
     // device tables
-    DBuf<int> str_b, bend_b, lnk_b, mot_b, ev_b, bnd_b, bnd_e;
+    DBuf<int> str_b, bend_b, lnk_b, mot_b, ev_b, bnd_b, bnd_e, lnk_ord, mot_ord;
+    std::vector<int> xl_ord[2];   // linkers, motors: device table position -> crosslink index
     DBuf<double> str_k, str_eql, bend_k, bend_eqt, lnk_k, lnk_eql, lnk_p1, lnk_p2, lnk_sf;
     DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
     DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
@@ -279,6 +280,7 @@ void refresh_ptrs(mdn_ctx* ctx) {
     s.bend_b = ctx->bend_b.p; s.bend_k = ctx->bend_k.p; s.bend_eqt = ctx->bend_eqt.p;
     s.lnk_b = ctx->lnk_b.p; s.lnk_k = ctx->lnk_k.p; s.lnk_eql = ctx->lnk_eql.p; s.lnk_p1 = ctx->lnk_p1.p;
     s.lnk_p2 = ctx->lnk_p2.p; s.lnk_sf = ctx->lnk_sf.p;
+    s.lnk_ord = ctx->lnk_ord.p; s.mot_ord = ctx->mot_ord.p;
     s.mot_b = ctx->mot_b.p; s.mot_k = ctx->mot_k.p; s.mot_eql = ctx->mot_eql.p; s.mot_p1 = ctx->mot_p1.p;
     s.mot_p2 = ctx->mot_p2.p; s.mot_sf = ctx->mot_sf.p;
     s.ev_b = ctx->ev_b.p; s.ev_ks = ctx->ev_ks.p;
@@ -502,7 +504,8 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
         if (ctx->graph_exec[q]) (void)hipGraphExecDestroy(ctx->graph_exec[q]);
         if (ctx->graph[q]) (void)hipGraphDestroy(ctx->graph[q]);
     }
-    DBuf<int>* ib[] = {&ctx->str_b, &ctx->bend_b, &ctx->lnk_b, &ctx->mot_b, &ctx->ev_b, &ctx->bnd_b, &ctx->bnd_e,
+    DBuf<int>* ib[] = {&ctx->str_b, &ctx->bend_b, &ctx->lnk_b, &ctx->mot_b, &ctx->lnk_ord, &ctx->mot_ord, &ctx->ev_b,
+                       &ctx->bnd_b, &ctx->bnd_e,
                        &ctx->inc_s_start, &ctx->inc_s, &ctx->inc_d_start, &ctx->inc_d, &ctx->inc_cnt,
                        &ctx->inc_cursor, &ctx->d_cyl_b1, &ctx->d_cyl_b2, &ctx->d_cyl_fil, &ctx->d_cyl_pos,
                        &ctx->d_cyl_type, &ctx->bin, &ctx->bin_count, &ctx->bin_start, &ctx->bin_cursor,
@@ -592,13 +595,41 @@ int set_crosslinks(mdn_ctx* ctx, const mdn_crosslink* linkers, int n_linkers, co
         std::vector<int> cur(cnt.begin(), cnt.end() - 1);
         each_static([&](int b, long sl) { inc[cur[b]++] = (int)sl; });   // visited in slot order
     }
-    HIPCHK(upload(ctx, ctx->lnk_b, xb[0])); HIPCHK(upload(ctx, ctx->lnk_k, xk[0])); HIPCHK(upload(ctx, ctx->lnk_eql, xe[0]));
-    HIPCHK(upload(ctx, ctx->lnk_p1, xp1[0])); HIPCHK(upload(ctx, ctx->lnk_p2, xp2[0]));
-    HIPCHK(ctx->lnk_sf.ensure(std::max<size_t>(1, xk[0].size())));
-    HIPCHK(upload(ctx, ctx->mot_b, xb[1])); HIPCHK(upload(ctx, ctx->mot_k, xk[1])); HIPCHK(upload(ctx, ctx->mot_eql, xe[1]));
-    HIPCHK(upload(ctx, ctx->mot_p1, xp1[1])); HIPCHK(upload(ctx, ctx->mot_p2, xp2[1]));
-    HIPCHK(ctx->mot_sf.ensure(std::max<size_t>(1, xk[1].size())));
     HIPCHK(upload(ctx, ctx->inc_s_start, cnt)); HIPCHK(upload(ctx, ctx->inc_s, inc));
+    // The device tables in the threads' order: by the first bead of the first cylinder (stable:
+    // ties keep the interaction order); ord[i] is the crosslink's own index (its slots).
+    // MDN_XL_ORDER=0 keeps the interaction order (the forces are the same).
+    const char* env = getenv("MDN_XL_ORDER");
+    const bool sorted = !(env && env[0] == '0');
+    std::vector<int> sb[2];
+    std::vector<double> sk[2], se[2], sp1[2], sp2[2];
+    for (int w = 0; w < 2; w++) {
+        std::vector<int>& ord = ctx->xl_ord[w];
+        const size_t n = xk[w].size();
+        ord.resize(n);
+        for (size_t j = 0; j < n; j++) ord[j] = (int)j;
+        const std::vector<int>& b = xb[w];
+        if (sorted)
+            std::stable_sort(ord.begin(), ord.end(), [&b](int i, int j) { return b[4 * (size_t)i] < b[4 * (size_t)j]; });
+        sb[w].resize(4 * n); sk[w].resize(n); se[w].resize(n); sp1[w].resize(n); sp2[w].resize(n);
+        for (size_t i = 0; i < n; i++) {
+            const int t = ord[i];
+            for (int q = 0; q < 4; q++) sb[w][4 * i + q] = b[4 * (size_t)t + q];
+            sk[w][i] = xk[w][t]; se[w][i] = xe[w][t]; sp1[w][i] = xp1[w][t]; sp2[w][i] = xp2[w][t];
+        }
+    }
+    for (int w = 0; w < 2; w++) {
+        std::vector<int> ordu(ctx->xl_ord[w]);
+        if (ordu.empty()) ordu.push_back(0);
+        HIPCHK(upload(ctx, w ? ctx->mot_ord : ctx->lnk_ord, ordu));
+        HIPCHK(upload(ctx, w ? ctx->mot_b : ctx->lnk_b, sb[w]));
+        HIPCHK(upload(ctx, w ? ctx->mot_k : ctx->lnk_k, sk[w]));
+        HIPCHK(upload(ctx, w ? ctx->mot_eql : ctx->lnk_eql, se[w]));
+        HIPCHK(upload(ctx, w ? ctx->mot_p1 : ctx->lnk_p1, sp1[w]));
+        HIPCHK(upload(ctx, w ? ctx->mot_p2 : ctx->lnk_p2, sp2[w]));
+        HIPCHK((w ? ctx->mot_sf : ctx->lnk_sf).ensure(std::max<size_t>(1, xk[w].size())));
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // the host vectors above end with this scope
+    }
     return MDN_OK;
 }
 
@@ -1128,10 +1159,16 @@ int mdn_ff_forces(mdn_ctx* ctx, int32_t ff, const double* coord, double* force, 
 int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor_out) {
     int st = need_system(ctx, true);
     if (st) return st;
-    if (linker_out && ctx->sys.n[FF_LINKER])
-        HIPCHK(d2h(ctx, linker_out, ctx->lnk_sf.p, sizeof(double) * ctx->sys.n[FF_LINKER]));
-    if (motor_out && ctx->sys.n[FF_MOTOR])
-        HIPCHK(d2h(ctx, motor_out, ctx->mot_sf.p, sizeof(double) * ctx->sys.n[FF_MOTOR]));
+    // the tensions are stored in the device tables' order: back to the crosslinks' own order
+    for (int w = 0; w < 2; w++) {
+        double* out = w ? motor_out : linker_out;
+        const int n = ctx->sys.n[w ? FF_MOTOR : FF_LINKER];
+        if (!out || n == 0) continue;
+        std::vector<double> tmp(n);
+        HIPCHK(d2h(ctx, tmp.data(), (w ? ctx->mot_sf : ctx->lnk_sf).p, sizeof(double) * n));
+        const std::vector<int>& ord = ctx->xl_ord[w];
+        for (int i = 0; i < n; i++) out[ord[i]] = tmp[i];
+    }
     return MDN_OK;
 }
 
