@@ -110,8 +110,12 @@ struct DevSys {
     // (their neighbours along the filament are the adjacent term indices; out of range where the
     // term does not exist). blk_nb = the interaction blocks of the other force fields, which are
     // dispatched first.
+    // bcontig: the filaments' beads are numbered consecutively, filament after filament (the
+    // bead order of serializeDof), so a bead's neighbours are the adjacent indices and its
+    // bending term is 2 bterm[0][bead] - bead - 1: the bead blocks then read bterm[0] alone
     const int* bnbr;
     const int* bterm;
+    int bcontig;
     double* bsum;
     int bonded_gather;
     int blk_nb;
@@ -464,9 +468,17 @@ template <typename R>
 __device__ inline D3 bonded_at_bead(const DevSys& s, const double* __restrict__ x, int b, unsigned m, D3 acc) {
     const long nb = s.n_beads;
     const int ns = s.n[FF_STRETCH], nd = s.n[FF_BEND];
-    const int id[5] = {s.bnbr[b], s.bnbr[nb + b], b, s.bnbr[2 * nb + b], s.bnbr[3 * nb + b]};
-    const int t1 = s.bterm[b];        // stretching term (bead, next)
-    const int tc = s.bterm[nb + b];   // bending term (previous, bead, next)
+    int id[5], t1, tc;   // t1: stretching term (bead, next); tc: bending term (previous, bead, next)
+    if (s.bcontig) {     // absent neighbours: in-range adjacent beads, dropped by the mask
+        const int hi = (int)nb - 1;
+        id[0] = max(b - 2, 0); id[1] = max(b - 1, 0); id[2] = b; id[3] = min(b + 1, hi); id[4] = min(b + 2, hi);
+        t1 = s.bterm[b];
+        tc = 2 * t1 - b - 1;
+    } else {
+        id[0] = s.bnbr[b]; id[1] = s.bnbr[nb + b]; id[2] = b; id[3] = s.bnbr[2 * nb + b]; id[4] = s.bnbr[3 * nb + b];
+        t1 = s.bterm[b];
+        tc = s.bterm[nb + b];
+    }
     D3 c[5];   // along the filament: two before .. two after
 #pragma unroll
     for (int j = 0; j < 5; j++) c[j] = D3{x[3L * id[j]], x[3L * id[j] + 1], x[3L * id[j] + 2]};
@@ -1044,8 +1056,11 @@ __device__ inline double bead_bonded_energy(const DevSys& s, const double* __res
     const long nb = s.n_beads;
     const int ns = s.n[FF_STRETCH], nd = s.n[FF_BEND];
     const unsigned m = s.pmask[b];
-    const int ip = s.bnbr[nb + b], in = s.bnbr[2 * nb + b];
-    const int t1 = min(max(s.bterm[b], 0), max(ns - 1, 0)), tc = min(max(s.bterm[nb + b], 0), max(nd - 1, 0));
+    const int t1r = s.bterm[b];
+    const int ip = s.bcontig ? max(b - 1, 0) : s.bnbr[nb + b];
+    const int in = s.bcontig ? min(b + 1, (int)nb - 1) : s.bnbr[2 * nb + b];
+    const int tcr = s.bcontig ? 2 * t1r - b - 1 : s.bterm[nb + b];
+    const int t1 = min(max(t1r, 0), max(ns - 1, 0)), tc = min(max(tcr, 0), max(nd - 1, 0));
     auto X = [x](int i) { return D3{x[3L * i], x[3L * i + 1], x[3L * i + 2]}; };
     const D3 cp = X(ip), c0 = X(b), cn = X(in);
     const double sk = ns > 0 ? s.str_k[t1] : 0.0, sl = ns > 0 ? s.str_eql[t1] : 0.0;

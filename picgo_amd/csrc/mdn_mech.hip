@@ -766,6 +766,13 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
             return fail(ctx, MDN_ERR_STATE, "bending term %d: inconsistent filament tables", t);
     HIPCHK(upload(ctx, ctx->bnbr, bnbr));
     HIPCHK(upload(ctx, ctx->bterm, bterm));
+    {   // the bead blocks' neighbours from the indices alone when beads run filament by filament
+        const int total = d->n_filaments > 0 ? d->fil_bead_offset[d->n_filaments] : 0;
+        bool contig = true;
+        for (int k = 0; k < total && contig; k++) contig = d->fil_beads[k] == k;
+        const char* env = getenv("MDN_BEAD_TABLES");   // 1: always read the neighbour tables
+        ctx->sys.bcontig = contig && !(env && env[0] == '1') ? 1 : 0;
+    }
     HIPCHK(ctx->bsum.ensure(3 * (size_t)nbd));
     HIPCHK(upload(ctx, ctx->str_b, str_b)); HIPCHK(upload(ctx, ctx->str_k, str_k)); HIPCHK(upload(ctx, ctx->str_eql, str_eql));
     HIPCHK(upload(ctx, ctx->bend_b, bend_b)); HIPCHK(upload(ctx, ctx->bend_k, bend_k)); HIPCHK(upload(ctx, ctx->bend_eqt, bend_eqt));

@@ -104,6 +104,34 @@ def test_large_network_minimization_equals_pieces(gpu, prec):
     np.testing.assert_array_equal(a.compute_forces(net.coord, precision=prec), b.compute_forces(net.coord, precision=prec))
 
 
+@pytest.mark.parametrize("prec", [abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED])
+def test_bead_neighbours_from_indices_equal_tables(gpu, prec):
+    """With the beads numbered filament after filament (the generator's and serializeDof's order)
+    the bead blocks take a bead's neighbours from the adjacent indices and its bending term from
+    its stretching term; MDN_BEAD_TABLES=1 makes them read the neighbour tables instead. On the
+    420,000-bead network (where the line search's bead blocks also take the filament energies)
+    the forces, energies and a 40-call trajectory must be bit-identical."""
+    net = synth.Network.config(4)
+    os.environ["MDN_BEAD_TABLES"] = "1"
+    try:
+        a = gpu()
+        a.upload(net.desc, net.coord)
+    finally:
+        os.environ.pop("MDN_BEAD_TABLES", None)
+    a.reset_neighbor_lists(net.coord)
+    b = _ctx(gpu, net, None)
+    x = net.coord + np.random.default_rng(12).normal(0.0, 3.0, net.coord.size)
+    np.testing.assert_array_equal(a.compute_forces(x, precision=prec), b.compute_forces(x, precision=prec))
+    ea, eb = a.compute_individual_energies(x, precision=prec), b.compute_individual_energies(x, precision=prec)
+    assert list(ea.ff) == list(eb.ff)
+    p = abi.default_cg_params(precision=prec, energy_call_limit=40)
+    xa, fa, ra = a.minimize(p, net.coord)
+    xb, fb, rb = b.minimize(p, net.coord)
+    assert (ra.num_iter, ra.num_energy_call, ra.num_force_call) == (rb.num_iter, rb.num_energy_call, rb.num_force_call)
+    np.testing.assert_array_equal(xa, xb)
+    np.testing.assert_array_equal(fa, fb)
+
+
 def test_batched_with_recomputing_contexts(gpu):
     nets = [synth.Network.config(3, index=i) for i in range(3)]
     sa = [_ctx(gpu, n, 0) for n in nets]
