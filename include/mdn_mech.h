@@ -179,7 +179,8 @@ const char* mdn_last_error(const mdn_ctx* ctx);
 
 /* Upload the system and its coordinates, and build the bonded and crosslink tables
  * (ForceFieldManager::vectorizeAllForceFields, ForceFieldManager.cpp:35-50, for the
- * stretching, bending, linker and motor force fields). */
+ * stretching, bending, linker and motor force fields). coord NULL: the coordinates that
+ * mdn_dof_serialize gathered on the device for sys->n_beads beads. */
 int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* sys, const double* coord);
 
 /* Counts: n_beads, n_cylinders, and interaction counts per force field (after mdn_nl_build
@@ -287,6 +288,16 @@ int mdn_displace(mdn_ctx* ctx, const double* delta, int32_t n_var);
  * perturbation between successive minimizations, with no buffer and no host transfer. The
  * neighbor lists are not rebuilt. */
 int mdn_perturb(mdn_ctx* ctx, uint64_t seed, uint64_t stream, double sigma);
+
+/* serializeDof (DofSerializer.hpp:64-92) on the device: the next system's bead coordinates,
+ * n_beads of them in the order of Bead::getBeads(), gathered from the current device coordinates
+ * (the last upload, displacement or minimization result) with only the new beads' coordinates
+ * from the host. src[i] >= 0: bead i of the next system is bead src[i] of the current one;
+ * src[i] < 0: it is a new bead, whose coordinates are the next 3 values of new_coord (new beads
+ * in ascending i). The result is kept on the device until the next mdn_system_upload(ctx, desc,
+ * NULL) adopts it as that system's coordinates (the current system stays usable meanwhile).
+ * Beads that chemistry destroyed are the current indices that no src entry names. */
+int mdn_dof_serialize(mdn_ctx* ctx, int32_t n_beads, const int32_t* src, const double* new_coord);
 
 /* Mixed precision (MDN_PREC_MIXED): the bending and boundary terms and the force components of
  * the harmonic springs (stretching, linkers, motors) are computed in FP32 from bond vectors and

@@ -81,6 +81,17 @@ public:
         check(mdn_system_upload(ctx_, &sys, coord.data()));
         nvar_ = 3 * sys.n_beads;
     }
+    // serializeDof (DofSerializer.hpp:64-92) on the device, for the next system's beads in
+    // Bead::getBeads() order: src[i] = the bead's index in the current system, or -1 for a new
+    // bead (its coordinates the next three values of newCoord); then vectorizeAllForceFields(sys)
+    // uploads the next system with those device coordinates
+    void serializeDof(const std::vector<int32_t>& src, const std::vector<double>& newCoord) {
+        check(mdn_dof_serialize(ctx_, (int32_t)src.size(), src.data(), newCoord.empty() ? nullptr : newCoord.data()));
+    }
+    void vectorizeAllForceFields(const mdn_system_desc& sys) {
+        check(mdn_system_upload(ctx_, &sys, nullptr));
+        nvar_ = 3 * sys.n_beads;
+    }
     // LinkerStretching / MotorGhostStretching::vectorize alone, between minimizations whose
     // chemistry only bound or unbound crosslinks (filaments, neighbor lists kept)
     void vectorizeCrosslinks(const std::vector<mdn_crosslink>& linkers, const std::vector<mdn_crosslink>& motors) {

@@ -638,6 +638,18 @@ __device__ inline D3 gather_block(const DevSys& s, int bmask, const R* __restric
 // Plain (non-CG) kernels
 // =======================================================================================
 
+// serializeDof from the device coordinates: out[3i + q] = xo[3 g[i] + q] for a current bead
+// (g[i] >= 0, below the current bead count) or xn[3 (-1 - g[i]) + q] for a new one (the host
+// checked both ranges). One thread per coordinate.
+__global__ void __launch_bounds__(BLOCK) k_dof_gather(int n, const int* __restrict__ g, const double* __restrict__ xo,
+                                                      const double* __restrict__ xn, double* __restrict__ out) {
+    const long e = (long)blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= 3L * n) return;
+    const int i = (int)(e / 3), q = (int)(e - 3L * i);
+    const int s = g[i];
+    out[e] = s >= 0 ? xo[3L * s + q] : xn[3L * (-1 - s) + q];
+}
+
 // Slots of every interaction at x (ForceFieldManager::computeForces before the gather).
 // Excluded-volume pairs needing the numerical fallback are listed in bad_list.
 // Grid: the interaction blocks. The mixed instantiation is held to 4 waves per SIMD (126 VGPRs,

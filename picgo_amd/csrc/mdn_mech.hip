@@ -95,6 +95,9 @@ struct mdn_ctx {
     // device tables
     DBuf<int> str_b, bend_b, lnk_b, mot_b, ev_b, bnd_b, bnd_e, lnk_ord, mot_ord;
     std::vector<int> xl_ord[2];   // linkers, motors: device table position -> crosslink index
+    DBuf<double> dof_next, dof_new;   // mdn_dof_serialize: the gathered coordinates, new beads
+    DBuf<int> dof_src;
+    int dof_next_n = -1;              // beads in dof_next (-1: none pending)
     DBuf<double> str_k, str_eql, bend_k, bend_eqt, lnk_k, lnk_eql, lnk_p1, lnk_p2, lnk_sf;
     DBuf<double> mot_k, mot_eql, mot_p1, mot_p2, mot_sf, ev_ks, bnd_krep, bnd_slen;
     DBuf<int> inc_s_start, inc_s, inc_d_start, inc_d, inc_cnt, inc_cursor, bstart, sorted_id, pos;
@@ -511,7 +514,7 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                        &ctx->d_cyl_type, &ctx->bin, &ctx->bin_count, &ctx->bin_start, &ctx->bin_cursor,
                        &ctx->members, &ctx->nl_count, &ctx->nl_off, &ctx->pairs, &ctx->bnd_cnt, &ctx->bnd_off,
                        &ctx->nl_err, &ctx->bad_count, &ctx->bad_list, &ctx->fallback_ok, &ctx->bstart,
-                       &ctx->sorted_id, &ctx->pos};
+                       &ctx->sorted_id, &ctx->pos, &ctx->dof_src};
     for (auto* b : ib) b->release();
     DBuf<double>* db[] = {&ctx->str_k, &ctx->str_eql, &ctx->bend_k, &ctx->bend_eqt, &ctx->lnk_k, &ctx->lnk_eql,
                           &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
@@ -519,7 +522,7 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                           &ctx->d_cyl_kexvol, &ctx->d_cyl_eql, &ctx->d_krep_e, &ctx->d_slen_e, &ctx->mid,
                           &ctx->x[0], &ctx->x[1], &ctx->f, &ctx->f2, &ctx->fp, &ctx->xh, &ctx->d[0], &ctx->d[1],
                           &ctx->xmin[0], &ctx->xmin[1], &ctx->scratch_f, &ctx->ext, &ctx->partial, &ctx->eout,
-                          &ctx->vstat, &ctx->fallback_partial};
+                          &ctx->vstat, &ctx->fallback_partial, &ctx->dof_next, &ctx->dof_new};
     for (auto* b : db) b->release();
     ctx->d_cyl_minus.release();
     ctx->d_fil_ends.release();
@@ -634,9 +637,12 @@ int set_crosslinks(mdn_ctx* ctx, const mdn_crosslink* linkers, int n_linkers, co
 }
 
 int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coord) {
-    if (!ctx || !d || !coord) return fail(ctx, MDN_ERR_ARG, "null argument");
+    if (!ctx || !d) return fail(ctx, MDN_ERR_ARG, "null argument");
     HIPCHK(hipSetDevice(ctx->device));
     if (d->n_beads <= 0 || d->n_filaments < 0) return fail(ctx, MDN_ERR_ARG, "bad bead / filament counts");
+    if (!coord && ctx->dof_next_n != d->n_beads)
+        return fail(ctx, MDN_ERR_ARG, "no coordinates: coord is NULL and mdn_dof_serialize gathered %d beads, not %d",
+                    ctx->dof_next_n, d->n_beads);
     if (d->n_boundary > MAXELEM) return fail(ctx, MDN_ERR_UNSUPPORTED, "at most %d boundary elements", MAXELEM);
     ctx->uploaded = ctx->nl_built = false;
     int st = MDN_OK;
@@ -812,12 +818,54 @@ int mdn_system_upload(mdn_ctx* ctx, const mdn_system_desc* d, const double* coor
                     &ctx->xmin[0], &ctx->xmin[1], &ctx->scratch_f, &ctx->ext})
         HIPCHK(b->ensure(nv));
     ctx->cur = 0;
-    HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->x[0].p, coord, sizeof(double) * nv, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(ctx->x[0].p, ctx->dof_next.p, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ctx->dof_next_n = -1;
     refresh_ptrs(ctx);
     if ((st = ensure_eval_buffers(ctx))) return st;
     refresh_ptrs(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->uploaded = true;
+    return MDN_OK;
+}
+
+// serializeDof (DofSerializer.hpp:64-92) from the device coordinates: bead i of the next system is
+// current bead src[i], or (src[i] < 0) the next new bead of new_coord.
+int mdn_dof_serialize(mdn_ctx* ctx, int32_t n_beads, const int32_t* src, const double* new_coord) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    if (n_beads <= 0 || !src) return fail(ctx, MDN_ERR_ARG, "n_beads must be positive and src given");
+    const int n_old = ctx->n_beads;
+    std::vector<char> used(n_old, 0);
+    int n_new = 0;
+    for (int i = 0; i < n_beads; i++) {
+        const int s = src[i];
+        if (s < 0) { n_new++; continue; }
+        if (s >= n_old) return fail(ctx, MDN_ERR_ARG, "src[%d] = %d: the current system has %d beads", i, s, n_old);
+        if (used[s]) return fail(ctx, MDN_ERR_ARG, "src[%d] = %d: current bead named twice", i, s);
+        used[s] = 1;
+    }
+    if (n_new > 0 && !new_coord) return fail(ctx, MDN_ERR_ARG, "%d new beads and no new_coord", n_new);
+    HIPCHK(hipSetDevice(ctx->device));
+    // the gather map: >= 0 a current bead, -1 - k the k-th new bead
+    std::vector<int> g(n_beads);
+    for (int i = 0, k = 0; i < n_beads; i++) g[i] = src[i] >= 0 ? src[i] : -1 - k++;
+    HIPCHK(upload(ctx, ctx->dof_src, g));
+    HIPCHK(ctx->dof_new.ensure(3 * (size_t)std::max(1, n_new)));
+    if (n_new > 0)
+        HIPCHK(hipMemcpyAsync(ctx->dof_new.p, new_coord, sizeof(double) * 3 * n_new, hipMemcpyHostToDevice,
+                              ctx->stream));
+    HIPCHK(ctx->dof_next.ensure(3 * (size_t)n_beads));
+    const long ne = 3L * n_beads;
+    hipLaunchKernelGGL(k_dof_gather, dim3(ceil_div(ne, BLOCK)), dim3(BLOCK), 0, ctx->stream, n_beads,
+                       (const int*)ctx->dof_src.p, (const double*)ctx->x[ctx->cur].p, (const double*)ctx->dof_new.p,
+                       ctx->dof_next.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));   // g ends with this scope
+    ctx->dof_next_n = n_beads;
     return MDN_OK;
 }
 

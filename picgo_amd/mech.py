@@ -32,6 +32,7 @@ EXPORTED = [
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
     "mdn_forces_noalloc", "mdn_displace", "mdn_perturb", "mdn_param_grad_mixed", "mdn_hvp_mixed",
+    "mdn_dof_serialize",
 ]
 
 
@@ -89,6 +90,7 @@ def lib():
         L.mdn_forces_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
         L.mdn_displace.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
         L.mdn_perturb.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double]
+        L.mdn_dof_serialize.argtypes = [C.c_void_p, C.c_int32, i32p, dp]
         _lib = L
     return _lib
 
@@ -112,11 +114,12 @@ class ForceFieldManager:
         if st != 0:
             raise MdnError(st, self._L.mdn_last_error(self._h).decode())
 
-    # ForceFieldManager::vectorizeAllForceFields (bonded + crosslink tables)
+    # ForceFieldManager::vectorizeAllForceFields (bonded + crosslink tables); coord None: the
+    # coordinates gathered on the device by dof_serialize
     def vectorize_all_force_fields(self, desc, coord):
         ptr = desc if isinstance(desc, C._Pointer) else C.pointer(desc)
-        c = _f64(coord)
-        self._chk(self._L.mdn_system_upload(self._h, ptr, c.ctypes.data_as(dp)))
+        c = None if coord is None else _f64(coord)
+        self._chk(self._L.mdn_system_upload(self._h, ptr, None if c is None else c.ctypes.data_as(dp)))
         self.n_beads = ptr.contents.n_beads
         self.n_filaments = ptr.contents.n_filaments
         return self
@@ -209,6 +212,15 @@ class ForceFieldManager:
         self._chk(self._L.mdn_minimize(self._h, C.byref(params), None if x is None else x.ctypes.data_as(dp),
                                        f.ctypes.data_as(dp) if want_forces else None, C.byref(res)))
         return x, f, res
+
+    def dof_serialize(self, src, new_coord=None):
+        """serializeDof on the device (mdn_dof_serialize): bead i of the next system is current
+        bead src[i], or (src[i] < 0) the next new bead of ``new_coord`` (3 values each). The next
+        ``upload(desc, None)`` adopts the result."""
+        sv = np.ascontiguousarray(src, dtype=np.int32)
+        nc = None if new_coord is None else _f64(new_coord)
+        self._chk(self._L.mdn_dof_serialize(self._h, int(sv.size), sv.ctypes.data_as(i32p),
+                                            None if nc is None else nc.ctypes.data_as(dp)))
 
     def perturb(self, seed, stream, sigma):
         """x_i += sigma z_i with seeded standard normals z generated on the device (mdn_perturb;

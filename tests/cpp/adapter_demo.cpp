@@ -1,7 +1,7 @@
 // C++ caller of the drop-in adapters (include/mdn_mech.hpp): the MController-shaped sequence
 // serialize/vectorize -> reset neighbor lists -> minimize -> stretch forces and boundary load
 // forces, on the seeded 50-filament network, and the per-force-field adapters (ForceField).
-// Then the device-resident calls. Prints "pairs energy_before energy_after iterations max_force
+// Then the device-resident calls and serializeDof on the device. Prints "pairs energy_before energy_after iterations max_force
 // converged n_linker_stretch_forces n_loaded_filament_ends force_field_adapters_ok
 // device_resident_ok".
 #include <cmath>
@@ -79,6 +79,22 @@ int main() {
         for (size_t i = 0; i < xn.size(); i++) ss += (xn[i] - xd[i]) * (xn[i] - xd[i]);
         const double rms = std::sqrt(ss / (double)xn.size());
         dev_ok &= rms > 0.9 && rms < 1.1;
+        // serializeDof on the device for the same bead set in reversed order (all beads current),
+        // then the next system's upload from those device coordinates
+        {
+            const int nb = d->n_beads;
+            std::vector<int32_t> src(nb), rb(d->fil_bead_offset[d->n_filaments]);
+            for (int i = 0; i < nb; i++) src[i] = nb - 1 - i;
+            for (size_t k = 0; k < rb.size(); k++) rb[k] = nb - 1 - d->fil_beads[k];
+            mdn_system_desc d2 = *d;
+            d2.fil_beads = rb.data();
+            ffm.serializeDof(src, {});
+            ffm.vectorizeAllForceFields(d2);
+            std::vector<double> xr;
+            ffm.getCoords(xr);
+            for (int i = 0; i < nb; i++)
+                for (int q = 0; q < 3; q++) dev_ok &= xr[3 * i + q] == xn[3 * (nb - 1 - i) + q];
+        }
         std::printf("%ld %.10e %.10e %d %.6e %d %zu %d %d %d\n", pairs, e0, res.energies_after.total, res.num_iter, mx,
                     (int)res.converged(), lsf.size(), loaded, ff_ok, dev_ok);
     } catch (const mdn::Error& e) {

@@ -255,3 +255,77 @@ def test_quadratic_line_search_refits_near_the_line_minimizer():
         assert lam_bt < lam_q <= lam_true * (1 + 1e-3), (ratio, lam_bt / lam_true, lam_q / lam_true)
         assert lam_true - lam_q <= near * lam_true
         assert energy(x0 + lam_q * f0) < energy(x0 + lam_bt * f0)
+
+
+def test_database_restatement_reproduces_the_reference_fixture():
+    """TestDatabase.cpp:100-154 ("vectorized data"): the element indices that serializeDof orders
+    beads by (DatabaseBase: append, swap-with-last removal) and the stable indices with their
+    rearrange (Database<T, true>), at every point the reference's test checks."""
+    from oracle.database import ElementList, StableIndex
+    el, si = ElementList(), StableIndex()
+
+    def make(v):
+        el.create(v)
+        si.create(v)
+
+    def drop(v):
+        el.destroy(v)
+        si.destroy(v)
+
+    make(0)
+    assert el.index(0) == 0 and si.where[0] == 0
+    for v in (1, 2, 3, 4):
+        make(v)
+    drop(3)
+    make(5)
+    assert len(si.stable) == 5 and el.index(5) == 4 and si.where[5] == 3 and si.stable[3] == 5
+    make(6)
+    drop(5)
+    drop(1)
+    drop(2)
+    assert len(el.elems) == 3 and len(si.stable) == 6
+    assert el.index(4) == 2 and si.where[4] == 4 and si.stable[4] == 4
+    assert el.index(6) == 1 and si.where[6] == 5 and si.stable[5] == 6
+    si.rearrange()
+    assert len(el.elems) == 3 and len(si.stable) == 3
+    assert el.index(4) == 2 and si.stable[1] == 4 and si.where[4] == 1
+    assert el.index(6) == 1 and si.stable[2] == 6 and si.where[6] == 2
+
+
+def test_database_restatement_invariants():
+    """Random create / destroy sequences: the element order holds every live element once; after
+    rearrange the stable indices are 0 .. n-1, every live element keeps its stable index when it
+    was below the final size, and serialize_map names each surviving element's previous index."""
+    from oracle.database import ElementList, StableIndex, serialize_map
+    rng = np.random.default_rng(5)
+    el, si = ElementList(), StableIndex()
+    nxt = 0
+    for _ in range(40):
+        el.create(nxt)
+        si.create(nxt)
+        nxt += 1
+    for _ in range(20):
+        before = list(el.elems)
+        live = set(el.elems)
+        for _ in range(int(rng.integers(1, 15))):
+            if rng.random() < 0.5 and len(el.elems) > 1:
+                v = el.elems[int(rng.integers(len(el.elems)))]
+                el.destroy(v)
+                si.destroy(v)
+                live.discard(v)
+            else:
+                el.create(nxt)
+                si.create(nxt)
+                live.add(nxt)
+                nxt += 1
+        assert sorted(el.elems) == sorted(live)
+        keep = {v: i for v, i in si.where.items()}
+        final = len(si.stable) - len(si.deleted)
+        si.rearrange()
+        assert len(si.stable) == len(live) and sorted(si.stable) == sorted(live)
+        for v, i in keep.items():
+            if i < final:
+                assert si.where[v] == i
+        m = serialize_map(before, el.elems)
+        for i, v in enumerate(el.elems):
+            assert m[i] == (before.index(v) if v in before else -1)
