@@ -299,6 +299,39 @@ int mdn_perturb(mdn_ctx* ctx, uint64_t seed, uint64_t stream, double sigma);
  * Beads that chemistry destroyed are the current indices that no src entry names. */
 int mdn_dof_serialize(mdn_ctx* ctx, int32_t n_beads, const int32_t* src, const double* new_coord);
 
+/* ---- binding-site pair search (HybridBindingSearchManager) ---------------------------- */
+
+/* One filament-type pair and distance range of a binder (setbindingsearchparameter,
+ * HybridBindingSearchManager.cpp:34-108, and addtoHNeighborList, :1024-1039). */
+typedef struct mdn_binding_params {
+    int32_t ftype1, ftype2;    /* the filament-type pair                                      */
+    int32_t n_sites;           /* binding sites per cylinder (at most 1 << shift_bits)        */
+    int32_t shift_bits;        /* pair entry t = (cylinder << shift_bits) | site              */
+    const float* site_pos;     /* [n_sites] relative positions (binding-site monomer /
+                                  cylinderNumMon, in float); the reference uses the first
+                                  type's list for both cylinders (bindingsites2 = bs1)        */
+    float p2_first, p2_last;   /* the second type's first and last site positions, bounds of
+                                  the early rejections (minparamcyl2 / maxparamcyl2)          */
+    float rmin, rmax;          /* the binder's distance range (nm), squared in float          */
+    float cyl_size;            /* the larger cylinderSize of the two types (the list's range) */
+    int32_t site_step;         /* > 0: only sites with index % site_step == 0 (linkers:
+                                  linkerbindingskip - 1, :847-851); 0: every site             */
+    const float* site_free;    /* [n_cylinders * n_sites], 1 = the site is free
+                                  (speciesboundvec of the binder); NULL: every site free      */
+} mdn_binding_params;
+
+/* HybridBindingSearchManager::updateAllPossibleBindingsstencilHYBD
+ * (HybridBindingSearchManager.cpp:759-966) at coord (host, 3*n_beads; NULL: the device
+ * coordinates): every pair of free binding sites on two cylinders of different filaments whose
+ * distance lies in [rmin, rmax], found through the hybrid neighbor list with the binder's range
+ * (cylinder midpoints within cyl_size + rmax, at least max(rmin - cyl_size, 0), the
+ * same-filament rule) and the reference's per-site quadratic test. Cylinders are this ABI's
+ * global indices; out receives n pairs (t1, t2), t1 on the cylinder with the larger index,
+ * ordered by (cylinder 1, cylinder 2, site 1, site 2). out NULL: *n_out = the count only;
+ * capacity below the count: MDN_ERR_CAPACITY with *n_out = the count. */
+int mdn_binding_pairs(mdn_ctx* ctx, const mdn_binding_params* p, const double* coord, uint32_t* out,
+                      int64_t capacity, int64_t* n_out);
+
 /* Mixed precision (MDN_PREC_MIXED): the bending and boundary terms and the force components of
  * the harmonic springs (stretching, linkers, motors) are computed in FP32 from bond vectors and
  * coordinate differences formed in FP64 (absolute coordinates never enter FP32 arithmetic; spring

@@ -195,6 +195,13 @@ int or_system_ncyl(const or_system* s);
 int or_system_nbend(const or_system* s);
 const double* or_system_stretchforce(const or_system* s, int motor);
 
+/* HybridBindingSearchManager::updateAllPossibleBindingsstencilHYBD (HybridBindingSearchManager.cpp:
+ * 759-966) by brute force over all cylinder pairs at coord: the pairs (t1, t2) in the order
+ * (cylinder 1, cylinder 2, site 1, site 2), t = (cylinder << shift_bits) | site. Returns the
+ * count, or -1 when cap is too small (*needed holds the count). */
+long or_binding_pairs(const or_system* s, const double* coord, const mdn_binding_params* p, uint32_t* out,
+                      long cap, long* needed);
+
 #ifdef __cplusplus
 }
 #endif

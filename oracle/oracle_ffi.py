@@ -67,6 +67,9 @@ def lib(fast=False):
         L.or_nl_cells.restype = C.c_long
         L.or_nl_cells.argtypes = [dp, ip, ip, C.c_int, C.c_float, C.c_float, dp, ip, C.c_long,
                                   C.POINTER(C.c_long)]
+        L.or_binding_pairs.restype = C.c_long
+        L.or_binding_pairs.argtypes = [C.c_void_p, dp, C.POINTER(abi.BindingParams), C.POINTER(C.c_uint32), C.c_long,
+                                       C.POINTER(C.c_long)]
         L.or_system_create.restype = C.c_void_p
         L.or_system_create.argtypes = [C.POINTER(abi.SystemDesc), C.POINTER(C.c_int)]
         L.or_system_destroy.argtypes = [C.c_void_p]
@@ -295,6 +298,17 @@ class OracleSystem:
     @property
     def n_pairs(self):
         return self._L.or_system_npairs(self._h)
+
+    def binding_pairs(self, params, coord):
+        """or_binding_pairs: (n, 2) uint32 (cylinder << shift_bits | site) pairs by brute force."""
+        c = _arr(coord, np.float64)
+        need = C.c_long()
+        self._L.or_binding_pairs(self._h, _p(c), C.byref(params), None, 0, C.byref(need))
+        out = np.zeros(2 * max(1, need.value), np.uint32)
+        n = self._L.or_binding_pairs(self._h, _p(c), C.byref(params), out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                     need.value, C.byref(need))
+        assert n == need.value
+        return out[: 2 * n].reshape(-1, 2)
 
     def pairs(self):
         n = self.n_pairs

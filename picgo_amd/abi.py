@@ -58,6 +58,26 @@ class SystemDesc(C.Structure):
     ]
 
 
+class BindingParams(C.Structure):
+    _fields_ = [("ftype1", C.c_int32), ("ftype2", C.c_int32), ("n_sites", C.c_int32), ("shift_bits", C.c_int32),
+                ("site_pos", C.POINTER(C.c_float)), ("p2_first", C.c_float), ("p2_last", C.c_float),
+                ("rmin", C.c_float), ("rmax", C.c_float), ("cyl_size", C.c_float), ("site_step", C.c_int32),
+                ("site_free", C.POINTER(C.c_float))]
+
+
+def binding_params(site_pos, rmin, rmax, cyl_size, site_step=0, site_free=None, ftype=(0, 0), shift_bits=4):
+    """A BindingParams with its arrays kept alive on the returned object (``.keep``)."""
+    import numpy as np
+    sp = np.ascontiguousarray(site_pos, dtype=np.float32)
+    sf = None if site_free is None else np.ascontiguousarray(site_free, dtype=np.float32)
+    p = BindingParams(ftype1=ftype[0], ftype2=ftype[1], n_sites=sp.size, shift_bits=shift_bits,
+                      site_pos=sp.ctypes.data_as(C.POINTER(C.c_float)), p2_first=float(sp[0]),
+                      p2_last=float(sp[-1]), rmin=rmin, rmax=rmax, cyl_size=cyl_size, site_step=site_step,
+                      site_free=None if sf is None else sf.ctypes.data_as(C.POINTER(C.c_float)))
+    p.keep = (sp, sf)
+    return p
+
+
 class EnergyReport(C.Structure):
     _fields_ = [("total", C.c_double), ("ff", C.c_double * MDN_NUM_FF)]
 

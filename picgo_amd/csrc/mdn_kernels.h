@@ -1759,6 +1759,7 @@ struct NLDev {
     double side;                // bin side
     int nb[3];
     double grid[3];
+    int ta, tb;                 // the filament-type pair a list is restricted to (ta <= tb; -1: any)
 };
 
 // midpoints 0.5*(x1 + x2) (Cylinder.cpp:48) and bin index; flags out-of-grid cylinders
@@ -1802,6 +1803,10 @@ __device__ inline bool nl_keep(const NLDev& n, const double* mid, int c, int cn)
     if (n.cyl_fil[c] == n.cyl_fil[cn]) {
         int sep = abs(n.cyl_pos[c] - n.cyl_pos[cn]);
         if (sep <= 2) return false;
+    }
+    if (n.ta >= 0) {   // the type pair, in ascending order (HybridNeighborListImpl.cpp:423-430)
+        const int t1 = n.cyl_type[c], t2 = n.cyl_type[cn];
+        if (min(t1, t2) != n.ta || max(t1, t2) != n.tb) return false;
     }
     double d2 = 0.0;
     for (int q = 0; q < 3; q++) {
@@ -1852,6 +1857,83 @@ __global__ void k_nl_pairs(NLDev n, const double* __restrict__ mid, const int* _
         pairs[2 * (long)(base + j + 1) + 1] = v;
     }
     for (int i = 0; i < cnt; i++) pairs[2 * (long)(base + i)] = c;
+}
+
+// HybridBindingSearchManager::updateAllPossibleBindingsstencilHYBD (HybridBindingSearchManager.cpp:
+// 759-966) for one list pair (c, cn): the distance between site point p1 = x1 + s1 (x2 - x1) and
+// x3 + t (x4 - x3) is a quadratic A t^2 + B t + C in t; its roots at rmin^2 and rmax^2 bound the
+// sites s2 of cn in range. The expression types are the reference's: site positions and squared
+// bounds in float, the rest in FP64.
+struct BindDev {
+    int n_sites, shift, step, ftype1, ftype2;
+    const float* site;
+    const float* free;    // [n_cyl * n_sites] or null
+    float rminsq, rmaxsq, p2_first, p2_last;
+};
+
+template <int PASS>
+__global__ void k_bind_sites(NLDev n, BindDev b, const double* __restrict__ x, long n_pairs,
+                             const int* __restrict__ pairs, int* __restrict__ count, const int* __restrict__ off,
+                             uint32_t* __restrict__ out) {
+    const long pi = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= n_pairs) return;
+    const int c = pairs[2 * pi], cn = pairs[2 * pi + 1];
+    int cnt = 0;
+    const long base = PASS ? off[pi] : 0;
+    // same filament, and the reference's type test (it compares c's own type with the
+    // complementary one, so only a pair of equal types passes)
+    const int tc = n.cyl_type[c];
+    const bool typed = tc == b.ftype1 || tc == b.ftype2;
+    const int compl_t = tc == b.ftype1 ? b.ftype2 : b.ftype1;
+    if (n.cyl_fil[c] != n.cyl_fil[cn] && typed && tc == compl_t) {
+        const D3 x1 = ld3(x, n.cyl_b1[c]), x2 = ld3(x, n.cyl_b2[c]);
+        const D3 x3 = ld3(x, n.cyl_b1[cn]), x4 = ld3(x, n.cyl_b2[cn]);
+        const double a12[3] = {x2.x - x1.x, x2.y - x1.y, x2.z - x1.z};
+        const double a13[3] = {x3.x - x1.x, x3.y - x1.y, x3.z - x1.z};
+        const double a34[3] = {x4.x - x3.x, x4.y - x3.y, x4.z - x3.z};
+        auto dot = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+        const double s13 = dot(a13, a13), s12 = dot(a12, a12), d13_12 = dot(a13, a12);
+        const double s34 = dot(a34, a34), d13_34 = dot(a13, a34), d34_12 = dot(a34, a12);
+        for (int p1 = 0; p1 < b.n_sites; p1++) {
+            if (b.step > 0 && p1 % b.step != 0) continue;
+            if (b.free && b.free[(long)c * b.n_sites + p1] != 1.0f) continue;
+            const float mp1 = b.site[p1];
+            const double A = s34;
+            const double B = 2.0 * d13_34 - 2.0 * mp1 * d34_12;
+            const double C = s13 + (double)(mp1 * mp1) * s12 - 2.0 * mp1 * d13_12;
+            const double Bsq = B * B;
+            const double C1 = C - (double)b.rminsq, C2 = C - (double)b.rmaxsq;
+            const double q1 = Bsq - 4 * A * C1, q2 = Bsq - 4 * A * C2;
+            const bool st1 = q1 < 0, st2 = q2 < 0;
+            if (st1 && st2) continue;
+            double mn[2] = {0.0, 0.0}, mx[2] = {0.0, 0.0};
+            if (!st1) {
+                const double r1 = (-B + sqrt(q1)) / (2 * A), r2 = (-B - sqrt(q1)) / (2 * A);
+                mn[0] = r1 < r2 ? r1 : r2;
+                mn[1] = r1 < r2 ? r2 : r1;
+                if (mn[0] < b.p2_first && mn[1] > b.p2_last) continue;   // every site too close
+            }
+            if (!st2) {
+                const double r1 = (-B + sqrt(q2)) / (2 * A), r2 = (-B - sqrt(q2)) / (2 * A);
+                mx[0] = r1 < r2 ? r1 : r2;
+                mx[1] = r1 < r2 ? r2 : r1;
+                if (mx[0] > b.p2_last || mx[1] < b.p2_first) continue;   // every site too far
+            }
+            for (int p2 = 0; p2 < b.n_sites; p2++) {
+                if (b.step > 0 && p2 % b.step != 0) continue;
+                if (b.free && b.free[(long)cn * b.n_sites + p2] != 1.0f) continue;
+                const float mp2 = b.site[p2];
+                if (!st2 && (mp2 < mx[0] || mp2 > mx[1])) continue;
+                if (!st1 && (mp2 > mn[0] && mp2 < mn[1])) continue;
+                if (PASS) {
+                    out[2 * (base + cnt)] = ((uint32_t)c << b.shift) | (uint32_t)p1;
+                    out[2 * (base + cnt) + 1] = ((uint32_t)cn << b.shift) | (uint32_t)p2;
+                }
+                cnt++;
+            }
+        }
+    }
+    if (!PASS) count[pi] = cnt;
 }
 
 // CylinderExclVolume::vectorize (CylinderExclVolume.cpp:73-138)

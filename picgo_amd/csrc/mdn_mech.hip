@@ -121,6 +121,11 @@ struct mdn_ctx {
     // NL workspace
     DBuf<double> mid;
     DBuf<int> bin, bin_count, bin_start, bin_cursor, members, nl_count, nl_off, pairs, bnd_cnt, bnd_off;
+    // the binding-site search's own list (mdn_binding_pairs) and its per-pair counts
+    DBuf<double> b_mid;
+    DBuf<int> b_bin, b_bin_count, b_bin_start, b_bin_cursor, b_members, b_count, b_off, b_pairs, b_scount, b_soff;
+    DBuf<float> b_site, b_free;
+    DBuf<uint32_t> b_out;
     DBuf<int> nl_err;
     DBuf<unsigned char> cub_tmp;
     long n_pairs = 0;
@@ -331,6 +336,7 @@ NLDev nl_view(mdn_ctx* ctx) {
         n.grid[q] = ctx->grid[q];
         n.nb[q] = std::max(1, (int)std::ceil(ctx->grid[q] / n.side));
     }
+    n.ta = n.tb = -1;
     return n;
 }
 
@@ -514,7 +520,13 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                        &ctx->d_cyl_type, &ctx->bin, &ctx->bin_count, &ctx->bin_start, &ctx->bin_cursor,
                        &ctx->members, &ctx->nl_count, &ctx->nl_off, &ctx->pairs, &ctx->bnd_cnt, &ctx->bnd_off,
                        &ctx->nl_err, &ctx->bad_count, &ctx->bad_list, &ctx->fallback_ok, &ctx->bstart,
-                       &ctx->sorted_id, &ctx->pos, &ctx->dof_src};
+                       &ctx->sorted_id, &ctx->pos, &ctx->dof_src, &ctx->b_bin, &ctx->b_bin_count,
+                       &ctx->b_bin_start, &ctx->b_bin_cursor, &ctx->b_members, &ctx->b_count, &ctx->b_off,
+                       &ctx->b_pairs, &ctx->b_scount, &ctx->b_soff};
+    ctx->b_mid.release();
+    ctx->b_site.release();
+    ctx->b_free.release();
+    ctx->b_out.release();
     for (auto* b : ib) b->release();
     DBuf<double>* db[] = {&ctx->str_k, &ctx->str_eql, &ctx->bend_k, &ctx->bend_eqt, &ctx->lnk_k, &ctx->lnk_eql,
                           &ctx->lnk_p1, &ctx->lnk_p2, &ctx->lnk_sf, &ctx->mot_k, &ctx->mot_eql, &ctx->mot_p1,
@@ -989,6 +1001,61 @@ int relayout(mdn_ctx* ctx) {
     return MDN_OK;
 }
 
+// The buffers of one hybrid neighbor list build (midpoints, bins, members, per-cylinder counts
+// and offsets, the pair array).
+struct NlBufs {
+    DBuf<double>& mid;
+    DBuf<int>& bin; DBuf<int>& bin_count; DBuf<int>& bin_start; DBuf<int>& bin_cursor; DBuf<int>& members;
+    DBuf<int>& count; DBuf<int>& off; DBuf<int>& pairs;
+};
+
+// HybridCylinderCylinderNL::reset for one list: midpoints, counting sort into bins, the
+// 27-bin stencil (count, scan, emit, per-cylinder sort). *np = the pairs, ordered by cylinder
+// then neighbour.
+static int build_pairs(mdn_ctx* ctx, const NLDev& n, const double* x, NlBufs& B, long* np, const char* what) {
+    int st = MDN_OK;
+    const long nbins = (long)n.nb[0] * n.nb[1] * n.nb[2];
+    if (nbins > (1L << 30)) return fail(ctx, MDN_ERR_UNSUPPORTED, "neighbor-list grid too large (%ld bins)", nbins);
+    const int nc = n.n_cyl;
+    HIPCHK(B.mid.ensure(3 * (size_t)std::max(1, nc)));
+    HIPCHK(B.bin.ensure(std::max(1, nc)));
+    HIPCHK(B.bin_count.ensure(nbins + 1));
+    HIPCHK(B.bin_start.ensure(nbins + 1));
+    HIPCHK(B.bin_cursor.ensure(nbins + 1));
+    HIPCHK(B.members.ensure(std::max(1, nc)));
+    HIPCHK(B.count.ensure(nc + 1));
+    HIPCHK(B.off.ensure(nc + 1));
+    HIPCHK(hipMemsetAsync(B.bin_count.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(B.bin_cursor.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->nl_err.p, 0, sizeof(int), ctx->stream));
+    const int G = std::max(1, ceil_div(nc, BLOCK));
+    hipLaunchKernelGGL(k_nl_bin, dim3(G), dim3(BLOCK), 0, ctx->stream, n, x, B.mid.p, B.bin.p, B.bin_count.p,
+                       ctx->nl_err.p);
+    HIPCHK(hipGetLastError());
+    if ((st = exclusive_scan(ctx, B.bin_count.p, B.bin_start.p, nbins + 1))) return st;
+    hipLaunchKernelGGL(k_nl_fill, dim3(G), dim3(BLOCK), 0, ctx->stream, nc, B.bin.p, B.bin_start.p, B.bin_cursor.p,
+                       B.members.p);
+    hipLaunchKernelGGL(k_nl_pairs<0>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, B.mid.p, B.bin.p, B.bin_start.p,
+                       B.members.p, B.count.p, (const int*)nullptr, (int*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(B.count.p + nc, 0, sizeof(int), ctx->stream));
+    if ((st = exclusive_scan(ctx, B.count.p, B.off.p, nc + 1))) return st;
+    HIPCHK(hipMemcpyAsync(ctx->h_int, B.off.p + nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_int + 1, ctx->nl_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_int[1] != 0) {
+        return fail(ctx, MDN_ERR_NL_BOUNDS, "%d cylinder midpoints outside the neighbor-list grid [0, %g)x[0, %g)x[0, %g)",
+                    ctx->h_int[1], ctx->grid[0], ctx->grid[1], ctx->grid[2]);
+    }
+    *np = ctx->h_int[0];
+    if (*np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many %s pairs (%ld)", what, *np);
+    HIPCHK(B.pairs.ensure(2 * (size_t)std::max(1L, *np)));
+    hipLaunchKernelGGL(k_nl_pairs<1>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, B.mid.p, B.bin.p, B.bin_start.p,
+                       B.members.p, B.count.p, B.off.p, B.pairs.p);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+}
+
 int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
     int st = need_system(ctx, false);
     if (st) return st;
@@ -1000,45 +1067,13 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
     if ((st = set_coords_dev(ctx, coord))) return st;
     const double* x = ctx->x[ctx->cur].p;
     NLDev n = nl_view(ctx);
-    const long nbins = (long)n.nb[0] * n.nb[1] * n.nb[2];
-    if (nbins > (1L << 30)) return fail(ctx, MDN_ERR_UNSUPPORTED, "neighbor-list grid too large (%ld bins)", nbins);
     const int nc = ctx->n_cyl;
-    HIPCHK(ctx->mid.ensure(3 * (size_t)std::max(1, nc)));
-    HIPCHK(ctx->bin.ensure(std::max(1, nc)));
-    HIPCHK(ctx->bin_count.ensure(nbins + 1));
-    HIPCHK(ctx->bin_start.ensure(nbins + 1));
-    HIPCHK(ctx->bin_cursor.ensure(nbins + 1));
-    HIPCHK(ctx->members.ensure(std::max(1, nc)));
-    HIPCHK(ctx->nl_count.ensure(nc + 1));
-    HIPCHK(ctx->nl_off.ensure(nc + 1));
-    HIPCHK(hipMemsetAsync(ctx->bin_count.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->bin_cursor.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->nl_err.p, 0, sizeof(int), ctx->stream));
-    const int G = std::max(1, ceil_div(nc, BLOCK));
-    hipLaunchKernelGGL(k_nl_bin, dim3(G), dim3(BLOCK), 0, ctx->stream, n, x, ctx->mid.p, ctx->bin.p,
-                       ctx->bin_count.p, ctx->nl_err.p);
-    HIPCHK(hipGetLastError());
-    if ((st = exclusive_scan(ctx, ctx->bin_count.p, ctx->bin_start.p, nbins + 1))) return st;
-    hipLaunchKernelGGL(k_nl_fill, dim3(G), dim3(BLOCK), 0, ctx->stream, nc, ctx->bin.p, ctx->bin_start.p,
-                       ctx->bin_cursor.p, ctx->members.p);
-    hipLaunchKernelGGL(k_nl_pairs<0>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, ctx->mid.p, ctx->bin.p,
-                       ctx->bin_start.p, ctx->members.p, ctx->nl_count.p, (const int*)nullptr, (int*)nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemsetAsync(ctx->nl_count.p + nc, 0, sizeof(int), ctx->stream));
-    if ((st = exclusive_scan(ctx, ctx->nl_count.p, ctx->nl_off.p, nc + 1))) return st;
-    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->nl_off.p + nc, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->h_int + 1, ctx->nl_err.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (ctx->h_int[1] != 0) {
-        return fail(ctx, MDN_ERR_NL_BOUNDS, "%d cylinder midpoints outside the neighbor-list grid [0, %g)x[0, %g)x[0, %g)",
-                    ctx->h_int[1], ctx->grid[0], ctx->grid[1], ctx->grid[2]);
+    long np = 0;
+    {
+        NlBufs B{ctx->mid, ctx->bin, ctx->bin_count, ctx->bin_start, ctx->bin_cursor, ctx->members,
+                 ctx->nl_count, ctx->nl_off, ctx->pairs};
+        if ((st = build_pairs(ctx, n, x, B, &np, "excluded-volume"))) return st;
     }
-    const long np = ctx->h_int[0];
-    if (np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many excluded-volume pairs (%ld)", np);
-    HIPCHK(ctx->pairs.ensure(2 * (size_t)std::max(1L, np)));
-    hipLaunchKernelGGL(k_nl_pairs<1>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, ctx->mid.p, ctx->bin.p,
-                       ctx->bin_start.p, ctx->members.p, ctx->nl_count.p, ctx->nl_off.p, ctx->pairs.p);
-    HIPCHK(hipGetLastError());
     // CylinderExclVolume::vectorize
     HIPCHK(ctx->ev_b.ensure(4 * (size_t)std::max(1L, np)));
     HIPCHK(ctx->ev_ks.ensure((size_t)std::max(1L, np)));
@@ -1126,6 +1161,88 @@ int mdn_boundary_load_forces(mdn_ctx* ctx, const double* coord, int32_t n_types,
                               ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MDN_OK;
+}
+
+// HybridBindingSearchManager::updateAllPossibleBindingsstencilHYBD (HybridBindingSearchManager.cpp:
+// 759-966): the binder's hybrid list (addtoHNeighborList, :1024-1039: range cyl_size + rmax,
+// lower bound max(rmin - cyl_size, 0), in float), then the site pairs of every listed pair,
+// counted, scanned and emitted in list order.
+int mdn_binding_pairs(mdn_ctx* ctx, const mdn_binding_params* p, const double* coord, uint32_t* out,
+                      int64_t capacity, int64_t* n_out) {
+    int st = need_system(ctx, false);
+    if (st) return st;
+    if (!p || !n_out) return fail(ctx, MDN_ERR_ARG, "null argument");
+    if (p->n_sites <= 0 || !p->site_pos) return fail(ctx, MDN_ERR_ARG, "n_sites must be positive, site_pos given");
+    if (p->shift_bits <= 0 || p->shift_bits > 16 || p->n_sites > (1 << p->shift_bits))
+        return fail(ctx, MDN_ERR_ARG, "shift_bits %d cannot hold %d sites", p->shift_bits, p->n_sites);
+    if (((long)ctx->n_cyl << p->shift_bits) > 0xffffffffL)
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "%d cylinders << %d bits exceed 32 bits", ctx->n_cyl, p->shift_bits);
+    if (!(p->rmax > 0) || p->rmin < 0 || p->site_step < 0) return fail(ctx, MDN_ERR_ARG, "bad range or site step");
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * 3 * ctx->n_beads, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    // the list's range in float, as addtoHNeighborList and setneighborsearchparameters form it
+    const float rmaxsq_b = p->rmax * p->rmax, rminsq_b = p->rmin * p->rmin;
+    const float rmax_l = p->cyl_size + std::sqrt(rmaxsq_b);
+    const float rmin_l = std::max(std::sqrt(rminsq_b) - p->cyl_size, 0.0f);
+    NLDev n = nl_view(ctx);
+    n.rmaxsq = rmax_l * rmax_l;
+    n.rminsq = rmin_l * rmin_l;
+    n.side = 1.125 * std::sqrt(n.rmaxsq);
+    for (int q = 0; q < 3; q++) n.nb[q] = std::max(1, (int)std::ceil(ctx->grid[q] / n.side));
+    n.ta = std::min(p->ftype1, p->ftype2);
+    n.tb = std::max(p->ftype1, p->ftype2);
+    long np = 0;
+    {
+        NlBufs B{ctx->b_mid, ctx->b_bin, ctx->b_bin_count, ctx->b_bin_start, ctx->b_bin_cursor, ctx->b_members,
+                 ctx->b_count, ctx->b_off, ctx->b_pairs};
+        if ((st = build_pairs(ctx, n, x, B, &np, "binding-list"))) return st;
+    }
+    BindDev b{};
+    b.n_sites = p->n_sites;
+    b.shift = p->shift_bits;
+    b.step = p->site_step;
+    b.ftype1 = std::min(p->ftype1, p->ftype2);   // the reference's _filamentIDvec pair, ascending
+    b.ftype2 = std::max(p->ftype1, p->ftype2);
+    b.rminsq = rminsq_b;
+    b.rmaxsq = rmaxsq_b;
+    b.p2_first = p->p2_first;
+    b.p2_last = p->p2_last;
+    {
+        std::vector<float> site(p->site_pos, p->site_pos + p->n_sites);
+        HIPCHK(upload(ctx, ctx->b_site, site));
+        if (p->site_free) {
+            std::vector<float> fr(p->site_free, p->site_free + (size_t)ctx->n_cyl * p->n_sites);
+            HIPCHK(upload(ctx, ctx->b_free, fr));
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));   // the host vectors end with this scope
+    }
+    b.site = ctx->b_site.p;
+    b.free = p->site_free ? ctx->b_free.p : nullptr;
+    HIPCHK(ctx->b_scount.ensure(np + 1));
+    HIPCHK(ctx->b_soff.ensure(np + 1));
+    const int G = std::max(1, ceil_div(np, BLOCK));
+    hipLaunchKernelGGL(k_bind_sites<0>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, b, x, np, (const int*)ctx->b_pairs.p,
+                       ctx->b_scount.p, (const int*)nullptr, (uint32_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemsetAsync(ctx->b_scount.p + np, 0, sizeof(int), ctx->stream));
+    if ((st = exclusive_scan(ctx, ctx->b_scount.p, ctx->b_soff.p, np + 1))) return st;
+    HIPCHK(d2h(ctx, ctx->h_int, ctx->b_soff.p + np, sizeof(int)));
+    const long total = ctx->h_int[0];
+    *n_out = total;
+    if (!out) return MDN_OK;
+    if (capacity < total)
+        return fail(ctx, MDN_ERR_CAPACITY, "%ld binding pairs, capacity %ld", total, (long)capacity);
+    if (total == 0) return MDN_OK;
+    HIPCHK(ctx->b_out.ensure(2 * (size_t)total));
+    hipLaunchKernelGGL(k_bind_sites<1>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, b, x, np, (const int*)ctx->b_pairs.p,
+                       ctx->b_scount.p, (const int*)ctx->b_soff.p, ctx->b_out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(d2h(ctx, out, ctx->b_out.p, sizeof(uint32_t) * 2 * total));
     return MDN_OK;
 }
 

@@ -32,7 +32,7 @@ EXPORTED = [
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
     "mdn_forces_noalloc", "mdn_displace", "mdn_perturb", "mdn_param_grad_mixed", "mdn_hvp_mixed",
-    "mdn_dof_serialize",
+    "mdn_dof_serialize", "mdn_binding_pairs",
 ]
 
 
@@ -91,6 +91,8 @@ def lib():
         L.mdn_displace.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
         L.mdn_perturb.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double]
         L.mdn_dof_serialize.argtypes = [C.c_void_p, C.c_int32, i32p, dp]
+        L.mdn_binding_pairs.argtypes = [C.c_void_p, C.POINTER(abi.BindingParams), dp, C.POINTER(C.c_uint32),
+                                        C.c_int64, i64p]
         _lib = L
     return _lib
 
@@ -212,6 +214,17 @@ class ForceFieldManager:
         self._chk(self._L.mdn_minimize(self._h, C.byref(params), None if x is None else x.ctypes.data_as(dp),
                                        f.ctypes.data_as(dp) if want_forces else None, C.byref(res)))
         return x, f, res
+
+    def binding_pairs(self, params, coord=None):
+        """HybridBindingSearchManager's site pairs (mdn_binding_pairs): an (n, 2) uint32 array of
+        (cylinder << shift_bits | site) entries, the first on the larger cylinder index."""
+        cp = None if coord is None else _f64(coord).ctypes.data_as(dp)
+        n = C.c_int64()
+        self._chk(self._L.mdn_binding_pairs(self._h, C.byref(params), cp, None, 0, C.byref(n)))
+        out = np.zeros(2 * max(1, n.value), np.uint32)
+        self._chk(self._L.mdn_binding_pairs(self._h, C.byref(params), cp, out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                            n.value, C.byref(n)))
+        return out[: 2 * n.value].reshape(-1, 2)
 
     def dof_serialize(self, src, new_coord=None):
         """serializeDof on the device (mdn_dof_serialize): bead i of the next system is current

@@ -384,3 +384,67 @@ def test_cg_safe_mode_then_recovery_move():
     assert res.num_safe_mode >= 1 and res.num_recovery >= 1
     assert moves[0] == pytest.approx(0.25)
     assert res.call_limit_exceeded
+
+
+def _direct_binding_sets(desc, coord, sites, rmin, rmax, free=None, step=0):
+    """Every site pair (c > c', different filaments) by its distance, straight from the geometry:
+    the pairs surely inside [rmin, rmax] and those surely outside (relative margin 1e-9 on the
+    squared distance), as (c << 4 | s1, c' << 4 | s2) entries."""
+    nf = desc.n_filaments
+    off = np.ctypeslib.as_array(desc.fil_bead_offset, shape=(nf + 1,))
+    fb = np.ctypeslib.as_array(desc.fil_beads, shape=(int(off[-1]),))
+    b1, b2, fil = [], [], []
+    for f in range(nf):
+        for k in range(off[f], off[f + 1] - 1):
+            b1.append(fb[k]); b2.append(fb[k + 1]); fil.append(f)
+    b1, b2, fil = np.array(b1), np.array(b2), np.array(fil)
+    x = coord.reshape(-1, 3)
+    s = np.asarray(sites, np.float32).astype(np.float64)
+    use = np.array([step == 0 or i % step == 0 for i in range(len(s))])
+    pts = x[b1][:, None, :] + s[None, :, None] * (x[b2] - x[b1])[:, None, :]      # (ncyl, nsite, 3)
+    lo2, hi2 = np.float32(rmin) * np.float32(rmin), np.float32(rmax) * np.float32(rmax)
+    inside, outside = set(), set()
+    for c in range(len(b1)):
+        cn = np.arange(c)
+        cn = cn[fil[cn] != fil[c]]
+        if cn.size == 0:
+            continue
+        d2 = ((pts[c][:, None, None, :] - pts[cn][None, :, :, :]) ** 2).sum(-1)     # (s1, c', s2)
+        ok1 = use[:, None, None] & use[None, None, :]
+        if free is not None:
+            ok1 = ok1 & (free[c][:, None, None] == 1) & (free[cn][None, :, :] == 1)
+        sure_in = ok1 & (d2 >= lo2 * (1 + 1e-9)) & (d2 <= hi2 * (1 - 1e-9))
+        sure_out = ~ok1 | (d2 < lo2 * (1 - 1e-9)) | (d2 > hi2 * (1 + 1e-9))
+        for s1, j, s2 in zip(*np.nonzero(sure_in)):
+            inside.add((c << 4 | int(s1), int(cn[j]) << 4 | int(s2)))
+        for s1, j, s2 in zip(*np.nonzero(~sure_out)):
+            outside.add((c << 4 | int(s1), int(cn[j]) << 4 | int(s2)))   # "not surely outside"
+    return inside, outside
+
+
+@pytest.mark.parametrize("rmin,rmax,step,with_free", [(30.0, 40.0, 0, False), (30.0, 40.0, 1, True),
+                                                     (175.0, 225.0, 0, False), (175.0, 225.0, 2, True)])
+def test_binding_pairs_are_the_sites_in_range(rmin, rmax, step, with_free):
+    """The binding-site search (HybridBindingSearchManager.cpp:759-966) finds the pairs of free
+    sites on cylinders of different filaments at a distance in [rMin, rMax], whatever its list and
+    quadratic-root shortcuts: its set equals the pairs found from the site-to-site distances
+    directly, up to pairs within 1e-9 of a bound (none on these seeded networks). The list range
+    uses the largest cylinder length, so its midpoint prefilter drops no pair."""
+    net = synth.Network.config(0)
+    o = of.OracleSystem(net.desc)
+    x = net.coord + np.random.default_rng(int(rmax)).normal(0.0, 4.0, net.coord.size)
+    ncyl = net.n_cylinders
+    sites = np.arange(4, 40, 4) / 40.0
+    free = (np.random.default_rng(3).random((ncyl, len(sites))) < 0.7).astype(np.float32) if with_free else None
+    xb = x.reshape(-1, 3)
+    nf = net.desc.n_filaments
+    off = np.ctypeslib.as_array(net.desc.fil_bead_offset, shape=(nf + 1,))
+    fb = np.ctypeslib.as_array(net.desc.fil_beads, shape=(int(off[-1]),))
+    lens = [np.linalg.norm(xb[fb[k + 1]] - xb[fb[k]]) for f in range(nf) for k in range(off[f], off[f + 1] - 1)]
+    bp = abi.binding_params(sites, rmin, rmax, float(max(lens)) + 1.0, site_step=step,
+                            site_free=None if free is None else free.reshape(-1))
+    got = {(int(a), int(b)) for a, b in o.binding_pairs(bp, x)}
+    inside, maybe = _direct_binding_sets(net.desc, x, sites, rmin, rmax, free, step)
+    assert len(inside) > 50
+    assert inside <= got <= maybe
+    assert got == inside
