@@ -415,8 +415,9 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
 /* The schedule of the last batched minimization on a lead context: out3 = [workgroup size,
  * largest-first order (1) or given order (0), tuned in that call (1) or reused (0)]; tune_ms (may
  * be null, MDN_BATCH_CANDIDATES values) = the tuning times in ms of the candidates 128 given,
- * 128 largest first, 256 given, 256 largest first, 512 given, 512 largest first. */
-#define MDN_BATCH_CANDIDATES 6
+ * 128 largest first, 256 given, 256 largest first, 512 given, 512 largest first, 1024 given,
+ * 1024 largest first. */
+#define MDN_BATCH_CANDIDATES 8
 int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms);
 
 #ifdef __cplusplus

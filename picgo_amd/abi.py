@@ -28,6 +28,7 @@ MDN_BND_PLANE, MDN_BND_SPHERE = 0, 1
 MDN_CG_STEEPEST, MDN_CG_FLETCHER_REEVES, MDN_CG_POLAK_RIBIERE = 0, 1, 2
 MDN_PREC_FP64, MDN_PREC_MIXED = 0, 1
 MDN_LS_BACKTRACKING, MDN_LS_QUADRATIC = 0, 1
+MDN_BATCH_CANDIDATES = 8   # schedules the batched minimizer's tuner times (include/mdn_mech.h)
 MDN_MIN_ERR_LINESEARCH = 1
 
 c_int32_p = C.POINTER(C.c_int32)

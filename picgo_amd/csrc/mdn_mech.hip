@@ -159,7 +159,7 @@ struct mdn_ctx {
     DBuf<int> batch_order, batch_queue;
     long batch_key = -1;          // (systems, precision, beads) of the tuned schedule
     int batch_nt = 256, batch_sorted = 1, batch_tuned = 0;
-    double batch_tune_ms[MDN_BATCH_CANDIDATES] = {0, 0, 0, 0, 0, 0};
+    double batch_tune_ms[MDN_BATCH_CANDIDATES] = {};
 
     bool use_graph = true;   // replay the CG slots as a captured hipGraph (MDN_CG_GRAPH=0: launch them directly)
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // per precision (MDN_PREC_*)
@@ -1867,12 +1867,18 @@ static int launch_batched(mdn_ctx* L, int n, const BatchParams& P) {
     return MDN_OK;
 }
 
+template <typename R>
+static int run_batched_t(mdn_ctx* L, int n, const BatchParams& P, int nt) {
+    switch (nt) {
+    case 128: return launch_batched<R, 128>(L, n, P);
+    case 512: return launch_batched<R, 512>(L, n, P);
+    case 1024: return launch_batched<R, 1024>(L, n, P);
+    default: return launch_batched<R, 256>(L, n, P);
+    }
+}
+
 static int run_batched(mdn_ctx* L, int n, const BatchParams& P, int prec, int nt) {
-    if (prec == MDN_PREC_MIXED)
-        return nt == 512 ? launch_batched<float, 512>(L, n, P)
-               : nt == 128 ? launch_batched<float, 128>(L, n, P) : launch_batched<float, 256>(L, n, P);
-    return nt == 512 ? launch_batched<double, 512>(L, n, P)
-           : nt == 128 ? launch_batched<double, 128>(L, n, P) : launch_batched<double, 256>(L, n, P);
+    return prec == MDN_PREC_MIXED ? run_batched_t<float>(L, n, P, nt) : run_batched_t<double>(L, n, P, nt);
 }
 
 extern "C" {
@@ -1952,7 +1958,7 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
         HIPCHK_CTX(L, hipEventCreate(&e0));
         HIPCHK_CTX(L, hipEventCreate(&e1));
         float best = 1e30f;
-        const int nts[3] = {128, 256, 512};
+        const int nts[4] = {128, 256, 512, 1024};
         for (int c = 0; c < MDN_BATCH_CANDIDATES; c++) {
             const int nt = nts[c >> 1], sorted = c & 1;
             for (int i = 0; i < n; i++)   // every candidate starts from the given coordinates

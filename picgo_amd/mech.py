@@ -370,7 +370,7 @@ def minimize_batched(systems, params, coords, want_forces=True):
 def last_batch_schedule(lead):
     """(workgroup size, largest-first order, tuned in the last call, tuning times in ms)."""
     out = (C.c_int32 * 3)()
-    ms = (C.c_double * 6)()
+    ms = (C.c_double * abi.MDN_BATCH_CANDIDATES)()
     st = lib().mdn_last_batch_schedule(lead._h, out, ms)
     if st != 0:
         raise MdnError(st, "mdn_last_batch_schedule")

@@ -64,6 +64,29 @@ int main(void) {
     assert sizes == py
 
 
+def test_header_constants_match_the_python_mirror():
+    """The array length the batched schedule query fills and the other header constants the
+    Python side sizes buffers or checks with."""
+    text = open(os.path.join(ROOT, "include", "mdn_mech.h")).read()
+    defs = dict(re.findall(r"^#define\s+(MDN_\w+)\s+(\d+)", text, flags=re.M))
+    assert int(defs["MDN_BATCH_CANDIDATES"]) == abi.MDN_BATCH_CANDIDATES
+    enums = dict(re.findall(r"(MDN_LS_\w+)\s*=\s*(\d+)", text))
+    assert int(enums["MDN_LS_BACKTRACKING"]) == abi.MDN_LS_BACKTRACKING
+    assert int(enums["MDN_LS_QUADRATIC"]) == abi.MDN_LS_QUADRATIC
+    binding = """
+#include <stdio.h>
+#include "mdn_mech.h"
+int main(void) { printf("%zu\\n", sizeof(mdn_binding_params)); return 0; }
+"""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        cpath, exe = os.path.join(d, "b.c"), os.path.join(d, "b")
+        open(cpath, "w").write(binding)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, cpath], check=True)
+        size = int(subprocess.run([exe], capture_output=True, text=True, check=True).stdout)
+    assert size == C.sizeof(abi.BindingParams)
+
+
 def test_context_without_gpu_fails_cleanly():
     """On a host without a visible GPU the library reports MDN_ERR_HIP instead of crashing."""
     try:
