@@ -1760,7 +1760,17 @@ struct NLDev {
     int nb[3];
     double grid[3];
     int ta, tb;                 // the filament-type pair a list is restricted to (ta <= tb; -1: any)
+    // bins are numbered tile by tile: tiles of T x T x T bins (T = 2, 4 or 8), nt[] tiles per
+    // axis; bin id = tile * T^3 + lx + T (ly + T lz)
+    int T, lt;                  // lt = log2 T
+    int nt[3];
 };
+
+__device__ inline int nl_bin_id(const NLDev& n, int x, int y, int z) {
+    const int m = n.T - 1;
+    const int tile = (x >> n.lt) + n.nt[0] * ((y >> n.lt) + n.nt[1] * (z >> n.lt));
+    return (tile << (3 * n.lt)) + (x & m) + ((y & m) << n.lt) + ((z & m) << (2 * n.lt));
+}
 
 // midpoints 0.5*(x1 + x2) (Cylinder.cpp:48) and bin index; flags out-of-grid cylinders
 __global__ void k_nl_bin(NLDev n, const double* __restrict__ x, double* __restrict__ mid, int* __restrict__ bin,
@@ -1783,80 +1793,175 @@ __global__ void k_nl_bin(NLDev n, const double* __restrict__ x, double* __restri
         bin[c] = 0;
         return;
     }
-    int bb = ix[0] + n.nb[0] * (ix[1] + n.nb[1] * ix[2]);
+    int bb = nl_bin_id(n, ix[0], ix[1], ix[2]);
     bin[c] = bb;
     atomicAdd(&bin_count[bb], 1);
 }
 
+// the members of each bin, with their bin and midpoint in the same (bin) order
 __global__ void k_nl_fill(int n_cyl, const int* __restrict__ bin, const int* __restrict__ bin_start,
-                          int* __restrict__ cursor, int* __restrict__ members) {
+                          int* __restrict__ cursor, const double* __restrict__ mid, int* __restrict__ members,
+                          int* __restrict__ mbin, double* __restrict__ mmid) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_cyl) return;
     int bb = bin[c];
     int k = atomicAdd(&cursor[bb], 1);
-    members[bin_start[bb] + k] = c;
+    const int m = bin_start[bb] + k;
+    members[m] = c;
+    mbin[m] = bb;
+    for (int q = 0; q < 3; q++) mmid[3 * (long)m + q] = mid[3 * (long)c + q];
 }
 
-// HybridNeighborListImpl.cpp:411-439 predicate
-__device__ inline bool nl_keep(const NLDev& n, const double* mid, int c, int cn) {
+// HybridNeighborListImpl.cpp:411-439 predicate, on explicit midpoints (a candidate's from LDS)
+__device__ inline bool nl_keep_at(const NLDev& n, int c, int cn, const double* mc, const double* mn) {
     if (c <= cn) return false;
     if (n.cyl_fil[c] == n.cyl_fil[cn]) {
         int sep = abs(n.cyl_pos[c] - n.cyl_pos[cn]);
         if (sep <= 2) return false;
     }
-    if (n.ta >= 0) {   // the type pair, in ascending order (HybridNeighborListImpl.cpp:423-430)
+    if (n.ta >= 0) {
         const int t1 = n.cyl_type[c], t2 = n.cyl_type[cn];
         if (min(t1, t2) != n.ta || max(t1, t2) != n.tb) return false;
     }
     double d2 = 0.0;
     for (int q = 0; q < 3; q++) {
-        double v = mid[3 * cn + q] - mid[3 * c + q];
+        double v = mn[q] - mc[q];
         d2 += v * v;
     }
-    if (d2 < n.rminsq || d2 > n.rmaxsq) return false;
-    return true;
+    return !(d2 < n.rminsq || d2 > n.rmaxsq);
 }
 
-// pass 0: count per cylinder; pass 1: emit into [off[c], off[c+1]) and sort ascending
+// Cell-list pairs, one workgroup per tile of T^3 bins (HybridCylinderCylinderNL::updateNeighborsbin,
+// HybridNeighborListImpl.cpp:332-471, over the 27 bins around each cylinder's bin):
+//   1. each thread takes halo bins (the tile's bins and one bin around them, (T + 2)^3) and
+//      reads their member ranges; a block scan of the counts places every halo bin's members
+//      contiguously, in halo-bin order, in LDS with their midpoints,
+//   2. every cylinder of the tile tests the members of its 27 bins, read from LDS as nine
+//      x-rows of three consecutive halo bins.
+// A halo with more than NL_CAP members is read from global memory instead (the same 27 bins). The
+// bins are wider than the range, so the pair set is the predicate's whatever the bin size.
+// PASS 0: count per cylinder; PASS 1: emit into [off[c], off[c + 1]) and sort ascending.
+constexpr int NL_CAP = 768;
+constexpr int NL_HMAX = 10 * 10 * 10;   // (T + 2)^3 for T <= 8
+
 template <int PASS>
-__global__ void k_nl_pairs(NLDev n, const double* __restrict__ mid, const int* __restrict__ bin,
-                           const int* __restrict__ bin_start, const int* __restrict__ members,
-                           int* __restrict__ count, const int* __restrict__ off, int* __restrict__ pairs) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n.n_cyl) return;
-    const int bb = bin[c];
-    const int bx = bb % n.nb[0], by = (bb / n.nb[0]) % n.nb[1], bz = bb / (n.nb[0] * n.nb[1]);
-    int cnt = 0;
-    int base = PASS ? off[c] : 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                int X = bx + dx, Y = by + dy, Z = bz + dz;
-                if (X < 0 || Y < 0 || Z < 0 || X >= n.nb[0] || Y >= n.nb[1] || Z >= n.nb[2]) continue;
-                int nbin = X + n.nb[0] * (Y + n.nb[1] * Z);
-                for (int m = bin_start[nbin]; m < bin_start[nbin + 1]; m++) {
-                    int cn = members[m];
-                    if (nl_keep(n, mid, c, cn)) {
-                        if (PASS) pairs[2 * (long)(base + cnt) + 1] = cn;
-                        cnt++;
+__global__ void __launch_bounds__(BLOCK) k_nl_tile(NLDev n, const int* __restrict__ bin_start,
+                                                   const int* __restrict__ members, const int* __restrict__ mbin,
+                                                   const double* __restrict__ mmid, int* __restrict__ count,
+                                                   const int* __restrict__ off, int* __restrict__ pairs) {
+    __shared__ int s_a[NL_HMAX], s_start[NL_HMAX + 1], s_part[BLOCK];
+    __shared__ int s_sc[NL_CAP];
+    __shared__ double s_mid[3 * NL_CAP];
+    const int T = n.T, lt = n.lt, T3 = 1 << (3 * lt), H = T + 2, H3 = H * H * H;
+    const int tile = blockIdx.x;
+    const int i0 = bin_start[(long)tile * T3], i1 = bin_start[(long)(tile + 1) * T3];
+    if (i0 == i1) return;   // an empty tile (the whole workgroup)
+    const int tx = tile % n.nt[0], ty = (tile / n.nt[0]) % n.nt[1], tz = tile / (n.nt[0] * n.nt[1]);
+    const int hx0 = tx * T - 1, hy0 = ty * T - 1, hz0 = tz * T - 1;
+    // 1. the halo bins' member ranges (bins outside the grid are empty)
+    const int run = (H3 + BLOCK - 1) / BLOCK, r0 = threadIdx.x * run, r1 = min(r0 + run, H3);
+    int tot = 0;
+    for (int h = r0; h < r1; h++) {
+        const int X = hx0 + h % H, Y = hy0 + (h / H) % H, Z = hz0 + h / (H * H);
+        int a = 0, c = 0;
+        if (X >= 0 && Y >= 0 && Z >= 0 && X < n.nb[0] && Y < n.nb[1] && Z < n.nb[2]) {
+            const int g = nl_bin_id(n, X, Y, Z);
+            a = bin_start[g];
+            c = bin_start[g + 1] - a;
+        }
+        s_a[h] = a;
+        s_start[h] = c;
+        tot += c;
+    }
+    s_part[threadIdx.x] = tot;
+    __syncthreads();
+    for (int step = 1; step < BLOCK; step *= 2) {   // inclusive scan of the runs' totals
+        const int v = threadIdx.x >= step ? s_part[threadIdx.x - step] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    {
+        int acc = s_part[threadIdx.x] - tot;   // exclusive prefix of this run
+        for (int h = r0; h < r1; h++) {
+            const int v = s_start[h];
+            s_start[h] = acc;
+            acc += v;
+        }
+        if (threadIdx.x == BLOCK - 1) s_start[H3] = s_part[BLOCK - 1];
+    }
+    __syncthreads();
+    const int nh = s_start[H3];
+    const bool staged = nh <= NL_CAP;   // the same for the whole workgroup
+    if (staged) {
+        int h = 0;
+        for (int j = threadIdx.x; j < nh; j += BLOCK) {
+            // the halo bin holding position j: the last h with s_start[h] <= j (binary search)
+            int lo = 0, hi = H3 - 1;
+            while (lo < hi) {
+                const int md = (lo + hi + 1) >> 1;
+                if (s_start[md] <= j) lo = md;
+                else hi = md - 1;
+            }
+            h = lo;
+            const int m = s_a[h] + (j - s_start[h]);
+            s_sc[j] = members[m];
+            for (int q = 0; q < 3; q++) s_mid[3 * j + q] = mmid[3 * (long)m + q];
+        }
+        __syncthreads();
+    }
+    // 2. the tile's cylinders
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        const int c = members[i];
+        const double mc[3] = {mmid[3 * (long)i], mmid[3 * (long)i + 1], mmid[3 * (long)i + 2]};
+        const int loc = mbin[i] & (T3 - 1);
+        const int lx = (loc & (T - 1)) + 1, ly = ((loc >> lt) & (T - 1)) + 1, lz = (loc >> (2 * lt)) + 1;   // halo coords
+        int cnt = 0;
+        const long base = PASS ? off[c] : 0;
+        if (staged) {
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++) {
+                    const int h = (lx - 1) + H * ((ly + dy) + H * (lz + dz));
+                    for (int k = s_start[h]; k < s_start[h + 3]; k++) {
+                        const int cn = s_sc[k];
+                        if (nl_keep_at(n, c, cn, mc, &s_mid[3 * k])) {
+                            if (PASS) pairs[2 * (base + cnt) + 1] = cn;
+                            cnt++;
+                        }
                     }
                 }
-            }
-    if (!PASS) {
-        count[c] = cnt;
-        return;
-    }
-    // insertion sort of the neighbour ids, then the owning cylinder id
-    for (int i = 1; i < cnt; i++) {
-        int v = pairs[2 * (long)(base + i) + 1];
-        int j = i - 1;
-        while (j >= 0 && pairs[2 * (long)(base + j) + 1] > v) {
-            pairs[2 * (long)(base + j + 1) + 1] = pairs[2 * (long)(base + j) + 1];
-            j--;
+        } else {
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        const int X = hx0 + lx + dx, Y = hy0 + ly + dy, Z = hz0 + lz + dz;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= n.nb[0] || Y >= n.nb[1] || Z >= n.nb[2]) continue;
+                        const int nbin = nl_bin_id(n, X, Y, Z);
+                        for (int m = bin_start[nbin]; m < bin_start[nbin + 1]; m++) {
+                            const int cn = members[m];
+                            if (nl_keep_at(n, c, cn, mc, &mmid[3 * (long)m])) {
+                                if (PASS) pairs[2 * (base + cnt) + 1] = cn;
+                                cnt++;
+                            }
+                        }
+                    }
         }
-        pairs[2 * (long)(base + j + 1) + 1] = v;
+        if (!PASS) {
+            count[c] = cnt;
+            continue;
+        }
+        // insertion sort of the neighbour ids, then the owning cylinder id
+        for (int a = 1; a < cnt; a++) {
+            const int v = pairs[2 * (base + a) + 1];
+            int j = a - 1;
+            while (j >= 0 && pairs[2 * (base + j) + 1] > v) {
+                pairs[2 * (base + j + 1) + 1] = pairs[2 * (base + j) + 1];
+                j--;
+            }
+            pairs[2 * (base + j + 1) + 1] = v;
+        }
+        for (int a = 0; a < cnt; a++) pairs[2 * (base + a)] = c;
     }
-    for (int i = 0; i < cnt; i++) pairs[2 * (long)(base + i)] = c;
 }
 
 // HybridBindingSearchManager::updateAllPossibleBindingsstencilHYBD (HybridBindingSearchManager.cpp:

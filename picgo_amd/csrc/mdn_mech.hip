@@ -122,8 +122,9 @@ struct mdn_ctx {
     DBuf<double> mid;
     DBuf<int> bin, bin_count, bin_start, bin_cursor, members, nl_count, nl_off, pairs, bnd_cnt, bnd_off;
     // the binding-site search's own list (mdn_binding_pairs) and its per-pair counts
-    DBuf<double> b_mid;
+    DBuf<double> b_mid, mmid, b_mmid;
     DBuf<int> b_bin, b_bin_count, b_bin_start, b_bin_cursor, b_members, b_count, b_off, b_pairs, b_scount, b_soff;
+    DBuf<int> mbin, b_mbin;
     DBuf<float> b_site, b_free;
     DBuf<uint32_t> b_out;
     DBuf<int> nl_err;
@@ -319,6 +320,21 @@ int ensure_eval_buffers(mdn_ctx* ctx) {
     return MDN_OK;
 }
 
+// The tile edge of the list kernel: the largest of 8, 4, 2 bins that still gives 1024 tiles (a
+// workgroup each), so that small grids keep the chip busy.
+void nl_tiles(NLDev& n) {
+    for (int T = 8, lt = 3; T >= 2; T /= 2, lt--) {
+        n.T = T;
+        n.lt = lt;
+        long tiles = 1;
+        for (int q = 0; q < 3; q++) {
+            n.nt[q] = (n.nb[q] + T - 1) / T;
+            tiles *= n.nt[q];
+        }
+        if (tiles >= 1024) break;
+    }
+}
+
 NLDev nl_view(mdn_ctx* ctx) {
     NLDev n{};
     n.n_cyl = ctx->n_cyl;
@@ -337,6 +353,7 @@ NLDev nl_view(mdn_ctx* ctx) {
         n.nb[q] = std::max(1, (int)std::ceil(ctx->grid[q] / n.side));
     }
     n.ta = n.tb = -1;
+    nl_tiles(n);
     return n;
 }
 
@@ -524,6 +541,10 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
                        &ctx->b_bin_start, &ctx->b_bin_cursor, &ctx->b_members, &ctx->b_count, &ctx->b_off,
                        &ctx->b_pairs, &ctx->b_scount, &ctx->b_soff};
     ctx->b_mid.release();
+    ctx->mmid.release();
+    ctx->b_mmid.release();
+    ctx->mbin.release();
+    ctx->b_mbin.release();
     ctx->b_site.release();
     ctx->b_free.release();
     ctx->b_out.release();
@@ -1007,6 +1028,7 @@ struct NlBufs {
     DBuf<double>& mid;
     DBuf<int>& bin; DBuf<int>& bin_count; DBuf<int>& bin_start; DBuf<int>& bin_cursor; DBuf<int>& members;
     DBuf<int>& count; DBuf<int>& off; DBuf<int>& pairs;
+    DBuf<int>& mbin; DBuf<double>& mmid;   // each member's bin and midpoint, in bin order
 };
 
 // HybridCylinderCylinderNL::reset for one list: midpoints, counting sort into bins, the
@@ -1014,7 +1036,8 @@ struct NlBufs {
 // then neighbour.
 static int build_pairs(mdn_ctx* ctx, const NLDev& n, const double* x, NlBufs& B, long* np, const char* what) {
     int st = MDN_OK;
-    const long nbins = (long)n.nb[0] * n.nb[1] * n.nb[2];
+    const long ntiles = (long)n.nt[0] * n.nt[1] * n.nt[2];
+    const long nbins = ntiles * n.T * n.T * n.T;   // tile by tile, the last tiles padded
     if (nbins > (1L << 30)) return fail(ctx, MDN_ERR_UNSUPPORTED, "neighbor-list grid too large (%ld bins)", nbins);
     const int nc = n.n_cyl;
     HIPCHK(B.mid.ensure(3 * (size_t)std::max(1, nc)));
@@ -1023,6 +1046,8 @@ static int build_pairs(mdn_ctx* ctx, const NLDev& n, const double* x, NlBufs& B,
     HIPCHK(B.bin_start.ensure(nbins + 1));
     HIPCHK(B.bin_cursor.ensure(nbins + 1));
     HIPCHK(B.members.ensure(std::max(1, nc)));
+    HIPCHK(B.mbin.ensure(std::max(1, nc)));
+    HIPCHK(B.mmid.ensure(3 * (size_t)std::max(1, nc)));
     HIPCHK(B.count.ensure(nc + 1));
     HIPCHK(B.off.ensure(nc + 1));
     HIPCHK(hipMemsetAsync(B.bin_count.p, 0, sizeof(int) * (nbins + 1), ctx->stream));
@@ -1034,9 +1059,10 @@ static int build_pairs(mdn_ctx* ctx, const NLDev& n, const double* x, NlBufs& B,
     HIPCHK(hipGetLastError());
     if ((st = exclusive_scan(ctx, B.bin_count.p, B.bin_start.p, nbins + 1))) return st;
     hipLaunchKernelGGL(k_nl_fill, dim3(G), dim3(BLOCK), 0, ctx->stream, nc, B.bin.p, B.bin_start.p, B.bin_cursor.p,
-                       B.members.p);
-    hipLaunchKernelGGL(k_nl_pairs<0>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, B.mid.p, B.bin.p, B.bin_start.p,
-                       B.members.p, B.count.p, (const int*)nullptr, (int*)nullptr);
+                       (const double*)B.mid.p, B.members.p, B.mbin.p, B.mmid.p);
+    hipLaunchKernelGGL(k_nl_tile<0>, dim3(ntiles), dim3(BLOCK), 0, ctx->stream, n, (const int*)B.bin_start.p,
+                       (const int*)B.members.p, (const int*)B.mbin.p, (const double*)B.mmid.p, B.count.p,
+                       (const int*)nullptr, (int*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemsetAsync(B.count.p + nc, 0, sizeof(int), ctx->stream));
     if ((st = exclusive_scan(ctx, B.count.p, B.off.p, nc + 1))) return st;
@@ -1050,8 +1076,9 @@ static int build_pairs(mdn_ctx* ctx, const NLDev& n, const double* x, NlBufs& B,
     *np = ctx->h_int[0];
     if (*np > 0x7fffffffL / 4) return fail(ctx, MDN_ERR_UNSUPPORTED, "too many %s pairs (%ld)", what, *np);
     HIPCHK(B.pairs.ensure(2 * (size_t)std::max(1L, *np)));
-    hipLaunchKernelGGL(k_nl_pairs<1>, dim3(G), dim3(BLOCK), 0, ctx->stream, n, B.mid.p, B.bin.p, B.bin_start.p,
-                       B.members.p, B.count.p, B.off.p, B.pairs.p);
+    hipLaunchKernelGGL(k_nl_tile<1>, dim3(ntiles), dim3(BLOCK), 0, ctx->stream, n, (const int*)B.bin_start.p,
+                       (const int*)B.members.p, (const int*)B.mbin.p, (const double*)B.mmid.p, B.count.p,
+                       (const int*)B.off.p, B.pairs.p);
     HIPCHK(hipGetLastError());
     return MDN_OK;
 }
@@ -1071,7 +1098,7 @@ int mdn_nl_build(mdn_ctx* ctx, const double* coord, int64_t* n_pairs_out) {
     long np = 0;
     {
         NlBufs B{ctx->mid, ctx->bin, ctx->bin_count, ctx->bin_start, ctx->bin_cursor, ctx->members,
-                 ctx->nl_count, ctx->nl_off, ctx->pairs};
+                 ctx->nl_count, ctx->nl_off, ctx->pairs, ctx->mbin, ctx->mmid};
         if ((st = build_pairs(ctx, n, x, B, &np, "excluded-volume"))) return st;
     }
     // CylinderExclVolume::vectorize
@@ -1194,12 +1221,13 @@ int mdn_binding_pairs(mdn_ctx* ctx, const mdn_binding_params* p, const double* c
     n.rminsq = rmin_l * rmin_l;
     n.side = 1.125 * std::sqrt(n.rmaxsq);
     for (int q = 0; q < 3; q++) n.nb[q] = std::max(1, (int)std::ceil(ctx->grid[q] / n.side));
+    nl_tiles(n);
     n.ta = std::min(p->ftype1, p->ftype2);
     n.tb = std::max(p->ftype1, p->ftype2);
     long np = 0;
     {
         NlBufs B{ctx->b_mid, ctx->b_bin, ctx->b_bin_count, ctx->b_bin_start, ctx->b_bin_cursor, ctx->b_members,
-                 ctx->b_count, ctx->b_off, ctx->b_pairs};
+                 ctx->b_count, ctx->b_off, ctx->b_pairs, ctx->b_mbin, ctx->b_mmid};
         if ((st = build_pairs(ctx, n, x, B, &np, "binding-list"))) return st;
     }
     BindDev b{};
