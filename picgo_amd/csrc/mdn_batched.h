@@ -355,6 +355,22 @@ __global__ void __launch_bounds__(256) k_batch_collect(const BatchItem* __restri
     }
 }
 
+// One launch for the start copies of a whole batch (each system's device coordinates into its
+// x[0], the start's backup for the schedule tuning and its restores): copy i moves c[i].n doubles
+// from c[i].src to c[i].dst. Grid (blocks per copy, systems), both strided.
+struct BatchCopy {
+    const double* src;
+    double* dst;
+    long n;
+};
+__global__ void __launch_bounds__(256) k_batch_copy(const BatchCopy* __restrict__ c, int n) {
+    for (int i = blockIdx.y; i < n; i += gridDim.y) {
+        const BatchCopy e = c[i];
+        for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < e.n; k += (long)gridDim.x * blockDim.x)
+            e.dst[k] = e.src[k];
+    }
+}
+
 __global__ void k_fill_int(int* p, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }

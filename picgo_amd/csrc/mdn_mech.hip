@@ -158,6 +158,7 @@ struct mdn_ctx {
     DBuf<CgState> batch_states;   // the batch's final states and reports, gathered for one read-back
     DBuf<double> batch_reports;
     DBuf<int> batch_order, batch_queue;
+    DBuf<BatchCopy> batch_copy;   // [start copies | start backups | restores], one per system each
     long batch_key = -1;          // (systems, precision, beads) of the tuned schedule
     int batch_nt = 256, batch_sorted = 1, batch_tuned = 0;
     double batch_tune_ms[MDN_BATCH_CANDIDATES] = {};
@@ -579,6 +580,7 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     ctx->batch_reports.release();
     ctx->batch_order.release();
     ctx->batch_queue.release();
+    ctx->batch_copy.release();
     ctx->cg.release();
     if (ctx->h_cg) (void)hipHostFree(ctx->h_cg);
     if (ctx->h_eout) (void)hipHostFree(ctx->h_eout);
@@ -1937,16 +1939,23 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     for (int i = 1; i < n; i++) HIPCHK_CTX(L, hipStreamSynchronize(ctxs[i]->stream));
     std::vector<BatchItem> items(n);
     std::vector<long> size(n);
+    // device-to-device copies in three sets of n (a zero count copies nothing), each set one launch:
+    // the systems that start from their device coordinates (x[cur] -> x[0]), the start's backup for
+    // the schedule tuning runs (x[0] -> ext) and its restore before each of them (ext -> x[0])
+    std::vector<BatchCopy> copies(3 * (size_t)n);
+    long max_nv = 1;
     for (int i = 0; i < n; i++) {
         mdn_ctx* c = ctxs[i];
-        const size_t bytes = sizeof(double) * 3 * c->n_beads;
+        const long nv = 3L * c->n_beads;
+        const size_t bytes = sizeof(double) * nv;
+        max_nv = std::max(max_nv, nv);
         if (coords[i])
             HIPCHK_CTX(L, hipMemcpyAsync(c->x[0].p, coords[i], bytes, hipMemcpyHostToDevice, L->stream));
         else if (c->cur != 0)   // the context's device coordinates
-            HIPCHK_CTX(L, hipMemcpyAsync(c->x[0].p, c->x[c->cur].p, bytes, hipMemcpyDeviceToDevice, L->stream));
+            copies[i] = BatchCopy{c->x[c->cur].p, c->x[0].p, nv};
         c->cur = 0;
-        // the start, kept on the device for the schedule tuning runs (each starts from it)
-        HIPCHK_CTX(L, hipMemcpyAsync(c->ext.p, c->x[0].p, bytes, hipMemcpyDeviceToDevice, L->stream));
+        copies[n + i] = BatchCopy{c->x[0].p, c->ext.p, nv};
+        copies[2L * n + i] = BatchCopy{c->ext.p, c->x[0].p, nv};
         items[i].s = c->sys;
         items[i].B = cg_bufs(c);
         items[i].st = c->cg.p;
@@ -1969,13 +1978,23 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
     HIPCHK_CTX(L, L->batch_items.ensure(n));
     HIPCHK_CTX(L, L->batch_order.ensure(n));
     HIPCHK_CTX(L, L->batch_queue.ensure(1));
+    HIPCHK_CTX(L, L->batch_copy.ensure(3 * (size_t)n));
     HIPCHK_CTX(L, hipMemcpyAsync(L->batch_items.p, items.data(), sizeof(BatchItem) * n, hipMemcpyHostToDevice,
                                  L->stream));
+    HIPCHK_CTX(L, hipMemcpyAsync(L->batch_copy.p, copies.data(), sizeof(BatchCopy) * 3 * n, hipMemcpyHostToDevice,
+                                 L->stream));
+    const dim3 cgrid((unsigned)std::min(64L, std::max(1L, (long)ceil_div(max_nv, 256L))), (unsigned)std::min(n, 4096));
+    auto copy_set = [&](int set) {   // set 0: start copies, 1: backups, 2: restores
+        hipLaunchKernelGGL(k_batch_copy, cgrid, dim3(256), 0, L->stream,
+                           (const BatchCopy*)(L->batch_copy.p + (size_t)set * n), n);
+        return hipGetLastError();
+    };
+    HIPCHK_CTX(L, copy_set(0));
     std::vector<int> order_given(n), order_sorted(n);
     for (int i = 0; i < n; i++) order_given[i] = order_sorted[i] = i;
     std::stable_sort(order_sorted.begin(), order_sorted.end(), [&](int a, int b) { return size[a] > size[b]; });
     int st;
-    // schedule: workgroup size (128, 256, 512) x system order (given, largest first), chosen once
+    // schedule: workgroup size (128, 256, 512, 1024) x system order (given, largest first), chosen once
     // per (systems, precision, beads) by timing each candidate on this batch, run to completion
     // from the caller's coordinates: the batch time is set by its slowest systems, whose iteration
     // counts only a complete run shows (a short run measures the start-up instead)
@@ -1987,11 +2006,10 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
         HIPCHK_CTX(L, hipEventCreate(&e1));
         float best = 1e30f;
         const int nts[4] = {128, 256, 512, 1024};
+        HIPCHK_CTX(L, copy_set(1));   // the start, kept for the tuning runs
         for (int c = 0; c < MDN_BATCH_CANDIDATES; c++) {
             const int nt = nts[c >> 1], sorted = c & 1;
-            for (int i = 0; i < n; i++)   // every candidate starts from the given coordinates
-                HIPCHK_CTX(L, hipMemcpyAsync(ctxs[i]->x[0].p, ctxs[i]->ext.p, sizeof(double) * 3 * ctxs[i]->n_beads,
-                                             hipMemcpyDeviceToDevice, L->stream));
+            HIPCHK_CTX(L, copy_set(2));   // every candidate starts from the given coordinates
             HIPCHK_CTX(L, hipMemcpyAsync(L->batch_order.p, sorted ? order_sorted.data() : order_given.data(),
                                          sizeof(int) * n, hipMemcpyHostToDevice, L->stream));
             HIPCHK_CTX(L, hipEventRecord(e0, L->stream));
@@ -2012,9 +2030,7 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
         L->batch_key = key;
         L->batch_tuned = 1;
         // the tuning runs moved the coordinates: start again from the given ones
-        for (int i = 0; i < n; i++)
-            HIPCHK_CTX(L, hipMemcpyAsync(ctxs[i]->x[0].p, ctxs[i]->ext.p, sizeof(double) * 3 * ctxs[i]->n_beads,
-                                         hipMemcpyDeviceToDevice, L->stream));
+        HIPCHK_CTX(L, copy_set(2));
     } else {
         L->batch_tuned = 0;
     }
@@ -2079,7 +2095,7 @@ int mdn_minimize_batched(mdn_ctx* const* ctxs, int32_t n, const mdn_cg_params* p
 
 // The schedule of the last batched minimization on a lead context: [workgroup size,
 // largest-first order (1) or given order (0), tuned in that call (1) or reused (0)], and the
-// tuning times (ms) of the candidates (128, 256, 512 threads, each in given then sorted order).
+// tuning times (ms) of the candidates (128, 256, 512, 1024 threads, each in given then sorted order).
 int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms) {
     if (!ctx || !out3) return MDN_ERR_ARG;
     out3[0] = ctx->batch_nt;
@@ -2090,8 +2106,8 @@ int mdn_last_batch_schedule(mdn_ctx* ctx, int32_t* out3, double* tune_ms) {
     return MDN_OK;
 }
 
-// Launch statistics of the last minimization: [line-search launches that did work, move
-// launches, gather launches, graph batches].
+// Launch statistics of the last minimization: [line-search launches that did work, steps resumed
+// after the host-run forceN fallback, force gathers, CG iterations].
 int mdn_last_min_stats(mdn_ctx* ctx, int64_t* out) {
     if (!ctx || !out) return MDN_ERR_ARG;
     out[0] = ctx->last_cg.n_ls_launch;
