@@ -167,6 +167,7 @@ struct mdn_ctx {
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // per precision (MDN_PREC_*)
     hipGraph_t graph[2] = {nullptr, nullptr};
     long sys_gen = 0, graph_gen[2] = {-1, -1};
+    long min_gen[2] = {-1, -1};   // the system generation of the last minimization, per precision
     CgState last_cg{};
 };
 
@@ -1747,7 +1748,12 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
     HIPCHK(hipMemsetAsync(ctx->d[0].p, 0, sizeof(double) * nv, ctx->stream));   // searchDir.assign(numDof, 0)
     HIPCHK(hipMemcpyAsync(ctx->cg.p, &S, sizeof(CgState), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    if (ctx->use_graph && (st = cg_graph(ctx, prec))) return st;
+    // The slots are captured into a graph once a layout is minimized a second time: a graph made
+    // for one minimization (a layout that the next neighbor-list rebuild replaces) costs more to
+    // instantiate than its replays save over launching the same kernels directly.
+    const bool graph = ctx->use_graph && (ctx->graph_gen[prec] == ctx->sys_gen || ctx->min_gen[prec] == ctx->sys_gen);
+    ctx->min_gen[prec] = ctx->sys_gen;
+    if (graph && (st = cg_graph(ctx, prec))) return st;
     const DevSys slot_sys = cg_sys(ctx, prec);
     const CgBufs slot_bufs = cg_bufs(ctx);
 
@@ -1766,7 +1772,7 @@ int mdn_minimize(mdn_ctx* ctx, const mdn_cg_params* p, double* coord, double* fo
         int k = 0;
         long here = 0;
         while (!done && launched < max_batches) {
-            if (ctx->use_graph) {
+            if (graph) {
                 HIPCHK(hipGraphLaunch(ctx->graph_exec[prec], ctx->stream));
             } else {
                 for (int i = 0; i < CG_BATCH; i++) {
