@@ -55,6 +55,37 @@ def test_mixed_force_error_unrelaxed(gpu, cfg):
     assert abs(ev32 - ev64) <= 1e-12 * abs(ev64)
 
 
+def test_mixed_force_error_batch_networks(gpu):
+    """Config 3's networks (20-400 filaments x 5-30 cylinders), every eighth of the 256: the mixed
+    path's force error against FP64. At the generated start states many networks hold almost no
+    force (filaments and crosslinks at their rest lengths, no excluded-volume pair or boundary
+    contact: max|F| can be 1e-10 pN); where max|F| < 1 pN the error is bounded in pN against the
+    1 pN force tolerance, elsewhere normwise. After seeded 5 nm bead noise every network holds
+    real forces. In every network the FP32 path really ran."""
+    g = gpu()
+    rng = np.random.default_rng(3)
+    worst_abs, worst_rel, n_rel = 0.0, 0.0, 0
+    for i in range(0, 256, 8):
+        net = synth.Network.config(3, index=i)
+        g.upload(net.desc, net.coord)
+        for x in (net.coord, net.coord + 5.0 * rng.standard_normal(net.coord.size)):
+            g.reset_neighbor_lists(x)
+            f64 = g.compute_forces(x)
+            f32 = g.compute_forces(x, precision=abi.MDN_PREC_MIXED)
+            d, fmax = float(np.abs(f32 - f64).max()), float(np.abs(f64).max())
+            if fmax >= 1.0:
+                worst_rel = max(worst_rel, d / fmax)
+                n_rel += 1
+                assert d <= 1e-6 * fmax, (i, d, fmax)
+            else:
+                worst_abs = max(worst_abs, d)
+                assert d <= 1e-6, (i, d)                    # pN
+        assert not np.array_equal(f32, f64), i
+    assert n_rel >= 32   # every perturbed state, at least
+    print(f"config 3 (32 networks, start and perturbed): worst normwise mixed force error {worst_rel:.3e} "
+          f"over {n_rel} states with max|F| >= 1 pN, worst {worst_abs:.3e} pN elsewhere", flush=True)
+
+
 def test_mixed_force_error_relaxed(gpu):
     net, g = _sys(gpu, 1)
     x, _, res = g.minimize(abi.default_cg_params(), net.coord)
