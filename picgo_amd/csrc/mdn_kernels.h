@@ -463,9 +463,12 @@ constexpr long GCHUNK_PAD = GCHUNK_BYTES / 4;   // values (>= 3 * gchunk<R>() fo
 // recomputed from the coordinates x and added to acc: the same term functions on the same inputs
 // as in term_ef, so the values the piece arrays would hold (rounded to R, accumulated in FP64).
 // Every load is issued up front and every term evaluated (absent pieces see in-range beads and
-// terms and are dropped by the select), so no load waits behind a branch.
-template <typename R>
-__device__ inline D3 bonded_at_bead(const DevSys& s, const double* __restrict__ x, int b, unsigned m, D3 acc) {
+// terms and are dropped by the select), so no load waits behind a branch. With E, *e also
+// receives the energies of the filament terms bead b starts, as bead_bonded_energy forms them
+// (the same term functions on the same coordinates and constants, already loaded here).
+template <typename R, bool E = false>
+__device__ inline D3 bonded_at_bead(const DevSys& s, const double* __restrict__ x, int b, unsigned m, D3 acc,
+                                    double* e = nullptr) {
     const long nb = s.n_beads;
     const int ns = s.n[FF_STRETCH], nd = s.n[FF_BEND];
     int id[5], t1, tc;   // t1: stretching term (bead, next); tc: bending term (previous, bead, next)
@@ -510,6 +513,14 @@ __device__ inline D3 bonded_at_bead(const DevSys& s, const double* __restrict__ 
         acc.x = on ? acc.x + v[k].x : acc.x;
         acc.y = on ? acc.y + v[k].y : acc.y;
         acc.z = on ? acc.z + v[k].z : acc.z;
+    }
+    if constexpr (E) {   // the stretching term (b, next) and the bending term centred on b
+        const double es = (double)stretch_energy<R>(c[2], c[3], sk[1], sl[1]);
+        const double eb = (double)bend_energy<R>(c[1], c[2], c[3], bk[1], bt[1]);
+        double en = 0.0;
+        en = (m & 0x02u) ? en + es : en;   // bead 1 of a stretching term
+        en = (m & 0x08u) ? en + eb : en;   // bead 2 of a bending term
+        *e = en;
     }
     return acc;
 }
@@ -1096,16 +1107,17 @@ __device__ inline void bead_ls(const DevSys& s, const CgBufs& B, int j, bool mov
     const int b = j * BLOCK + threadIdx.x;
     const bool on = b < s.n_beads;
     const double* __restrict__ xn = B.x[par ^ 1];
+    const bool with_e = !move && energies;   // block-uniform
+    double v[3] = {0.0, 0.0, 0.0};
     if (on) {
-        const D3 acc = bonded_at_bead<R>(s, xn, b, s.pmask[b], D3{0.0, 0.0, 0.0});
+        const D3 acc = with_e ? bonded_at_bead<R, true>(s, xn, b, s.pmask[b], D3{0.0, 0.0, 0.0}, &v[0])
+                              : bonded_at_bead<R>(s, xn, b, s.pmask[b], D3{0.0, 0.0, 0.0});
         s.bsum[3L * b] = acc.x;
         s.bsum[3L * b + 1] = acc.y;
         s.bsum[3L * b + 2] = acc.z;
     }
-    if (move || !energies) return;   // block-uniform
-    double v[3] = {0.0, 0.0, 0.0};
+    if (!with_e) return;
     if (on) {
-        v[0] = bead_bonded_energy<R>(s, xn, b);
         if (!first) v[1] = bead_bonded_energy<R>(s, B.xh, b);
         if (need_e0) v[2] = bead_bonded_energy<R>(s, B.x[par], b);
     }
