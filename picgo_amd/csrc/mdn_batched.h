@@ -124,14 +124,37 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     const double* __restrict__ dc = B.d[dpar];
     R* slots = static_cast<R*>(B.slots);
     const double* __restrict__ fcur = B.f[S.fpar];
-    const Pos p1{xc, dc, fcur, beta, lam0, pend ? 2 : 1};
-    const Pos p2{xc, dc, fcur, beta, lam0 * 0.5, pend ? 2 : 1};
-    const Pos pz{xc, nullptr, nullptr, 0.0, 0.0, 0};
+    const double lamh = lam0 * 0.5;
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, INFINITY};
+    // the bead pass first (k_cg_pos): the trial coordinates x + lambda0 (F + beta d) (the
+    // speculative x_next) and, after a rejection, x + (lambda0 / 2) d, written once per bead with
+    // the expression pos_of forms on the fly, so the interactions below read one coordinate
+    // vector per bead instead of three
+    const double maxd = S.max_dist;
+    double* __restrict__ xn = B.x[par ^ 1];
+    double* __restrict__ dn = B.d[dpar ^ 1];
+    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) {
+        const double fv = fcur[i];
+        const double de = pend ? fv + beta * dc[i] : dc[i];
+        xn[i] = xc[i] + lam0 * de;   // speculative x_next
+        if (!first) B.xh[i] = xc[i] + lamh * de;
+        if (first) {
+            acc[6] += de * fv;                            // d.F
+            acc[7] = rmin(acc[7], maxd / fabs(de));       // MController.h:293-305
+            if (pend) {
+                dn[i] = de;                               // shiftSearchDir materialized
+                B.fp[i] = fv;                             // forcePrev = force
+            }
+        }
+    }
+    __syncthreads();
+    const Pos p1{xn, nullptr, nullptr, 0.0, 0.0, 0};
+    const Pos p2{B.xh, nullptr, nullptr, 0.0, 0.0, 0};
+    const Pos pz{xc, nullptr, nullptr, 0.0, 0.0, 0};
     int fbad = 0;
     // one pass per evaluation keeps a single interaction's working set live at a time
     for (int ff = 0; ff < NFF; ff++)
-        for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R>(s, ff, t, p1, slots, 0, &fbad);
+        for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R, NT == 1024>(s, ff, t, p1, slots, 0, &fbad);
     if (fbad) atomicOr(&sh_bad, 1);
     if (!first)   // the first launch of a search evaluates lambda0 alone (see ls_control)
         for (int ff = 0; ff < NFF; ff++)
@@ -145,22 +168,6 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
                 int fb2 = 0;
                 acc[4] += term_ef<R>(s, ff, t, pz, (R*)nullptr, 2, &fb2);
             }
-    const double maxd = S.max_dist;
-    double* __restrict__ xn = B.x[par ^ 1];
-    double* __restrict__ dn = B.d[dpar ^ 1];
-    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) {
-        const double fv = fcur[i];
-        const double de = pend ? fv + beta * dc[i] : dc[i];
-        xn[i] = xc[i] + lam0 * de;   // speculative x_next
-        if (first) {
-            acc[6] += de * fv;                            // d.F
-            acc[7] = rmin(acc[7], maxd / fabs(de));       // MController.h:293-305
-            if (pend) {
-                dn[i] = de;                               // shiftSearchDir materialized
-                B.fp[i] = fv;                             // forcePrev = force
-            }
-        }
-    }
     const int op8[8] = {0, 0, 1, 1, 0, 1, 0, 2};
     block_reduce_n<8, NT>(acc, op8, red);
     double dsum[3];
@@ -189,15 +196,17 @@ __device__ inline void blk_move(const DevSys& s, const CgBufs& B, const CgState&
     const double lam = S.lambda;
     const double* __restrict__ x = B.x[S.parity];
     const double* __restrict__ d = B.d[S.dpar];
-    const Pos p{x, d, nullptr, 0.0, lam, 1};
+    double* __restrict__ xn = B.x[S.parity ^ 1];
+    // moveAlongSearchDir (CGMethod.cpp:350-354) / the recovery move (MController.h:318-321),
+    // first, so that the interactions read the new coordinates
+    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) xn[i] = x[i] + lam * d[i];
+    __syncthreads();
+    const Pos p{xn, nullptr, nullptr, 0.0, 0.0, 0};
     R* slots = static_cast<R*>(B.slots);
     int fbad = 0;
     for (int ff = 0; ff < NFF; ff++)
         for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p, slots, -1, &fbad);
     if (fbad) atomicOr(&sh_bad, 1);
-    double* __restrict__ xn = B.x[S.parity ^ 1];
-    // moveAlongSearchDir (CGMethod.cpp:350-354) / the recovery move (MController.h:318-321)
-    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) xn[i] = x[i] + lam * d[i];
     __syncthreads();
 }
 

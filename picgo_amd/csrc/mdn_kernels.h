@@ -356,7 +356,10 @@ __device__ inline void push_defer(const DevSys& s, int code, int t, D3 c1, D3 c2
 // form divides a bracket that vanishes like the squared triple product of the pair by that
 // square, so FP32 rounding is amplified like 1/tau^2 for near-coplanar pairs (tau the normalized
 // triple product), and FP32 gains nothing on gfx950 for this latency-bound term.
-template <typename R>
+// SPLIT: an excluded-volume pair's energy and forces by two calls, each forming the pair's
+// invariants (the same values bit for bit): more arithmetic, fewer registers live at once, for
+// kernels whose register budget the shared form overflows.
+template <typename R, bool SPLIT = false>
 __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R* __restrict__ slots,
                                  int code, int* fbad, int* bad_t = nullptr) {
     D3 out[4];
@@ -411,9 +414,18 @@ __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R
         if (slots) {
             bool ok = true;
             if (fast) {
-                if (code < 0) ok = exvol_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], out);
-                else U = exvol_energy_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
+                if (code < 0) {
+                    ok = exvol_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], out);
+                } else if (SPLIT) {
+                    U = exvol_energy_fast(c1, c2, c3, c4, s.ev_ks[tp], &deferred);
+                    ok = exvol_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], out);
+                } else {
+                    U = exvol_energy_forces_fast(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
+                }
             } else if (code < 0) {
+                ok = exvol_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], out);
+            } else if (SPLIT) {
+                U = exvol_energy<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred);
                 ok = exvol_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], out);
             } else {
                 U = exvol_energy_forces<double>(c1, c2, c3, c4, s.ev_ks[tp], &deferred, &ok, out);
