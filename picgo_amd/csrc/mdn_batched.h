@@ -152,9 +152,12 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     const Pos p2{B.xh, nullptr, nullptr, 0.0, 0.0, 0};
     const Pos pz{xc, nullptr, nullptr, 0.0, 0.0, 0};
     int fbad = 0;
-    // one pass per evaluation keeps a single interaction's working set live at a time
+    // one pass per evaluation keeps a single interaction's working set live at a time; an
+    // excluded-volume pair's energy and forces by two calls (the whole minimizer is one kernel,
+    // whose register budget the shared form overflows; measured faster in the mixed path,
+    // unchanged in FP64)
     for (int ff = 0; ff < NFF; ff++)
-        for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R, NT == 1024>(s, ff, t, p1, slots, 0, &fbad);
+        for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R, true>(s, ff, t, p1, slots, 0, &fbad);
     if (fbad) atomicOr(&sh_bad, 1);
     if (!first)   // the first launch of a search evaluates lambda0 alone (see ls_control)
         for (int ff = 0; ff < NFF; ff++)

@@ -357,8 +357,9 @@ __device__ inline void push_defer(const DevSys& s, int code, int t, D3 c1, D3 c2
 // square, so FP32 rounding is amplified like 1/tau^2 for near-coplanar pairs (tau the normalized
 // triple product), and FP32 gains nothing on gfx950 for this latency-bound term.
 // SPLIT: an excluded-volume pair's energy and forces by two calls, each forming the pair's
-// invariants (the same values bit for bit): more arithmetic, fewer registers live at once, for
-// kernels whose register budget the shared form overflows.
+// invariants: more arithmetic, fewer registers live at once, for kernels whose register budget
+// the shared form overflows. The FP64 path's values are the same bit for bit; the mixed path's
+// FMA-contracted forms may fuse differently in the other context (differences at rounding level).
 template <typename R, bool SPLIT = false>
 __device__ inline double term_ef(const DevSys& s, int ff, int t, const Pos& p, R* __restrict__ slots,
                                  int code, int* fbad, int* bad_t = nullptr) {
@@ -1185,7 +1186,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_cg_ls(DevSys s, const CgState* __r
     double v[3] = {0.0, 0.0, 0.0};
     if (t < s.n[ff]) {
         int fbad = 0;
-        v[0] = term_ef<R>(s, ff, t, p1, tslots, 0, &fbad);
+        v[0] = term_ef<R, true>(s, ff, t, p1, tslots, 0, &fbad);   // split: no register spills at 3 waves
         if (fbad) atomicAdd(B.bad_count, 1);
         if (!first) {   // the first launch of a search evaluates lambda0 alone (see ls_control)
             const Pos p2{B.xh, nullptr, nullptr, 0.0, 0.0, 0};
