@@ -24,6 +24,9 @@ struct BatchItem {
     double* report;   // out: [0..8] FP64 energy report at the start, [9..17] at the result
 };
 
+// values per thread per step of the workgroup's streaming passes over a system's vectors
+constexpr int BU = 4;
+
 struct BatchParams {
     int method, try_recover, force_call_limit, energy_call_limit, report_on_call_limit;
     int limit_override;   // > 0: force-call limit of every system (schedule tuning runs)
@@ -77,25 +80,44 @@ __device__ inline void blk_report(const DevSys& s, const double* x, double* rep,
     __syncthreads();
 }
 
-// force[b] = the bead's bonded pieces + its slots (the order of gather_block), with the partial
+// The filament stretching and bending forces as per-bead bonded sums at x (bonded_at_bead into
+// s.bsum: each bead's pieces recomputed from the coordinates, FP64 sums in piece order, the values
+// the piece arrays would hold) and, with E, the energies of the terms each bead starts added to
+// *e. Beads are adjacent or read from the neighbour tables: one round of loads per bead instead of
+// an index load, then coordinate loads, per term.
+template <typename R, int NT, bool E>
+__device__ inline void blk_bonded(const DevSys& s, const double* __restrict__ x, double* e) {
+    for (int b = threadIdx.x; b < s.n_beads; b += NT) {
+        double eb = 0.0;
+        const D3 acc = bonded_at_bead<R, E>(s, x, b, s.pmask[b], D3{0.0, 0.0, 0.0}, &eb);
+        s.bsum[3L * b] = acc.x;
+        s.bsum[3L * b + 1] = acc.y;
+        s.bsum[3L * b + 2] = acc.z;
+        if constexpr (E) *e += eb;
+    }
+}
+// the energies of the filament terms each bead starts at x, added to *e
+template <typename R, int NT>
+__device__ inline void blk_bonded_energy(const DevSys& s, const double* __restrict__ x, double* e) {
+    for (int b = threadIdx.x; b < s.n_beads; b += NT) *e += bead_bonded_energy<R>(s, x, b);
+}
+__device__ inline bool is_bonded(int ff) { return ff == FF_STRETCH || ff == FF_BEND; }
+
+// force[b] = the bead's bonded sum + its slots (the order of gather_block), with the partial
 // dot products F.F, F.F_prev (fp may be null) and max|F| of this thread's beads, and the
 // minimum-energy backup candidate xmin_dst = xn when cand.
 template <typename R, int NT>
 __device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, double* __restrict__ f,
                                   const double* __restrict__ fp, bool cand, const double* __restrict__ xn,
                                   double* __restrict__ xmin_dst, double* v) {
-    const R* pc = static_cast<const R*>(s.piece);
     for (int b = threadIdx.x; b < s.n_beads; b += NT) {
-        D3 acc{0.0, 0.0, 0.0};
-        const unsigned m = s.pmask[b];
-#pragma unroll
-        for (int k = 0; k < NBOND; k++)
-            if ((m >> k) & 1) {
-                const R* q = pc + 3L * ((long)k * s.n_beads + b);
-                acc.x += (double)q[0]; acc.y += (double)q[1]; acc.z += (double)q[2];
-            }
-        const int q1 = s.bstart[b + 1];
-        for (int q = s.bstart[b]; q < q1; q++) {
+        // every load of the bead before the first use
+        const int q0 = s.bstart[b], q1 = s.bstart[b + 1];
+        D3 acc{s.bsum[3L * b], s.bsum[3L * b + 1], s.bsum[3L * b + 2]};
+        double fpv[3] = {0.0, 0.0, 0.0};
+        if (fp)
+            for (int c = 0; c < 3; c++) fpv[c] = fp[3 * b + c];
+        for (int q = q0; q < q1; q++) {
             const R* w = slots + 3L * q;
             acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
         }
@@ -103,7 +125,7 @@ __device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, 
         for (int c = 0; c < 3; c++) {
             f[3 * b + c] = fv[c];
             v[0] += fv[c] * fv[c];
-            if (fp) v[1] += fv[c] * fp[3 * b + c];
+            if (fp) v[1] += fv[c] * fpv[c];
             v[2] = rmax(v[2], fabs(fv[c]));
             if (cand) xmin_dst[3 * b + c] = xn[3 * b + c];
         }
@@ -133,17 +155,33 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     const double maxd = S.max_dist;
     double* __restrict__ xn = B.x[par ^ 1];
     double* __restrict__ dn = B.d[dpar ^ 1];
-    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) {
-        const double fv = fcur[i];
-        const double de = pend ? fv + beta * dc[i] : dc[i];
-        xn[i] = xc[i] + lam0 * de;   // speculative x_next
-        if (!first) B.xh[i] = xc[i] + lamh * de;
-        if (first) {
-            acc[6] += de * fv;                            // d.F
-            acc[7] = rmin(acc[7], maxd / fabs(de));       // MController.h:293-305
-            if (pend) {
-                dn[i] = de;                               // shiftSearchDir materialized
-                B.fp[i] = fv;                             // forcePrev = force
+    // BU values per thread per step, every load issued before the first use (one workgroup
+    // streams the whole system: latency-bound with one load in flight per thread); each thread
+    // still visits i = tid, tid + NT, ... in order, so the d.F partial sums are unchanged
+    const int n3 = 3 * s.n_beads;
+    for (int i0 = threadIdx.x; i0 < n3; i0 += BU * NT) {
+        double fv[BU], dv[BU], xv[BU];
+#pragma unroll
+        for (int u = 0; u < BU; u++) {
+            const int i = i0 + u * NT;
+            fv[u] = i < n3 ? fcur[i] : 0.0;
+            dv[u] = i < n3 ? dc[i] : 0.0;
+            xv[u] = i < n3 ? xc[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < BU; u++) {
+            const int i = i0 + u * NT;
+            if (i >= n3) break;
+            const double de = pend ? fv[u] + beta * dv[u] : dv[u];
+            xn[i] = xv[u] + lam0 * de;   // speculative x_next
+            if (!first) B.xh[i] = xv[u] + lamh * de;
+            if (first) {
+                acc[6] += de * fv[u];                         // d.F
+                acc[7] = rmin(acc[7], maxd / fabs(de));       // MController.h:293-305
+                if (pend) {
+                    dn[i] = de;                               // shiftSearchDir materialized
+                    B.fp[i] = fv[u];                          // forcePrev = force
+                }
             }
         }
     }
@@ -157,20 +195,28 @@ __device__ inline void blk_ls(const DevSys& s, const CgBufs& B, CgState& S, doub
     // whose register budget the shared form overflows; measured faster in the mixed path,
     // unchanged in FP64)
     for (int ff = 0; ff < NFF; ff++)
-        for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R, true>(s, ff, t, p1, slots, 0, &fbad);
+        if (!is_bonded(ff))
+            for (int t = threadIdx.x; t < s.n[ff]; t += NT) acc[0] += term_ef<R, true>(s, ff, t, p1, slots, 0, &fbad);
+    blk_bonded<R, NT, true>(s, xn, &acc[0]);
     if (fbad) atomicOr(&sh_bad, 1);
-    if (!first)   // the first launch of a search evaluates lambda0 alone (see ls_control)
+    if (!first) {   // the first launch of a search evaluates lambda0 alone (see ls_control)
         for (int ff = 0; ff < NFF; ff++)
-            for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
-                int fb2 = 0;
-                acc[1] += term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fb2);
-            }
-    if (need_e0)
+            if (!is_bonded(ff))
+                for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
+                    int fb2 = 0;
+                    acc[1] += term_ef<R>(s, ff, t, p2, (R*)nullptr, 1, &fb2);
+                }
+        blk_bonded_energy<R, NT>(s, B.xh, &acc[1]);
+    }
+    if (need_e0) {
         for (int ff = 0; ff < NFF; ff++)
-            for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
-                int fb2 = 0;
-                acc[4] += term_ef<R>(s, ff, t, pz, (R*)nullptr, 2, &fb2);
-            }
+            if (!is_bonded(ff))
+                for (int t = threadIdx.x; t < s.n[ff]; t += NT) {
+                    int fb2 = 0;
+                    acc[4] += term_ef<R>(s, ff, t, pz, (R*)nullptr, 2, &fb2);
+                }
+        blk_bonded_energy<R, NT>(s, xc, &acc[4]);
+    }
     const int op8[8] = {0, 0, 1, 1, 0, 1, 0, 2};
     block_reduce_n<8, NT>(acc, op8, red);
     double dsum[3];
@@ -202,13 +248,27 @@ __device__ inline void blk_move(const DevSys& s, const CgBufs& B, const CgState&
     double* __restrict__ xn = B.x[S.parity ^ 1];
     // moveAlongSearchDir (CGMethod.cpp:350-354) / the recovery move (MController.h:318-321),
     // first, so that the interactions read the new coordinates
-    for (int i = threadIdx.x; i < 3 * s.n_beads; i += NT) xn[i] = x[i] + lam * d[i];
+    const int n3 = 3 * s.n_beads;
+    for (int i0 = threadIdx.x; i0 < n3; i0 += BU * NT) {
+        double xv[BU], dv[BU];
+#pragma unroll
+        for (int u = 0; u < BU; u++) {
+            const int i = i0 + u * NT;
+            xv[u] = i < n3 ? x[i] : 0.0;
+            dv[u] = i < n3 ? d[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < BU; u++)
+            if (i0 + u * NT < n3) xn[i0 + u * NT] = xv[u] + lam * dv[u];
+    }
     __syncthreads();
     const Pos p{xn, nullptr, nullptr, 0.0, 0.0, 0};
     R* slots = static_cast<R*>(B.slots);
     int fbad = 0;
     for (int ff = 0; ff < NFF; ff++)
-        for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p, slots, -1, &fbad);
+        if (!is_bonded(ff))
+            for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p, slots, -1, &fbad);
+    blk_bonded<R, NT, false>(s, xn, nullptr);
     if (fbad) atomicOr(&sh_bad, 1);
     __syncthreads();
 }
@@ -258,7 +318,9 @@ __device__ inline void blk_minimize(const BatchItem& it, const BatchParams& P, C
         R* slots = static_cast<R*>(B.slots);
         int fbad = 0;
         for (int ff = 0; ff < NFF; ff++)
-            for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p0, slots, -1, &fbad);
+            if (!is_bonded(ff))
+                for (int t = threadIdx.x; t < s.n[ff]; t += NT) (void)term_ef<R>(s, ff, t, p0, slots, -1, &fbad);
+        blk_bonded<R, NT, false>(s, B.x[0], nullptr);
         if (fbad) atomicOr(&sh_bad, 1);
         for (int i = threadIdx.x; i < 3 * nb; i += NT) B.d[0][i] = 0.0;
     }
