@@ -25,7 +25,7 @@ struct BatchItem {
 };
 
 // values per thread per step of the workgroup's streaming passes over a system's vectors
-constexpr int BU = 4;
+constexpr int BU = 8;
 
 struct BatchParams {
     int method, try_recover, force_call_limit, energy_call_limit, report_on_call_limit;
@@ -85,21 +85,39 @@ __device__ inline void blk_report(const DevSys& s, const double* x, double* rep,
 // the piece arrays would hold) and, with E, the energies of the terms each bead starts added to
 // *e. Beads are adjacent or read from the neighbour tables: one round of loads per bead instead of
 // an index load, then coordinate loads, per term.
+// Two beads per step (b and b + NT, both beads' loads before either's stores, so the second
+// bead's loads are in flight with the first's); each thread still visits its beads in order.
 template <typename R, int NT, bool E>
 __device__ inline void blk_bonded(const DevSys& s, const double* __restrict__ x, double* e) {
-    for (int b = threadIdx.x; b < s.n_beads; b += NT) {
-        double eb = 0.0;
-        const D3 acc = bonded_at_bead<R, E>(s, x, b, s.pmask[b], D3{0.0, 0.0, 0.0}, &eb);
-        s.bsum[3L * b] = acc.x;
-        s.bsum[3L * b + 1] = acc.y;
-        s.bsum[3L * b + 2] = acc.z;
-        if constexpr (E) *e += eb;
+    const int nb = s.n_beads;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += 2 * NT) {
+        const int b1 = b0 + NT, b1c = min(b1, nb - 1);   // an absent second bead: any bead, dropped
+        double e0 = 0.0, e1 = 0.0;
+        const D3 a0 = bonded_at_bead<R, E>(s, x, b0, s.pmask[b0], D3{0.0, 0.0, 0.0}, &e0);
+        const D3 a1 = bonded_at_bead<R, E>(s, x, b1c, s.pmask[b1c], D3{0.0, 0.0, 0.0}, &e1);
+        s.bsum[3L * b0] = a0.x;
+        s.bsum[3L * b0 + 1] = a0.y;
+        s.bsum[3L * b0 + 2] = a0.z;
+        if constexpr (E) *e += e0;
+        if (b1 < nb) {
+            s.bsum[3L * b1] = a1.x;
+            s.bsum[3L * b1 + 1] = a1.y;
+            s.bsum[3L * b1 + 2] = a1.z;
+            if constexpr (E) *e += e1;
+        }
     }
 }
-// the energies of the filament terms each bead starts at x, added to *e
+// the energies of the filament terms each bead starts at x, added to *e (two beads per step)
 template <typename R, int NT>
 __device__ inline void blk_bonded_energy(const DevSys& s, const double* __restrict__ x, double* e) {
-    for (int b = threadIdx.x; b < s.n_beads; b += NT) *e += bead_bonded_energy<R>(s, x, b);
+    const int nb = s.n_beads;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += 2 * NT) {
+        const int b1 = b0 + NT;
+        const double e0 = bead_bonded_energy<R>(s, x, b0);
+        const double e1 = bead_bonded_energy<R>(s, x, min(b1, nb - 1));
+        *e += e0;
+        if (b1 < nb) *e += e1;
+    }
 }
 __device__ inline bool is_bonded(int ff) { return ff == FF_STRETCH || ff == FF_BEND; }
 
@@ -110,24 +128,40 @@ template <typename R, int NT>
 __device__ inline void blk_gather(const DevSys& s, const R* __restrict__ slots, double* __restrict__ f,
                                   const double* __restrict__ fp, bool cand, const double* __restrict__ xn,
                                   double* __restrict__ xmin_dst, double* v) {
-    for (int b = threadIdx.x; b < s.n_beads; b += NT) {
-        // every load of the bead before the first use
-        const int q0 = s.bstart[b], q1 = s.bstart[b + 1];
-        D3 acc{s.bsum[3L * b], s.bsum[3L * b + 1], s.bsum[3L * b + 2]};
-        double fpv[3] = {0.0, 0.0, 0.0};
-        if (fp)
-            for (int c = 0; c < 3; c++) fpv[c] = fp[3 * b + c];
-        for (int q = q0; q < q1; q++) {
-            const R* w = slots + 3L * q;
-            acc.x += (double)w[0]; acc.y += (double)w[1]; acc.z += (double)w[2];
+    const int nb = s.n_beads;
+    for (int b0 = threadIdx.x; b0 < nb; b0 += 2 * NT) {
+        // two beads per step (b0, b0 + NT), every load of both before the first use; each
+        // thread still adds its beads in order
+        int bb[2] = {b0, min(b0 + NT, nb - 1)};
+        int q0[2], q1[2];
+        D3 acc[2];
+        double fpv[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int b = bb[u];
+            q0[u] = s.bstart[b];
+            q1[u] = s.bstart[b + 1];
+            acc[u] = D3{s.bsum[3L * b], s.bsum[3L * b + 1], s.bsum[3L * b + 2]};
+            if (fp)
+                for (int c = 0; c < 3; c++) fpv[u][c] = fp[3 * b + c];
         }
-        const double fv[3] = {acc.x, acc.y, acc.z};
-        for (int c = 0; c < 3; c++) {
-            f[3 * b + c] = fv[c];
-            v[0] += fv[c] * fv[c];
-            if (fp) v[1] += fv[c] * fpv[c];
-            v[2] = rmax(v[2], fabs(fv[c]));
-            if (cand) xmin_dst[3 * b + c] = xn[3 * b + c];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (u == 1 && b0 + NT >= nb) break;
+            const int b = bb[u];
+            D3 a = acc[u];
+            for (int q = q0[u]; q < q1[u]; q++) {
+                const R* w = slots + 3L * q;
+                a.x += (double)w[0]; a.y += (double)w[1]; a.z += (double)w[2];
+            }
+            const double fv[3] = {a.x, a.y, a.z};
+            for (int c = 0; c < 3; c++) {
+                f[3 * b + c] = fv[c];
+                v[0] += fv[c] * fv[c];
+                if (fp) v[1] += fv[c] * fpv[u][c];
+                v[2] = rmax(v[2], fabs(fv[c]));
+                if (cand) xmin_dst[3 * b + c] = xn[3 * b + c];
+            }
         }
     }
 }
