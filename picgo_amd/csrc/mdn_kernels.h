@@ -250,10 +250,13 @@ __device__ inline void block_reduce_n(double* v, const int* op, double* lds /* >
 
 // The same for a large block whose waves would otherwise each run NV full wave reductions on
 // shared SIMDs: every wave folds groups of 8 lanes (three DPP row shifts), the NTH / 8 group
-// results go through LDS, and wave 0 alone combines them and runs the full wave reductions.
-// Fixed order, so deterministic. Results valid in every lane of wave 0.
+// results go through LDS, and wave i combines value i (NV <= NTH / 64) from its NTH / 8 group
+// results with one full wave reduction, leaving it in res[i]. Fixed order, so deterministic;
+// every thread returns with all NV results.
 template <int NV, int NTH>
-__device__ inline void block_reduce_fold(double* v, const int* op, double* lds /* >= NV * NTH / 8 */) {
+__device__ inline void block_reduce_fold(double* v, const int* op, double* lds /* >= NV * NTH / 8 */,
+                                             double* res /* NV */) {
+    static_assert(NV <= NTH / 64, "one wave per value");
     constexpr int NP = NTH / 8;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -263,17 +266,20 @@ __device__ inline void block_reduce_fold(double* v, const int* op, double* lds /
 #pragma unroll
         for (int i = 0; i < NV; i++) lds[i * NP + w * 8 + (lane >> 3)] = v[i];
     __syncthreads();
-    if (w != 0) return;
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-        const double id = op[i] == 0 ? 0.0 : op[i] == 1 ? -INFINITY : INFINITY;
-        double r = lane < NP ? lds[i * NP + lane] : id;
+    if (w < NV) {   // wave-uniform
+        const int o = op[w];
+        const double id = o == 0 ? 0.0 : o == 1 ? -INFINITY : INFINITY;
+        double r = lane < NP ? lds[w * NP + lane] : id;
         for (int q = lane + 64; q < NP; q += 64) {
-            const double u = lds[i * NP + q];
-            r = op[i] == 0 ? r + u : op[i] == 1 ? rmax(r, u) : rmin(r, u);
+            const double u = lds[w * NP + q];
+            r = o == 0 ? r + u : o == 1 ? rmax(r, u) : rmin(r, u);
         }
-        v[i] = op[i] == 0 ? wave_sum(r) : op[i] == 1 ? wave_max(r) : wave_min(r);
+        r = o == 0 ? wave_sum(r) : o == 1 ? wave_max(r) : wave_min(r);
+        if (lane == 0) res[w] = r;
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = res[i];
 }
 
 // Pins values loaded at a kernel's entry before its first branch: without a use there, the
@@ -1547,6 +1553,7 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
     static_assert(NW <= FIN, "the state is staged one word per thread");
     __shared__ CgState Sl;
     __shared__ double lds[8 * (FIN / 8)];
+    __shared__ double res8[8];
     __shared__ int sh_ndef, sh_bad;
     if (threadIdx.x < NW) reinterpret_cast<int*>(&Sl)[threadIdx.x] = reinterpret_cast<const int*>(st)[threadIdx.x];
     if (threadIdx.x == 0) {
@@ -1561,25 +1568,25 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
     const int ne = ne0 + neb, p0c = T + s.nbb;
     // E(lambda), E(lambda/2), -, -, E(x), -, d.F, max|d| (then the recovery step) | F.F, F.F_prev, max|F|
     double acc[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // both column ranges' loads in flight before any sum (one round trip when they fit)
+    // E(lambda / 2) is read only after a launch that evaluated it (a search's later launches) and
+    // E(x) only when the first launch of a search evaluated it: the common launch reads one energy
+    // column; the flags are loaded up front and only those rarer loads wait for them
+    const int ls_active = st->ls_active, ecur_valid = st->ecur_valid;
+    // the other columns' loads in flight before any sum (one round trip when they fit)
     const int nc = ne > s.nbb ? ne : s.nbb;
     for (int base = 0; base < nc; base += 8 * FIN) {
-        double p0[8], p1[8], p2[8], q0[8], q1[8];
+        double p0[8], q0[8], q1[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int c = base + threadIdx.x + j * FIN;
             const long col = c < ne0 ? c : T + (c - ne0);
             p0[j] = c < ne ? B.partial[col] : 0.0;
-            p1[j] = c < ne ? B.partial[np + col] : 0.0;
-            p2[j] = c < ne ? B.partial[2L * np + col] : 0.0;
             q0[j] = c < s.nbb ? B.partial[p0c + c] : 0.0;
             q1[j] = c < s.nbb ? B.partial[np + p0c + c] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             acc[0] += p0[j];
-            acc[1] += p1[j];
-            acc[4] += p2[j];
             acc[6] += q0[j];
             acc[7] = rmax(acc[7], q1[j]);
         }
@@ -1598,13 +1605,30 @@ __global__ void __launch_bounds__(FIN) k_cg_fin(DevSys s, CgState* __restrict__ 
             acc[10] = rmax(acc[10], p[j][2]);
         }
     }
+    const bool want_half = ls_active != 0, want_x = ls_active == 0 && ecur_valid == 0;
+    if (want_half || want_x)   // uniform
+        for (int base = 0; base < ne; base += 8 * FIN) {
+            double p1[8], p2[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int c = base + threadIdx.x + j * FIN;
+                const long col = c < ne0 ? c : T + (c - ne0);
+                p1[j] = want_half && c < ne ? B.partial[np + col] : 0.0;
+                p2[j] = want_x && c < ne ? B.partial[2L * np + col] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                acc[1] += p1[j];
+                acc[4] += p2[j];
+            }
+        }
     __syncthreads();
     const int phase0 = Sl.phase;
     if (Sl.done || (phase0 != PH_LS && phase0 != PH_MOVE)) return;
     // the eight reduced values (the three flags of acc are set from the deferred pass below)
     double r8[8] = {acc[0], acc[1], acc[4], acc[6], acc[7], acc[8], acc[9], acc[10]};
     const int op[8] = {0, 0, 0, 0, 1, 0, 0, 1};
-    block_reduce_fold<8, FIN>(r8, op, lds);
+    block_reduce_fold<8, FIN>(r8, op, lds, res8);
     acc[0] = r8[0]; acc[1] = r8[1]; acc[4] = r8[2]; acc[6] = r8[3];
     // the recovery step, min over components of maxDist / |d| (MController.h:293-305), equals
     // maxDist / max|d| under correctly rounded division: one division instead of one per component
