@@ -53,6 +53,33 @@ def test_batched_trajectory_prefix_vs_oracle(gpu, ncalls):
         np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
 
 
+def test_batched_relabelled_beads_prefix_vs_oracle(gpu):
+    """Dense networks with their beads renumbered by a random permutation, so a filament's beads
+    are not consecutive: the batched workgroups then take each bead's filament neighbours from the
+    neighbour tables. The trajectory prefix equals the oracle's on the renumbered systems."""
+    from tests.test_gpu_bonded_gather import _relabelled
+    nets = [synth.Network.config(3, index=i, dense=True) for i in range(4)]
+    systems, ffms = [], []
+    for k, net in enumerate(nets):
+        d2, x2, _ = _relabelled(net.desc, net.coord, 11 + k)
+        g = gpu()
+        g.upload(d2, x2)
+        g.reset_neighbor_lists(x2)
+        systems.append((d2, x2))
+        ffms.append(g)
+    p = abi.default_cg_params(energy_call_limit=22)
+    xs, _, res = mech.minimize_batched(ffms, p, [x for _, x in systems])
+    for i, (d2, x2) in enumerate(systems):
+        o = of.OracleSystem(d2)
+        o.build_nl(x2)
+        st, xo, fo, ro = o.minimize(p, x2)
+        rb = res[i]
+        print(f"relabelled system {i}: batched {rb.num_iter}/{rb.num_energy_call} oracle {ro.num_iter}/{ro.num_energy_call} "
+              f"max|dx| {float(np.abs(xs[i] - xo).max()):.3e}", flush=True)
+        assert (rb.num_iter, rb.num_energy_call, rb.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+        np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
+
+
 def test_batched_quadratic_line_search_prefix_vs_oracle(gpu):
     """The quadratic line search in the persistent kernel (the same ls_control as the
     single-system minimizer): networks 0-3 of the batch (on three of them the oracle's parabola
