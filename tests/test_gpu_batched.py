@@ -80,6 +80,35 @@ def test_batched_relabelled_beads_prefix_vs_oracle(gpu):
         np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=0)
 
 
+def test_batched_short_filaments_vs_oracle(gpu):
+    """Filaments of 2 to 5 beads (every absent-neighbour case of the bead-by-bead bonded sums), as
+    given and with the beads renumbered, minimized in one batch beside a network: each follows the
+    oracle's whole minimization (counts identical, coordinates 1e-8)."""
+    from tests.test_gpu_bonded_gather import _relabelled, _short_filaments
+    hs = _short_filaments()
+    d2, x2, _ = _relabelled(hs.desc, hs.coord, 7)
+    net = synth.Network.config(3, index=5)
+    systems = [(hs.desc, hs.coord), (d2, x2), (net.desc, net.coord)]
+    ffms = []
+    for d, x in systems:
+        g = gpu()
+        g.upload(d, x)
+        g.reset_neighbor_lists(x)
+        ffms.append(g)
+    p = abi.default_cg_params()
+    xs, _, res = mech.minimize_batched(ffms, p, [x for _, x in systems])
+    for i, (d, x) in enumerate(systems[:2]):
+        o = of.OracleSystem(d)
+        o.build_nl(x)
+        st, xo, fo, ro = o.minimize(p, x)
+        rb = res[i]
+        print(f"short filaments {i}: batched {rb.num_iter}/{rb.num_energy_call} oracle {ro.num_iter}/{ro.num_energy_call} "
+              f"max|dx| {float(np.abs(xs[i] - xo).max()):.3e}", flush=True)
+        assert (rb.num_iter, rb.num_energy_call, rb.num_force_call) == (ro.num_iter, ro.num_energy_call, ro.num_force_call)
+        np.testing.assert_allclose(xs[i], xo, rtol=1e-8, atol=1e-9)
+    assert res[2].converged_grad or res[2].converged_energy
+
+
 def test_batched_quadratic_line_search_prefix_vs_oracle(gpu):
     """The quadratic line search in the persistent kernel (the same ls_control as the
     single-system minimizer): networks 0-3 of the batch (on three of them the oracle's parabola
