@@ -403,7 +403,7 @@ int mdn_set_params(mdn_ctx* ctx, const double* kstr, const double* eql, const do
  * (coords[i] NULL: the start is the context's device coordinates, which receive the result, with
  * no coordinate transfer over the host link);
  * forces_out may be null, or forces_out[i] may be null; results[i] is filled for every system.
- * The schedule (workgroup size 128, 256 or 512 threads, given or largest-first order) is chosen on
+ * The schedule (workgroup size 128, 256, 512 or 1024 threads, given or largest-first order) is chosen on
  * the first call for a given (n, precision, total beads) by timing a complete run of each candidate
  * from the given coordinates.
  * Returns MDN_OK when every system succeeded, else the status of the first failing system (its
