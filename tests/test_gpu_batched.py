@@ -109,6 +109,38 @@ def test_batched_short_filaments_vs_oracle(gpu):
     assert res[2].converged_grad or res[2].converged_energy
 
 
+def test_batched_mixed_reaches_the_force_tolerance(gpu):
+    """The mixed batched minimizer with the energy criterion off: every network of a batch (32 of
+    the 256-network set, after seeded 5 nm bead noise so that each holds real forces) reaches the
+    reference's force tolerance max|F| < 1 pN, and FP64 re-evaluation at each result confirms it:
+    the mixed forces there are within 1e-5 relative (normwise) of FP64."""
+    rng = np.random.default_rng(9)
+    nets = [synth.Network.config(3, index=8 * i) for i in range(32)]
+    ffms, starts = [], []
+    for net in nets:
+        g = gpu()
+        x = net.coord + 5.0 * rng.standard_normal(net.coord.size)
+        g.upload(net.desc, x)
+        g.reset_neighbor_lists(x)
+        ffms.append(g)
+        starts.append(x)
+    p = abi.default_cg_params(precision=abi.MDN_PREC_MIXED, energy_rel_tol=0.0)
+    xs, fs, res = mech.minimize_batched(ffms, p, starts)
+    worst_rel, worst_f64 = 0.0, 0.0
+    for i, (g, r) in enumerate(zip(ffms, res)):
+        assert r.converged_grad and not r.call_limit_exceeded and r.error_flags == 0, i
+        assert r.grad_inf_norm < 1.0, i
+        f64 = g.compute_forces(xs[i])
+        fm = g.compute_forces(xs[i], precision=abi.MDN_PREC_MIXED)
+        np.testing.assert_array_equal(fm, fs[i])   # the returned forces are the mixed path's at the result
+        rel = float(np.abs(fm - f64).max() / np.abs(f64).max())
+        worst_rel, worst_f64 = max(worst_rel, rel), max(worst_f64, float(np.abs(f64).max()))
+        assert rel <= 1e-5, (i, rel)
+        assert float(np.abs(f64).max()) < 1.0 + 1e-5, i
+    print(f"32 networks, force-only mixed batch: worst FP64 max|F| {worst_f64:.6f} pN, worst relative "
+          f"force error {worst_rel:.3e}", flush=True)
+
+
 def test_batched_quadratic_line_search_prefix_vs_oracle(gpu):
     """The quadratic line search in the persistent kernel (the same ls_control as the
     single-system minimizer): networks 0-3 of the batch (on three of them the oracle's parabola
