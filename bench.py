@@ -24,10 +24,13 @@ and FP64 reductions); the FP64 path runs the same sequence beside it. The line a
                    bounded sample of the first minimization, scaled to minimizations/s;
   network_42k      the 42,000-bead network (5 um cube): force evaluations/s and whole
                    minimizations in both precisions;
+  network_42k_adjoint  the same network with lognormal per-cylinder parameters near its relaxed
+                   state: Hessian-vector products and dE/dtheta, device time per call (HIP events)
+                   and 64 HVPs through the host-buffer entry points, mixed error against FP64;
   batch_256        the 256-network batch in the persistent batched kernel.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
-       python bench.py --workload single|batched|minimize ...   (one part alone, for profiling)
+       python bench.py --workload single|adjoint|batched|minimize ...   (one part alone, for profiling)
 """
 import argparse
 import json
@@ -243,6 +246,54 @@ def single_network(args):
     return out
 
 
+def adjoint_42k(args):
+    """dE/dtheta and Hessian-vector products on the 42,000-bead network with lognormal
+    per-cylinder parameters, near its relaxed state (an FP64 minimization, then 0.01 nm of seeded
+    noise, which moves every pair off the coplanar test's kink): 64 seeded unit vectors."""
+    net = synth.Network.config(2)
+    ffm = mech.ForceFieldManager(0)
+    ffm.upload(net.desc, net.coord)
+    ffm.reset_neighbor_lists(net.coord)
+    x, _, res = ffm.minimize(abi.default_cg_params(), net.coord, want_forces=False)
+    x = x + np.random.default_rng(22).normal(0.0, 0.01, x.size)
+    ffm.set_coords(x)
+    rng = np.random.default_rng(64)
+    vs = rng.standard_normal((64, x.size))
+    vs /= np.linalg.norm(vs, axis=1, keepdims=True)
+    counts = ffm.counts()
+    B_hvp = force_eval_bytes(counts, net.n_beads) + 24 * net.n_beads   # + the tangent vector in
+    out = {"workload": "adjoint: " + SINGLE + "; lognormal per-cylinder parameters; 64 seeded unit vectors",
+           "n_beads": net.n_beads, "relax_cg_iterations": res.num_iter}
+    hv = {}
+    for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
+        ffm.bench_adjoint(8, vs[0], prec)
+        ms_hvp, ms_grad = ffm.bench_adjoint(64, vs[0], prec)
+        ffm.hvp(vs[0], precision=prec)
+        t0 = time.perf_counter()
+        hv[name] = np.stack([ffm.hvp(v, precision=prec) for v in vs])
+        t_host = time.perf_counter() - t0
+        ffm.param_grad(precision=prec)
+        t0 = time.perf_counter()
+        for _ in range(16):
+            ffm.param_grad(precision=prec)
+        t_grad_host = (time.perf_counter() - t0) / 16
+        out[name] = {"us_per_hvp_device": 1e3 * ms_hvp, "us_per_param_grad_device": 1e3 * ms_grad,
+                     "ms_64_hvps_device": 64 * ms_hvp,
+                     "ms_64_hvps_host_buffers": 1e3 * t_host,
+                     "us_per_param_grad_host_buffers": 1e6 * t_grad_host,
+                     "hvp_roofline_frac": B_hvp / (ms_hvp * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    err = np.abs(hv["mixed"] - hv["fp64"]).max(axis=1) / np.abs(hv["fp64"]).max(axis=1)
+    g64, g32 = ffm.param_grad(), ffm.param_grad(precision=abi.MDN_PREC_MIXED)
+    out["mixed_max_rel_hvp_error"] = float(err.max())
+    out["mixed_param_grad_rel_error"] = [float(np.abs(g32[k] - g64[k]).max() / np.abs(g64[k]).max())
+                                         for k in range(3)]
+    out["hvp_algorithmic_bytes"] = B_hvp
+    log(f"42k adjoint: HVP {out['mixed']['us_per_hvp_device']:.1f} us mixed / "
+        f"{out['fp64']['us_per_hvp_device']:.1f} us FP64, dE/dtheta {out['fp64']['us_per_param_grad_device']:.1f} us")
+    ffm.close()
+    return out
+
+
 def batch_256(args):
     nets = [synth.Network.config(3, index=i) for i in range(args.batch_count)]
     ffms = []
@@ -360,6 +411,7 @@ def run_stress(args):
     ffm.close()
     if not args.no_extras:
         line["network_42k"] = single_network(args)
+        line["network_42k_adjoint"] = adjoint_42k(args)
         line["batch_256"] = batch_256(args)
     if not args.no_cpu:
         first = mech.perturbation(NOISE_SEED, 0, NOISE_SIGMA, 3 * net.n_beads)
@@ -383,6 +435,14 @@ def run_batched(args):
     out = batch_256(args)
     line = base_line(out["mixed"]["minimizations_per_s"], "minimizations/s", 1, 1, out["mixed"]["batch_ms"],
                      {"workload": BATCH})
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
+def run_adjoint(args):
+    out = adjoint_42k(args)
+    line = base_line(1e6 / out["mixed"]["us_per_hvp_device"], "HVPs/s", 64, 8,
+                     out["mixed"]["us_per_hvp_device"] / 1e3, {"workload": out["workload"]})
     line.update(out)
     print(json.dumps(line), flush=True)
 
@@ -432,7 +492,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the 42,000-bead network and the 256-network batch")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=["stress", "single", "batched", "minimize", "forces"], default="stress")
+    ap.add_argument("--workload", choices=["stress", "single", "adjoint", "batched", "minimize", "forces"], default="stress")
     ap.add_argument("--net", type=int, choices=[1, 4], default=4)
     ap.add_argument("--prec", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--batch-count", type=int, default=256)
@@ -456,7 +516,7 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         sys.exit("--steps must be at least 1")
-    {"stress": run_stress, "single": run_single, "batched": run_batched, "minimize": run_minimize,
+    {"stress": run_stress, "single": run_single, "adjoint": run_adjoint, "batched": run_batched, "minimize": run_minimize,
      "forces": run_force_evals}[args.workload](args)
 
 

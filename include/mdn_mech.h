@@ -271,6 +271,16 @@ int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
 /* Time `iters` total-energy evaluations the same way. */
 int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval);
 
+/* Time `iters` Hessian-vector products (mdn_hvp / mdn_hvp_mixed semantics, tangent v of n_var =
+ * 3 * n_beads host doubles, uploaded once before the timed launches) and then `iters` parameter
+ * gradients (mdn_param_grad / mdn_param_grad_mixed) on the context's current device coordinates,
+ * with HIP events on the context's stream: *ms_per_hvp and *ms_per_param_grad receive the mean
+ * device time of one, without the host transfers of the per-call entry points. Fails with
+ * MDN_ERR_UNSUPPORTED when an excluded-volume pair at these coordinates needs a fallback (no
+ * tangent, or the Simpson energy), whose host round trips these timings would leave out. */
+int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const double* v, int32_t n_var,
+                      double* ms_per_hvp, double* ms_per_param_grad);
+
 /* Set the current device coordinates from host memory (no NL rebuild). */
 int mdn_set_coords(mdn_ctx* ctx, const double* coord);
 int mdn_get_coords(mdn_ctx* ctx, double* coord);

@@ -12,10 +12,14 @@ STATUS_NAMES = {
     4: "MDN_ERR_NONFINITE", 5: "MDN_ERR_LINESEARCH", 6: "MDN_ERR_NL_BOUNDS",
     7: "MDN_ERR_RECOVERY", 8: "MDN_ERR_NOMEM", 9: "MDN_ERR_UNSUPPORTED", 10: "MDN_ERR_CAPACITY",
 }
+MDN_ERR_ARG = 1
+MDN_ERR_HIP = 2
+MDN_ERR_STATE = 3
 MDN_ERR_NONFINITE = 4
 MDN_ERR_LINESEARCH = 5
 MDN_ERR_NL_BOUNDS = 6
 MDN_ERR_RECOVERY = 7
+MDN_ERR_NOMEM = 8
 MDN_ERR_UNSUPPORTED = 9
 MDN_ERR_CAPACITY = 10
 

@@ -1738,15 +1738,18 @@ __global__ void __launch_bounds__(BLOCK) k_tangent_terms(DevSys s, const double*
     if (fbad) atomicAdd(bad_count, 1);
 }
 
-// Per-pair excluded-volume energies (closed form, FP64); NaN marks a pair whose closed form is
-// non-finite or < -1 (the reference then integrates numerically: k_ev_pair_energyN).
-__global__ void k_ev_pair_energy(DevSys s, const double* __restrict__ x, double* __restrict__ U) {
+// Per-pair excluded-volume energies (closed form, FP64). A pair whose closed form is non-finite
+// or < -1 (the reference then integrates numerically: k_ev_pair_energyN) gets NaN and is listed
+// in bad_list (bad_count of them, in no particular order).
+__global__ void k_ev_pair_energy(DevSys s, const double* __restrict__ x, double* __restrict__ U, int* bad_count,
+                                 int* bad_list) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= s.n[FF_EXVOL]) return;
     int deferred = 0;
     const double u = exvol_energy<double>(ld3(x, s.ev_b[4 * t]), ld3(x, s.ev_b[4 * t + 1]), ld3(x, s.ev_b[4 * t + 2]),
                                           ld3(x, s.ev_b[4 * t + 3]), s.ev_ks[t], &deferred);
     U[t] = deferred ? NAN : u;
+    if (deferred) bad_list[atomicAdd(bad_count, 1)] = t;
 }
 // The Simpson energy (energyN) of pair t into U[t] (-1 when it stays non-finite). One block.
 __global__ void __launch_bounds__(BLOCK) k_ev_pair_energyN(DevSys s, const double* __restrict__ x, int t,
@@ -1768,33 +1771,46 @@ __global__ void __launch_bounds__(BLOCK) k_ev_pair_energyN(DevSys s, const doubl
 //     dE/dL0 = -k (L - L0);
 //   excluded volume (kRepScaled = krep L0_i L0_j, CylinderExclVolume.cpp:73-138): every pair with
 //     cylinder c adds U / L0_c to dE/dL0_c; those pairs are the ones in the incidence list of c's
-//     first bead with the bead as the pair's bead 1 or bead 3 (ascending pair order);
+//     first bead with the bead as the pair's bead 1 or bead 3;
 //   bending term whose second cylinder is c (its k, FilamentBending.cpp:29-60): dE/dk = U / k.
 // out[c], out[n_cyl + c], out[2 n_cyl + c]. R = float: the mixed path's arithmetic (the
 // extension formed in FP64 and the stretching derivatives rounded to FP32 as the mixed stretching
 // energy is, FP32 bending energy, FP64 excluded-volume energies).
+// PG_LANES lanes per cylinder (in one wave): they stride over the incidence list, their sums of
+// pair energies meet in a fixed xor tree (the same on every run), and lane 0 writes the
+// stretching derivatives, lane 1 the bending one. Grid: n_cyl * PG_LANES threads.
+constexpr int PG_LANES = 8;
 template <typename R>
-__global__ void k_param_grad(DevSys s, int n_cyl, const int* __restrict__ cyl_b1, const int* __restrict__ cyl_bend,
-                             const double* __restrict__ x, const double* __restrict__ U, double* __restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_cyl) return;
-    const V3<double> r = dif<double>(ld3(x, s.str_b[2 * c + 1]), ld3(x, s.str_b[2 * c]));
-    const double dl = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) - s.str_eql[c];
-    out[c] = (double)(R)(0.5 * dl * dl);
-    double gl = (double)(R)(-s.str_k[c] * dl);
-    const int b = cyl_b1[c];
-    const long e0 = s.slot0[FF_EXVOL], e1 = s.slot0[FF_EXVOL + 1];
-    for (int q = s.bstart[b]; q < s.bstart[b + 1]; q++) {
-        const long id = s.sorted_id[q];
-        if (id < e0 || id >= e1) continue;
-        const long loc = id - e0;
-        if ((loc & 1) == 0) gl += U[loc >> 2] / s.str_eql[c];   // bead 1 or bead 3 of the pair
+__global__ void __launch_bounds__(BLOCK) k_param_grad(DevSys s, int n_cyl, const int* __restrict__ cyl_b1,
+                                                      const int* __restrict__ cyl_bend, const double* __restrict__ x,
+                                                      const double* __restrict__ U, double* __restrict__ out) {
+    static_assert(64 % PG_LANES == 0, "a cylinder's lanes share one wave");
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = g / PG_LANES, lane = g % PG_LANES;
+    double eu = 0.0;   // sum of U over c's pairs, this lane's share
+    if (c < n_cyl) {
+        const int b = cyl_b1[c];
+        const long e0 = s.slot0[FF_EXVOL], e1 = s.slot0[FF_EXVOL + 1];
+        for (int q = s.bstart[b] + lane; q < s.bstart[b + 1]; q += PG_LANES) {
+            const long id = s.sorted_id[q];
+            const long loc = id - e0;
+            if (id >= e0 && id < e1 && (loc & 1) == 0) eu += U[loc >> 2];   // bead 1 or bead 3 of the pair
+        }
     }
-    out[n_cyl + c] = gl;
-    const int tb = cyl_bend[c];
-    out[2 * n_cyl + c] = tb >= 0 ? (double)bend_energy<R>(ld3(x, s.bend_b[3 * tb]), ld3(x, s.bend_b[3 * tb + 1]),
-                                                          ld3(x, s.bend_b[3 * tb + 2]), 1.0, s.bend_eqt[tb])
-                                 : 0.0;
+#pragma unroll
+    for (int m = PG_LANES / 2; m >= 1; m /= 2) eu += __shfl_xor(eu, m, PG_LANES);
+    if (c >= n_cyl) return;
+    if (lane == 0) {
+        const V3<double> r = dif<double>(ld3(x, s.str_b[2 * c + 1]), ld3(x, s.str_b[2 * c]));
+        const double dl = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) - s.str_eql[c];
+        out[c] = (double)(R)(0.5 * dl * dl);
+        out[n_cyl + c] = (double)(R)(-s.str_k[c] * dl) + eu / s.str_eql[c];
+    } else if (lane == 1) {
+        const int tb = cyl_bend[c];
+        out[2 * n_cyl + c] = tb >= 0 ? (double)bend_energy<R>(ld3(x, s.bend_b[3 * tb]), ld3(x, s.bend_b[3 * tb + 1]),
+                                                              ld3(x, s.bend_b[3 * tb + 2]), 1.0, s.bend_eqt[tb])
+                                     : 0.0;
+    }
 }
 
 // =======================================================================================

@@ -1375,51 +1375,67 @@ int mdn_crosslink_stretch_forces(mdn_ctx* ctx, double* linker_out, double* motor
     return MDN_OK;
 }
 
+// dE/dtheta at device coordinates x into ctx->grad_out: the per-pair excluded-volume energies,
+// then one thread per cylinder. check: settle the pairs whose closed-form energy is deferred (a
+// host sync on a device count, then the Simpson energy of each listed pair); without it the
+// caller has established that no pair at x is deferred.
+static int param_grad_launch(mdn_ctx* ctx, const double* x, int prec, bool check) {
+    const DevSys& s = ctx->sys;
+    const int np = s.n[FF_EXVOL];
+    if (np > 0) {
+        HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_ev_pair_energy, dim3(ceil_div(np, BLOCK)), dim3(BLOCK), 0, ctx->stream, s, x, ctx->ev_U.p,
+                           ctx->bad_count.p, ctx->bad_list.p);
+        HIPCHK(hipGetLastError());
+    }
+    if (np > 0 && check) {
+        HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        const int nbad = ctx->h_int[0];
+        if (nbad > 0) {
+            std::vector<int> redo(nbad);
+            HIPCHK(d2h(ctx, redo.data(), ctx->bad_list.p, sizeof(int) * nbad));
+            std::sort(redo.begin(), redo.end());
+            for (int t : redo)
+                hipLaunchKernelGGL(k_ev_pair_energyN, dim3(1), dim3(BLOCK), 0, ctx->stream, s, x, t, ctx->ev_U.p);
+            HIPCHK(hipGetLastError());
+            std::vector<double> U(nbad);
+            for (int i = 0; i < nbad; i++)
+                HIPCHK(hipMemcpyAsync(&U[i], ctx->ev_U.p + redo[i], sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            for (int i = 0; i < nbad; i++)
+                if (U[i] <= -1.0)
+                    return fail(ctx, MDN_ERR_NONFINITE,
+                                "excluded-volume pair %d: energy non-finite after the Simpson fallback", redo[i]);
+        }
+    }
+    if (ctx->n_cyl > 0) {
+        if (prec == MDN_PREC_MIXED)
+            hipLaunchKernelGGL(k_param_grad<float>, dim3(ceil_div(ctx->n_cyl * PG_LANES, BLOCK)), dim3(BLOCK), 0, ctx->stream, s,
+                               ctx->n_cyl, (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x,
+                               (const double*)ctx->ev_U.p, ctx->grad_out.p);
+        else
+            hipLaunchKernelGGL(k_param_grad<double>, dim3(ceil_div(ctx->n_cyl * PG_LANES, BLOCK)), dim3(BLOCK), 0, ctx->stream, s,
+                               ctx->n_cyl, (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x,
+                               (const double*)ctx->ev_U.p, ctx->grad_out.p);
+    }
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+}
+
 static int param_grad_impl(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out, int prec) {
     int st = need_system(ctx, true);
     if (st) return st;
     if (!out || n_out != 3L * ctx->n_cyl) return fail(ctx, MDN_ERR_ARG, "n_out must be 3*n_cylinders = %d", 3 * ctx->n_cyl);
     HIPCHK(hipSetDevice(ctx->device));
-    const DevSys& s = ctx->sys;
     const double* x = ctx->x[ctx->cur].p;
     if (coord) {
         HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * 3 * ctx->n_beads, hipMemcpyHostToDevice, ctx->stream));
         x = ctx->ext.p;
     }
-    const int np = s.n[FF_EXVOL];
-    HIPCHK(ctx->ev_U.ensure(std::max(1, np)));
+    HIPCHK(ctx->ev_U.ensure(std::max(1, ctx->sys.n[FF_EXVOL])));
     HIPCHK(ctx->grad_out.ensure(3 * (size_t)std::max(1, ctx->n_cyl)));
-    if (np > 0) {
-        hipLaunchKernelGGL(k_ev_pair_energy, dim3(ceil_div(np, BLOCK)), dim3(BLOCK), 0, ctx->stream, s, x, ctx->ev_U.p);
-        HIPCHK(hipGetLastError());
-        std::vector<double> U(np);
-        HIPCHK(hipMemcpyAsync(U.data(), ctx->ev_U.p, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        std::vector<int> redo;
-        for (int t = 0; t < np; t++)
-            if (U[t] != U[t]) redo.push_back(t);
-        for (int t : redo)
-            hipLaunchKernelGGL(k_ev_pair_energyN, dim3(1), dim3(BLOCK), 0, ctx->stream, s, x, t, ctx->ev_U.p);
-        if (!redo.empty()) {
-            for (int t : redo)
-                HIPCHK(hipMemcpyAsync(&U[t], ctx->ev_U.p + t, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            for (int t : redo)
-                if (U[t] <= -1.0)
-                    return fail(ctx, MDN_ERR_NONFINITE, "excluded-volume pair %d: energy non-finite after the Simpson fallback", t);
-        }
-    }
-    if (ctx->n_cyl > 0) {
-        if (prec == MDN_PREC_MIXED)
-            hipLaunchKernelGGL(k_param_grad<float>, dim3(ceil_div(ctx->n_cyl, BLOCK)), dim3(BLOCK), 0, ctx->stream, s,
-                               ctx->n_cyl, (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x,
-                               (const double*)ctx->ev_U.p, ctx->grad_out.p);
-        else
-            hipLaunchKernelGGL(k_param_grad<double>, dim3(ceil_div(ctx->n_cyl, BLOCK)), dim3(BLOCK), 0, ctx->stream, s,
-                               ctx->n_cyl, (const int*)ctx->d_cyl_b1.p, (const int*)ctx->d_cyl_bend.p, x,
-                               (const double*)ctx->ev_U.p, ctx->grad_out.p);
-    }
-    HIPCHK(hipGetLastError());
+    if ((st = param_grad_launch(ctx, x, prec, true))) return st;
     HIPCHK(hipMemcpyAsync(out, ctx->grad_out.p, sizeof(double) * 3 * ctx->n_cyl, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MDN_OK;
@@ -1433,19 +1449,11 @@ int mdn_param_grad_mixed(mdn_ctx* ctx, const double* coord, double* out, int64_t
     return param_grad_impl(ctx, coord, out, n_out, MDN_PREC_MIXED);
 }
 
-static int hvp_impl(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var, int prec) {
-    int st = need_system(ctx, true);
-    if (st) return st;
-    if (!v || !out || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
-    HIPCHK(hipSetDevice(ctx->device));
+// -H v into ctx->scratch_f for the device tangent ctx->tan_v at device coordinates x (the tangent
+// pieces of every interaction, then the FP64 gather); excluded-volume pairs without a tangent are
+// counted in bad_count.
+static int hvp_launch(mdn_ctx* ctx, const double* x, int prec) {
     const DevSys& s = ctx->sys;
-    const double* x = ctx->x[ctx->cur].p;
-    if (coord) {
-        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
-        x = ctx->ext.p;
-    }
-    HIPCHK(ctx->tan_v.ensure(n_var));
-    HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
     if (s.blk[NFF] > 0) {
         if (prec == MDN_PREC_MIXED)
@@ -1457,7 +1465,22 @@ static int hvp_impl(mdn_ctx* ctx, const double* coord, const double* v, double* 
     }
     HIPCHK(hipGetLastError());
     // the tangent pieces are in the piece arrays
-    if ((st = gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, false))) return st;
+    return gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, false);
+}
+
+static int hvp_impl(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var, int prec) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (!v || !out || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    HIPCHK(ctx->tan_v.ensure(n_var));
+    HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = hvp_launch(ctx, x, prec))) return st;
     HIPCHK(hipMemcpyAsync(out, ctx->scratch_f.p, sizeof(double) * n_var, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -2174,6 +2197,50 @@ int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
     HIPCHK(hipSetDevice(ctx->device));
     return precision == MDN_PREC_MIXED ? bench_forces_t<float>(ctx, iters, ms_per_eval, kernel_ms)
                                        : bench_forces_t<double>(ctx, iters, ms_per_eval, kernel_ms);
+}
+
+int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const double* v, int32_t n_var,
+                      double* ms_per_hvp, double* ms_per_param_grad) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (precision != MDN_PREC_FP64 && precision != MDN_PREC_MIXED)
+        return fail(ctx, MDN_ERR_ARG, "unknown precision %d", precision);
+    if (iters <= 0) return fail(ctx, MDN_ERR_ARG, "iters must be positive");
+    if (!v || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    HIPCHK(hipSetDevice(ctx->device));
+    const double* x = ctx->x[ctx->cur].p;
+    HIPCHK(ctx->tan_v.ensure(n_var));
+    HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx->ev_U.ensure(std::max(1, ctx->sys.n[FF_EXVOL])));
+    HIPCHK(ctx->grad_out.ensure(3 * (size_t)std::max(1, ctx->n_cyl)));
+    // once with the checks: every pair has a tangent and a closed-form energy at x, so the timed
+    // launches below do exactly the work of mdn_hvp and mdn_param_grad on the device
+    if ((st = hvp_launch(ctx, x, precision))) return st;
+    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_int[0] != 0)
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "%d excluded-volume pairs need the numerical force fallback (no tangent)",
+                    ctx->h_int[0]);
+    if ((st = param_grad_launch(ctx, x, precision, true))) return st;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->sys.n[FF_EXVOL] > 0 && ctx->h_int[0] != 0)
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "%d excluded-volume pairs need the Simpson energy", ctx->h_int[0]);
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2));
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iters && !st; i++) st = hvp_launch(ctx, x, precision);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    for (int i = 0; i < iters && !st; i++) st = param_grad_launch(ctx, x, precision, false);
+    HIPCHK(hipEventRecord(e2, ctx->stream));
+    HIPCHK(hipEventSynchronize(e2));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, e0, e1));
+    HIPCHK(hipEventElapsedTime(&b, e1, e2));
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
+    if (st) return st;
+    if (ms_per_hvp) *ms_per_hvp = a / iters;
+    if (ms_per_param_grad) *ms_per_param_grad = b / iters;
+    return MDN_OK;
 }
 
 int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval) {

@@ -32,7 +32,7 @@ EXPORTED = [
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
     "mdn_forces_noalloc", "mdn_displace", "mdn_perturb", "mdn_param_grad_mixed", "mdn_hvp_mixed",
-    "mdn_dof_serialize", "mdn_binding_pairs",
+    "mdn_dof_serialize", "mdn_binding_pairs", "mdn_bench_adjoint",
 ]
 
 
@@ -74,6 +74,7 @@ def lib():
         L.mdn_minimize.argtypes = [C.c_void_p, C.POINTER(abi.CgParams), dp, dp, C.POINTER(abi.MinResult)]
         L.mdn_bench_forces.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, dp]
         L.mdn_bench_energy.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
+        L.mdn_bench_adjoint.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, C.c_int32, dp, dp]
         L.mdn_set_coords.argtypes = [C.c_void_p, dp]
         L.mdn_get_coords.argtypes = [C.c_void_p, dp]
         L.mdn_forces_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int32]
@@ -311,6 +312,15 @@ class ForceFieldManager:
         a = C.c_double()
         self._chk(self._L.mdn_bench_energy(self._h, precision, iters, C.byref(a)))
         return a.value
+
+    def bench_adjoint(self, iters, v, precision=abi.MDN_PREC_FP64):
+        """Mean device ms of one Hessian-vector product along v and of one dE/dtheta at the current
+        device coordinates (mdn_bench_adjoint)."""
+        vv = _f64(v)
+        a, b = C.c_double(), C.c_double()
+        self._chk(self._L.mdn_bench_adjoint(self._h, precision, iters, vv.ctypes.data_as(dp), vv.size, C.byref(a),
+                                            C.byref(b)))
+        return a.value, b.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_ffi as of
-from picgo_amd import abi, synth
+from picgo_amd import abi, mech, synth
 from tests.exvol_conditioning import exvol_bead_bars as _exvol_bead_bars, well_conditioned as _well_conditioned
 
 pytestmark = pytest.mark.gpu
@@ -221,3 +221,22 @@ def test_mixed_hvp_and_param_grad(relaxed):
     # straightest triples of the relaxed network, where 1 - x in FP32 gave 0.12); dE/dL0 =
     # -k (L - L0) + sum U / L0 can cancel between its two parts, so it is held normwise only
     assert comp[0] <= 1e-6 and comp[1] <= 1e-2
+
+
+def test_bench_adjoint_times_the_entry_points_work(relaxed):
+    """mdn_bench_adjoint times, on the device, the launches that mdn_hvp and mdn_param_grad make
+    (tangent terms and gather; pair energies and the per-cylinder kernel): after it, the
+    context's results are those of the per-call entry points at the same coordinates."""
+    net, g, x = relaxed
+    rng = np.random.default_rng(9)
+    v = _unit(rng, x.size)
+    g.set_coords(x)
+    for prec in (abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED):
+        ms_hvp, ms_grad = g.bench_adjoint(4, v, precision=prec)
+        print(f"precision {prec}: {1e3 * ms_hvp:.1f} us per HVP, {1e3 * ms_grad:.1f} us per dE/dtheta", flush=True)
+        assert 0.0 < ms_hvp < 100.0 and 0.0 < ms_grad < 100.0
+        np.testing.assert_array_equal(g.hvp(v, precision=prec), g.hvp(v, x, precision=prec))
+        np.testing.assert_array_equal(g.param_grad(precision=prec), g.param_grad(x, precision=prec))
+    with pytest.raises(mech.MdnError) as ei:
+        g.bench_adjoint(4, v[:-1])
+    assert ei.value.code == abi.MDN_ERR_ARG

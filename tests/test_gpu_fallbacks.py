@@ -133,3 +133,30 @@ def test_batched_refuses_forceN(gpu):
         mech.minimize_batched([g2, g], abi.default_cg_params(), [net.coord, hs.coord])
     print(ei.value, flush=True)
     assert ei.value.code == 9 and "numerical force fallback" in str(ei.value)   # MDN_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("case", ["parallel", "collinear"])
+def test_param_grad_takes_the_simpson_pair_energy(gpu, case):
+    """dE/dL0 of each cylinder is -k (L - L0) plus U / L0 for every excluded-volume pair it is in
+    (kRepScaled = krep L0_i L0_j): here U is the pair's Simpson energy, the closed form being
+    deferred, so the device lists the pair, the host runs energyN for it and the per-cylinder
+    kernel reads the result. The benchmark helper, whose timed launches leave that host-run step
+    out, refuses these coordinates."""
+    hs = _parallel() if case == "parallel" else _collinear()
+    g, o = _setup(gpu, hs)
+    x = hs.coord
+    U = o.energy(x)[1].ff[abi.MDN_FF_EXVOL]
+    X = x.reshape(-1, 3)
+    for prec in (abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED):
+        grad = g.param_grad(x, precision=prec)
+        for c in (0, 1):
+            dl = float(np.sqrt(np.sum((X[2 * c + 1] - X[2 * c]) ** 2))) - 108.0
+            stretch = -100.0 * dl   # exact in both precisions: L - L0 is 0 or -48 nm here
+            print(f"{case} precision {prec} cylinder {c}: dE/dL0 {grad[1, c]:.15e}, stretching {stretch}, "
+                  f"U / L0 {U / 108.0:.15e}", flush=True)
+            assert grad[1, c] - stretch == pytest.approx(U / 108.0, rel=1e-12)
+            assert grad[0, c] == 0.5 * dl * dl and grad[2, c] == 0.0
+        with pytest.raises(mech.MdnError) as ei:
+            g.bench_adjoint(1, np.full(x.size, 0.5), precision=prec)
+        print(ei.value, flush=True)
+        assert ei.value.code == abi.MDN_ERR_UNSUPPORTED
