@@ -25,8 +25,9 @@ and FP64 reductions); the FP64 path runs the same sequence beside it. The line a
   network_42k      the 42,000-bead network (5 um cube): force evaluations/s and whole
                    minimizations in both precisions;
   network_42k_adjoint  the same network with lognormal per-cylinder parameters near its relaxed
-                   state: Hessian-vector products and dE/dtheta, device time per call (HIP events)
-                   and 64 HVPs through the host-buffer entry points, mixed error against FP64;
+                   state: Hessian-vector products and dE/dtheta, device time per product (HIP
+                   events; one per launch and 16 per launch), 64 HVPs through the host-buffer
+                   entry points (one call each, and one mdn_hvp_batch call), mixed error against FP64;
   batch_256        the 256-network batch in the persistent batched kernel.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extras]
@@ -268,10 +269,17 @@ def adjoint_42k(args):
     for name, prec in (("fp64", abi.MDN_PREC_FP64), ("mixed", abi.MDN_PREC_MIXED)):
         ffm.bench_adjoint(8, vs[0], prec)
         ms_hvp, ms_grad = ffm.bench_adjoint(64, vs[0], prec)
+        ffm.bench_adjoint(2, vs[:16], prec)
+        ms_hvp16, _ = ffm.bench_adjoint(4, vs[:16], prec)   # 4 launches of 16: the 64 vectors' device work
         ffm.hvp(vs[0], precision=prec)
         t0 = time.perf_counter()
         hv[name] = np.stack([ffm.hvp(v, precision=prec) for v in vs])
         t_host = time.perf_counter() - t0
+        ffm.hvp_batch(vs[:16], precision=prec)
+        t0 = time.perf_counter()
+        hb = ffm.hvp_batch(vs, precision=prec)
+        t_batch = time.perf_counter() - t0
+        batch_same = bool(np.array_equal(hb, hv[name]))
         ffm.param_grad(precision=prec)
         t0 = time.perf_counter()
         for _ in range(16):
@@ -279,7 +287,11 @@ def adjoint_42k(args):
         t_grad_host = (time.perf_counter() - t0) / 16
         out[name] = {"us_per_hvp_device": 1e3 * ms_hvp, "us_per_param_grad_device": 1e3 * ms_grad,
                      "ms_64_hvps_device": 64 * ms_hvp,
+                     "us_per_hvp_device_16_per_launch": 1e3 * ms_hvp16,
+                     "ms_64_hvps_device_16_per_launch": 64 * ms_hvp16,
                      "ms_64_hvps_host_buffers": 1e3 * t_host,
+                     "ms_64_hvps_host_buffers_one_call": 1e3 * t_batch,
+                     "hvp_batch_equals_single_calls": batch_same,
                      "us_per_param_grad_host_buffers": 1e6 * t_grad_host,
                      "hvp_roofline_frac": B_hvp / (ms_hvp * 1e-3) / 1e9 / HBM_PEAK_GBS}
     err = np.abs(hv["mixed"] - hv["fp64"]).max(axis=1) / np.abs(hv["fp64"]).max(axis=1)
@@ -289,7 +301,8 @@ def adjoint_42k(args):
                                          for k in range(3)]
     out["hvp_algorithmic_bytes"] = B_hvp
     log(f"42k adjoint: HVP {out['mixed']['us_per_hvp_device']:.1f} us mixed / "
-        f"{out['fp64']['us_per_hvp_device']:.1f} us FP64, dE/dtheta {out['fp64']['us_per_param_grad_device']:.1f} us")
+        f"{out['fp64']['us_per_hvp_device']:.1f} us FP64 ({out['fp64']['us_per_hvp_device_16_per_launch']:.1f} us "
+        f"16 per launch), dE/dtheta {out['fp64']['us_per_param_grad_device']:.1f} us")
     ffm.close()
     return out
 

@@ -271,15 +271,18 @@ int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
 /* Time `iters` total-energy evaluations the same way. */
 int mdn_bench_energy(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_per_eval);
 
-/* Time `iters` Hessian-vector products (mdn_hvp / mdn_hvp_mixed semantics, tangent v of n_var =
- * 3 * n_beads host doubles, uploaded once before the timed launches) and then `iters` parameter
- * gradients (mdn_param_grad / mdn_param_grad_mixed) on the context's current device coordinates,
- * with HIP events on the context's stream: *ms_per_hvp and *ms_per_param_grad receive the mean
- * device time of one, without the host transfers of the per-call entry points. Fails with
- * MDN_ERR_UNSUPPORTED when an excluded-volume pair at these coordinates needs a fallback (no
- * tangent, or the Simpson energy), whose host round trips these timings would leave out. */
+/* Time `iters` rounds of n_vec Hessian-vector products (n_vec = 1: mdn_hvp / mdn_hvp_mixed
+ * semantics; 2 .. MDN_HVP_BATCH_CHUNK: one chunk of mdn_hvp_batch, its tangent terms in one
+ * launch) along the tangents v (n_vec rows of n_var = 3 * n_beads host doubles, uploaded once
+ * before the timed launches), then `iters` parameter gradients (mdn_param_grad /
+ * mdn_param_grad_mixed), on the context's current device coordinates, with HIP events on the
+ * context's stream: *ms_per_hvp receives the mean device time per product and
+ * *ms_per_param_grad per gradient, without the host transfers of the per-call entry points.
+ * Fails with MDN_ERR_UNSUPPORTED when an excluded-volume pair at these coordinates needs a
+ * fallback (no tangent, or the Simpson energy), whose host round trips these timings would
+ * leave out. */
 int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const double* v, int32_t n_var,
-                      double* ms_per_hvp, double* ms_per_param_grad);
+                      int32_t n_vec, double* ms_per_hvp, double* ms_per_param_grad);
 
 /* Set the current device coordinates from host memory (no NL rebuild). */
 int mdn_set_coords(mdn_ctx* ctx, const double* coord);
@@ -396,6 +399,18 @@ int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int
  * boundary and accumulation (the precision split of mdn_forces_mixed). */
 int mdn_param_grad_mixed(mdn_ctx* ctx, const double* coord, double* out, int64_t n_out);
 int mdn_hvp_mixed(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var);
+
+/* n_vec Hessian-vector products at one point: out[k n_var + i] = (H V_k)_i for the tangents
+ * V_k = V + k n_var (each n_var = 3 * n_beads doubles), with the results of mdn_hvp (precision
+ * MDN_PREC_FP64) or mdn_hvp_mixed (MDN_PREC_MIXED) bit for bit. Up to MDN_HVP_BATCH_CHUNK
+ * products share one launch of the tangent terms (a grid dimension per product, each with its
+ * own slot and piece arrays on the device), and every chunk crosses the host link in one copy
+ * each way, with one synchronization per call. Like mdn_hvp, the reference's counterpart is the
+ * dense finite-difference Hessian (ForceFieldManager.cpp:354-500), whose columns these products
+ * replace one direction at a time. */
+#define MDN_HVP_BATCH_CHUNK 16
+int mdn_hvp_batch(mdn_ctx* ctx, int32_t precision, const double* coord, const double* V, double* out,
+                  int32_t n_var, int32_t n_vec);
 
 /* Replace the per-cylinder parameters (any pointer may be null: unchanged): stretching k and
  * rest length L0 (the excluded-volume kRepScaled of the current neighbor list is rebuilt) per

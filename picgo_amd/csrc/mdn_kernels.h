@@ -723,6 +723,26 @@ __global__ void __launch_bounds__(BLOCK) k_gather_plain(DevSys s, int bmask, con
     f[3 * b] = acc.x; f[3 * b + 1] = acc.y; f[3 * b + 2] = acc.z;
 }
 
+// The FP64 gathers of several tangent products in one launch (k_gather_plain's gather of the
+// piece arrays and every slot, from zero): product k = blockIdx.y from its slots at
+// slots + k slot_stride and its pieces at piece + k piece_stride into f + k n_var.
+template <int GB>
+__global__ void __launch_bounds__(BLOCK) k_gather_tangents(DevSys s, const double* __restrict__ slots, long slot_stride,
+                                                           double* piece, long piece_stride, double* __restrict__ f,
+                                                           long n_var) {
+    __shared__ double L[GCHUNK_BYTES / sizeof(double)];
+    const long k = blockIdx.y;
+    DevSys sk = s;
+    sk.piece = piece + k * piece_stride;
+    const int b = blockIdx.x * GB + threadIdx.x;
+    const bool mine = threadIdx.x < GB && b < s.n_beads;
+    const D3 acc = gather_block<double, GB, false>(sk, 3, slots + k * slot_stride, 0L, s.slot0[NFF], true,
+                                                   D3{0.0, 0.0, 0.0}, L);
+    if (!mine) return;
+    double* o = f + k * n_var;
+    o[3 * b] = acc.x; o[3 * b + 1] = acc.y; o[3 * b + 2] = acc.z;
+}
+
 // Block-parallel Simpson energies (energyN) of the pairs deferred by the preceding kernel.
 // Thread 0 receives, per code, the sum of the energies in (code, pair index) order and a flag for
 // a pair that stays non-finite (the reference's -1 sentinel); the deferred list is reset.
@@ -1677,22 +1697,28 @@ __device__ inline DualD3 pos_dual(const double* __restrict__ x, const double* __
 // reference would switch to forceN, which has no tangent here).
 template <typename RT>
 __device__ inline void term_tangent(const DevSys& s, int ff, int t, const double* __restrict__ x,
-                                    const double* __restrict__ v, double* __restrict__ slots, int* fbad) {
+                                    const double* __restrict__ v, double* __restrict__ slots,
+                                    double* __restrict__ piece, int* fbad) {
+    // the filament terms' tangents go to the bead-major pieces of this product: piece[k][bead]
+    auto put = [&](int k, int bead, D3 d) {
+        double* q = piece + 3L * ((long)k * s.n_beads + bead);
+        q[0] = d.x; q[1] = d.y; q[2] = d.z;
+    };
     DualD3 out[4];
     switch (ff) {
     case FF_STRETCH: {
         const int b1 = s.str_b[2 * t], b2 = s.str_b[2 * t + 1];
         stretch_forces<RT>(pos_dual(x, v, b1), pos_dual(x, v, b2), s.str_k[t], s.str_eql[t], out);
-        put_piece<double>(s, 1, b1, out[0].d);
-        put_piece<double>(s, 0, b2, out[1].d);
+        put(1, b1, out[0].d);
+        put(0, b2, out[1].d);
         return;
     }
     case FF_BEND: {
         const int b1 = s.bend_b[3 * t], b2 = s.bend_b[3 * t + 1], b3 = s.bend_b[3 * t + 2];
         bend_forces<RT>(pos_dual(x, v, b1), pos_dual(x, v, b2), pos_dual(x, v, b3), s.bend_k[t], s.bend_eqt[t], out);
-        put_piece<double>(s, 4, b1, out[0].d);
-        put_piece<double>(s, 3, b2, out[1].d);
-        put_piece<double>(s, 2, b3, out[2].d);
+        put(4, b1, out[0].d);
+        put(3, b2, out[1].d);
+        put(2, b3, out[2].d);
         return;
     }
     case FF_LINKER:
@@ -1725,16 +1751,21 @@ __device__ inline void term_tangent(const DevSys& s, int ff, int t, const double
     }
 }
 
+// Grid: (interaction blocks, products). Product k = blockIdx.y takes its tangent from
+// v + k v_stride and writes its slots and pieces at slots + k slot_stride, piece + k piece_stride.
 template <typename RT>
 __global__ void __launch_bounds__(BLOCK) k_tangent_terms(DevSys s, const double* __restrict__ x,
-                                                         const double* __restrict__ v, double* __restrict__ slots,
+                                                         const double* __restrict__ v, long v_stride,
+                                                         double* __restrict__ slots, long slot_stride,
+                                                         double* __restrict__ piece, long piece_stride,
                                                          int* bad_count) {
     const int ff = ff_of_block(s, blockIdx.x);
     if (ff >= NFF) return;
     const int t = (blockIdx.x - s.fblk[ff]) * BLOCK + threadIdx.x;
     if (t >= s.n[ff]) return;
+    const long k = blockIdx.y;
     int fbad = 0;
-    term_tangent<RT>(s, ff, t, x, v, slots, &fbad);
+    term_tangent<RT>(s, ff, t, x, v + k * v_stride, slots + k * slot_stride, piece + k * piece_stride, &fbad);
     if (fbad) atomicAdd(bad_count, 1);
 }
 

@@ -111,6 +111,7 @@ struct mdn_ctx {
     DBuf<int> d_cyl_bend;
     std::vector<int> bend_cyl;
     DBuf<double> ev_U, grad_out, tan_v;
+    DBuf<double> hb_v, hb_slots, hb_piece, hb_out;   // mdn_hvp_batch: per-product tangents, slots, pieces, results
     DBuf<double> piece;   // NBOND * 3 * n_beads values (FP64; the mixed path uses the first half as float)
     // cylinder tables for the NL
     DBuf<int> d_cyl_b1, d_cyl_b2, d_cyl_fil, d_cyl_pos, d_cyl_type;
@@ -570,6 +571,10 @@ int mdn_ctx_destroy(mdn_ctx* ctx) {
     ctx->ev_U.release();
     ctx->grad_out.release();
     ctx->tan_v.release();
+    ctx->hb_v.release();
+    ctx->hb_slots.release();
+    ctx->hb_piece.release();
+    ctx->hb_out.release();
     ctx->piece.release();
     ctx->cub_tmp.release();
     ctx->slots.release();
@@ -1449,23 +1454,54 @@ int mdn_param_grad_mixed(mdn_ctx* ctx, const double* coord, double* out, int64_t
     return param_grad_impl(ctx, coord, out, n_out, MDN_PREC_MIXED);
 }
 
+// The tangent terms of nv products at device coordinates x: product k's tangent at v + k n_var,
+// its slots at slots + k slot_stride, its pieces at piece + k piece_stride. Excluded-volume pairs
+// without a tangent are counted in bad_count (zeroed by the caller).
+static void launch_tangents(mdn_ctx* ctx, const double* x, int prec, const double* v, int nv, double* slots,
+                            long slot_stride, double* piece, long piece_stride) {
+    const DevSys& s = ctx->sys;
+    if (s.blk[NFF] == 0 || nv == 0) return;
+    const long n_var = 3L * ctx->n_beads;
+    const dim3 g(s.blk[NFF], nv);
+    if (prec == MDN_PREC_MIXED)
+        hipLaunchKernelGGL(k_tangent_terms<DualF>, g, dim3(BLOCK), 0, ctx->stream, s, x, v, n_var, slots, slot_stride,
+                           piece, piece_stride, ctx->bad_count.p);
+    else
+        hipLaunchKernelGGL(k_tangent_terms<DualD>, g, dim3(BLOCK), 0, ctx->stream, s, x, v, n_var, slots, slot_stride,
+                           piece, piece_stride, ctx->bad_count.p);
+}
+
+// (dF/dx) v of nv products into out + k n_var from their slots and pieces (one launch).
+static void gather_tangents(mdn_ctx* ctx, int nv, const double* slots, long slot_stride, double* piece,
+                            long piece_stride, double* out) {
+    const DevSys& s = ctx->sys;
+    const dim3 g(gather_blocks(ctx), nv), b(BLOCK);
+    const long n_var = 3L * ctx->n_beads;
+    if (gather_size(ctx) == 128)
+        hipLaunchKernelGGL(k_gather_tangents<128>, g, b, 0, ctx->stream, s, slots, slot_stride, piece, piece_stride, out,
+                           n_var);
+    else
+        hipLaunchKernelGGL(k_gather_tangents<64>, g, b, 0, ctx->stream, s, slots, slot_stride, piece, piece_stride, out,
+                           n_var);
+}
+
+// (dF/dx) v of one product into out from its slots and pieces (the FP64 gather).
+static void gather_tangent(mdn_ctx* ctx, const double* slots, double* piece, double* out) {
+    DevSys s = ctx->sys;
+    s.piece = piece;
+    launch_gather_plain<double>(ctx, s, 3, slots, 0L, s.slot0[NFF], (const double*)nullptr, out, false);
+}
+
 // -H v into ctx->scratch_f for the device tangent ctx->tan_v at device coordinates x (the tangent
 // pieces of every interaction, then the FP64 gather); excluded-volume pairs without a tangent are
 // counted in bad_count.
 static int hvp_launch(mdn_ctx* ctx, const double* x, int prec) {
-    const DevSys& s = ctx->sys;
     HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
-    if (s.blk[NFF] > 0) {
-        if (prec == MDN_PREC_MIXED)
-            hipLaunchKernelGGL(k_tangent_terms<DualF>, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
-                               (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
-        else
-            hipLaunchKernelGGL(k_tangent_terms<DualD>, dim3(s.blk[NFF]), dim3(BLOCK), 0, ctx->stream, s, x,
-                               (const double*)ctx->tan_v.p, ctx->slots.p, ctx->bad_count.p);
-    }
+    launch_tangents(ctx, x, prec, ctx->tan_v.p, 1, ctx->slots.p, 0L, ctx->piece.p, 0L);
     HIPCHK(hipGetLastError());
-    // the tangent pieces are in the piece arrays
-    return gather(ctx, 3, 0, s.slot0[NFF], nullptr, ctx->scratch_f.p, MDN_PREC_FP64, false);
+    gather_tangent(ctx, ctx->slots.p, ctx->piece.p, ctx->scratch_f.p);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
 }
 
 static int hvp_impl(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var, int prec) {
@@ -1497,6 +1533,51 @@ int mdn_hvp(mdn_ctx* ctx, const double* coord, const double* v, double* out, int
 
 int mdn_hvp_mixed(mdn_ctx* ctx, const double* coord, const double* v, double* out, int32_t n_var) {
     return hvp_impl(ctx, coord, v, out, n_var, MDN_PREC_MIXED);
+}
+
+int mdn_hvp_batch(mdn_ctx* ctx, int32_t precision, const double* coord, const double* V, double* out, int32_t n_var,
+                  int32_t n_vec) {
+    int st = need_system(ctx, true);
+    if (st) return st;
+    if (precision != MDN_PREC_FP64 && precision != MDN_PREC_MIXED)
+        return fail(ctx, MDN_ERR_ARG, "unknown precision %d", precision);
+    if (!V || !out || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    if (n_vec < 1) return fail(ctx, MDN_ERR_ARG, "n_vec must be positive");
+    HIPCHK(hipSetDevice(ctx->device));
+    const DevSys& s = ctx->sys;
+    const double* x = ctx->x[ctx->cur].p;
+    if (coord) {
+        HIPCHK(hipMemcpyAsync(ctx->ext.p, coord, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+        x = ctx->ext.p;
+    }
+    // per product: its tangent, its slots (with the gather's padding), its pieces, its result
+    const int chunk = std::min<int>(n_vec, MDN_HVP_BATCH_CHUNK);
+    const long slot_stride = 3 * std::max(1L, s.slot0[NFF]) + GCHUNK_PAD;
+    const long piece_stride = (long)NBOND * 3 * ctx->n_beads;
+    HIPCHK(ctx->hb_v.ensure((size_t)chunk * n_var));
+    HIPCHK(ctx->hb_slots.ensure((size_t)chunk * slot_stride));
+    HIPCHK(ctx->hb_piece.ensure((size_t)chunk * piece_stride));
+    HIPCHK(ctx->hb_out.ensure((size_t)chunk * n_var));
+    HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+    for (int k0 = 0; k0 < n_vec; k0 += chunk) {
+        const int nv = std::min(chunk, n_vec - k0);
+        HIPCHK(hipMemcpyAsync(ctx->hb_v.p, V + (size_t)k0 * n_var, sizeof(double) * nv * n_var, hipMemcpyHostToDevice,
+                              ctx->stream));
+        launch_tangents(ctx, x, precision, ctx->hb_v.p, nv, ctx->hb_slots.p, slot_stride, ctx->hb_piece.p, piece_stride);
+        HIPCHK(hipGetLastError());
+        gather_tangents(ctx, nv, ctx->hb_slots.p, slot_stride, ctx->hb_piece.p, piece_stride, ctx->hb_out.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out + (size_t)k0 * n_var, ctx->hb_out.p, sizeof(double) * nv * n_var,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_int[0] != 0)
+        return fail(ctx, MDN_ERR_UNSUPPORTED, "%d excluded-volume pairs need the numerical force fallback (no tangent)",
+                    ctx->h_int[0]);
+    const size_t total = (size_t)n_vec * n_var;
+    for (size_t i = 0; i < total; i++) out[i] = -out[i];   // H v = -(dF/dx) v
+    return MDN_OK;
 }
 
 int mdn_set_params(mdn_ctx* ctx, const double* kstr, const double* eql, const double* kbend) {
@@ -2199,7 +2280,17 @@ int mdn_bench_forces(mdn_ctx* ctx, int32_t precision, int32_t iters, double* ms_
                                        : bench_forces_t<double>(ctx, iters, ms_per_eval, kernel_ms);
 }
 
-int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const double* v, int32_t n_var,
+// n_vec products (tangents at hb_v) as mdn_hvp_batch launches them, into hb_out.
+static int hvp_batch_launch(mdn_ctx* ctx, const double* x, int prec, int n_vec, long slot_stride, long piece_stride) {
+    HIPCHK(hipMemsetAsync(ctx->bad_count.p, 0, sizeof(int), ctx->stream));
+    launch_tangents(ctx, x, prec, ctx->hb_v.p, n_vec, ctx->hb_slots.p, slot_stride, ctx->hb_piece.p, piece_stride);
+    HIPCHK(hipGetLastError());
+    gather_tangents(ctx, n_vec, ctx->hb_slots.p, slot_stride, ctx->hb_piece.p, piece_stride, ctx->hb_out.p);
+    HIPCHK(hipGetLastError());
+    return MDN_OK;
+}
+
+int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const double* v, int32_t n_var, int32_t n_vec,
                       double* ms_per_hvp, double* ms_per_param_grad) {
     int st = need_system(ctx, true);
     if (st) return st;
@@ -2207,15 +2298,32 @@ int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const doub
         return fail(ctx, MDN_ERR_ARG, "unknown precision %d", precision);
     if (iters <= 0) return fail(ctx, MDN_ERR_ARG, "iters must be positive");
     if (!v || n_var != 3 * ctx->n_beads) return fail(ctx, MDN_ERR_ARG, "n_var must be 3*n_beads = %d", 3 * ctx->n_beads);
+    if (n_vec < 1 || n_vec > MDN_HVP_BATCH_CHUNK)
+        return fail(ctx, MDN_ERR_ARG, "n_vec must be in [1, %d]", MDN_HVP_BATCH_CHUNK);
     HIPCHK(hipSetDevice(ctx->device));
+    const DevSys& s = ctx->sys;
     const double* x = ctx->x[ctx->cur].p;
-    HIPCHK(ctx->tan_v.ensure(n_var));
-    HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx->ev_U.ensure(std::max(1, ctx->sys.n[FF_EXVOL])));
+    const long slot_stride = 3 * std::max(1L, s.slot0[NFF]) + GCHUNK_PAD;
+    const long piece_stride = (long)NBOND * 3 * ctx->n_beads;
+    if (n_vec == 1) {
+        HIPCHK(ctx->tan_v.ensure(n_var));
+        HIPCHK(hipMemcpyAsync(ctx->tan_v.p, v, sizeof(double) * n_var, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        HIPCHK(ctx->hb_v.ensure((size_t)n_vec * n_var));
+        HIPCHK(ctx->hb_slots.ensure((size_t)n_vec * slot_stride));
+        HIPCHK(ctx->hb_piece.ensure((size_t)n_vec * piece_stride));
+        HIPCHK(ctx->hb_out.ensure((size_t)n_vec * n_var));
+        HIPCHK(hipMemcpyAsync(ctx->hb_v.p, v, sizeof(double) * n_vec * n_var, hipMemcpyHostToDevice, ctx->stream));
+    }
+    auto hvps = [&]() {
+        return n_vec == 1 ? hvp_launch(ctx, x, precision)
+                          : hvp_batch_launch(ctx, x, precision, n_vec, slot_stride, piece_stride);
+    };
+    HIPCHK(ctx->ev_U.ensure(std::max(1, s.n[FF_EXVOL])));
     HIPCHK(ctx->grad_out.ensure(3 * (size_t)std::max(1, ctx->n_cyl)));
     // once with the checks: every pair has a tangent and a closed-form energy at x, so the timed
-    // launches below do exactly the work of mdn_hvp and mdn_param_grad on the device
-    if ((st = hvp_launch(ctx, x, precision))) return st;
+    // launches below do exactly the work of mdn_hvp (mdn_hvp_batch) and mdn_param_grad on the device
+    if ((st = hvps())) return st;
     HIPCHK(hipMemcpyAsync(ctx->h_int, ctx->bad_count.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (ctx->h_int[0] != 0)
@@ -2223,12 +2331,12 @@ int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const doub
                     ctx->h_int[0]);
     if ((st = param_grad_launch(ctx, x, precision, true))) return st;
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (ctx->sys.n[FF_EXVOL] > 0 && ctx->h_int[0] != 0)
+    if (s.n[FF_EXVOL] > 0 && ctx->h_int[0] != 0)
         return fail(ctx, MDN_ERR_UNSUPPORTED, "%d excluded-volume pairs need the Simpson energy", ctx->h_int[0]);
     hipEvent_t e0, e1, e2;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1)); HIPCHK(hipEventCreate(&e2));
     HIPCHK(hipEventRecord(e0, ctx->stream));
-    for (int i = 0; i < iters && !st; i++) st = hvp_launch(ctx, x, precision);
+    for (int i = 0; i < iters && !st; i++) st = hvps();
     HIPCHK(hipEventRecord(e1, ctx->stream));
     for (int i = 0; i < iters && !st; i++) st = param_grad_launch(ctx, x, precision, false);
     HIPCHK(hipEventRecord(e2, ctx->stream));
@@ -2238,7 +2346,7 @@ int mdn_bench_adjoint(mdn_ctx* ctx, int32_t precision, int32_t iters, const doub
     HIPCHK(hipEventElapsedTime(&b, e1, e2));
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     if (st) return st;
-    if (ms_per_hvp) *ms_per_hvp = a / iters;
+    if (ms_per_hvp) *ms_per_hvp = a / iters / n_vec;
     if (ms_per_param_grad) *ms_per_param_grad = b / iters;
     return MDN_OK;
 }

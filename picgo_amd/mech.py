@@ -32,7 +32,7 @@ EXPORTED = [
     "mdn_last_min_stats", "mdn_energy_mixed", "mdn_minimize_batched", "mdn_last_batch_schedule",
     "mdn_param_grad", "mdn_hvp", "mdn_set_params", "mdn_boundary_load_forces", "mdn_crosslinks_update",
     "mdn_forces_noalloc", "mdn_displace", "mdn_perturb", "mdn_param_grad_mixed", "mdn_hvp_mixed",
-    "mdn_dof_serialize", "mdn_binding_pairs", "mdn_bench_adjoint",
+    "mdn_dof_serialize", "mdn_binding_pairs", "mdn_bench_adjoint", "mdn_hvp_batch",
 ]
 
 
@@ -74,7 +74,7 @@ def lib():
         L.mdn_minimize.argtypes = [C.c_void_p, C.POINTER(abi.CgParams), dp, dp, C.POINTER(abi.MinResult)]
         L.mdn_bench_forces.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, dp]
         L.mdn_bench_energy.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp]
-        L.mdn_bench_adjoint.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, C.c_int32, dp, dp]
+        L.mdn_bench_adjoint.argtypes = [C.c_void_p, C.c_int32, C.c_int32, dp, C.c_int32, C.c_int32, dp, dp]
         L.mdn_set_coords.argtypes = [C.c_void_p, dp]
         L.mdn_get_coords.argtypes = [C.c_void_p, dp]
         L.mdn_forces_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int32]
@@ -85,6 +85,7 @@ def lib():
         L.mdn_param_grad.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.mdn_hvp.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
         L.mdn_hvp_mixed.argtypes = [C.c_void_p, dp, dp, dp, C.c_int32]
+        L.mdn_hvp_batch.argtypes = [C.c_void_p, C.c_int32, dp, dp, dp, C.c_int32, C.c_int32]
         L.mdn_param_grad_mixed.argtypes = [C.c_void_p, dp, dp, C.c_int64]
         L.mdn_set_params.argtypes = [C.c_void_p, dp, dp, dp]
         L.mdn_last_min_stats.argtypes = [C.c_void_p, i64p]
@@ -297,6 +298,16 @@ class ForceFieldManager:
         self._chk(fn(self._h, cp, vv.ctypes.data_as(dp), out.ctypes.data_as(dp), out.size))
         return out
 
+    def hvp_batch(self, V, coord=None, precision=abi.MDN_PREC_FP64):
+        """Hessian-vector products of the rows of V (n_vec x 3 n_beads) at one point, as rows
+        (mdn_hvp_batch)."""
+        VV = _f64(V).reshape(-1, 3 * self.n_beads)
+        out = np.zeros_like(VV)
+        cp = _f64(coord).ctypes.data_as(dp) if coord is not None else None
+        self._chk(self._L.mdn_hvp_batch(self._h, precision, cp, VV.ctypes.data_as(dp), out.ctypes.data_as(dp),
+                                        VV.shape[1], VV.shape[0]))
+        return out
+
     def set_params(self, kstr=None, eql=None, kbend=None):
         """Replace per-cylinder stretching k, rest length and bending k (mdn_set_params)."""
         arrs = [None if a is None else _f64(a) for a in (kstr, eql, kbend)]
@@ -314,12 +325,13 @@ class ForceFieldManager:
         return a.value
 
     def bench_adjoint(self, iters, v, precision=abi.MDN_PREC_FP64):
-        """Mean device ms of one Hessian-vector product along v and of one dE/dtheta at the current
-        device coordinates (mdn_bench_adjoint)."""
-        vv = _f64(v)
+        """Mean device ms per Hessian-vector product along the rows of v (one row: mdn_hvp's
+        launches; several: one chunk of mdn_hvp_batch) and per dE/dtheta at the current device
+        coordinates (mdn_bench_adjoint)."""
+        vv = _f64(v).reshape(-1, 3 * self.n_beads)
         a, b = C.c_double(), C.c_double()
-        self._chk(self._L.mdn_bench_adjoint(self._h, precision, iters, vv.ctypes.data_as(dp), vv.size, C.byref(a),
-                                            C.byref(b)))
+        self._chk(self._L.mdn_bench_adjoint(self._h, precision, iters, vv.ctypes.data_as(dp), vv.shape[1],
+                                            vv.shape[0], C.byref(a), C.byref(b)))
         return a.value, b.value
 
     def close(self):

@@ -237,6 +237,28 @@ def test_bench_adjoint_times_the_entry_points_work(relaxed):
         assert 0.0 < ms_hvp < 100.0 and 0.0 < ms_grad < 100.0
         np.testing.assert_array_equal(g.hvp(v, precision=prec), g.hvp(v, x, precision=prec))
         np.testing.assert_array_equal(g.param_grad(precision=prec), g.param_grad(x, precision=prec))
+    with pytest.raises(mech.MdnError) as ei:   # more products than one launch takes
+        g.bench_adjoint(1, np.tile(v, (17, 1)))
+    assert ei.value.code == abi.MDN_ERR_ARG
+
+
+def test_hvp_batch_equals_single_products(relaxed):
+    """mdn_hvp_batch: 20 products (a chunk of 16 in one tangent launch, then 4) equal mdn_hvp's
+    and mdn_hvp_mixed's bit for bit, at given coordinates and at the device's current ones; the
+    batched launches as timed by mdn_bench_adjoint leave the same products on the device."""
+    net, g, x = relaxed
+    rng = np.random.default_rng(12)
+    V = np.stack([_unit(rng, x.size) for _ in range(20)])
+    g.set_coords(x)
+    for prec in (abi.MDN_PREC_FP64, abi.MDN_PREC_MIXED):
+        single = np.stack([g.hvp(v, x, precision=prec) for v in V])
+        np.testing.assert_array_equal(g.hvp_batch(V, x, precision=prec), single)
+        np.testing.assert_array_equal(g.hvp_batch(V[:3], precision=prec), single[:3])
+        ms_one, _ = g.bench_adjoint(2, V[0], precision=prec)
+        ms_16, _ = g.bench_adjoint(2, V[:16], precision=prec)
+        print(f"precision {prec}: {1e3 * ms_one:.1f} us per product alone, {1e3 * ms_16:.1f} us per product "
+              f"in a launch of 16", flush=True)
+        np.testing.assert_array_equal(g.hvp_batch(V[:1], precision=prec), single[:1])
     with pytest.raises(mech.MdnError) as ei:
-        g.bench_adjoint(4, v[:-1])
+        g.hvp_batch(V[:0])
     assert ei.value.code == abi.MDN_ERR_ARG
